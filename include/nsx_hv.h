@@ -1,0 +1,122 @@
+/* nsx_hv.h -- C ABI of the MI355X proton-neutron J-scheme Hamiltonian (H*v)
+ * and thick-restart Lanczos, a drop-in for NuShellX's NuTRL/NuLnz hot path.
+ *
+ * Plain C: pointers, sizes and return codes only.  Every function returns 0 on
+ * success and a negative NSX_E* code on failure, with the message available
+ * from nsx_last_error() (thread-local).  Nothing here exits or throws.
+ */
+#ifndef NSX_HV_H
+#define NSX_HV_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NSX_OK = 0,
+  NSX_EIO = -1,     /* a file could not be opened or read */
+  NSX_EFORMAT = -2, /* a record has an unexpected size or content */
+  NSX_ETAG = -3,    /* .sgt x tag differs from the .tri tag */
+  NSX_ECAP = -4,    /* unsupported input (e.g. truncated basis) or size limit */
+  NSX_EGPU = -5,    /* HIP runtime error */
+  NSX_EARG = -6     /* invalid argument */
+};
+
+/* Arithmetic of the block contractions.  Everything outside them (the Lanczos
+ * recurrence, reorthogonalisation, Ritz solve, residuals) is FP64 in all modes. */
+enum {
+  NSX_FP64 = 0,   /* FP64 GEMMs: the plain drop-in */
+  NSX_FP32 = 1,   /* FP32 operands, FP32 accumulation */
+  NSX_BF16X3 = 2  /* BF16 hi/lo split operands (3 products), FP32 accumulation */
+};
+
+typedef struct nsx_hv nsx_hv;
+
+/* Reads the files program NuTRL reads for one nucleus and total J and uploads
+ * the operators.  Replaces Lnz_TRD's setup, src/NuTRL/NuCoreX.f90:807-1367,
+ * with the file set of :1634-1670:
+ *   nus_prefix  <Nucleus+Jcode>      -> <nus_prefix>.nus
+ *   opfile_a    <Nucleus>a           -> .tri/.trd, and .mat/.sgt per J of side a
+ *   opfile_b    <Nucleus>b           -> .tri/.trd, and .mat/.sgt per J of side b
+ *   intfile     <Nucleus><xtra>      -> .oph and .spe
+ * Names are used as given after trimming blanks (Fortran blank-padded names). */
+int nsx_hv_create(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
+                  const char *intfile, int precision, nsx_hv **h);
+void nsx_hv_destroy(nsx_hv *h);
+
+/* N = no_state, basd = number of (2Ja,2Jb) blocks (src/NuTRL/NuCoreX.f90:975-988). */
+int nsx_hv_dims(const nsx_hv *h, int64_t *N, int32_t *basd);
+/* Block b: 2Ja, 2Jb, offset posnv (0-based) and size dimnv. */
+int nsx_hv_block(const nsx_hv *h, int32_t b, int32_t *ja2, int32_t *jb2, int64_t *pos, int64_t *dim);
+/* lcoef parity mask of the basis (1 = state of the requested parity), N bytes. */
+int nsx_hv_lcoef(const nsx_hv *h, uint8_t *mask);
+/* Algorithmic work of one H*v: the reference loop nest's sparse flops (all terms,
+ * and the like-particle H_a V + V H_b^T part of them) and FP64 bytes
+ * (v read, w written, every operator value read once). */
+int nsx_hv_work(const nsx_hv *h, double *flops, double *flops_like, int64_t *bytes);
+
+/* v_out = H v_in on host vectors.  The REAL*4 form is the drop-in for
+ *   call multiplyx(); call multiplya(); call multiplyb()
+ * with coefR -> v_in and coefL -> v_out (src/NuTRL/NuCoreX.f90:317-319,340-342);
+ * v_out is overwritten, as multiplyx does. */
+int nsx_hv_apply_f32(nsx_hv *h, const float *v_in, float *v_out);
+int nsx_hv_apply_f64(nsx_hv *h, const double *v_in, double *v_out);
+/* Device vectors (FP64, N each) on a HIP stream (NULL = the handle's stream). */
+int nsx_hv_apply_dev(nsx_hv *h, const double *d_in, double *d_out, void *stream);
+
+/* ---------------------------------------------------------------- batches
+ * Many instances (nuclei, J values, interaction samples) in one launch pair;
+ * vectors are concatenated in the order given (offsets from nsx_batch_offset). */
+typedef struct nsx_batch nsx_batch;
+int nsx_batch_create(nsx_hv *const *hs, int32_t n, int precision, nsx_batch **b);
+void nsx_batch_destroy(nsx_batch *b);
+int nsx_batch_dims(const nsx_batch *b, int64_t *N_total);
+int nsx_batch_offset(const nsx_batch *b, int32_t i, int64_t *off);
+int nsx_batch_work(const nsx_batch *b, double *flops, double *flops_like, int64_t *bytes);
+int nsx_batch_apply_dev(nsx_batch *b, const double *d_in, double *d_out, void *stream);
+
+/* ---------------------------------------------------------------- Lanczos
+ * Thick-restart Lanczos for the lowest eigenpairs (replaces TRL.Lnz_OP,
+ * src/NuTRL/NuCoreX.f90:240-542, with device-resident basis vectors).
+ * The H*v runs in the handle's precision; everything else is FP64. */
+typedef struct {
+  int32_t nev;        /* eigenvalues wanted (lowest) */
+  int32_t max_basis;  /* Krylov basis size before a restart (e.g. 50) */
+  int32_t keep;       /* Ritz vectors kept at a restart (e.g. 10) */
+  int32_t max_iter;   /* cap on H*v applications */
+  double tol;         /* residual |beta * s_last| bound per wanted pair (MeV) */
+  uint64_t seed;      /* random start when no start vector is given */
+} nsx_trl_opts;
+
+typedef struct {
+  int32_t matvecs, restarts, converged;
+  double seconds;     /* wall time of the solve */
+} nsx_trl_stats;
+
+/* start: optional host start vector (N doubles, NULL = seeded random on lcoef).
+ * evals: nev doubles.  evecs: optional host N*nev doubles (column-major). */
+int nsx_trl_solve(nsx_hv *h, const nsx_trl_opts *opt, const double *start, double *evals,
+                  double *evecs, double *residuals, nsx_trl_stats *stats);
+
+/* ---------------------------------------------------------------- device helpers
+ * For hosts without their own HIP binding (Fortran, ctypes): device buffers,
+ * copies on the default stream, and HIP-event timing of the H*v kernels.
+ * These have no counterpart in the reference (it keeps everything on the host). */
+int nsx_dev_alloc(int64_t bytes, void **ptr);
+int nsx_dev_free(void *ptr);
+int nsx_dev_upload(void *dst, const void *src, int64_t bytes);
+int nsx_dev_download(void *dst, const void *src, int64_t bytes);
+int nsx_dev_sync(void);
+/* warmup + iters batched H*v on the batch stream; HIP events bracket every
+ * launch.  Averages in ms: whole H*v, pn kernel, reduce kernel. */
+int nsx_batch_time(nsx_batch *b, const double *d_in, double *d_out, int32_t warmup, int32_t iters,
+                   double *ms_hv, double *ms_pn, double *ms_reduce);
+/* device name and compute-unit count of the current device */
+int nsx_device_info(char *name, int32_t len, int32_t *cus);
+
+const char *nsx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

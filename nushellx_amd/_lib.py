@@ -1,0 +1,143 @@
+"""ctypes binding of ``libnsx.so`` (the C ABI declared in ``include/nsx_hv.h``).
+
+The library is built in-tree by ``nushellx_amd.build``.  Loading fails loudly
+when it is missing: there is no CPU fallback for the H*v.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnsx.so")
+
+NSX_FP64, NSX_FP32, NSX_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": NSX_FP64, "fp32": NSX_FP32, "bf16x3": NSX_BF16X3}
+
+# every symbol include/nsx_hv.h declares
+EXPORTS = [
+    "nsx_hv_create", "nsx_hv_destroy", "nsx_hv_dims", "nsx_hv_block", "nsx_hv_lcoef",
+    "nsx_hv_work", "nsx_hv_apply_f32", "nsx_hv_apply_f64", "nsx_hv_apply_dev",
+    "nsx_batch_create", "nsx_batch_destroy", "nsx_batch_dims", "nsx_batch_offset",
+    "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
+    "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
+    "nsx_last_error",
+]
+
+
+class TrlOpts(C.Structure):
+    _fields_ = [("nev", C.c_int32), ("max_basis", C.c_int32), ("keep", C.c_int32),
+                ("max_iter", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64)]
+
+
+class TrlStats(C.Structure):
+    _fields_ = [("matvecs", C.c_int32), ("restarts", C.c_int32), ("converged", C.c_int32),
+                ("seconds", C.c_double)]
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m nushellx_amd.build` "
+                           "(the H*v has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    L.nsx_hv_create.argtypes = [C.c_char_p] * 4 + [C.c_int, C.POINTER(vp)]
+    L.nsx_hv_destroy.argtypes = [vp]
+    L.nsx_hv_destroy.restype = None
+    L.nsx_hv_dims.argtypes = [vp, C.POINTER(i64), C.POINTER(i32)]
+    L.nsx_hv_block.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]
+    L.nsx_hv_lcoef.argtypes = [vp, vp]
+    L.nsx_hv_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i64)]
+    L.nsx_hv_apply_f32.argtypes = [vp, vp, vp]
+    L.nsx_hv_apply_f64.argtypes = [vp, vp, vp]
+    L.nsx_hv_apply_dev.argtypes = [vp, vp, vp, vp]
+    L.nsx_batch_create.argtypes = [C.POINTER(vp), i32, C.c_int, C.POINTER(vp)]
+    L.nsx_batch_destroy.argtypes = [vp]
+    L.nsx_batch_destroy.restype = None
+    L.nsx_batch_dims.argtypes = [vp, C.POINTER(i64)]
+    L.nsx_batch_offset.argtypes = [vp, i32, C.POINTER(i64)]
+    L.nsx_batch_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i64)]
+    L.nsx_batch_apply_dev.argtypes = [vp, vp, vp, vp]
+    L.nsx_trl_solve.argtypes = [vp, C.POINTER(TrlOpts), vp, vp, vp, vp, C.POINTER(TrlStats)]
+    L.nsx_dev_alloc.argtypes = [i64, C.POINTER(vp)]
+    L.nsx_dev_free.argtypes = [vp]
+    L.nsx_dev_upload.argtypes = [vp, vp, i64]
+    L.nsx_dev_download.argtypes = [vp, vp, i64]
+    L.nsx_dev_sync.argtypes = []
+    L.nsx_batch_time.argtypes = [vp, vp, vp, i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]
+    L.nsx_device_info.argtypes = [C.c_char_p, i32, C.POINTER(i32)]
+    L.nsx_last_error.restype = C.c_char_p
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().nsx_last_error().decode(errors="replace")
+        raise NsxError(rc, f"{what} failed ({rc}): {msg}")
+
+
+class NsxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class DeviceBuffer:
+    """A device allocation owned through the library's own HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(lib().nsx_dev_alloc(int(nbytes), C.byref(p)), "nsx_dev_alloc")
+        self.ptr = p
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a) -> None:
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().nsx_dev_upload(self.ptr, a.ctypes.data, a.nbytes), "nsx_dev_upload")
+
+    def download(self, a) -> None:
+        assert a.nbytes <= self.nbytes and a.flags["C_CONTIGUOUS"]
+        check(lib().nsx_dev_download(a.ctypes.data, self.ptr, a.nbytes), "nsx_dev_download")
+
+    def data_ptr(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().nsx_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync() -> None:
+    check(lib().nsx_dev_sync(), "nsx_dev_sync")
+
+
+def device_info():
+    name = C.create_string_buffer(128)
+    cus = C.c_int32()
+    check(lib().nsx_device_info(name, 128, C.byref(cus)), "nsx_device_info")
+    return name.value.decode(), int(cus.value)

@@ -1,0 +1,324 @@
+// C ABI (include/nsx_hv.h): handles, device plans, host/device apply, batches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nsx_hv.h"
+#include "nsx_api_internal.hpp"
+
+namespace nsx {
+
+static thread_local std::string g_err;
+void set_error(const std::string& e) { g_err = e; }
+
+static int code_of(const std::string& e) {
+  if (e.rfind("cannot open", 0) == 0) return NSX_EIO;
+  if (e.find("not consistent") != std::string::npos) return NSX_ETAG;
+  if (e.find("not supported") != std::string::npos) return NSX_ECAP;
+  return NSX_EFORMAT;
+}
+
+template <typename T>
+static std::string up(T*& dst, const std::vector<T>& src) {
+  dst = nullptr;
+  if (src.empty()) return "";
+  if (hipMalloc(&dst, sizeof(T) * src.size()) != hipSuccess) return "hipMalloc failed";
+  if (hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return "hipMemcpy failed";
+  return "";
+}
+
+std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
+  std::string e;
+  if (!(e = up(d.items, hp.items)).empty()) return e;
+  if (!(e = up(d.aterms, hp.aterms)).empty()) return e;
+  if (!(e = up(d.rterms, hp.rterms)).empty()) return e;
+  if (!(e = up(d.blocks, hp.blocks)).empty()) return e;
+  if (!(e = up(d.part_list, hp.part_list)).empty()) return e;
+  if (!(e = up(d.Ad, hp.Ad)).empty()) return e;
+  if (!(e = up(d.Bh, hp.Bh)).empty()) return e;
+  if (!(e = up(d.H, hp.H)).empty()) return e;
+  if (hp.part_size > 0 && hipMalloc(&d.part, sizeof(double) * hp.part_size) != hipSuccess)
+    return "hipMalloc failed (partials)";
+  d.n_items = (int)hp.items.size();
+  d.n_blocks = (int)hp.blocks.size();
+  d.lds_bytes = sizeof(double) * std::max(hp.max_lds, 1);
+  d.N = hp.N;
+  if (d.lds_bytes > 64 * 1024) {
+    if (hipFuncSetAttribute((const void*)k_pn_partial_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)d.lds_bytes) != hipSuccess)
+      return "LDS request too large";
+  }
+  return "";
+}
+
+void free_plan(DevicePlan& d) {
+  (void)hipFree(d.items); (void)hipFree(d.aterms); (void)hipFree(d.rterms); (void)hipFree(d.blocks);
+  (void)hipFree(d.part_list); (void)hipFree(d.Ad); (void)hipFree(d.Bh); (void)hipFree(d.H); (void)hipFree(d.part);
+  d = DevicePlan();
+}
+
+}  // namespace nsx
+
+using namespace nsx;
+
+static std::string trimmed(const char* s) {
+  std::string t = s ? s : "";
+  size_t a = t.find_first_not_of(" \t");
+  size_t b = t.find_last_not_of(" \t");
+  if (a == std::string::npos) return "";
+  return t.substr(a, b - a + 1);
+}
+
+#define NSX_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      set_error(std::string("HIP error: ") + hipGetErrorString(_e));         \
+      return NSX_EGPU;                                                       \
+    }                                                                        \
+  } while (0)
+
+extern "C" {
+
+const char* nsx_last_error(void) { return nsx::g_err.c_str(); }
+
+int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfile_b,
+                  const char* intfile, int precision, nsx_hv** out) {
+  if (!out) { set_error("null output handle"); return NSX_EARG; }
+  *out = nullptr;
+  if (precision < NSX_FP64 || precision > NSX_BF16X3) { set_error("unknown precision"); return NSX_EARG; }
+  std::unique_ptr<nsx_hv> h(new nsx_hv());
+  h->precision = precision;
+  std::string e = build_model(trimmed(nus_prefix), trimmed(opfile_a), trimmed(opfile_b),
+                              trimmed(intfile), h->model);
+  if (!e.empty()) { set_error(e); return code_of(e); }
+  plan_add(h->hplan, h->model);
+  if (precision != NSX_FP64) {
+    e = build_lowp(h->hplan, precision, h->lowp);
+    if (!e.empty()) { set_error(e); return NSX_EGPU; }
+  }
+  e = upload_plan(h->hplan, h->dplan);
+  if (!e.empty()) { set_error(e); return NSX_EGPU; }
+  NSX_HIP(hipStreamCreate(&h->stream));
+  const int64_t N = std::max<int64_t>(h->model.N, 1);
+  NSX_HIP(hipMalloc(&h->d_in, sizeof(double) * N));
+  NSX_HIP(hipMalloc(&h->d_out, sizeof(double) * N));
+  *out = h.release();
+  return NSX_OK;
+}
+
+void nsx_hv_destroy(nsx_hv* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_plan(h->dplan);
+  free_lowp(h->lowp);
+  (void)hipFree(h->d_in);
+  (void)hipFree(h->d_out);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int nsx_hv_dims(const nsx_hv* h, int64_t* N, int32_t* basd) {
+  if (!h) { set_error("null handle"); return NSX_EARG; }
+  if (N) *N = h->model.N;
+  if (basd) *basd = h->model.basd;
+  return NSX_OK;
+}
+
+int nsx_hv_block(const nsx_hv* h, int32_t b, int32_t* ja2, int32_t* jb2, int64_t* pos, int64_t* dim) {
+  if (!h || b < 0 || b >= h->model.basd) { set_error("bad block index"); return NSX_EARG; }
+  *ja2 = h->model.bas_a[b];
+  *jb2 = h->model.bas_b[b];
+  *pos = h->model.posnv[b];
+  *dim = int64_t(h->model.da[b]) * h->model.db[b];
+  return NSX_OK;
+}
+
+int nsx_hv_lcoef(const nsx_hv* h, uint8_t* mask) {
+  if (!h || !mask) { set_error("null argument"); return NSX_EARG; }
+  std::memcpy(mask, h->model.lcoef.data(), h->model.lcoef.size());
+  return NSX_OK;
+}
+
+int nsx_hv_work(const nsx_hv* h, double* flops, double* flops_like, int64_t* bytes) {
+  if (!h) { set_error("null handle"); return NSX_EARG; }
+  if (flops) *flops = h->hplan.flops_ref;
+  if (flops_like) *flops_like = h->hplan.flops_like;
+  if (bytes) *bytes = h->hplan.bytes_alg;
+  return NSX_OK;
+}
+
+int nsx_hv_apply_dev(nsx_hv* h, const double* d_in, double* d_out, void* stream) {
+  if (!h || (!d_in && h->model.N) || (!d_out && h->model.N)) { set_error("null argument"); return NSX_EARG; }
+  if (d_in == d_out && h->model.N) { set_error("in-place H*v is not supported"); return NSX_EARG; }
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  hipError_t e = h->precision == NSX_FP64 ? launch_hv_f64(h->dplan, d_in, d_out, s)
+                                          : launch_hv_lowp(h->dplan, h->lowp, d_in, d_out, s);
+  if (e != hipSuccess) { set_error(std::string("HIP error: ") + hipGetErrorString(e)); return NSX_EGPU; }
+  return NSX_OK;
+}
+
+int nsx_hv_apply_f64(nsx_hv* h, const double* v_in, double* v_out) {
+  if (!h || (h->model.N && (!v_in || !v_out))) { set_error("null argument"); return NSX_EARG; }
+  const size_t bytes = sizeof(double) * h->model.N;
+  if (!bytes) return NSX_OK;
+  NSX_HIP(hipMemcpyAsync(h->d_in, v_in, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = nsx_hv_apply_dev(h, h->d_in, h->d_out, h->stream);
+  if (rc) return rc;
+  NSX_HIP(hipMemcpyAsync(v_out, h->d_out, bytes, hipMemcpyDeviceToHost, h->stream));
+  NSX_HIP(hipStreamSynchronize(h->stream));
+  return NSX_OK;
+}
+
+int nsx_hv_apply_f32(nsx_hv* h, const float* v_in, float* v_out) {
+  if (!h || (h->model.N && (!v_in || !v_out))) { set_error("null argument"); return NSX_EARG; }
+  std::vector<double> a(v_in, v_in + h->model.N), b(h->model.N);
+  int rc = nsx_hv_apply_f64(h, a.data(), b.data());
+  if (rc) return rc;
+  for (int64_t i = 0; i < h->model.N; ++i) v_out[i] = (float)b[i];
+  return NSX_OK;
+}
+
+int nsx_batch_create(nsx_hv* const* hs, int32_t n, int precision, nsx_batch** out) {
+  if (!out || !hs || n <= 0) { set_error("bad batch arguments"); return NSX_EARG; }
+  *out = nullptr;
+  std::unique_ptr<nsx_batch> b(new nsx_batch());
+  b->precision = precision;
+  for (int i = 0; i < n; ++i) {
+    if (!hs[i]) { set_error("null handle in batch"); return NSX_EARG; }
+    plan_add(b->hplan, hs[i]->model);
+  }
+  std::string e;
+  if (precision != NSX_FP64 && !(e = build_lowp(b->hplan, precision, b->lowp)).empty()) {
+    set_error(e);
+    return NSX_EGPU;
+  }
+  if (!(e = upload_plan(b->hplan, b->dplan)).empty()) { set_error(e); return NSX_EGPU; }
+  NSX_HIP(hipStreamCreate(&b->stream));
+  *out = b.release();
+  return NSX_OK;
+}
+
+void nsx_batch_destroy(nsx_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_plan(b->dplan);
+  free_lowp(b->lowp);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int nsx_batch_dims(const nsx_batch* b, int64_t* N) {
+  if (!b || !N) { set_error("null argument"); return NSX_EARG; }
+  *N = b->hplan.N;
+  return NSX_OK;
+}
+
+int nsx_batch_offset(const nsx_batch* b, int32_t i, int64_t* off) {
+  if (!b || !off || i < 0 || i >= (int)b->hplan.vbase.size()) { set_error("bad instance index"); return NSX_EARG; }
+  *off = b->hplan.vbase[i];
+  return NSX_OK;
+}
+
+int nsx_batch_work(const nsx_batch* b, double* flops, double* flops_like, int64_t* bytes) {
+  if (!b) { set_error("null batch"); return NSX_EARG; }
+  if (flops) *flops = b->hplan.flops_ref;
+  if (flops_like) *flops_like = b->hplan.flops_like;
+  if (bytes) *bytes = b->hplan.bytes_alg;
+  return NSX_OK;
+}
+
+int nsx_batch_apply_dev(nsx_batch* b, const double* d_in, double* d_out, void* stream) {
+  if (!b || !d_in || !d_out || d_in == d_out) { set_error("bad batch apply arguments"); return NSX_EARG; }
+  hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+  hipError_t e = b->precision == NSX_FP64 ? launch_hv_f64(b->dplan, d_in, d_out, s)
+                                          : launch_hv_lowp(b->dplan, b->lowp, d_in, d_out, s);
+  if (e != hipSuccess) { set_error(std::string("HIP error: ") + hipGetErrorString(e)); return NSX_EGPU; }
+  return NSX_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int nsx_dev_alloc(int64_t bytes, void** ptr) {
+  if (!ptr || bytes < 0) { set_error("bad allocation request"); return NSX_EARG; }
+  *ptr = nullptr;
+  NSX_HIP(hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 8)));
+  return NSX_OK;
+}
+int nsx_dev_free(void* ptr) {
+  NSX_HIP(hipFree(ptr));
+  return NSX_OK;
+}
+int nsx_dev_upload(void* dst, const void* src, int64_t bytes) {
+  if (bytes > 0) NSX_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return NSX_OK;
+}
+int nsx_dev_download(void* dst, const void* src, int64_t bytes) {
+  if (bytes > 0) NSX_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return NSX_OK;
+}
+int nsx_dev_sync(void) {
+  NSX_HIP(hipDeviceSynchronize());
+  return NSX_OK;
+}
+
+int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warmup, int32_t iters,
+                   double* ms_hv, double* ms_pn, double* ms_reduce) {
+  if (!b || iters <= 0) { set_error("bad timing arguments"); return NSX_EARG; }
+  hipStream_t s = b->stream;
+  for (int i = 0; i < warmup; ++i) {
+    int rc = nsx_batch_apply_dev(b, d_in, d_out, s);
+    if (rc) return rc;
+  }
+  std::vector<hipEvent_t> ev(3 * (size_t)iters);
+  for (auto& e : ev) NSX_HIP(hipEventCreate(&e));
+  for (int i = 0; i < iters; ++i) {
+    NSX_HIP(hipEventRecord(ev[3 * i], s));
+    hipError_t e = b->precision == NSX_FP64 ? launch_pn_f64(b->dplan, d_in, s)
+                                            : launch_pn_lowp(b->dplan, b->lowp, d_in, s);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
+    NSX_HIP(hipEventRecord(ev[3 * i + 1], s));
+    e = b->precision == NSX_FP64 ? launch_reduce_f64(b->dplan, d_in, d_out, s)
+                                 : launch_reduce_lowp(b->dplan, b->lowp, d_in, d_out, s);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
+    NSX_HIP(hipEventRecord(ev[3 * i + 2], s));
+  }
+  NSX_HIP(hipStreamSynchronize(s));
+  double pn = 0.0, red = 0.0;
+  for (int i = 0; i < iters; ++i) {
+    float a = 0.f, c = 0.f;
+    NSX_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+    NSX_HIP(hipEventElapsedTime(&c, ev[3 * i + 1], ev[3 * i + 2]));
+    pn += a;
+    red += c;
+  }
+  float tot = 0.f;
+  NSX_HIP(hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)iters - 1]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (ms_hv) *ms_hv = tot / iters;
+  if (ms_pn) *ms_pn = pn / iters;
+  if (ms_reduce) *ms_reduce = red / iters;
+  return NSX_OK;
+}
+
+int nsx_device_info(char* name, int32_t len, int32_t* cus) {
+  int dev = 0;
+  NSX_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  NSX_HIP(hipGetDeviceProperties(&p, dev));
+  if (name && len > 0) {
+    std::strncpy(name, p.gcnArchName, (size_t)len - 1);
+    name[len - 1] = 0;
+  }
+  if (cus) *cus = p.multiProcessorCount;
+  return NSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// Device-side plan and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "nsx_plan.hpp"
+
+namespace nsx {
+
+struct DevicePlan {
+  DItem* items = nullptr;
+  DATerm* aterms = nullptr;
+  DRTerm* rterms = nullptr;
+  DBlock* blocks = nullptr;
+  int64_t* part_list = nullptr;
+  double *Ad = nullptr, *Bh = nullptr, *H = nullptr, *part = nullptr;
+  int n_items = 0, n_blocks = 0;
+  size_t lds_bytes = 0;
+  int64_t N = 0;
+};
+
+// Uploads a host plan; returns "" or an error message.
+std::string upload_plan(const HostPlan& hp, DevicePlan& d);
+void free_plan(DevicePlan& d);
+
+hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
+hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s);
+hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
+const void* k_pn_partial_ptr();
+
+// Reduced-precision operands of the block contractions (FP32 or BF16 hi/lo).
+struct LowpPlan {
+  int precision = 0;
+  void* Ad = nullptr;   // operand copies in the working precision
+  void* Bh = nullptr;
+  void* H = nullptr;
+  void* vin = nullptr;  // input vector converted per call
+};
+std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp);
+void free_lowp(LowpPlan& lp);
+hipError_t launch_hv_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W,
+                          hipStream_t s);
+hipError_t launch_pn_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, hipStream_t s);
+hipError_t launch_reduce_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W,
+                              hipStream_t s);
+
+}  // namespace nsx

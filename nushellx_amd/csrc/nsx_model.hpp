@@ -1,0 +1,74 @@
+// Host setup of one pn J-scheme H*v instance (one nucleus, one total J).
+//
+// Mirrors what NuTRL's Lnz_TRD builds before its Lanczos loop
+// (src/NuTRL/NuCoreX.f90:807-1367): the (2Ja,2Jb) block basis, the Racah
+// recoupling scalars (:1071-1127) and the drdbh term tables (:1428-1577).
+// Instead of per-thread bitmask walks the terms are regrouped for the GPU:
+//
+//   W_L = H_a(Ja) V_L + V_L H_b(Jb)^T
+//       + sum_{Ja'} sum_{k=(Ja,Ja',sa,sa',lam)} A_k  sum_{Jb'} r(Ja,Jb,Ja',Jb',lam)
+//                                                   V_(Ja',Jb') Bhat(Jb,Jb',lam,sa,sa')^T
+//
+// with Bhat = sum_{sb,sb'} opmeV(sa,sa',sb,sb',lam) B(Jb,Jb',sb,sb',lam) folded once
+// (the b-density list of a drdbh entry depends only on (Jb,Jb',lam,sa,sa')).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nsx_io.hpp"
+
+namespace nsx {
+
+struct RTerm {
+  int R;        // right block index
+  int bhat;     // index into Model::bhat
+  double r;     // Racah factor r(Ja,Jb,Ja',Jb',lam)
+};
+struct ATerm {
+  int adens;    // index into Model::adense
+  std::vector<RTerm> rt;
+};
+struct Item {   // one (L, Ja') work item
+  int L;
+  int ajr;
+  std::vector<ATerm> at;
+};
+
+struct Model {
+  NusInput nus;
+  SideFiles A, B;
+  int J2 = 0, basd = 0, no_level = 5;
+  double renorm = 1.0;
+  int64_t N = 0;
+  bool azero = false, bzero = false;
+  std::vector<int> bas_a, bas_b, da, db;   // per block: 2Ja, 2Jb, d(Ja), d(Jb)
+  std::vector<int64_t> posnv;
+  std::vector<uint8_t> lcoef;              // parity mask of the basis states
+  std::vector<Item> items;
+  // dense a-densities d(Jl) x d(Jr), column-major, and their keys
+  std::vector<std::vector<double>> adense;
+  std::vector<std::array<int, 2>> adense_J;  // (2Jl, 2Jr)
+  // folded b-operators d(Jb) x d(Jb'), column-major
+  std::vector<std::vector<double>> bhat;
+  std::vector<std::array<int, 2>> bhat_J;    // (2Jb, 2Jb')
+  // algorithmic (reference sparse) work of one H*v, and term counts
+  double flops_ref = 0.0;               // all terms
+  double flops_like = 0.0;              // H_a V + V H_b^T part of flops_ref
+  int64_t n_aterms = 0, n_rterms = 0, n_bterms = 0;
+};
+
+// Reads every file of one NuTRL run and builds the model.  Returns "" on success.
+std::string build_model(const std::string& nus_prefix, const std::string& opfile_a,
+                        const std::string& opfile_b, const std::string& intfile, Model& m);
+
+// Restatement of the reference's float-bisection key search
+// (src/NuCommonX/NuCommonX.f90:147-183); 1-based index or 0.
+int tfind_ref(const std::vector<std::array<uint8_t, 8>>& keys, const uint8_t* key);
+
+double sixj(int a, int b, int c, int d, int e, int f);
+double racah_w(int j1, int j2, int j3, int j4, int J, int K);
+
+}  // namespace nsx

@@ -1,0 +1,176 @@
+"""Host-side mirror of NuTRL's H*v interface over the C ABI.
+
+NuTRL is launched as ``NuTRL <Nucleus+Jcode> <Nucleus>a <Nucleus>b <Nucleus><xtra>``
+(``src/NuShellX/NuShellx.f90:1060-1063``) and, after its setup, calls
+``multiplyx(); multiplya(); multiplyb()`` to turn ``coefR`` into ``coefL = H coefR``
+(``src/NuTRL/NuCoreX.f90:317-319``).  ``Hamiltonian`` takes the same four names,
+and ``multiply`` is that call sequence.  Errors surface as ``NsxError`` carrying
+the negative code of ``include/nsx_hv.h`` (the reference stops with a message).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._lib import NSX_FP64, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, check, device_sync, lib
+
+
+class Hamiltonian:
+    def __init__(self, nus_prefix: str, opfile_a: str, opfile_b: str, intfile: str,
+                 precision: str = "fp64"):
+        L = lib()
+        self.precision = precision
+        h = C.c_void_p()
+        check(L.nsx_hv_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
+                              intfile.encode(), PRECISIONS[precision], C.byref(h)), "nsx_hv_create")
+        self._h = h
+        n, b = C.c_int64(), C.c_int32()
+        check(L.nsx_hv_dims(h, C.byref(n), C.byref(b)), "nsx_hv_dims")
+        self.N, self.basd = int(n.value), int(b.value)
+
+    @classmethod
+    def from_args(cls, args: Sequence[str], precision: str = "fp64") -> "Hamiltonian":
+        return cls(*args, precision=precision)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nsx_hv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- basis ---------------------------------------------------------------
+    def blocks(self) -> List[Tuple[int, int, int, int]]:
+        out = []
+        a, b, p, d = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int64()
+        for i in range(self.basd):
+            check(lib().nsx_hv_block(self._h, i, C.byref(a), C.byref(b), C.byref(p), C.byref(d)),
+                  "nsx_hv_block")
+            out.append((a.value, b.value, p.value, d.value))
+        return out
+
+    def lcoef(self) -> np.ndarray:
+        m = np.zeros(max(self.N, 1), dtype=np.uint8)
+        check(lib().nsx_hv_lcoef(self._h, m.ctypes.data), "nsx_hv_lcoef")
+        return m[:self.N].astype(bool)
+
+    def work(self) -> Tuple[float, float, int]:
+        """(flops, like-particle flops, FP64 bytes) of one H*v (algorithmic counts)."""
+        f, fl, b = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().nsx_hv_work(self._h, C.byref(f), C.byref(fl), C.byref(b)), "nsx_hv_work")
+        return f.value, fl.value, int(b.value)
+
+    # ---- H*v -----------------------------------------------------------------
+    def multiply(self, v: np.ndarray) -> np.ndarray:
+        """coefL = H coefR on host vectors (float32 in -> float32 out, else float64)."""
+        if v.shape != (self.N,):
+            raise ValueError(f"vector of length {self.N} expected, got {v.shape}")
+        if v.dtype == np.float32:
+            v = np.ascontiguousarray(v)
+            out = np.empty_like(v)
+            check(lib().nsx_hv_apply_f32(self._h, v.ctypes.data, out.ctypes.data), "nsx_hv_apply_f32")
+            return out
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty_like(v)
+        check(lib().nsx_hv_apply_f64(self._h, v.ctypes.data, out.ctypes.data), "nsx_hv_apply_f64")
+        return out
+
+    def multiply_dev(self, d_in, d_out, stream=None) -> None:
+        """H*v on device buffers (``DeviceBuffer`` or raw pointers) on a HIP stream."""
+        pin = d_in.data_ptr() if hasattr(d_in, "data_ptr") else int(d_in)
+        pout = d_out.data_ptr() if hasattr(d_out, "data_ptr") else int(d_out)
+        s = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        check(lib().nsx_hv_apply_dev(self._h, C.c_void_p(pin), C.c_void_p(pout), C.c_void_p(s)),
+              "nsx_hv_apply_dev")
+
+    # ---- Lanczos ----------------------------------------------------------------
+    def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,
+                max_iter: int = 5000, seed: int = 5, start: Optional[np.ndarray] = None,
+                vectors: bool = False):
+        """Thick-restart Lanczos for the lowest ``nev`` states (NuTRL's Lnz_OP)."""
+        opts = TrlOpts(nev, max_basis, keep, max_iter, tol, seed)
+        ev = np.zeros(nev)
+        res = np.zeros(nev)
+        vec = np.zeros((nev, self.N)) if vectors else None
+        st = TrlStats()
+        s = None
+        if start is not None:
+            start = np.ascontiguousarray(start, dtype=np.float64)
+            s = start.ctypes.data
+        check(lib().nsx_trl_solve(self._h, C.byref(opts), s, ev.ctypes.data,
+                                  vec.ctypes.data if vectors else None, res.ctypes.data, C.byref(st)),
+              "nsx_trl_solve")
+        out = {"evals": ev, "residuals": res, "matvecs": st.matvecs, "restarts": st.restarts,
+               "converged": st.converged, "seconds": st.seconds}
+        if vectors:
+            out["evecs"] = vec
+        return out
+
+
+class Batch:
+    """Many instances in one launch pair (vectors concatenated in instance order)."""
+
+    def __init__(self, hams: Sequence[Hamiltonian], precision: str = "fp64"):
+        arr = (C.c_void_p * len(hams))(*[h.handle for h in hams])
+        b = C.c_void_p()
+        check(lib().nsx_batch_create(arr, len(hams), PRECISIONS[precision], C.byref(b)), "nsx_batch_create")
+        self._b = b
+        self._hams = list(hams)
+        n = C.c_int64()
+        check(lib().nsx_batch_dims(b, C.byref(n)), "nsx_batch_dims")
+        self.N = int(n.value)
+        self.offsets = []
+        for i in range(len(hams)):
+            o = C.c_int64()
+            check(lib().nsx_batch_offset(b, i, C.byref(o)), "nsx_batch_offset")
+            self.offsets.append(int(o.value))
+
+    def work(self) -> Tuple[float, float, int]:
+        f, fl, by = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().nsx_batch_work(self._b, C.byref(f), C.byref(fl), C.byref(by)), "nsx_batch_work")
+        return f.value, fl.value, int(by.value)
+
+    def multiply_dev(self, d_in, d_out, stream=None) -> None:
+        pin = d_in.data_ptr() if hasattr(d_in, "data_ptr") else int(d_in)
+        pout = d_out.data_ptr() if hasattr(d_out, "data_ptr") else int(d_out)
+        s = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        check(lib().nsx_batch_apply_dev(self._b, C.c_void_p(pin), C.c_void_p(pout), C.c_void_p(s)),
+              "nsx_batch_apply_dev")
+
+    def time(self, d_in, d_out, warmup: int = 10, iters: int = 100):
+        """HIP-event timing (ms per batched H*v, pn kernel, reduce kernel)."""
+        hv, pn, red = C.c_double(), C.c_double(), C.c_double()
+        check(lib().nsx_batch_time(self._b, C.c_void_p(d_in.data_ptr()), C.c_void_p(d_out.data_ptr()),
+                                   warmup, iters, C.byref(hv), C.byref(pn), C.byref(red)),
+              "nsx_batch_time")
+        return hv.value, pn.value, red.value
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().nsx_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["Hamiltonian", "Batch", "DeviceBuffer", "device_sync", "NsxError", "NSX_FP64"]
