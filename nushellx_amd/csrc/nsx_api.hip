@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -34,6 +35,19 @@ static std::string up(T*& dst, const std::vector<T>& src) {
 
 std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
   std::string e;
+  if (!(e = up(d.items2, hp.items2)).empty()) return e;
+  if (!(e = up(d.phases, hp.phases)).empty()) return e;
+  if (!(e = up(d.pairs, hp.pairs)).empty()) return e;
+  d.n_items2 = (int)hp.items2.size();
+  d.lds_y = hp.lds_y;
+  d.lds_buf = hp.lds_buf;
+  d.lds2_bytes = sizeof(double) * (size_t)(hp.lds_buf + hp.lds_y);
+  if (d.lds2_bytes > 64 * 1024)
+    for (int i = 0; i < 4; ++i)
+      if (hipFuncSetAttribute(k_pn_staged_ptr(i), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)d.lds2_bytes) != hipSuccess)
+        return "LDS request too large (staged kernel)";
+  if (const char* v = std::getenv("NSX_STAGED_VARIANT")) d.staged_variant = std::atoi(v);
   if (!(e = up(d.items, hp.items)).empty()) return e;
   if (!(e = up(d.aterms, hp.aterms)).empty()) return e;
   if (!(e = up(d.rterms, hp.rterms)).empty()) return e;
@@ -57,6 +71,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
 }
 
 void free_plan(DevicePlan& d) {
+  (void)hipFree(d.items2); (void)hipFree(d.phases); (void)hipFree(d.pairs);
   (void)hipFree(d.items); (void)hipFree(d.aterms); (void)hipFree(d.rterms); (void)hipFree(d.blocks);
   (void)hipFree(d.part_list); (void)hipFree(d.Ad); (void)hipFree(d.Bh); (void)hipFree(d.H); (void)hipFree(d.part);
   d = DevicePlan();

@@ -34,9 +34,9 @@ __global__ __launch_bounds__(kThreads) void k_pn_partial(
       for (int u = T.r_begin; u < T.r_end; ++u) {
         const DRTerm R = rterms[u];
         const double* v = V + R.v_off + i;
-        const double* b = Bh + R.b_off + j;
+        const double* b = Bh + R.b_off + (int64_t)R.dbr * j;  // Bhat^T column j
         double s = 0.0;
-        for (int k = 0; k < R.dbr; ++k) s = fma(v[(int64_t)dar * k], b[(int64_t)db * k], s);
+        for (int k = 0; k < R.dbr; ++k) s = fma(v[(int64_t)dar * k], b[k], s);
         y = fma(R.r, s, y);
       }
       Y[e] = y;
@@ -65,6 +65,155 @@ __global__ __launch_bounds__(kThreads) void k_pn_partial(
   }
 }
 
+// Staged pn kernel for blocks with all sides <= kSmallMax.  One workgroup (NT
+// threads) runs a chunk of an item's phases.  A phase's three operand runs are
+// either prefetched into registers while the previous phase computes (PF > 0)
+// or copied at the start of the phase (PF == 0, fewer registers and so more
+// resident workgroups per CU).  Y (dar x db) and W (da x db) accumulate in
+// registers, 4 columns per thread, so one LDS read of V (or A) feeds 4 FMAs
+// against broadcast reads of Bhat^T (or Y).
+template <int NT, int PF>
+__global__ __launch_bounds__(NT) void k_pn_staged(
+    const DItem2* __restrict__ items, const DPhase* __restrict__ phases,
+    const DPair* __restrict__ pairs, const double* __restrict__ V, const double* __restrict__ Ad,
+    const double* __restrict__ Bt, double* __restrict__ part, int lds_buf) {
+  constexpr int MU = (kSmallMax * ((kSmallMax + kColBlk - 1) / kColBlk) + NT - 1) / NT;
+  extern __shared__ double sm[];
+  double* buf = sm;
+  double* sY = sm + lds_buf;
+  const DItem2 it = items[blockIdx.x];
+  const int da = it.da, dar = it.dar, db = it.db;
+  const int ncb = (db + kColBlk - 1) / kColBlk;
+  const int nuY = dar * ncb, nuW = da * ncb;
+  const int tid = threadIdx.x;
+  double w[MU][kColBlk], y[MU][kColBlk];
+  double pre[PF > 0 ? PF : 1];
+#pragma unroll
+  for (int q = 0; q < MU; ++q)
+#pragma unroll
+    for (int c = 0; c < kColBlk; ++c) { w[q][c] = 0.0; y[q][c] = 0.0; }
+
+  auto src_of = [&](const DPhase& P, int e) -> const double* {
+    const int nvb = P.nv + P.nb;
+    return e < P.nv ? V + P.v_off + e : (e < nvb ? Bt + P.b_off + (e - P.nv) : Ad + P.a_off + (e - nvb));
+  };
+  auto prefetch = [&](const DPhase& P) {
+    const int nt = P.nv + P.nb + P.na;
+#pragma unroll
+    for (int q = 0; q < (PF > 0 ? PF : 1); ++q) {
+      const int e = tid + NT * q;
+      if (PF > 0 && e < nt) pre[q] = *src_of(P, e);
+    }
+  };
+
+  if (PF > 0 && it.ph_begin < it.ph_end) prefetch(phases[it.ph_begin]);
+  for (int p = it.ph_begin; p < it.ph_end; ++p) {
+    const DPhase P = phases[p];
+    const int nt = P.nv + P.nb + P.na;
+    __syncthreads();  // the previous phase has finished reading buf and sY
+    if (PF > 0) {
+#pragma unroll
+      for (int q = 0; q < (PF > 0 ? PF : 1); ++q) {
+        const int e = tid + NT * q;
+        if (e < nt) buf[e] = pre[q];
+      }
+    } else {
+#pragma unroll 8
+      for (int e = tid; e < nt; e += NT) buf[e] = *src_of(P, e);
+    }
+    __syncthreads();
+    if (PF > 0 && p + 1 < it.ph_end) prefetch(phases[p + 1]);
+    // Y += r V_R Bhat^T for every pair of the phase
+    for (int pr = P.pair_begin; pr < P.pair_end; ++pr) {
+      const DPair Q = pairs[pr];
+      const double* sV = buf + Q.v_lds;
+      const double* sB = buf + Q.b_lds;
+      const int dbr = Q.dbr;
+#pragma unroll
+      for (int q = 0; q < MU; ++q) {
+        const int u = tid + NT * q;
+        if (u < nuY) {
+          const int i = u % dar, j0 = (u / dar) * kColBlk;
+          const double* bc[kColBlk];
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c) bc[c] = sB + dbr * min(j0 + c, db - 1);
+          double s0[kColBlk] = {0.0, 0.0, 0.0, 0.0}, s1[kColBlk] = {0.0, 0.0, 0.0, 0.0};
+          int k = 0;
+          for (; k + 1 < dbr; k += 2) {
+            const double v0 = sV[i + dar * k], v1 = sV[i + dar * (k + 1)];
+#pragma unroll
+            for (int c = 0; c < kColBlk; ++c) {
+              s0[c] = fma(v0, bc[c][k], s0[c]);
+              s1[c] = fma(v1, bc[c][k + 1], s1[c]);
+            }
+          }
+          if (k < dbr) {
+            const double v0 = sV[i + dar * k];
+#pragma unroll
+            for (int c = 0; c < kColBlk; ++c) s0[c] = fma(v0, bc[c][k], s0[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c) y[q][c] = fma(Q.r, s0[c] + s1[c], y[q][c]);
+        }
+      }
+    }
+    if (P.na > 0) {
+      // publish Y, then W += A Y with A at the end of the buffer
+#pragma unroll
+      for (int q = 0; q < MU; ++q) {
+        const int u = tid + NT * q;
+        if (u < nuY) {
+          const int i = u % dar, j0 = (u / dar) * kColBlk;
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c)
+            if (j0 + c < db) sY[i + dar * (j0 + c)] = y[q][c];
+        }
+#pragma unroll
+        for (int c = 0; c < kColBlk; ++c) y[q][c] = 0.0;
+      }
+      __syncthreads();
+      const double* sA = buf + P.nv + P.nb;
+#pragma unroll
+      for (int q = 0; q < MU; ++q) {
+        const int u = tid + NT * q;
+        if (u < nuW) {
+          const int i = u % da, j0 = (u / da) * kColBlk;
+          const double* yc[kColBlk];
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c) yc[c] = sY + dar * min(j0 + c, db - 1);
+          double s0[kColBlk] = {0.0, 0.0, 0.0, 0.0}, s1[kColBlk] = {0.0, 0.0, 0.0, 0.0};
+          int k = 0;
+          for (; k + 1 < dar; k += 2) {
+            const double a0 = sA[i + da * k], a1 = sA[i + da * (k + 1)];
+#pragma unroll
+            for (int c = 0; c < kColBlk; ++c) {
+              s0[c] = fma(a0, yc[c][k], s0[c]);
+              s1[c] = fma(a1, yc[c][k + 1], s1[c]);
+            }
+          }
+          if (k < dar) {
+            const double a0 = sA[i + da * k];
+#pragma unroll
+            for (int c = 0; c < kColBlk; ++c) s0[c] = fma(a0, yc[c][k], s0[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c) w[q][c] += s0[c] + s1[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MU; ++q) {
+    const int u = tid + NT * q;
+    if (u < nuW) {
+      const int i = u % da, j0 = (u / da) * kColBlk;
+#pragma unroll
+      for (int c = 0; c < kColBlk; ++c)
+        if (j0 + c < db) part[it.part_off + i + (int64_t)da * (j0 + c)] = w[q][c];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_reduce(
     const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
     const double* __restrict__ part, const double* __restrict__ V, const double* __restrict__ H,
@@ -72,6 +221,43 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
   const DBlock b = blocks[blockIdx.x];
   const int da = b.da, db = b.db, nW = da * b.nc;
   const double* v = V + b.v_off;
+  if (da <= kSmallMax && db <= kSmallMax && b.j0 == 0 && b.nc == db) {
+    // small block: stage V_L, H_a, H_b in LDS, then contract from LDS
+    __shared__ double sv[kSmallMax * kSmallMax], sha[kSmallMax * kSmallMax],
+        shb[kSmallMax * kSmallMax];
+    const int nv = da * db;
+    for (int e = threadIdx.x; e < nv; e += kThreads) sv[e] = v[e];
+    if (b.ha_off >= 0)
+      for (int e = threadIdx.x; e < da * da; e += kThreads) sha[e] = H[b.ha_off + e];
+    if (b.hb_off >= 0)
+      for (int e = threadIdx.x; e < db * db; e += kThreads) shb[e] = H[b.hb_off + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nv; e += kThreads) {
+      const int i = e % da, j = e / da;
+      double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+      int p = b.p_begin;
+      for (; p + 3 < b.p_end; p += 4) {
+        w0 += part[part_list[p] + e];
+        w1 += part[part_list[p + 1] + e];
+        w2 += part[part_list[p + 2] + e];
+        w3 += part[part_list[p + 3] + e];
+      }
+      for (; p < b.p_end; ++p) w0 += part[part_list[p] + e];
+      double w = (w0 + w1) + (w2 + w3);
+      if (b.ha_off >= 0) {
+        double s = 0.0;
+        for (int k = 0; k < da; ++k) s = fma(sha[i + da * k], sv[k + da * j], s);
+        w += s;
+      }
+      if (b.hb_off >= 0) {
+        double s = 0.0;
+        for (int k = 0; k < db; ++k) s = fma(sv[i + da * k], shb[j + db * k], s);
+        w += s;
+      }
+      W[b.v_off + e] = w;
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < nW; e += kThreads) {
     const int i = e % da, j = b.j0 + e / da;
     const int64_t ij = i + (int64_t)da * j;
@@ -94,6 +280,28 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
 }
 
 hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s) {
+  if (d.n_items2 > 0) {
+    const size_t lds = d.lds2_bytes;
+    const int g = d.n_items2;
+    switch (d.staged_variant) {
+      case 1:  // 256 threads, register prefetch
+        if ((d.lds_buf + 255) / 256 <= 12)
+          hipLaunchKernelGGL((k_pn_staged<256, 12>), dim3(g), dim3(256), lds, s, d.items2, d.phases,
+                             d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
+        else
+          hipLaunchKernelGGL((k_pn_staged<256, kFatPF>), dim3(g), dim3(256), lds, s, d.items2, d.phases,
+                             d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
+        break;
+      case 2:  // 256 threads, copy at phase start
+        hipLaunchKernelGGL((k_pn_staged<256, 0>), dim3(g), dim3(256), lds, s, d.items2, d.phases,
+                           d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
+        break;
+      default:  // 128 threads, copy at phase start
+        hipLaunchKernelGGL((k_pn_staged<128, 0>), dim3(g), dim3(128), lds, s, d.items2, d.phases,
+                           d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
+        break;
+    }
+  }
   if (d.n_items > 0) {
     hipLaunchKernelGGL(k_pn_partial, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
                        d.aterms, d.rterms, V, d.Ad, d.Bh, d.part);
@@ -116,6 +324,15 @@ hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStr
 }
 
 const void* k_pn_partial_ptr() { return (const void*)k_pn_partial; }
+// every staged-kernel instance, for launch attributes
+const void* k_pn_staged_ptr(int i) {
+  switch (i) {
+    case 0: return (const void*)k_pn_staged<256, 12>;
+    case 1: return (const void*)k_pn_staged<256, kFatPF>;
+    case 2: return (const void*)k_pn_staged<256, 0>;
+    default: return (const void*)k_pn_staged<128, 0>;
+  }
+}
 
 std::string build_lowp(const HostPlan&, int, LowpPlan&) {
   return "reduced-precision H*v is not built yet";

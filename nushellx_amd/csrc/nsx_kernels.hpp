@@ -9,6 +9,12 @@
 namespace nsx {
 
 struct DevicePlan {
+  DItem2* items2 = nullptr;
+  DPhase* phases = nullptr;
+  DPair* pairs = nullptr;
+  int n_items2 = 0, lds_y = 1, lds_buf = 1;
+  int staged_variant = 0;  // 0: 128 threads copy-in, 1: 256 threads prefetch, 2: 256 threads copy-in
+  size_t lds2_bytes = 0;
   DItem* items = nullptr;
   DATerm* aterms = nullptr;
   DRTerm* rterms = nullptr;
@@ -28,6 +34,7 @@ hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStr
 hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s);
 hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
 const void* k_pn_partial_ptr();
+const void* k_pn_staged_ptr(int i);
 
 // Reduced-precision operands of the block contractions (FP32 or BF16 hi/lo).
 struct LowpPlan {

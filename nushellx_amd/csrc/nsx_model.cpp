@@ -233,6 +233,7 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
                 slot = (int)m.bhat.size();
                 m.bhat.push_back(std::move(acc));
                 m.bhat_J.push_back({bjl, bjr});
+                m.bhat_key.push_back(bk);
                 bhat_of[bk] = slot;
               } else {
                 slot = f->second;

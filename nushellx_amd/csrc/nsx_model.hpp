@@ -54,6 +54,7 @@ struct Model {
   // folded b-operators d(Jb) x d(Jb'), column-major
   std::vector<std::vector<double>> bhat;
   std::vector<std::array<int, 2>> bhat_J;    // (2Jb, 2Jb')
+  std::vector<std::array<int, 5>> bhat_key;  // (2Jb, 2Jb', 2lam, sa, sa')
   // algorithmic (reference sparse) work of one H*v, and term counts
   double flops_ref = 0.0;               // all terms
   double flops_like = 0.0;              // H_a V + V H_b^T part of flops_ref

@@ -1,8 +1,30 @@
 #include "nsx_plan.hpp"
 
 #include <algorithm>
+#include <map>
 
 namespace nsx {
+
+// FNV-1a over the bytes of a matrix
+static uint64_t content_hash(const double* v, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(v);
+  for (size_t i = 0; i < n * sizeof(double); ++i) { h ^= b[i]; h *= 1099511628211ull; }
+  return h ^ (uint64_t)n;
+}
+
+// Append a matrix to `store` unless an identical one is already there; returns its offset.
+static int64_t intern(std::vector<double>& store, std::unordered_map<uint64_t, std::vector<int64_t>>& idx,
+                      const double* v, size_t n) {
+  const uint64_t h = content_hash(v, n);
+  auto& cands = idx[h];
+  for (int64_t off : cands)
+    if (off + (int64_t)n <= (int64_t)store.size() && std::equal(v, v + n, store.begin() + off)) return off;
+  const int64_t off = (int64_t)store.size();
+  store.insert(store.end(), v, v + n);
+  cands.push_back(off);
+  return off;
+}
 
 static int col_tile(int rows) {
   // widest column tile whose da x nc outputs fit one workgroup's registers
@@ -15,26 +37,53 @@ void plan_add(HostPlan& p, const Model& m) {
   p.vbase.push_back(vb);
   // operator values of this instance
   std::vector<int64_t> a_off(m.adense.size()), b_off(m.bhat.size());
-  for (size_t k = 0; k < m.adense.size(); ++k) {
-    a_off[k] = (int64_t)p.Ad.size();
-    p.Ad.insert(p.Ad.end(), m.adense[k].begin(), m.adense[k].end());
-  }
-  for (size_t k = 0; k < m.bhat.size(); ++k) {
-    b_off[k] = (int64_t)p.Bh.size();
-    p.Bh.insert(p.Bh.end(), m.bhat[k].begin(), m.bhat[k].end());
+  for (size_t k = 0; k < m.adense.size(); ++k)
+    a_off[k] = intern(p.Ad, p.dedup_a, m.adense[k].data(), m.adense[k].size());
+  // Bhat^T blocks grouped by (Jb, lam, sa, sa') with every Jb' of the group's
+  // range adjacent (zeros where a Jb' has no operator), so that the blocks one
+  // a-term needs form a single contiguous run.
+  {
+    std::map<std::array<int, 4>, std::vector<int>> groups;
+    for (size_t k = 0; k < m.bhat.size(); ++k) {
+      const auto& key = m.bhat_key[k];
+      groups[{key[0], key[2], key[3], key[4]}].push_back((int)k);
+    }
+    std::vector<double> run;
+    for (auto& g : groups) {
+      const int bjl = g.first[0];
+      const int db = m.B.d(bjl);
+      int jmin = 1 << 30, jmax = -1;
+      std::map<int, int> slot_of;
+      for (int k : g.second) {
+        jmin = std::min(jmin, m.bhat_key[k][1]);
+        jmax = std::max(jmax, m.bhat_key[k][1]);
+        slot_of[m.bhat_key[k][1]] = k;
+      }
+      run.clear();
+      std::map<int, int64_t> within;
+      for (int bjr = jmin; bjr <= jmax; bjr += m.B.delJ2) {
+        const int dbr = m.B.d(bjr);
+        within[bjr] = (int64_t)run.size();
+        auto it = slot_of.find(bjr);
+        // Bhat^T (dbr x db, column-major): element (i, c) = Bhat(c, i)
+        for (int c = 0; c < db; ++c)
+          for (int i = 0; i < dbr; ++i)
+            run.push_back(it == slot_of.end() ? 0.0 : m.bhat[it->second][size_t(i) * db + c]);
+      }
+      const int64_t off = intern(p.Bh, p.dedup_b, run.data(), run.size());
+      for (int k : g.second) b_off[k] = off + within[m.bhat_key[k][1]];
+    }
   }
   std::vector<int64_t> ha_off(m.A.nJ(), -1), hb_off(m.B.nJ(), -1);
   int64_t nnz_like = 0;
   for (int j = 0; j < m.A.nJ(); ++j)
     if (!m.A.H.empty() && !m.A.H[j].empty()) {
-      ha_off[j] = (int64_t)p.H.size();
-      p.H.insert(p.H.end(), m.A.H[j].begin(), m.A.H[j].end());
+      ha_off[j] = intern(p.H, p.dedup_h, m.A.H[j].data(), m.A.H[j].size());
       for (double v : m.A.H[j]) nnz_like += v != 0.0;
     }
   for (int j = 0; j < m.B.nJ(); ++j)
     if (!m.B.H.empty() && !m.B.H[j].empty()) {
-      hb_off[j] = (int64_t)p.H.size();
-      p.H.insert(p.H.end(), m.B.H[j].begin(), m.B.H[j].end());
+      hb_off[j] = intern(p.H, p.dedup_h, m.B.H[j].data(), m.B.H[j].size());
       for (double v : m.B.H[j]) nnz_like += v != 0.0;
     }
   // items: partial tiles
@@ -62,6 +111,80 @@ void plan_add(HostPlan& p, const Model& m) {
       p.aterms.push_back(t);
     }
     const int a_end = (int)p.aterms.size();
+    int maxdbr = 0;
+    for (const ATerm& at : it.at)
+      for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
+    if (std::max(std::max(da, dar), std::max(db, maxdbr)) <= kSmallMax) {
+      // phases: one per a-term (split when its runs exceed kFatCap); items are
+      // cut into chunks of <= kMaxPhases phases, each chunk owning a partial
+      std::vector<DPhase> ph_item;
+      for (const ATerm& at : it.at) {
+        const DATerm& t = p.aterms[a_begin + (int)(&at - &it.at[0])];
+        const int na = da * dar;
+        int u0 = t.r_begin;
+        while (u0 < t.r_end) {
+          int u1 = u0;
+          int64_t v0 = p.rterms[u0].v_off, b0 = p.rterms[u0].b_off;
+          int nv = 0, nb = 0;
+          while (u1 < t.r_end) {
+            const DRTerm& r = p.rterms[u1];
+            if (r.v_off < v0 || r.b_off < b0) break;   // runs must grow monotonically
+            const int nv2 = (int)(r.v_off - v0) + dar * r.dbr;
+            const int nb2 = (int)(r.b_off - b0) + r.dbr * db;
+            const bool last = (u1 + 1 == t.r_end);
+            if (u1 > u0 && nv2 + nb2 + (last ? na : 0) > kFatCap) break;
+            nv = nv2; nb = nb2;
+            ++u1;
+          }
+          DPhase ph;
+          ph.v_off = v0; ph.b_off = b0;
+          ph.nv = nv; ph.nb = nb;
+          const bool has_a = (u1 == t.r_end);
+          ph.a_off = has_a ? t.a_off : -1;
+          ph.na = has_a ? na : 0;
+          ph.pair_begin = (int)p.pairs.size();
+          for (int u = u0; u < u1; ++u) {
+            const DRTerm& r = p.rterms[u];
+            DPair pr;
+            pr.r = r.r;
+            pr.v_lds = (int)(r.v_off - v0);
+            pr.b_lds = nv + (int)(r.b_off - b0);
+            pr.dbr = r.dbr;
+            pr.pad = 0;
+            p.pairs.push_back(pr);
+          }
+          ph.pair_end = (int)p.pairs.size();
+          ph.pad = 0;
+          p.lds_buf = std::max(p.lds_buf, ph.nv + ph.nb + ph.na);
+          ph_item.push_back(ph);
+          u0 = u1;
+        }
+      }
+      parts_of[it.L].pop_back();
+      p.part_size = base;
+      size_t q = 0;
+      while (q < ph_item.size()) {
+        DItem2 d2;
+        d2.part_off = p.part_size;
+        p.part_size += int64_t(da) * db;
+        parts_of[it.L].push_back(d2.part_off);
+        d2.da = da; d2.dar = dar; d2.db = db; d2.pad = 0;
+        d2.ph_begin = (int)p.phases.size();
+        int n = 0;
+        // cut only after a phase carrying A (an a-term never spans two chunks)
+        while (q < ph_item.size()) {
+          p.phases.push_back(ph_item[q]);
+          ++n;
+          const bool a_done = ph_item[q].na > 0;
+          ++q;
+          if (a_done && n >= kMaxPhases) break;
+        }
+        d2.ph_end = (int)p.phases.size();
+        p.items2.push_back(d2);
+      }
+      p.lds_y = std::max(p.lds_y, dar * db);
+      continue;
+    }
     int nc = std::min(col_tile(std::max(da, dar)), db);
     nc = std::min(nc, std::max(1, kMaxLdsDoubles / std::max(dar, 1)));
     for (int j0 = 0; j0 < db; j0 += nc) {

@@ -9,6 +9,7 @@
 //      partials of L in a fixed order, adds H_a V_L + V_L H_b^T, writes W_L.
 #pragma once
 #include <cstdint>
+#include <unordered_map>
 #include <vector>
 
 #include "nsx_model.hpp"
@@ -19,6 +20,13 @@ constexpr int kThreads = 256;
 constexpr int kMaxE = 8;                 // outputs per thread in one tile
 constexpr int kTileElems = kThreads * kMaxE;
 constexpr int kMaxLdsDoubles = 8192;     // Y tile capacity (64 KiB)
+constexpr int kSmallMax = 40;            // block sides up to this use the staged kernel
+constexpr int kPrefetch = (2 * kSmallMax * kSmallMax + kThreads - 1) / kThreads;
+constexpr int kColBlk = 4;               // output columns per thread in the staged kernel
+constexpr int kMaxU = (kSmallMax * ((kSmallMax + kColBlk - 1) / kColBlk) + kThreads - 1) / kThreads;
+constexpr int kMaxPhases = 6;            // phases per staged workgroup (long items are split)
+constexpr int kFatCap = 5120;            // doubles one phase stages in LDS (V run + Bhat run + A)
+constexpr int kFatPF = kFatCap / kThreads;
 
 struct DItem {       // one (L, Ja') partial, one column tile
   int64_t part_off;  // partial tile base (full da x db layout of the item)
@@ -30,9 +38,31 @@ struct DATerm {
   int64_t a_off;     // dense A (da x dar, column-major)
   int32_t r_begin, r_end;
 };
+// Staged kernel: an item is a list of phases.  A phase stages three contiguous
+// runs in LDS -- the V_R blocks (Ja', Jb') of consecutive Jb' (adjacent in the
+// vector), the Bhat^T blocks of the same Jb' (laid out adjacently per
+// (Jb, lam, sa, sa') group) and, in the last phase of an a-term, A -- and then
+// forms Y += sum_pairs r V_R Bhat^T and, with A, W += A Y.
+struct DPhase {
+  int64_t v_off, b_off, a_off;   // run starts: vector, Bhat^T store, A store (a_off < 0: no A)
+  int32_t nv, nb, na;            // doubles per run
+  int32_t pair_begin, pair_end;
+  int32_t pad;
+};
+struct DPair {
+  double r;
+  int32_t v_lds, b_lds;          // offsets in the phase's LDS buffer
+  int32_t dbr, pad;
+};
+struct DItem2 {
+  int64_t part_off;
+  int32_t da, dar, db;
+  int32_t ph_begin, ph_end;
+  int32_t pad;
+};
 struct DRTerm {
   int64_t v_off;     // V_R (dar x dbr, column-major) in the batched vector
-  int64_t b_off;     // Bhat (db x dbr, column-major)
+  int64_t b_off;     // Bhat^T (dbr x db, column-major)
   double r;
   int32_t dbr, pad;
 };
@@ -44,13 +74,20 @@ struct DBlock {      // one output block L, one column tile
 };
 
 struct HostPlan {
-  std::vector<DItem> items;
+  std::vector<DItem2> items2;        // staged (small-block) items
+  std::vector<DPhase> phases;
+  std::vector<DPair> pairs;
+  int lds_y = 1;                     // staged kernel Y region (doubles)
+  int lds_buf = 1;                   // staged kernel operand buffer (doubles): largest phase
+  std::vector<DItem> items;          // direct kernel (large-block) tiles
   std::vector<DATerm> aterms;
   std::vector<DRTerm> rterms;
   std::vector<DBlock> blocks;
   std::vector<int64_t> part_list;   // partial base of every item of each block, in item order
   std::vector<double> Ad, Bh, H;
   std::vector<int64_t> vbase;       // per instance
+  // identical operator matrices are stored once (content hash -> offsets)
+  std::unordered_map<uint64_t, std::vector<int64_t>> dedup_a, dedup_b, dedup_h;
   int64_t N = 0, part_size = 0;
   int max_lds = 0;                  // doubles of LDS the pn kernel needs
   double flops_ref = 0.0;           // algorithmic work of one batched H*v

@@ -1,0 +1,125 @@
+// Test helper (CPU only): executes the device work plan of libnsx.so with
+// plain loops, so the plan construction (block lists, phase runs, Bhat^T
+// layout, partial bookkeeping) is checked against the oracle without a GPU.
+// It mirrors the arithmetic of the HIP kernels, not their scheduling.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../nushellx_amd/csrc/nsx_model.hpp"
+#include "../nushellx_amd/csrc/nsx_plan.hpp"
+
+using namespace nsx;
+
+static thread_local std::string g_err;
+
+extern "C" const char* emu_last_error() { return g_err.c_str(); }
+
+extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const char* intf,
+                      const double* vin, double* vout, int64_t n) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  plan_add(p, m);
+  std::vector<double> part(p.part_size, 0.0), buf;
+  // staged items
+  for (const DItem2& it : p.items2) {
+    std::vector<double> W(size_t(it.da) * it.db, 0.0), Y(size_t(it.dar) * it.db, 0.0);
+    for (int ph = it.ph_begin; ph < it.ph_end; ++ph) {
+      const DPhase& P = p.phases[ph];
+      buf.assign(P.nv + P.nb + P.na, 0.0);
+      for (int i = 0; i < P.nv; ++i) buf[i] = vin[P.v_off + i];
+      for (int i = 0; i < P.nb; ++i) buf[P.nv + i] = p.Bh[P.b_off + i];
+      for (int i = 0; i < P.na; ++i) buf[P.nv + P.nb + i] = p.Ad[P.a_off + i];
+      for (int q = P.pair_begin; q < P.pair_end; ++q) {
+        const DPair& Q = p.pairs[q];
+        for (int j = 0; j < it.db; ++j)
+          for (int i = 0; i < it.dar; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < Q.dbr; ++k) s += buf[Q.v_lds + i + it.dar * k] * buf[Q.b_lds + k + Q.dbr * j];
+            Y[i + size_t(it.dar) * j] += Q.r * s;
+          }
+      }
+      if (P.na > 0) {
+        const double* A = buf.data() + P.nv + P.nb;
+        for (int j = 0; j < it.db; ++j)
+          for (int i = 0; i < it.da; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < it.dar; ++k) s += A[i + it.da * k] * Y[k + size_t(it.dar) * j];
+            W[i + size_t(it.da) * j] += s;
+          }
+        std::fill(Y.begin(), Y.end(), 0.0);
+      }
+    }
+    for (size_t q = 0; q < W.size(); ++q) part[it.part_off + q] = W[q];
+  }
+  // direct items (large blocks)
+  for (const DItem& it : p.items) {
+    for (int jj = 0; jj < it.nc; ++jj) {
+      const int j = it.j0 + jj;
+      std::vector<double> w(it.da, 0.0);
+      for (int a = it.a_begin; a < it.a_end; ++a) {
+        const DATerm& T = p.aterms[a];
+        std::vector<double> y(it.dar, 0.0);
+        for (int u = T.r_begin; u < T.r_end; ++u) {
+          const DRTerm& R = p.rterms[u];
+          for (int i = 0; i < it.dar; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < R.dbr; ++k) s += vin[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
+            y[i] += R.r * s;
+          }
+        }
+        for (int i = 0; i < it.da; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < it.dar; ++k) s += p.Ad[T.a_off + i + int64_t(it.da) * k] * y[k];
+          w[i] += s;
+        }
+      }
+      for (int i = 0; i < it.da; ++i) part[it.part_off + i + int64_t(it.da) * j] = w[i];
+    }
+  }
+  // reduce
+  for (const DBlock& b : p.blocks) {
+    for (int jj = 0; jj < b.nc; ++jj) {
+      const int j = b.j0 + jj;
+      for (int i = 0; i < b.da; ++i) {
+        const int64_t ij = i + int64_t(b.da) * j;
+        double w = 0.0;
+        for (int q = b.p_begin; q < b.p_end; ++q) w += part[p.part_list[q] + ij];
+        if (b.ha_off >= 0) {
+          double s = 0.0;
+          for (int k = 0; k < b.da; ++k) s += p.H[b.ha_off + i + int64_t(b.da) * k] * vin[b.v_off + k + int64_t(b.da) * j];
+          w += s;
+        }
+        if (b.hb_off >= 0) {
+          double s = 0.0;
+          for (int k = 0; k < b.db; ++k) s += vin[b.v_off + i + int64_t(b.da) * k] * p.H[b.hb_off + j + int64_t(b.db) * k];
+          w += s;
+        }
+        vout[b.v_off + ij] = w;
+      }
+    }
+  }
+  return 0;
+}
+
+// plan statistics for tests and tuning: items2, phases, pairs, direct items, blocks, partial doubles
+extern "C" int emu_plan_stats(const char* nus, const char* opa, const char* opb, const char* intf,
+                              int64_t* out8) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  HostPlan p;
+  plan_add(p, m);
+  out8[0] = (int64_t)p.items2.size();
+  out8[1] = (int64_t)p.phases.size();
+  out8[2] = (int64_t)p.pairs.size();
+  out8[3] = (int64_t)p.items.size();
+  out8[4] = (int64_t)p.blocks.size();
+  out8[5] = p.part_size;
+  out8[6] = p.lds_buf;
+  out8[7] = (int64_t)(p.Ad.size() + p.Bh.size() + p.H.size());
+  return 0;
+}
