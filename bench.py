@@ -84,6 +84,7 @@ def main():
     mf = make_workload(tmp)
     hams = [Hamiltonian(*mf.args(J2)) for J2 in mf.J2s]
     batch = Batch(hams)
+    tuned = batch.autotune(iters=10)   # staged-kernel shape, chosen before timing
     flops, flops_like, bytes_alg = batch.work()
     v = np.random.default_rng(1).standard_normal(batch.N)
     din = DeviceBuffer.from_array(v)
@@ -126,6 +127,7 @@ def main():
                    "flops_per_step": flops, "bytes_per_step": bytes_alg},
         "roofline": roofline,
         "kernel_ms": {"hv": ms_hv, "pn": ms_pn, "reduce": ms_red},
+        "tuned": tuned,
     }
     try:
         out["device"] = list(device_info())

@@ -66,7 +66,8 @@ int nsx_hv_apply_dev(nsx_hv *h, const double *d_in, double *d_out, void *stream)
 
 /* ---------------------------------------------------------------- batches
  * Many instances (nuclei, J values, interaction samples) in one launch pair;
- * vectors are concatenated in the order given (offsets from nsx_batch_offset). */
+ * vectors are concatenated in the order given (offsets from nsx_batch_offset).
+ * The instance handles must outlive the batch. */
 typedef struct nsx_batch nsx_batch;
 int nsx_batch_create(nsx_hv *const *hs, int32_t n, int precision, nsx_batch **b);
 void nsx_batch_destroy(nsx_batch *b);
@@ -74,6 +75,22 @@ int nsx_batch_dims(const nsx_batch *b, int64_t *N_total);
 int nsx_batch_offset(const nsx_batch *b, int32_t i, int64_t *off);
 int nsx_batch_work(const nsx_batch *b, double *flops, double *flops_like, int64_t *bytes);
 int nsx_batch_apply_dev(nsx_batch *b, const double *d_in, double *d_out, void *stream);
+
+/* ---------------------------------------------------------------- tuning
+ * Autotuner of the staged pn kernel: rebuilds the work plan for several phase
+ * capacities and chunk lengths and times every (threads, load mode) instance
+ * with HIP events on the instance's own operators; the fastest is kept.
+ * No counterpart in the reference (its loop nest has no tunable shape). */
+typedef struct {
+  int32_t fat_cap;     /* doubles one phase stages in LDS */
+  int32_t max_phases;  /* phases per workgroup */
+  int32_t threads;     /* workgroup size of the staged kernel */
+  int32_t mode;        /* 0: copy at phase start, 1: direct LDS loads one phase ahead */
+  double ms;           /* measured time of one H*v with this shape */
+} nsx_tune_result;
+int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);
+int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);
+int nsx_batch_set_config(nsx_batch *b, const nsx_tune_result *cfg);
 
 /* ---------------------------------------------------------------- Lanczos
  * Thick-restart Lanczos for the lowest eigenpairs (replaces TRL.Lnz_OP,

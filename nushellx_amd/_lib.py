@@ -21,13 +21,21 @@ EXPORTS = [
     "nsx_batch_create", "nsx_batch_destroy", "nsx_batch_dims", "nsx_batch_offset",
     "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
-    "nsx_last_error",
+    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_last_error",
 ]
 
 
 class TrlOpts(C.Structure):
     _fields_ = [("nev", C.c_int32), ("max_basis", C.c_int32), ("keep", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("fat_cap", C.c_int32), ("max_phases", C.c_int32), ("threads", C.c_int32),
+                ("mode", C.c_int32), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class TrlStats(C.Structure):
@@ -73,6 +81,9 @@ def lib() -> C.CDLL:
     L.nsx_batch_time.argtypes = [vp, vp, vp, i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]
     L.nsx_device_info.argtypes = [C.c_char_p, i32, C.POINTER(i32)]
+    L.nsx_hv_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
+    L.nsx_batch_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
+    L.nsx_batch_set_config.argtypes = [vp, C.POINTER(TuneResult)]
     L.nsx_last_error.restype = C.c_char_p
     _LIB = L
     return L

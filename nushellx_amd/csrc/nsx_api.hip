@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -41,13 +42,15 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
   d.n_items2 = (int)hp.items2.size();
   d.lds_y = hp.lds_y;
   d.lds_buf = hp.lds_buf;
-  d.lds2_bytes = sizeof(double) * (size_t)(hp.lds_buf + hp.lds_y);
+  d.max_units = hp.max_units;
+  d.lds2_bytes = sizeof(double) * (size_t)(2 * hp.lds_buf + hp.lds_y);
   if (d.lds2_bytes > 64 * 1024)
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < num_staged(); ++i)
       if (hipFuncSetAttribute(k_pn_staged_ptr(i), hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)d.lds2_bytes) != hipSuccess)
         return "LDS request too large (staged kernel)";
-  if (const char* v = std::getenv("NSX_STAGED_VARIANT")) d.staged_variant = std::atoi(v);
+  if (const char* v = std::getenv("NSX_STAGED_NT")) d.staged_nt = std::atoi(v);
+  if (const char* v = std::getenv("NSX_STAGED_MODE")) d.staged_mode = std::atoi(v);
   if (!(e = up(d.items, hp.items)).empty()) return e;
   if (!(e = up(d.aterms, hp.aterms)).empty()) return e;
   if (!(e = up(d.rterms, hp.rterms)).empty()) return e;
@@ -206,6 +209,7 @@ int nsx_batch_create(nsx_hv* const* hs, int32_t n, int precision, nsx_batch** ou
   b->precision = precision;
   for (int i = 0; i < n; ++i) {
     if (!hs[i]) { set_error("null handle in batch"); return NSX_EARG; }
+    b->models.push_back(&hs[i]->model);
     plan_add(b->hplan, hs[i]->model);
   }
   std::string e;
@@ -333,6 +337,120 @@ int nsx_device_info(char* name, int32_t len, int32_t* cus) {
     name[len - 1] = 0;
   }
   if (cus) *cus = p.multiProcessorCount;
+  return NSX_OK;
+}
+
+}  // extern "C"
+
+namespace nsx {
+
+std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
+                   HostPlan& hp, DevicePlan& d, LowpPlan& lp) {
+  const int nt = d.staged_nt, mode = d.staged_mode;
+  free_plan(d);
+  free_lowp(lp);
+  hp = HostPlan();
+  hp.cfg = cfg;
+  for (const Model* m : models) plan_add(hp, *m);
+  std::string e;
+  if (precision != NSX_FP64 && !(e = build_lowp(hp, precision, lp)).empty()) return e;
+  if (!(e = upload_plan(hp, d)).empty()) return e;
+  d.staged_nt = nt;
+  d.staged_mode = mode;
+  return "";
+}
+
+// Times one H*v (ms) over `iters` launches after 2 warm-up launches.
+static double time_hv(const DevicePlan& d, const LowpPlan& lp, int precision, const double* in,
+                      double* out, hipStream_t s, int iters) {
+  for (int i = 0; i < 2; ++i) {
+    hipError_t e = precision == NSX_FP64 ? launch_hv_f64(d, in, out, s) : launch_hv_lowp(d, lp, in, out, s);
+    if (e != hipSuccess) return -1.0;
+  }
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
+  (void)hipEventRecord(a, s);
+  for (int i = 0; i < iters; ++i) {
+    hipError_t e = precision == NSX_FP64 ? launch_hv_f64(d, in, out, s) : launch_hv_lowp(d, lp, in, out, s);
+    if (e != hipSuccess) return -1.0;
+  }
+  (void)hipEventRecord(b, s);
+  (void)hipEventSynchronize(b);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / iters;
+}
+
+static int autotune(const std::vector<const Model*>& models, int precision, HostPlan& hp, DevicePlan& d,
+                    LowpPlan& lp, hipStream_t s, int iters, nsx_tune_result* best_out) {
+  const int64_t N = std::max<int64_t>(hp.N, 1);
+  double *in = nullptr, *out = nullptr;
+  if (hipMalloc(&in, sizeof(double) * N) != hipSuccess || hipMalloc(&out, sizeof(double) * N) != hipSuccess) {
+    set_error("hipMalloc failed (autotune)");
+    return NSX_EGPU;
+  }
+  std::vector<double> h(N);
+  for (int64_t i = 0; i < N; ++i) h[i] = std::sin(0.37 * double(i + 1));
+  (void)hipMemcpy(in, h.data(), sizeof(double) * N, hipMemcpyHostToDevice);
+  nsx_tune_result best{0, 0, 0, 0, 1e300};
+  const int caps[] = {1536, 2560, 3584, kFatCap};
+  const int chunks[] = {3, 6, 12};
+  const int nts[] = {64, 128, 256};
+  for (int cap : caps)
+    for (int mp : chunks) {
+      PlanConfig cfg;
+      cfg.fat_cap = cap;
+      cfg.max_phases = mp;
+      std::string e = replan(models, cfg, precision, hp, d, lp);
+      if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+      for (int nt : nts)
+        for (int mode = 0; mode < 2; ++mode) {
+          d.staged_nt = nt;
+          d.staged_mode = mode;
+          const double ms = time_hv(d, lp, precision, in, out, s, iters);
+          if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, ms};
+        }
+    }
+  (void)hipFree(in);
+  (void)hipFree(out);
+  if (best.ms >= 1e300) { set_error("autotune: no configuration ran"); return NSX_EGPU; }
+  PlanConfig cfg;
+  cfg.fat_cap = best.fat_cap;
+  cfg.max_phases = best.max_phases;
+  d.staged_nt = best.threads;
+  d.staged_mode = best.mode;
+  std::string e = replan(models, cfg, precision, hp, d, lp);
+  if (!e.empty()) { set_error(e); return NSX_EGPU; }
+  if (best_out) *best_out = best;
+  return NSX_OK;
+}
+
+}  // namespace nsx
+
+extern "C" {
+
+int nsx_hv_autotune(nsx_hv* h, int32_t iters, nsx_tune_result* best) {
+  if (!h) { set_error("null handle"); return NSX_EARG; }
+  std::vector<const Model*> ms{&h->model};
+  return autotune(ms, h->precision, h->hplan, h->dplan, h->lowp, h->stream, iters > 0 ? iters : 10, best);
+}
+
+int nsx_batch_autotune(nsx_batch* b, int32_t iters, nsx_tune_result* best) {
+  if (!b) { set_error("null batch"); return NSX_EARG; }
+  return autotune(b->models, b->precision, b->hplan, b->dplan, b->lowp, b->stream, iters > 0 ? iters : 10, best);
+}
+
+int nsx_batch_set_config(nsx_batch* b, const nsx_tune_result* cfg) {
+  if (!b || !cfg) { set_error("null argument"); return NSX_EARG; }
+  PlanConfig pc;
+  pc.fat_cap = std::min(std::max(cfg->fat_cap, 256), kFatCap);
+  pc.max_phases = std::max(cfg->max_phases, 1);
+  b->dplan.staged_nt = cfg->threads;
+  b->dplan.staged_mode = cfg->mode;
+  std::string e = replan(b->models, pc, b->precision, b->hplan, b->dplan, b->lowp);
+  if (!e.empty()) { set_error(e); return NSX_EGPU; }
   return NSX_OK;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "nsx_kernels.hpp"
 #include "nsx_model.hpp"
@@ -20,6 +21,7 @@ struct nsx_hv {
 };
 
 struct nsx_batch {
+  std::vector<const nsx::Model*> models;  // owned by the instance handles
   nsx::HostPlan hplan;
   nsx::DevicePlan dplan;
   nsx::LowpPlan lowp;
@@ -29,4 +31,7 @@ struct nsx_batch {
 
 namespace nsx {
 void set_error(const std::string& e);
+// Rebuilds and uploads the plan of `models` with `cfg`; keeps kernel choices of `d`.
+std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
+                   HostPlan& hp, DevicePlan& d, LowpPlan& lp);
 }

@@ -65,64 +65,70 @@ __global__ __launch_bounds__(kThreads) void k_pn_partial(
   }
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Copies n dwords src[0..n) to LDS dst[0..n) with direct global->LDS loads: the
+// data bypasses VGPRs and lands at (wave-uniform base) + lane * 4 bytes.
+template <int NT>
+__device__ __forceinline__ void dma_run(const uint32_t* __restrict__ src, uint32_t* dst, int n) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int base = wave * 64; base < n; base += NT) {
+    const int e = base + lane;
+    if (e < n) __builtin_amdgcn_global_load_lds(src + e, (lds_void_t*)(dst + base), 4, 0, 0);
+  }
+}
+
 // Staged pn kernel for blocks with all sides <= kSmallMax.  One workgroup (NT
-// threads) runs a chunk of an item's phases.  A phase's three operand runs are
-// either prefetched into registers while the previous phase computes (PF > 0)
-// or copied at the start of the phase (PF == 0, fewer registers and so more
-// resident workgroups per CU).  Y (dar x db) and W (da x db) accumulate in
-// registers, 4 columns per thread, so one LDS read of V (or A) feeds 4 FMAs
-// against broadcast reads of Bhat^T (or Y).
-template <int NT, int PF>
+// threads) runs a chunk of an item's phases.  A phase's operand runs (V_R
+// blocks, Bhat^T blocks, A) are staged in LDS, either copied through registers
+// at the start of the phase (MODE 0, single buffer) or fetched one phase ahead
+// by direct global->LDS loads into a second buffer (MODE 1) so the copy of
+// phase p+1 overlaps the arithmetic of phase p.  Y (dar x db) and W (da x db)
+// accumulate in registers, MU (row, 4-column) units per thread: one LDS read of
+// V (or A) feeds 4 FMAs against broadcast reads of Bhat^T (or Y).
+template <int NT, int MU, int MODE>
 __global__ __launch_bounds__(NT) void k_pn_staged(
     const DItem2* __restrict__ items, const DPhase* __restrict__ phases,
     const DPair* __restrict__ pairs, const double* __restrict__ V, const double* __restrict__ Ad,
     const double* __restrict__ Bt, double* __restrict__ part, int lds_buf) {
-  constexpr int MU = (kSmallMax * ((kSmallMax + kColBlk - 1) / kColBlk) + NT - 1) / NT;
   extern __shared__ double sm[];
-  double* buf = sm;
-  double* sY = sm + lds_buf;
+  double* bufs[2] = {sm, sm + (MODE == 1 ? lds_buf : 0)};
+  double* sY = sm + (MODE == 1 ? 2 : 1) * lds_buf;
   const DItem2 it = items[blockIdx.x];
   const int da = it.da, dar = it.dar, db = it.db;
   const int ncb = (db + kColBlk - 1) / kColBlk;
   const int nuY = dar * ncb, nuW = da * ncb;
   const int tid = threadIdx.x;
   double w[MU][kColBlk], y[MU][kColBlk];
-  double pre[PF > 0 ? PF : 1];
 #pragma unroll
   for (int q = 0; q < MU; ++q)
 #pragma unroll
     for (int c = 0; c < kColBlk; ++c) { w[q][c] = 0.0; y[q][c] = 0.0; }
 
-  auto src_of = [&](const DPhase& P, int e) -> const double* {
-    const int nvb = P.nv + P.nb;
-    return e < P.nv ? V + P.v_off + e : (e < nvb ? Bt + P.b_off + (e - P.nv) : Ad + P.a_off + (e - nvb));
-  };
-  auto prefetch = [&](const DPhase& P) {
-    const int nt = P.nv + P.nb + P.na;
-#pragma unroll
-    for (int q = 0; q < (PF > 0 ? PF : 1); ++q) {
-      const int e = tid + NT * q;
-      if (PF > 0 && e < nt) pre[q] = *src_of(P, e);
-    }
+  auto issue = [&](const DPhase& P, double* dst) {
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    dma_run<NT>(reinterpret_cast<const uint32_t*>(V + P.v_off), d, 2 * P.nv);
+    dma_run<NT>(reinterpret_cast<const uint32_t*>(Bt + P.b_off), d + 2 * P.nv, 2 * P.nb);
+    if (P.na > 0)
+      dma_run<NT>(reinterpret_cast<const uint32_t*>(Ad + P.a_off), d + 2 * (P.nv + P.nb), 2 * P.na);
   };
 
-  if (PF > 0 && it.ph_begin < it.ph_end) prefetch(phases[it.ph_begin]);
+  if (MODE == 1 && it.ph_begin < it.ph_end) issue(phases[it.ph_begin], bufs[0]);
   for (int p = it.ph_begin; p < it.ph_end; ++p) {
     const DPhase P = phases[p];
-    const int nt = P.nv + P.nb + P.na;
-    __syncthreads();  // the previous phase has finished reading buf and sY
-    if (PF > 0) {
-#pragma unroll
-      for (int q = 0; q < (PF > 0 ? PF : 1); ++q) {
-        const int e = tid + NT * q;
-        if (e < nt) buf[e] = pre[q];
-      }
+    double* buf = bufs[(p - it.ph_begin) & 1];
+    if (MODE == 1) {
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of phase p have landed
+      __syncthreads();                // ... and every other wave's; phase p-1 is done
+      if (p + 1 < it.ph_end) issue(phases[p + 1], bufs[(p + 1 - it.ph_begin) & 1]);
     } else {
+      const int nvb = P.nv + P.nb, nt = nvb + P.na;
+      __syncthreads();  // the previous phase has finished reading buf and sY
 #pragma unroll 8
-      for (int e = tid; e < nt; e += NT) buf[e] = *src_of(P, e);
+      for (int e = tid; e < nt; e += NT)
+        buf[e] = e < P.nv ? V[P.v_off + e] : (e < nvb ? Bt[P.b_off + (e - P.nv)] : Ad[P.a_off + (e - nvb)]);
+      __syncthreads();
     }
-    __syncthreads();
-    if (PF > 0 && p + 1 < it.ph_end) prefetch(phases[p + 1]);
     // Y += r V_R Bhat^T for every pair of the phase
     for (int pr = P.pair_begin; pr < P.pair_end; ++pr) {
       const DPair Q = pairs[pr];
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
       }
     }
     if (P.na > 0) {
-      // publish Y, then W += A Y with A at the end of the buffer
+      // publish Y, then W += A Y with A at the end of the phase's buffer
 #pragma unroll
       for (int q = 0; q < MU; ++q) {
         const int u = tid + NT * q;
@@ -212,6 +218,34 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
         if (j0 + c < db) part[it.part_off + i + (int64_t)da * (j0 + c)] = w[q][c];
     }
   }
+}
+
+// Instances of the staged kernel: (threads, units per thread, load mode).
+struct StagedInst {
+  int nt, mu, mode;
+  const void* fn;
+};
+#define NSX_INST(NT, MU, MODE) {NT, MU, MODE, (const void*)k_pn_staged<NT, MU, MODE>}
+static const StagedInst kStaged[] = {
+    NSX_INST(64, 1, 0),  NSX_INST(64, 2, 0),  NSX_INST(64, 4, 0),  NSX_INST(64, 7, 0),
+    NSX_INST(128, 1, 0), NSX_INST(128, 2, 0), NSX_INST(128, 4, 0),
+    NSX_INST(256, 1, 0), NSX_INST(256, 2, 0),
+    NSX_INST(64, 1, 1),  NSX_INST(64, 2, 1),  NSX_INST(64, 4, 1),  NSX_INST(64, 7, 1),
+    NSX_INST(128, 1, 1), NSX_INST(128, 2, 1), NSX_INST(128, 4, 1),
+    NSX_INST(256, 1, 1), NSX_INST(256, 2, 1),
+};
+#undef NSX_INST
+constexpr int kNumStaged = sizeof(kStaged) / sizeof(kStaged[0]);
+
+// smallest instance with the requested threads and mode covering max_units
+static int pick_staged(int nt, int mode, int max_units) {
+  int best = -1;
+  for (int i = 0; i < kNumStaged; ++i) {
+    const StagedInst& s = kStaged[i];
+    if (s.nt != nt || s.mode != mode || s.nt * s.mu < max_units) continue;
+    if (best < 0 || s.mu < kStaged[best].mu) best = i;
+  }
+  return best;
 }
 
 __global__ __launch_bounds__(kThreads) void k_reduce(
@@ -281,26 +315,13 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
 
 hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s) {
   if (d.n_items2 > 0) {
-    const size_t lds = d.lds2_bytes;
-    const int g = d.n_items2;
-    switch (d.staged_variant) {
-      case 1:  // 256 threads, register prefetch
-        if ((d.lds_buf + 255) / 256 <= 12)
-          hipLaunchKernelGGL((k_pn_staged<256, 12>), dim3(g), dim3(256), lds, s, d.items2, d.phases,
-                             d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
-        else
-          hipLaunchKernelGGL((k_pn_staged<256, kFatPF>), dim3(g), dim3(256), lds, s, d.items2, d.phases,
-                             d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
-        break;
-      case 2:  // 256 threads, copy at phase start
-        hipLaunchKernelGGL((k_pn_staged<256, 0>), dim3(g), dim3(256), lds, s, d.items2, d.phases,
-                           d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
-        break;
-      default:  // 128 threads, copy at phase start
-        hipLaunchKernelGGL((k_pn_staged<128, 0>), dim3(g), dim3(128), lds, s, d.items2, d.phases,
-                           d.pairs, V, d.Ad, d.Bh, d.part, d.lds_buf);
-        break;
-    }
+    const int k = pick_staged(d.staged_nt, d.staged_mode, d.max_units);
+    if (k < 0) return hipErrorInvalidConfiguration;
+    const size_t lds = sizeof(double) * (size_t)((d.staged_mode == 1 ? 2 : 1) * d.lds_buf + d.lds_y);
+    hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt),
+                    (void*[]){(void*)&d.items2, (void*)&d.phases, (void*)&d.pairs, (void*)&V,
+                              (void*)&d.Ad, (void*)&d.Bh, (void*)&d.part, (void*)&d.lds_buf},
+                    lds, s);
   }
   if (d.n_items > 0) {
     hipLaunchKernelGGL(k_pn_partial, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
@@ -324,15 +345,9 @@ hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStr
 }
 
 const void* k_pn_partial_ptr() { return (const void*)k_pn_partial; }
-// every staged-kernel instance, for launch attributes
-const void* k_pn_staged_ptr(int i) {
-  switch (i) {
-    case 0: return (const void*)k_pn_staged<256, 12>;
-    case 1: return (const void*)k_pn_staged<256, kFatPF>;
-    case 2: return (const void*)k_pn_staged<256, 0>;
-    default: return (const void*)k_pn_staged<128, 0>;
-  }
-}
+// number of staged-kernel instances and their entry points (launch attributes)
+int num_staged() { return kNumStaged; }
+const void* k_pn_staged_ptr(int i) { return kStaged[i].fn; }
 
 std::string build_lowp(const HostPlan&, int, LowpPlan&) {
   return "reduced-precision H*v is not built yet";

@@ -13,7 +13,9 @@ struct DevicePlan {
   DPhase* phases = nullptr;
   DPair* pairs = nullptr;
   int n_items2 = 0, lds_y = 1, lds_buf = 1;
-  int staged_variant = 0;  // 0: 128 threads copy-in, 1: 256 threads prefetch, 2: 256 threads copy-in
+  int max_units = 1;       // staged kernel: largest units per item
+  int staged_nt = 128;     // staged kernel threads per workgroup
+  int staged_mode = 0;     // 0: copy at phase start, 1: direct LDS loads one phase ahead
   size_t lds2_bytes = 0;
   DItem* items = nullptr;
   DATerm* aterms = nullptr;
@@ -35,6 +37,7 @@ hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s);
 hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
 const void* k_pn_partial_ptr();
 const void* k_pn_staged_ptr(int i);
+int num_staged();
 
 // Reduced-precision operands of the block contractions (FP32 or BF16 hi/lo).
 struct LowpPlan {

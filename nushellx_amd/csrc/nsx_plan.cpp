@@ -132,7 +132,7 @@ void plan_add(HostPlan& p, const Model& m) {
             const int nv2 = (int)(r.v_off - v0) + dar * r.dbr;
             const int nb2 = (int)(r.b_off - b0) + r.dbr * db;
             const bool last = (u1 + 1 == t.r_end);
-            if (u1 > u0 && nv2 + nb2 + (last ? na : 0) > kFatCap) break;
+            if (u1 > u0 && nv2 + nb2 + (last ? na : 0) > p.cfg.fat_cap) break;
             nv = nv2; nb = nb2;
             ++u1;
           }
@@ -177,12 +177,13 @@ void plan_add(HostPlan& p, const Model& m) {
           ++n;
           const bool a_done = ph_item[q].na > 0;
           ++q;
-          if (a_done && n >= kMaxPhases) break;
+          if (a_done && n >= p.cfg.max_phases) break;
         }
         d2.ph_end = (int)p.phases.size();
         p.items2.push_back(d2);
       }
       p.lds_y = std::max(p.lds_y, dar * db);
+      p.max_units = std::max(p.max_units, std::max(da, dar) * ((db + kColBlk - 1) / kColBlk));
       continue;
     }
     int nc = std::min(col_tile(std::max(da, dar)), db);

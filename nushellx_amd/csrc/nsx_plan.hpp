@@ -73,7 +73,15 @@ struct DBlock {      // one output block L, one column tile
   int32_t p_begin, p_end;  // range in part_list of the item partials of L
 };
 
+// Tunable plan shape (chosen by the autotuner, see nsx_tune.cpp).
+struct PlanConfig {
+  int fat_cap = kFatCap;     // doubles one staged phase may load (<= kFatCap)
+  int max_phases = kMaxPhases;  // phases per staged workgroup
+};
+
 struct HostPlan {
+  PlanConfig cfg;
+  int max_units = 1;                 // staged kernel: largest (row, column block) count of an item
   std::vector<DItem2> items2;        // staged (small-block) items
   std::vector<DPhase> phases;
   std::vector<DPair> pairs;

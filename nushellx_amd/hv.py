@@ -14,7 +14,8 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import NSX_FP64, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, check, device_sync, lib
+from ._lib import (NSX_FP64, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, TuneResult, check,
+                   device_sync, lib)
 
 
 class Hamiltonian:
@@ -99,6 +100,12 @@ class Hamiltonian:
         check(lib().nsx_hv_apply_dev(self._h, C.c_void_p(pin), C.c_void_p(pout), C.c_void_p(s)),
               "nsx_hv_apply_dev")
 
+    def autotune(self, iters: int = 10) -> dict:
+        """Pick the staged-kernel shape by timing the candidates on this instance."""
+        r = TuneResult()
+        check(lib().nsx_hv_autotune(self._h, iters, C.byref(r)), "nsx_hv_autotune")
+        return r.as_dict()
+
     # ---- Lanczos ----------------------------------------------------------------
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,
                 max_iter: int = 5000, seed: int = 5, start: Optional[np.ndarray] = None,
@@ -152,6 +159,15 @@ class Batch:
         s = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
         check(lib().nsx_batch_apply_dev(self._b, C.c_void_p(pin), C.c_void_p(pout), C.c_void_p(s)),
               "nsx_batch_apply_dev")
+
+    def autotune(self, iters: int = 10) -> dict:
+        r = TuneResult()
+        check(lib().nsx_batch_autotune(self._b, iters, C.byref(r)), "nsx_batch_autotune")
+        return r.as_dict()
+
+    def set_config(self, cfg: dict) -> None:
+        r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"], 0.0)
+        check(lib().nsx_batch_set_config(self._b, C.byref(r)), "nsx_batch_set_config")
 
     def time(self, d_in, d_out, warmup: int = 10, iters: int = 100):
         """HIP-event timing (ms per batched H*v, pn kernel, reduce kernel)."""
