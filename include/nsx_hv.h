@@ -86,6 +86,8 @@ typedef struct {
   int32_t max_phases;  /* phases per workgroup */
   int32_t threads;     /* workgroup size of the staged kernel */
   int32_t mode;        /* 0: copy at phase start, 1: direct LDS loads one phase ahead */
+  int32_t atomic;      /* 0: deterministic partials + reduce; 1: FP64 atomic accumulation
+                          (summation order varies run to run at the 1e-16 relative level) */
   double ms;           /* measured time of one H*v with this shape */
 } nsx_tune_result;
 int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);

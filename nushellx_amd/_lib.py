@@ -32,7 +32,7 @@ class TrlOpts(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("fat_cap", C.c_int32), ("max_phases", C.c_int32), ("threads", C.c_int32),
-                ("mode", C.c_int32), ("ms", C.c_double)]
+                ("mode", C.c_int32), ("atomic", C.c_int32), ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

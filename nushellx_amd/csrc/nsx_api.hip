@@ -51,6 +51,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
         return "LDS request too large (staged kernel)";
   if (const char* v = std::getenv("NSX_STAGED_NT")) d.staged_nt = std::atoi(v);
   if (const char* v = std::getenv("NSX_STAGED_MODE")) d.staged_mode = std::atoi(v);
+  if (const char* v = std::getenv("NSX_ATOMIC")) d.atomic = std::atoi(v);
   if (!(e = up(d.items, hp.items)).empty()) return e;
   if (!(e = up(d.aterms, hp.aterms)).empty()) return e;
   if (!(e = up(d.rterms, hp.rterms)).empty()) return e;
@@ -67,7 +68,9 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
   d.N = hp.N;
   if (d.lds_bytes > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)k_pn_partial_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)d.lds_bytes) != hipSuccess)
+                            (int)d.lds_bytes) != hipSuccess ||
+        hipFuncSetAttribute((const void*)k_pn_partial_atomic_ptr(),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds_bytes) != hipSuccess)
       return "LDS request too large";
   }
   return "";
@@ -299,13 +302,21 @@ int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warm
   std::vector<hipEvent_t> ev(3 * (size_t)iters);
   for (auto& e : ev) NSX_HIP(hipEventCreate(&e));
   for (int i = 0; i < iters; ++i) {
+    // events: [pn kernels] [reduce / like-particle kernel], in launch order
+    const bool at = b->dplan.atomic != 0;
+    auto pn = [&]() {
+      return b->precision == NSX_FP64 ? launch_pn_f64(b->dplan, d_in, d_out, s)
+                                      : launch_pn_lowp(b->dplan, b->lowp, d_in, d_out, s);
+    };
+    auto red = [&]() {
+      return b->precision == NSX_FP64 ? launch_reduce_f64(b->dplan, d_in, d_out, s)
+                                      : launch_reduce_lowp(b->dplan, b->lowp, d_in, d_out, s);
+    };
     NSX_HIP(hipEventRecord(ev[3 * i], s));
-    hipError_t e = b->precision == NSX_FP64 ? launch_pn_f64(b->dplan, d_in, s)
-                                            : launch_pn_lowp(b->dplan, b->lowp, d_in, s);
+    hipError_t e = at ? red() : pn();
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
     NSX_HIP(hipEventRecord(ev[3 * i + 1], s));
-    e = b->precision == NSX_FP64 ? launch_reduce_f64(b->dplan, d_in, d_out, s)
-                                 : launch_reduce_lowp(b->dplan, b->lowp, d_in, d_out, s);
+    e = at ? pn() : red();
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
     NSX_HIP(hipEventRecord(ev[3 * i + 2], s));
   }
@@ -315,8 +326,7 @@ int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warm
     float a = 0.f, c = 0.f;
     NSX_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
     NSX_HIP(hipEventElapsedTime(&c, ev[3 * i + 1], ev[3 * i + 2]));
-    pn += a;
-    red += c;
+    if (b->dplan.atomic) { red += a; pn += c; } else { pn += a; red += c; }
   }
   float tot = 0.f;
   NSX_HIP(hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)iters - 1]));
@@ -346,7 +356,7 @@ namespace nsx {
 
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
                    HostPlan& hp, DevicePlan& d, LowpPlan& lp) {
-  const int nt = d.staged_nt, mode = d.staged_mode;
+  const int nt = d.staged_nt, mode = d.staged_mode, at = d.atomic;
   free_plan(d);
   free_lowp(lp);
   hp = HostPlan();
@@ -357,6 +367,7 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
   if (!(e = upload_plan(hp, d)).empty()) return e;
   d.staged_nt = nt;
   d.staged_mode = mode;
+  d.atomic = at;
   return "";
 }
 
@@ -394,9 +405,9 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
   std::vector<double> h(N);
   for (int64_t i = 0; i < N; ++i) h[i] = std::sin(0.37 * double(i + 1));
   (void)hipMemcpy(in, h.data(), sizeof(double) * N, hipMemcpyHostToDevice);
-  nsx_tune_result best{0, 0, 0, 0, 1e300};
-  const int caps[] = {1536, 2560, 3584, kFatCap};
-  const int chunks[] = {3, 6, 12};
+  nsx_tune_result best{0, 0, 0, 0, 0, 1e300};
+  const int caps[] = {768, 1536, 2560, kFatCap};
+  const int chunks[] = {1, 2, 4, 8};
   const int nts[] = {64, 128, 256};
   for (int cap : caps)
     for (int mp : chunks) {
@@ -406,12 +417,14 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
       std::string e = replan(models, cfg, precision, hp, d, lp);
       if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
       for (int nt : nts)
-        for (int mode = 0; mode < 2; ++mode) {
-          d.staged_nt = nt;
-          d.staged_mode = mode;
-          const double ms = time_hv(d, lp, precision, in, out, s, iters);
-          if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, ms};
-        }
+        for (int mode = 0; mode < 2; ++mode)
+          for (int at = 0; at < 2; ++at) {
+            d.staged_nt = nt;
+            d.staged_mode = mode;
+            d.atomic = at;
+            const double ms = time_hv(d, lp, precision, in, out, s, iters);
+            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms};
+          }
     }
   (void)hipFree(in);
   (void)hipFree(out);
@@ -421,6 +434,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
   cfg.max_phases = best.max_phases;
   d.staged_nt = best.threads;
   d.staged_mode = best.mode;
+  d.atomic = best.atomic;
   std::string e = replan(models, cfg, precision, hp, d, lp);
   if (!e.empty()) { set_error(e); return NSX_EGPU; }
   if (best_out) *best_out = best;
@@ -449,6 +463,7 @@ int nsx_batch_set_config(nsx_batch* b, const nsx_tune_result* cfg) {
   pc.max_phases = std::max(cfg->max_phases, 1);
   b->dplan.staged_nt = cfg->threads;
   b->dplan.staged_mode = cfg->mode;
+  b->dplan.atomic = cfg->atomic;
   std::string e = replan(b->models, pc, b->precision, b->hplan, b->dplan, b->lowp);
   if (!e.empty()) { set_error(e); return NSX_EGPU; }
   return NSX_OK;

@@ -14,10 +14,12 @@
 
 namespace nsx {
 
+template <bool ATOMIC>
 __global__ __launch_bounds__(kThreads) void k_pn_partial(
     const DItem* __restrict__ items, const DATerm* __restrict__ aterms,
     const DRTerm* __restrict__ rterms, const double* __restrict__ V,
-    const double* __restrict__ Ad, const double* __restrict__ Bh, double* __restrict__ part) {
+    const double* __restrict__ Ad, const double* __restrict__ Bh, double* __restrict__ part,
+    double* __restrict__ W) {
   extern __shared__ double Y[];
   const DItem it = items[blockIdx.x];
   const int da = it.da, dar = it.dar, db = it.db, nc = it.nc, j0 = it.j0;
@@ -60,7 +62,9 @@ __global__ __launch_bounds__(kThreads) void k_pn_partial(
     const int e = threadIdx.x + kThreads * q;
     if (e < nW) {
       const int i = e % da, jj = e / da;
-      part[it.part_off + i + (int64_t)da * (j0 + jj)] = w[q];
+      const int64_t ij = i + (int64_t)da * (j0 + jj);
+      if (ATOMIC) unsafeAtomicAdd(W + it.w_off + ij, w[q]);
+      else part[it.part_off + ij] = w[q];
     }
   }
 }
@@ -86,11 +90,11 @@ __device__ __forceinline__ void dma_run(const uint32_t* __restrict__ src, uint32
 // phase p+1 overlaps the arithmetic of phase p.  Y (dar x db) and W (da x db)
 // accumulate in registers, MU (row, 4-column) units per thread: one LDS read of
 // V (or A) feeds 4 FMAs against broadcast reads of Bhat^T (or Y).
-template <int NT, int MU, int MODE>
+template <int NT, int MU, int MODE, bool ATOMIC>
 __global__ __launch_bounds__(NT) void k_pn_staged(
     const DItem2* __restrict__ items, const DPhase* __restrict__ phases,
     const DPair* __restrict__ pairs, const double* __restrict__ V, const double* __restrict__ Ad,
-    const double* __restrict__ Bt, double* __restrict__ part, int lds_buf) {
+    const double* __restrict__ Bt, double* __restrict__ part, double* __restrict__ W, int lds_buf) {
   extern __shared__ double sm[];
   double* bufs[2] = {sm, sm + (MODE == 1 ? lds_buf : 0)};
   double* sY = sm + (MODE == 1 ? 2 : 1) * lds_buf;
@@ -215,44 +219,50 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
       const int i = u % da, j0 = (u / da) * kColBlk;
 #pragma unroll
       for (int c = 0; c < kColBlk; ++c)
-        if (j0 + c < db) part[it.part_off + i + (int64_t)da * (j0 + c)] = w[q][c];
+        if (j0 + c < db) {
+          const int64_t ij = i + (int64_t)da * (j0 + c);
+          if (ATOMIC) unsafeAtomicAdd(W + it.w_off + ij, w[q][c]);
+          else part[it.part_off + ij] = w[q][c];
+        }
     }
   }
 }
 
 // Instances of the staged kernel: (threads, units per thread, load mode).
 struct StagedInst {
-  int nt, mu, mode;
+  int nt, mu, mode, atomic;
   const void* fn;
 };
-#define NSX_INST(NT, MU, MODE) {NT, MU, MODE, (const void*)k_pn_staged<NT, MU, MODE>}
-static const StagedInst kStaged[] = {
-    NSX_INST(64, 1, 0),  NSX_INST(64, 2, 0),  NSX_INST(64, 4, 0),  NSX_INST(64, 7, 0),
-    NSX_INST(128, 1, 0), NSX_INST(128, 2, 0), NSX_INST(128, 4, 0),
-    NSX_INST(256, 1, 0), NSX_INST(256, 2, 0),
-    NSX_INST(64, 1, 1),  NSX_INST(64, 2, 1),  NSX_INST(64, 4, 1),  NSX_INST(64, 7, 1),
-    NSX_INST(128, 1, 1), NSX_INST(128, 2, 1), NSX_INST(128, 4, 1),
-    NSX_INST(256, 1, 1), NSX_INST(256, 2, 1),
-};
+#define NSX_INST(NT, MU, MODE, AT) {NT, MU, MODE, AT, (const void*)k_pn_staged<NT, MU, MODE, AT>}
+#define NSX_INSTS(MODE, AT)                                                                 \
+  NSX_INST(64, 1, MODE, AT), NSX_INST(64, 2, MODE, AT), NSX_INST(64, 4, MODE, AT),          \
+      NSX_INST(64, 7, MODE, AT), NSX_INST(128, 1, MODE, AT), NSX_INST(128, 2, MODE, AT),    \
+      NSX_INST(128, 4, MODE, AT), NSX_INST(256, 1, MODE, AT), NSX_INST(256, 2, MODE, AT)
+static const StagedInst kStaged[] = {NSX_INSTS(0, false), NSX_INSTS(1, false), NSX_INSTS(0, true),
+                                     NSX_INSTS(1, true)};
+#undef NSX_INSTS
 #undef NSX_INST
 constexpr int kNumStaged = sizeof(kStaged) / sizeof(kStaged[0]);
 
 // smallest instance with the requested threads and mode covering max_units
-static int pick_staged(int nt, int mode, int max_units) {
+static int pick_staged(int nt, int mode, bool atomic, int max_units) {
   int best = -1;
   for (int i = 0; i < kNumStaged; ++i) {
     const StagedInst& s = kStaged[i];
-    if (s.nt != nt || s.mode != mode || s.nt * s.mu < max_units) continue;
+    if (s.nt != nt || s.mode != mode || s.atomic != (int)atomic || s.nt * s.mu < max_units) continue;
     if (best < 0 || s.mu < kStaged[best].mu) best = i;
   }
   return best;
 }
 
+// W_L = (sum of the pn partials of L, when use_partials) + H_a V_L + V_L H_b^T.
+// In atomic mode this runs first (use_partials = 0) and the pn kernels add to W.
 __global__ __launch_bounds__(kThreads) void k_reduce(
     const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
     const double* __restrict__ part, const double* __restrict__ V, const double* __restrict__ H,
-    double* __restrict__ W) {
-  const DBlock b = blocks[blockIdx.x];
+    double* __restrict__ W, int use_partials) {
+  DBlock b = blocks[blockIdx.x];
+  if (!use_partials) b.p_end = b.p_begin;
   const int da = b.da, db = b.db, nW = da * b.nc;
   const double* v = V + b.v_off;
   if (da <= kSmallMax && db <= kSmallMax && b.j0 == 0 && b.nc == db) {
@@ -313,19 +323,25 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
   }
 }
 
-hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s) {
+hipError_t launch_pn_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s) {
+  const bool at = d.atomic != 0;
   if (d.n_items2 > 0) {
-    const int k = pick_staged(d.staged_nt, d.staged_mode, d.max_units);
+    const int k = pick_staged(d.staged_nt, d.staged_mode, at, d.max_units);
     if (k < 0) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(double) * (size_t)((d.staged_mode == 1 ? 2 : 1) * d.lds_buf + d.lds_y);
-    hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt),
-                    (void*[]){(void*)&d.items2, (void*)&d.phases, (void*)&d.pairs, (void*)&V,
-                              (void*)&d.Ad, (void*)&d.Bh, (void*)&d.part, (void*)&d.lds_buf},
-                    lds, s);
+    void* args[] = {(void*)&d.items2, (void*)&d.phases, (void*)&d.pairs, (void*)&V,
+                    (void*)&d.Ad,     (void*)&d.Bh,     (void*)&d.part,  (void*)&W,
+                    (void*)&d.lds_buf};
+    hipError_t e = hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt), args, lds, s);
+    if (e != hipSuccess) return e;
   }
   if (d.n_items > 0) {
-    hipLaunchKernelGGL(k_pn_partial, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
-                       d.aterms, d.rterms, V, d.Ad, d.Bh, d.part);
+    if (at)
+      hipLaunchKernelGGL(k_pn_partial<true>, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
+                         d.aterms, d.rterms, V, d.Ad, d.Bh, d.part, W);
+    else
+      hipLaunchKernelGGL(k_pn_partial<false>, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
+                         d.aterms, d.rterms, V, d.Ad, d.Bh, d.part, W);
   }
   return hipGetLastError();
 }
@@ -333,18 +349,26 @@ hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s) {
 hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s) {
   if (d.n_blocks > 0) {
     hipLaunchKernelGGL(k_reduce, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list,
-                       d.part, V, d.H, W);
+                       d.part, V, d.H, W, d.atomic ? 0 : 1);
   }
   return hipGetLastError();
 }
 
+// One H*v: deterministic mode runs pn partials then the reduce; atomic mode
+// writes the like-particle part first and lets the pn kernels add into W.
 hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s) {
-  hipError_t e = launch_pn_f64(d, V, s);
+  if (d.atomic) {
+    hipError_t e = launch_reduce_f64(d, V, W, s);
+    if (e != hipSuccess) return e;
+    return launch_pn_f64(d, V, W, s);
+  }
+  hipError_t e = launch_pn_f64(d, V, W, s);
   if (e != hipSuccess) return e;
   return launch_reduce_f64(d, V, W, s);
 }
 
-const void* k_pn_partial_ptr() { return (const void*)k_pn_partial; }
+const void* k_pn_partial_ptr() { return (const void*)k_pn_partial<false>; }
+const void* k_pn_partial_atomic_ptr() { return (const void*)k_pn_partial<true>; }
 // number of staged-kernel instances and their entry points (launch attributes)
 int num_staged() { return kNumStaged; }
 const void* k_pn_staged_ptr(int i) { return kStaged[i].fn; }
@@ -353,7 +377,7 @@ std::string build_lowp(const HostPlan&, int, LowpPlan&) {
   return "reduced-precision H*v is not built yet";
 }
 void free_lowp(LowpPlan& lp) { lp = LowpPlan(); }
-hipError_t launch_pn_lowp(const DevicePlan&, const LowpPlan&, const double*, hipStream_t) {
+hipError_t launch_pn_lowp(const DevicePlan&, const LowpPlan&, const double*, double*, hipStream_t) {
   return hipErrorNotSupported;
 }
 hipError_t launch_reduce_lowp(const DevicePlan&, const LowpPlan&, const double*, double*, hipStream_t) {

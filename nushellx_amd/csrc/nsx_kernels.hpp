@@ -16,6 +16,7 @@ struct DevicePlan {
   int max_units = 1;       // staged kernel: largest units per item
   int staged_nt = 128;     // staged kernel threads per workgroup
   int staged_mode = 0;     // 0: copy at phase start, 1: direct LDS loads one phase ahead
+  int atomic = 0;          // 1: pn kernels add into W with FP64 atomics (no partials)
   size_t lds2_bytes = 0;
   DItem* items = nullptr;
   DATerm* aterms = nullptr;
@@ -33,9 +34,10 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d);
 void free_plan(DevicePlan& d);
 
 hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
-hipError_t launch_pn_f64(const DevicePlan& d, const double* V, hipStream_t s);
+hipError_t launch_pn_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
 hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
 const void* k_pn_partial_ptr();
+const void* k_pn_partial_atomic_ptr();
 const void* k_pn_staged_ptr(int i);
 int num_staged();
 
@@ -51,7 +53,7 @@ std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp);
 void free_lowp(LowpPlan& lp);
 hipError_t launch_hv_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W,
                           hipStream_t s);
-hipError_t launch_pn_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, hipStream_t s);
+hipError_t launch_pn_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W, hipStream_t s);
 hipError_t launch_reduce_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W,
                               hipStream_t s);
 

@@ -165,6 +165,7 @@ void plan_add(HostPlan& p, const Model& m) {
       size_t q = 0;
       while (q < ph_item.size()) {
         DItem2 d2;
+        d2.w_off = vb + m.posnv[it.L];
         d2.part_off = p.part_size;
         p.part_size += int64_t(da) * db;
         parts_of[it.L].push_back(d2.part_off);
@@ -191,6 +192,7 @@ void plan_add(HostPlan& p, const Model& m) {
     for (int j0 = 0; j0 < db; j0 += nc) {
       DItem d;
       d.part_off = base;
+      d.w_off = vb + m.posnv[it.L];
       d.da = da; d.dar = dar; d.db = db;
       d.j0 = j0; d.nc = std::min(nc, db - j0);
       d.a_begin = a_begin; d.a_end = a_end; d.pad = 0;

@@ -30,6 +30,7 @@ constexpr int kFatPF = kFatCap / kThreads;
 
 struct DItem {       // one (L, Ja') partial, one column tile
   int64_t part_off;  // partial tile base (full da x db layout of the item)
+  int64_t w_off;     // output block in the batched vector (atomic accumulation)
   int32_t da, dar, db, j0, nc;
   int32_t a_begin, a_end;
   int32_t pad;
@@ -55,7 +56,8 @@ struct DPair {
   int32_t dbr, pad;
 };
 struct DItem2 {
-  int64_t part_off;
+  int64_t part_off;  // private partial (deterministic accumulation)
+  int64_t w_off;     // output block in the batched vector (atomic accumulation)
   int32_t da, dar, db;
   int32_t ph_begin, ph_end;
   int32_t pad;

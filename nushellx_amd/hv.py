@@ -166,7 +166,8 @@ class Batch:
         return r.as_dict()
 
     def set_config(self, cfg: dict) -> None:
-        r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"], 0.0)
+        r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"],
+                       cfg.get("atomic", 0), 0.0)
         check(lib().nsx_batch_set_config(self._b, C.byref(r)), "nsx_batch_set_config")
 
     def time(self, d_in, d_out, warmup: int = 10, iters: int = 100):

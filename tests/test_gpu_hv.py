@@ -74,13 +74,13 @@ def test_batch_matches_single(models):
     # every tuned shape computes the same H*v
     cfg = b.autotune(iters=3)
     assert cfg["ms"] > 0
-    for nt, mode in ((64, 0), (128, 1), (256, 0), (256, 1)):
-        b.set_config(dict(cfg, threads=nt, mode=mode))
+    for nt, mode, at in ((64, 0, 0), (128, 1, 0), (256, 0, 1), (256, 1, 1), (128, 0, 1)):
+        b.set_config(dict(cfg, threads=nt, mode=mode, atomic=at))
         b.multiply_dev(din, dout)
         device_sync()
         w2 = np.empty_like(v)
         dout.download(w2)
-        assert np.abs(w2 - w).max() <= 1e-13 * np.abs(w).max(), (nt, mode)
+        assert np.abs(w2 - w).max() <= 1e-13 * np.abs(w).max(), (nt, mode, at)
     for h, off in zip(hams, b.offsets):
         ws = h.multiply(v[off:off + h.N])
         assert np.abs(w[off:off + h.N] - ws).max() <= 1e-13 * max(np.abs(ws).max(), 1.0)
