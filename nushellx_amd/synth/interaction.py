@@ -72,6 +72,25 @@ class Interaction:
             a, b, c, d = c, d, a, b
         return ph * self.tbme.get((a, b, c, d, J2, T2), 0.0)
 
+    # ---- linear parametrisation: H = sum_i theta_i H_i over (SPEs, TBMEs) ----------
+    def parameter_keys(self) -> List[tuple]:
+        return [("spe", o) for o in range(len(self.orbits))] + [("v",) + k for k in sorted(self.tbme)]
+
+    def parameter_vector(self) -> np.ndarray:
+        return np.concatenate([np.asarray(self.spe, dtype=float),
+                               np.array([self.tbme[k] for k in sorted(self.tbme)], dtype=float)])
+
+    def unit(self, i: int) -> "Interaction":
+        """The interaction with parameter i equal to 1 and every other parameter 0."""
+        keys = sorted(self.tbme)
+        spe = np.zeros(len(self.orbits))
+        tb = {k: 0.0 for k in keys}
+        if i < len(spe):
+            spe[i] = 1.0
+        else:
+            tb[keys[i - len(spe)]] = 1.0
+        return Interaction(self.orbits, spe, tb)
+
     def perturbed(self, seed: int, rel_sigma: float = 0.1) -> "Interaction":
         """V_s = V_0 + eps, eps_i ~ N(0, (rel_sigma |V_0,i|)^2) independently."""
         rng = np.random.Generator(np.random.PCG64(seed))

@@ -369,6 +369,30 @@ class SideBasis:
                     np.add.at(H, (dst, src[free]), sg * V[x, y])
         return H
 
+    def hamiltonian_basis(self, inter: Interaction):
+        """Per 2J: (segment partition pairs, stack of per-parameter segment matrices)
+        so that H(theta) segments = sum_i theta_i M_i (parameters of
+        ``Interaction.parameter_keys``; T=0 TBMEs do not act between like nucleons)."""
+        keys = inter.parameter_keys()
+        out = {}
+        for J2 in self.J2s:
+            S = self.slice(J2)
+            ph = self.phi(J2)
+            pairs = [(p, q) for p in range(self.no_spart) if ph[p].shape[1]
+                     for q in range(p, self.no_spart)
+                     if ph[q].shape[1] and self.part_parity[p] == self.part_parity[q]]
+            mats = [np.zeros((len(keys), ph[q].shape[1], ph[p].shape[1])) for (p, q) in pairs]
+            for i, k in enumerate(keys):
+                if k[0] == "v" and k[-1] == 0:
+                    continue
+                Hm = self.hamiltonian_m(J2, inter.unit(i))
+                for n, (p, q) in enumerate(pairs):
+                    blk = Hm[S.start[q]:S.start[q] + S.count[q], S.start[p]:S.start[p] + S.count[p]]
+                    mats[n][i] = ph[q].T @ blk @ ph[p]
+            keep = [n for n, (p, q) in enumerate(pairs) if p == q or np.abs(mats[n]).max() > 1e-13]
+            out[J2] = ([(pairs[n][0] + 1, pairs[n][1] + 1) for n in keep], [mats[n] for n in keep])
+        return out
+
     def hamiltonian_segments(self, J2: int, inter: Interaction, tol: float = 1e-13):
         """Partition segments of H(J): list of (np, npr, m[dimptr x dimpt]), np<=npr, 1-based."""
         Hm = self.hamiltonian_m(J2, inter)

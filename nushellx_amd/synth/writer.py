@@ -74,8 +74,10 @@ def _partition_record(side: SideBasis) -> np.ndarray:
 
 
 def write_side_densities(side: SideBasis, path_base: str, xtag: float, max_lam: int,
-                         dens: Optional[Dict] = None) -> None:
-    """Write ``<path_base>.tri`` and ``<path_base>.trd``."""
+                         dens: Optional[Dict] = None, real_kind: int = 4) -> None:
+    """Write ``<path_base>.tri`` and ``<path_base>.trd`` (values REAL*4, or REAL*8
+    as an rkw=8 build of the reference would write them)."""
+    vdt = np.float32 if real_kind == 4 else np.float64
     if dens is None:
         dens = side.densities(max_lam)
     nsp = side.no_spart
@@ -116,7 +118,7 @@ def write_side_densities(side: SideBasis, path_base: str, xtag: float, max_lam: 
             if nl == 0:
                 continue
             vals.append(D[offl[pl]:offl[pl] + nl, offr[pr]:offr[pr] + nr].reshape(-1, order="F"))
-        v = np.concatenate(vals).astype(np.float32) if vals else np.zeros(0, np.float32)
+        v = np.concatenate(vals).astype(vdt) if vals else np.zeros(0, vdt)
         if len(v) == 0:
             continue
         recs.append((key, v, maskcols))
@@ -133,17 +135,20 @@ def write_side_densities(side: SideBasis, path_base: str, xtag: float, max_lam: 
 
 
 def write_side_matrices(side: SideBasis, side_nucleus: str, outdir: str, inter: Interaction,
-                        xtag: float, T2: int, nmp: int, xtra: str) -> None:
-    """Write the ``.mat``/``.sgt`` pair for every J of a side (one file pair per J)."""
+                        xtag: float, T2: int, nmp: int, xtra: str, real_kind: int = 4,
+                        segments=None) -> None:
+    """Write the ``.mat``/``.sgt`` pair for every J of a side (one file pair per J).
+    ``segments`` optionally maps 2J to precomputed segments."""
+    vdt = np.float32 if real_kind == 4 else np.float64
     for J2 in side.J2s:
         name = os.path.join(outdir, side_mat_name(side_nucleus, J2, T2, nmp, xtra))
-        segs = side.hamiltonian_segments(J2, inter)
+        segs = segments[J2] if segments is not None else side.hamiltonian_segments(J2, inter)
         with RecordWriter(name + ".mat") as w:
             w.write(side.no_spart)
             for (np_, npr, m) in segs:
-                occ = np.zeros(side.space.no_shells, dtype=np.int16)
-                occ[:] = side.partitions_global[np_ - 1]
-                w.write(m.shape[1], m.shape[0], np_, npr, m.astype(np.float32), 0, 0, occ)
+                occ = np.zeros(PTDIM, dtype=np.int16)
+                occ[:side.space.no_shells] = side.partitions_global[np_ - 1]
+                w.write(m.shape[1], m.shape[0], np_, npr, m.astype(vdt), 0, 0, occ)
         with RecordWriter(name + ".sgt") as w:
             w.write(side.no_spart)
             w.write(len(segs), side.dJ(J2), np.float32(xtag))
@@ -228,12 +233,26 @@ class SideCache:
     def __init__(self):
         self._sides: Dict[Tuple, SideBasis] = {}
         self._dens: Dict[Tuple, Dict] = {}
+        self._basis: Dict[Tuple, Dict] = {}
 
     def side(self, space: PNSpace, kind: str, n: int) -> SideBasis:
         key = (space.orbits, kind, n)
         if key not in self._sides:
             self._sides[key] = SideBasis(space, kind, n)
         return self._sides[key]
+
+    def segments(self, space: PNSpace, kind: str, n: int, inter: Interaction):
+        """Like-particle H segments per 2J; linear in (SPE, TBME), so they are
+        assembled from per-parameter basis matrices cached per side."""
+        key = (space.orbits, kind, n)
+        if key not in self._basis:
+            self._basis[key] = self.side(space, kind, n).hamiltonian_basis(inter)
+        basis = self._basis[key]
+        theta = inter.parameter_vector()
+        out = {}
+        for J2, (pairs, mats) in basis.items():
+            out[J2] = [(np_, npr, np.tensordot(theta, M, axes=(0, 0))) for (np_, npr), M in zip(pairs, mats)]
+        return out
 
     def dens(self, space: PNSpace, kind: str, n: int) -> Dict:
         key = (space.orbits, kind, n)
@@ -244,7 +263,7 @@ class SideCache:
 
 def write_model(model: PNModel, outdir: str, base: str, J2s: Sequence[int], xtra: str = "x",
                 no_level: int = 5, cache: Optional[SideCache] = None, seed_tag: int = 12345,
-                densities: bool = True) -> ModelFiles:
+                densities: bool = True, real_kind: int = 4) -> ModelFiles:
     """Write every file NuTRL needs for the listed total 2J values."""
     assert len(base) <= 5, "NuShellX nucleus names are at most 5 characters + J code"
     os.makedirs(outdir, exist_ok=True)
@@ -262,14 +281,15 @@ def write_model(model: PNModel, outdir: str, base: str, J2s: Sequence[int], xtra
         side_nuc = base + role
         if densities:
             write_side_densities(side, os.path.join(outdir, side_nuc), xtag, 2 * sp_.max_j2,
-                                 dens=cache.dens(sp_, kind, n))
+                                 dens=cache.dens(sp_, kind, n), real_kind=real_kind)
         else:
             # densities already on disk: keep their consistency tag
             from .fortio import read_records
             rec = next(read_records(os.path.join(outdir, side_nuc) + ".tri"))
             xtag = float(np.frombuffer(rec[20:24], dtype=np.float32)[0])
         side_nmp = -n if kind == "p" else n
-        write_side_matrices(side, side_nuc, outdir, model.inter, xtag, T2, side_nmp, xtra)
+        write_side_matrices(side, side_nuc, outdir, model.inter, xtag, T2, side_nmp, xtra,
+                            real_kind=real_kind, segments=cache.segments(sp_, kind, n, model.inter))
     for J2 in J2s:
         write_nus(os.path.join(outdir, base + an_code(J2 // 2)) + ".nus", base + an_code(J2 // 2),
                   sp_, nval, nmp, J2, J2, T2)

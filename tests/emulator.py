@@ -25,6 +25,8 @@ def lib():
         L.emu_hv.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64]
         L.emu_plan_stats.argtypes = [C.c_char_p] * 4 + [C.c_void_p]
         L.emu_last_error.restype = C.c_char_p
+        L.emu_sixj.argtypes = [C.c_int] * 6
+        L.emu_sixj.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -44,3 +46,7 @@ def plan_stats(args):
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return dict(zip(["items2", "phases", "pairs", "items", "blocks", "part_size", "lds_buf", "operator_doubles"], out.tolist()))
+
+
+def sixj(*a):
+    return lib().emu_sixj(*a)

@@ -123,3 +123,6 @@ extern "C" int emu_plan_stats(const char* nus, const char* opa, const char* opb,
   out8[7] = (int64_t)(p.Ad.size() + p.Bh.size() + p.H.size());
   return 0;
 }
+
+// the product's recoupling coefficients, for the golden-value tests
+extern "C" double emu_sixj(int a, int b, int c, int d, int e, int f) { return nsx::sixj(a, b, c, d, e, f); }
