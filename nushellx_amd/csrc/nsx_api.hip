@@ -64,15 +64,14 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
     return "hipMalloc failed (partials)";
   d.n_items = (int)hp.items.size();
   d.n_blocks = (int)hp.blocks.size();
-  d.lds_bytes = sizeof(double) * std::max(hp.max_lds, 1);
+  d.max_lds = std::max(hp.max_lds, 1);
   d.N = hp.N;
-  if (d.lds_bytes > 64 * 1024) {
-    if (hipFuncSetAttribute((const void*)k_pn_partial_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)d.lds_bytes) != hipSuccess ||
-        hipFuncSetAttribute((const void*)k_pn_partial_atomic_ptr(),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds_bytes) != hipSuccess)
-      return "LDS request too large";
-  }
+  if (sizeof(double) * (size_t)d.max_lds > 64 * 1024)
+    for (int prec = 0; prec < 3; ++prec)
+      for (int at = 0; at < 2; ++at)
+        if (hipFuncSetAttribute(k_pn_partial_ptr(prec, at != 0), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * d.max_lds)) != hipSuccess)
+          return "LDS request too large";
   return "";
 }
 
@@ -178,8 +177,7 @@ int nsx_hv_apply_dev(nsx_hv* h, const double* d_in, double* d_out, void* stream)
   if (!h || (!d_in && h->model.N) || (!d_out && h->model.N)) { set_error("null argument"); return NSX_EARG; }
   if (d_in == d_out && h->model.N) { set_error("in-place H*v is not supported"); return NSX_EARG; }
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  hipError_t e = h->precision == NSX_FP64 ? launch_hv_f64(h->dplan, d_in, d_out, s)
-                                          : launch_hv_lowp(h->dplan, h->lowp, d_in, d_out, s);
+  hipError_t e = launch_hv(h->dplan, h->lowp, h->precision, d_in, d_out, s);
   if (e != hipSuccess) { set_error(std::string("HIP error: ") + hipGetErrorString(e)); return NSX_EGPU; }
   return NSX_OK;
 }
@@ -258,8 +256,7 @@ int nsx_batch_work(const nsx_batch* b, double* flops, double* flops_like, int64_
 int nsx_batch_apply_dev(nsx_batch* b, const double* d_in, double* d_out, void* stream) {
   if (!b || !d_in || !d_out || d_in == d_out) { set_error("bad batch apply arguments"); return NSX_EARG; }
   hipStream_t s = stream ? (hipStream_t)stream : b->stream;
-  hipError_t e = b->precision == NSX_FP64 ? launch_hv_f64(b->dplan, d_in, d_out, s)
-                                          : launch_hv_lowp(b->dplan, b->lowp, d_in, d_out, s);
+  hipError_t e = launch_hv(b->dplan, b->lowp, b->precision, d_in, d_out, s);
   if (e != hipSuccess) { set_error(std::string("HIP error: ") + hipGetErrorString(e)); return NSX_EGPU; }
   return NSX_OK;
 }
@@ -304,14 +301,12 @@ int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warm
   for (int i = 0; i < iters; ++i) {
     // events: [pn kernels] [reduce / like-particle kernel], in launch order
     const bool at = b->dplan.atomic != 0;
-    auto pn = [&]() {
-      return b->precision == NSX_FP64 ? launch_pn_f64(b->dplan, d_in, d_out, s)
-                                      : launch_pn_lowp(b->dplan, b->lowp, d_in, d_out, s);
-    };
-    auto red = [&]() {
-      return b->precision == NSX_FP64 ? launch_reduce_f64(b->dplan, d_in, d_out, s)
-                                      : launch_reduce_lowp(b->dplan, b->lowp, d_in, d_out, s);
-    };
+    auto pn = [&]() { return launch_pn(b->dplan, b->lowp, b->precision, d_in, d_out, s); };
+    auto red = [&]() { return launch_reduce(b->dplan, b->lowp, b->precision, d_in, d_out, s); };
+    {
+      hipError_t ce = launch_convert(b->dplan, b->lowp, b->precision, d_in, s);
+      if (ce != hipSuccess) { set_error(hipGetErrorString(ce)); return NSX_EGPU; }
+    }
     NSX_HIP(hipEventRecord(ev[3 * i], s));
     hipError_t e = at ? red() : pn();
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
@@ -375,14 +370,14 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
 static double time_hv(const DevicePlan& d, const LowpPlan& lp, int precision, const double* in,
                       double* out, hipStream_t s, int iters) {
   for (int i = 0; i < 2; ++i) {
-    hipError_t e = precision == NSX_FP64 ? launch_hv_f64(d, in, out, s) : launch_hv_lowp(d, lp, in, out, s);
+    hipError_t e = launch_hv(d, lp, precision, in, out, s);
     if (e != hipSuccess) return -1.0;
   }
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
   (void)hipEventRecord(a, s);
   for (int i = 0; i < iters; ++i) {
-    hipError_t e = precision == NSX_FP64 ? launch_hv_f64(d, in, out, s) : launch_hv_lowp(d, lp, in, out, s);
+    hipError_t e = launch_hv(d, lp, precision, in, out, s);
     if (e != hipSuccess) return -1.0;
   }
   (void)hipEventRecord(b, s);

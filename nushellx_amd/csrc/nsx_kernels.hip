@@ -1,57 +1,98 @@
-// HIP kernels of the pn J-scheme H*v (FP64 path) for gfx950.
+// HIP kernels of the pn J-scheme H*v for gfx950.
 //
 // Math per output block L = (Ja, Jb) (column-major d(Ja) x d(Jb)):
 //   W_L = sum_items(L) sum_{a-terms} A_k Y_k        Y_k = sum_R r_R V_R Bhat_R^T
 //       + H_a V_L + V_L H_b^T
-// Kernel 1 owns one (item, column tile): Y lives in LDS, the A Y accumulation in
-// registers; partials go to a private buffer.  Kernel 2 owns one (block, column
-// tile): deterministic partial sum plus the like-particle products.  FP64 vector
-// FMA runs at the FP64 matrix-core rate on MI355X (78.6 TFLOP/s both), so these
-// ragged, tiny (<= 33 x 33 in the sd shell) contractions use VALU FMAs.
+// The block contractions are templated on the operand storage type T:
+//   double   FP64 operands, FP64 accumulation (the plain drop-in)
+//   float    FP32 operands, FP32 accumulation
+//   Bf16x2   BF16 hi/lo pairs x = hi + lo, a*b = ah*bh + ah*bl + al*bh, FP32 accumulation
+// W is always written in FP64.  FP64 vector FMA runs at the FP64 matrix-core
+// rate on MI355X (78.6 TFLOP/s both), so the ragged, tiny (<= 33 x 33 in the sd
+// shell) contractions use VALU FMAs with 4 output columns per thread.
 #include <hip/hip_runtime.h>
 
 #include "nsx_kernels.hpp"
 
 namespace nsx {
 
-template <bool ATOMIC>
+// ---------------------------------------------------------------- precision traits
+template <typename T>
+struct Prec;
+template <>
+struct Prec<double> {
+  using acc = double;
+  __device__ static double fma3(double a, double b, double s) { return fma(a, b, s); }
+  __device__ static double from(double x) { return x; }
+};
+template <>
+struct Prec<float> {
+  using acc = float;
+  __device__ static float fma3(float a, float b, float s) { return fmaf(a, b, s); }
+  __device__ static float from(float x) { return x; }
+};
+__device__ __forceinline__ float bf_hi(Bf16x2 x) { return __uint_as_float(x.bits & 0xffff0000u); }
+__device__ __forceinline__ float bf_lo(Bf16x2 x) { return __uint_as_float(x.bits << 16); }
+template <>
+struct Prec<Bf16x2> {
+  using acc = float;
+  // split product: the lo*lo term is below the pair's resolution and is dropped
+  __device__ static float fma3(Bf16x2 a, Bf16x2 b, float s) {
+    const float ah = bf_hi(a), al = bf_lo(a), bh = bf_hi(b), bl = bf_lo(b);
+    s = fmaf(al, bh, s);
+    s = fmaf(ah, bl, s);
+    return fmaf(ah, bh, s);
+  }
+  __device__ static Bf16x2 from(float x) { return bf16x2_split(x); }
+};
+
+// FP64 vector -> working precision (input of every reduced-precision H*v)
+template <typename T>
+__global__ __launch_bounds__(256) void k_convert(const double* __restrict__ x, T* __restrict__ y, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < n) y[i] = Prec<T>::from((typename Prec<T>::acc)x[i]);
+}
+
+// ---------------------------------------------------------------- direct kernel (large blocks)
+template <typename T, bool ATOMIC>
 __global__ __launch_bounds__(kThreads) void k_pn_partial(
     const DItem* __restrict__ items, const DATerm* __restrict__ aterms,
-    const DRTerm* __restrict__ rterms, const double* __restrict__ V,
-    const double* __restrict__ Ad, const double* __restrict__ Bh, double* __restrict__ part,
-    double* __restrict__ W) {
-  extern __shared__ double Y[];
+    const DRTerm* __restrict__ rterms, const T* __restrict__ V, const T* __restrict__ Ad,
+    const T* __restrict__ Bh, double* __restrict__ part, double* __restrict__ W) {
+  using Acc = typename Prec<T>::acc;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Y = reinterpret_cast<T*>(smem_raw);
   const DItem it = items[blockIdx.x];
   const int da = it.da, dar = it.dar, db = it.db, nc = it.nc, j0 = it.j0;
   const int nW = da * nc, nY = dar * nc;
-  double w[kMaxE];
+  Acc w[kMaxE];
 #pragma unroll
-  for (int q = 0; q < kMaxE; ++q) w[q] = 0.0;
+  for (int q = 0; q < kMaxE; ++q) w[q] = Acc(0);
   for (int a = it.a_begin; a < it.a_end; ++a) {
-    const DATerm T = aterms[a];
+    const DATerm Tm = aterms[a];
     for (int e = threadIdx.x; e < nY; e += kThreads) {
       const int i = e % dar, jj = e / dar;
       const int j = j0 + jj;
-      double y = 0.0;
-      for (int u = T.r_begin; u < T.r_end; ++u) {
+      Acc y = Acc(0);
+      for (int u = Tm.r_begin; u < Tm.r_end; ++u) {
         const DRTerm R = rterms[u];
-        const double* v = V + R.v_off + i;
-        const double* b = Bh + R.b_off + (int64_t)R.dbr * j;  // Bhat^T column j
-        double s = 0.0;
-        for (int k = 0; k < R.dbr; ++k) s = fma(v[(int64_t)dar * k], b[k], s);
-        y = fma(R.r, s, y);
+        const T* v = V + R.v_off + i;
+        const T* b = Bh + R.b_off + (int64_t)R.dbr * j;  // Bhat^T column j
+        Acc s = Acc(0);
+        for (int k = 0; k < R.dbr; ++k) s = Prec<T>::fma3(v[(int64_t)dar * k], b[k], s);
+        y += Acc(R.r) * s;
       }
-      Y[e] = y;
+      Y[e] = Prec<T>::from(y);
     }
     __syncthreads();
-    const double* A = Ad + T.a_off;
+    const T* A = Ad + Tm.a_off;
 #pragma unroll
     for (int q = 0; q < kMaxE; ++q) {
       const int e = threadIdx.x + kThreads * q;
       if (e < nW) {
         const int i = e % da, jj = e / da;
-        double s = 0.0;
-        for (int k = 0; k < dar; ++k) s = fma(A[i + (int64_t)da * k], Y[k + dar * jj], s);
+        Acc s = Acc(0);
+        for (int k = 0; k < dar; ++k) s = Prec<T>::fma3(A[i + (int64_t)da * k], Y[k + dar * jj], s);
         w[q] += s;
       }
     }
@@ -63,12 +104,13 @@ __global__ __launch_bounds__(kThreads) void k_pn_partial(
     if (e < nW) {
       const int i = e % da, jj = e / da;
       const int64_t ij = i + (int64_t)da * (j0 + jj);
-      if (ATOMIC) unsafeAtomicAdd(W + it.w_off + ij, w[q]);
-      else part[it.part_off + ij] = w[q];
+      if (ATOMIC) unsafeAtomicAdd(W + it.w_off + ij, (double)w[q]);
+      else part[it.part_off + ij] = (double)w[q];
     }
   }
 }
 
+// ---------------------------------------------------------------- staged kernel (sd-size blocks)
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 // Copies n dwords src[0..n) to LDS dst[0..n) with direct global->LDS loads: the
@@ -82,45 +124,48 @@ __device__ __forceinline__ void dma_run(const uint32_t* __restrict__ src, uint32
   }
 }
 
-// Staged pn kernel for blocks with all sides <= kSmallMax.  One workgroup (NT
-// threads) runs a chunk of an item's phases.  A phase's operand runs (V_R
-// blocks, Bhat^T blocks, A) are staged in LDS, either copied through registers
-// at the start of the phase (MODE 0, single buffer) or fetched one phase ahead
-// by direct global->LDS loads into a second buffer (MODE 1) so the copy of
-// phase p+1 overlaps the arithmetic of phase p.  Y (dar x db) and W (da x db)
-// accumulate in registers, MU (row, 4-column) units per thread: one LDS read of
-// V (or A) feeds 4 FMAs against broadcast reads of Bhat^T (or Y).
-template <int NT, int MU, int MODE, bool ATOMIC>
+// One workgroup (NT threads) runs a chunk of an item's phases.  A phase's
+// operand runs (V_R blocks, Bhat^T blocks, A) are staged in LDS, either copied
+// through registers at the start of the phase (MODE 0, single buffer) or
+// fetched one phase ahead by direct global->LDS loads into a second buffer
+// (MODE 1) so the copy of phase p+1 overlaps the arithmetic of phase p.  Y
+// (dar x db) and W (da x db) accumulate in registers, MU (row, 4-column) units
+// per thread: one LDS read of V (or A) feeds 4 FMAs against broadcast reads of
+// Bhat^T (or Y).
+template <typename T, int NT, int MU, int MODE, bool ATOMIC>
 __global__ __launch_bounds__(NT) void k_pn_staged(
     const DItem2* __restrict__ items, const DPhase* __restrict__ phases,
-    const DPair* __restrict__ pairs, const double* __restrict__ V, const double* __restrict__ Ad,
-    const double* __restrict__ Bt, double* __restrict__ part, double* __restrict__ W, int lds_buf) {
-  extern __shared__ double sm[];
-  double* bufs[2] = {sm, sm + (MODE == 1 ? lds_buf : 0)};
-  double* sY = sm + (MODE == 1 ? 2 : 1) * lds_buf;
+    const DPair* __restrict__ pairs, const T* __restrict__ V, const T* __restrict__ Ad,
+    const T* __restrict__ Bt, double* __restrict__ part, double* __restrict__ W, int lds_buf) {
+  using Acc = typename Prec<T>::acc;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  T* bufs[2] = {sm, sm + (MODE == 1 ? lds_buf : 0)};
+  T* sY = sm + (MODE == 1 ? 2 : 1) * lds_buf;
   const DItem2 it = items[blockIdx.x];
   const int da = it.da, dar = it.dar, db = it.db;
   const int ncb = (db + kColBlk - 1) / kColBlk;
   const int nuY = dar * ncb, nuW = da * ncb;
   const int tid = threadIdx.x;
-  double w[MU][kColBlk], y[MU][kColBlk];
+  constexpr int kDw = sizeof(T) / 4;  // dwords per element
+  Acc w[MU][kColBlk], y[MU][kColBlk];
 #pragma unroll
   for (int q = 0; q < MU; ++q)
 #pragma unroll
-    for (int c = 0; c < kColBlk; ++c) { w[q][c] = 0.0; y[q][c] = 0.0; }
+    for (int c = 0; c < kColBlk; ++c) { w[q][c] = Acc(0); y[q][c] = Acc(0); }
 
-  auto issue = [&](const DPhase& P, double* dst) {
+  auto issue = [&](const DPhase& P, T* dst) {
     uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-    dma_run<NT>(reinterpret_cast<const uint32_t*>(V + P.v_off), d, 2 * P.nv);
-    dma_run<NT>(reinterpret_cast<const uint32_t*>(Bt + P.b_off), d + 2 * P.nv, 2 * P.nb);
+    dma_run<NT>(reinterpret_cast<const uint32_t*>(V + P.v_off), d, kDw * P.nv);
+    dma_run<NT>(reinterpret_cast<const uint32_t*>(Bt + P.b_off), d + kDw * P.nv, kDw * P.nb);
     if (P.na > 0)
-      dma_run<NT>(reinterpret_cast<const uint32_t*>(Ad + P.a_off), d + 2 * (P.nv + P.nb), 2 * P.na);
+      dma_run<NT>(reinterpret_cast<const uint32_t*>(Ad + P.a_off), d + kDw * (P.nv + P.nb), kDw * P.na);
   };
 
   if (MODE == 1 && it.ph_begin < it.ph_end) issue(phases[it.ph_begin], bufs[0]);
   for (int p = it.ph_begin; p < it.ph_end; ++p) {
     const DPhase P = phases[p];
-    double* buf = bufs[(p - it.ph_begin) & 1];
+    T* buf = bufs[(p - it.ph_begin) & 1];
     if (MODE == 1) {
       __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of phase p have landed
       __syncthreads();                // ... and every other wave's; phase p-1 is done
@@ -136,34 +181,36 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
     // Y += r V_R Bhat^T for every pair of the phase
     for (int pr = P.pair_begin; pr < P.pair_end; ++pr) {
       const DPair Q = pairs[pr];
-      const double* sV = buf + Q.v_lds;
-      const double* sB = buf + Q.b_lds;
+      const T* sV = buf + Q.v_lds;
+      const T* sB = buf + Q.b_lds;
       const int dbr = Q.dbr;
 #pragma unroll
       for (int q = 0; q < MU; ++q) {
         const int u = tid + NT * q;
         if (u < nuY) {
           const int i = u % dar, j0 = (u / dar) * kColBlk;
-          const double* bc[kColBlk];
+          const T* bc[kColBlk];
 #pragma unroll
           for (int c = 0; c < kColBlk; ++c) bc[c] = sB + dbr * min(j0 + c, db - 1);
-          double s0[kColBlk] = {0.0, 0.0, 0.0, 0.0}, s1[kColBlk] = {0.0, 0.0, 0.0, 0.0};
+          Acc s0[kColBlk], s1[kColBlk];
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c) { s0[c] = Acc(0); s1[c] = Acc(0); }
           int k = 0;
           for (; k + 1 < dbr; k += 2) {
-            const double v0 = sV[i + dar * k], v1 = sV[i + dar * (k + 1)];
+            const T v0 = sV[i + dar * k], v1 = sV[i + dar * (k + 1)];
 #pragma unroll
             for (int c = 0; c < kColBlk; ++c) {
-              s0[c] = fma(v0, bc[c][k], s0[c]);
-              s1[c] = fma(v1, bc[c][k + 1], s1[c]);
+              s0[c] = Prec<T>::fma3(v0, bc[c][k], s0[c]);
+              s1[c] = Prec<T>::fma3(v1, bc[c][k + 1], s1[c]);
             }
           }
           if (k < dbr) {
-            const double v0 = sV[i + dar * k];
+            const T v0 = sV[i + dar * k];
 #pragma unroll
-            for (int c = 0; c < kColBlk; ++c) s0[c] = fma(v0, bc[c][k], s0[c]);
+            for (int c = 0; c < kColBlk; ++c) s0[c] = Prec<T>::fma3(v0, bc[c][k], s0[c]);
           }
 #pragma unroll
-          for (int c = 0; c < kColBlk; ++c) y[q][c] = fma(Q.r, s0[c] + s1[c], y[q][c]);
+          for (int c = 0; c < kColBlk; ++c) y[q][c] += Acc(Q.r) * (s0[c] + s1[c]);
         }
       }
     }
@@ -176,35 +223,37 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
           const int i = u % dar, j0 = (u / dar) * kColBlk;
 #pragma unroll
           for (int c = 0; c < kColBlk; ++c)
-            if (j0 + c < db) sY[i + dar * (j0 + c)] = y[q][c];
+            if (j0 + c < db) sY[i + dar * (j0 + c)] = Prec<T>::from(y[q][c]);
         }
 #pragma unroll
-        for (int c = 0; c < kColBlk; ++c) y[q][c] = 0.0;
+        for (int c = 0; c < kColBlk; ++c) y[q][c] = Acc(0);
       }
       __syncthreads();
-      const double* sA = buf + P.nv + P.nb;
+      const T* sA = buf + P.nv + P.nb;
 #pragma unroll
       for (int q = 0; q < MU; ++q) {
         const int u = tid + NT * q;
         if (u < nuW) {
           const int i = u % da, j0 = (u / da) * kColBlk;
-          const double* yc[kColBlk];
+          const T* yc[kColBlk];
 #pragma unroll
           for (int c = 0; c < kColBlk; ++c) yc[c] = sY + dar * min(j0 + c, db - 1);
-          double s0[kColBlk] = {0.0, 0.0, 0.0, 0.0}, s1[kColBlk] = {0.0, 0.0, 0.0, 0.0};
+          Acc s0[kColBlk], s1[kColBlk];
+#pragma unroll
+          for (int c = 0; c < kColBlk; ++c) { s0[c] = Acc(0); s1[c] = Acc(0); }
           int k = 0;
           for (; k + 1 < dar; k += 2) {
-            const double a0 = sA[i + da * k], a1 = sA[i + da * (k + 1)];
+            const T a0 = sA[i + da * k], a1 = sA[i + da * (k + 1)];
 #pragma unroll
             for (int c = 0; c < kColBlk; ++c) {
-              s0[c] = fma(a0, yc[c][k], s0[c]);
-              s1[c] = fma(a1, yc[c][k + 1], s1[c]);
+              s0[c] = Prec<T>::fma3(a0, yc[c][k], s0[c]);
+              s1[c] = Prec<T>::fma3(a1, yc[c][k + 1], s1[c]);
             }
           }
           if (k < dar) {
-            const double a0 = sA[i + da * k];
+            const T a0 = sA[i + da * k];
 #pragma unroll
-            for (int c = 0; c < kColBlk; ++c) s0[c] = fma(a0, yc[c][k], s0[c]);
+            for (int c = 0; c < kColBlk; ++c) s0[c] = Prec<T>::fma3(a0, yc[c][k], s0[c]);
           }
 #pragma unroll
           for (int c = 0; c < kColBlk; ++c) w[q][c] += s0[c] + s1[c];
@@ -221,54 +270,29 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
       for (int c = 0; c < kColBlk; ++c)
         if (j0 + c < db) {
           const int64_t ij = i + (int64_t)da * (j0 + c);
-          if (ATOMIC) unsafeAtomicAdd(W + it.w_off + ij, w[q][c]);
-          else part[it.part_off + ij] = w[q][c];
+          if (ATOMIC) unsafeAtomicAdd(W + it.w_off + ij, (double)w[q][c]);
+          else part[it.part_off + ij] = (double)w[q][c];
         }
     }
   }
 }
 
-// Instances of the staged kernel: (threads, units per thread, load mode).
-struct StagedInst {
-  int nt, mu, mode, atomic;
-  const void* fn;
-};
-#define NSX_INST(NT, MU, MODE, AT) {NT, MU, MODE, AT, (const void*)k_pn_staged<NT, MU, MODE, AT>}
-#define NSX_INSTS(MODE, AT)                                                                 \
-  NSX_INST(64, 1, MODE, AT), NSX_INST(64, 2, MODE, AT), NSX_INST(64, 4, MODE, AT),          \
-      NSX_INST(64, 7, MODE, AT), NSX_INST(128, 1, MODE, AT), NSX_INST(128, 2, MODE, AT),    \
-      NSX_INST(128, 4, MODE, AT), NSX_INST(256, 1, MODE, AT), NSX_INST(256, 2, MODE, AT)
-static const StagedInst kStaged[] = {NSX_INSTS(0, false), NSX_INSTS(1, false), NSX_INSTS(0, true),
-                                     NSX_INSTS(1, true)};
-#undef NSX_INSTS
-#undef NSX_INST
-constexpr int kNumStaged = sizeof(kStaged) / sizeof(kStaged[0]);
-
-// smallest instance with the requested threads and mode covering max_units
-static int pick_staged(int nt, int mode, bool atomic, int max_units) {
-  int best = -1;
-  for (int i = 0; i < kNumStaged; ++i) {
-    const StagedInst& s = kStaged[i];
-    if (s.nt != nt || s.mode != mode || s.atomic != (int)atomic || s.nt * s.mu < max_units) continue;
-    if (best < 0 || s.mu < kStaged[best].mu) best = i;
-  }
-  return best;
-}
-
+// ---------------------------------------------------------------- like-particle / reduce
 // W_L = (sum of the pn partials of L, when use_partials) + H_a V_L + V_L H_b^T.
 // In atomic mode this runs first (use_partials = 0) and the pn kernels add to W.
+template <typename T>
 __global__ __launch_bounds__(kThreads) void k_reduce(
     const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
-    const double* __restrict__ part, const double* __restrict__ V, const double* __restrict__ H,
+    const double* __restrict__ part, const T* __restrict__ V, const T* __restrict__ H,
     double* __restrict__ W, int use_partials) {
+  using Acc = typename Prec<T>::acc;
   DBlock b = blocks[blockIdx.x];
   if (!use_partials) b.p_end = b.p_begin;
   const int da = b.da, db = b.db, nW = da * b.nc;
-  const double* v = V + b.v_off;
+  const T* v = V + b.v_off;
   if (da <= kSmallMax && db <= kSmallMax && b.j0 == 0 && b.nc == db) {
     // small block: stage V_L, H_a, H_b in LDS, then contract from LDS
-    __shared__ double sv[kSmallMax * kSmallMax], sha[kSmallMax * kSmallMax],
-        shb[kSmallMax * kSmallMax];
+    __shared__ T sv[kSmallMax * kSmallMax], sha[kSmallMax * kSmallMax], shb[kSmallMax * kSmallMax];
     const int nv = da * db;
     for (int e = threadIdx.x; e < nv; e += kThreads) sv[e] = v[e];
     if (b.ha_off >= 0)
@@ -289,14 +313,14 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
       for (; p < b.p_end; ++p) w0 += part[part_list[p] + e];
       double w = (w0 + w1) + (w2 + w3);
       if (b.ha_off >= 0) {
-        double s = 0.0;
-        for (int k = 0; k < da; ++k) s = fma(sha[i + da * k], sv[k + da * j], s);
-        w += s;
+        Acc s = Acc(0);
+        for (int k = 0; k < da; ++k) s = Prec<T>::fma3(sha[i + da * k], sv[k + da * j], s);
+        w += (double)s;
       }
       if (b.hb_off >= 0) {
-        double s = 0.0;
-        for (int k = 0; k < db; ++k) s = fma(sv[i + da * k], shb[j + db * k], s);
-        w += s;
+        Acc s = Acc(0);
+        for (int k = 0; k < db; ++k) s = Prec<T>::fma3(sv[i + da * k], shb[j + db * k], s);
+        w += (double)s;
       }
       W[b.v_off + e] = w;
     }
@@ -308,83 +332,139 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
     double w = 0.0;
     for (int p = b.p_begin; p < b.p_end; ++p) w += part[part_list[p] + ij];
     if (b.ha_off >= 0) {
-      const double* Ha = H + b.ha_off;
-      double s = 0.0;
-      for (int k = 0; k < da; ++k) s = fma(Ha[i + (int64_t)da * k], v[k + (int64_t)da * j], s);
-      w += s;
+      const T* Ha = H + b.ha_off;
+      Acc s = Acc(0);
+      for (int k = 0; k < da; ++k) s = Prec<T>::fma3(Ha[i + (int64_t)da * k], v[k + (int64_t)da * j], s);
+      w += (double)s;
     }
     if (b.hb_off >= 0) {
-      const double* Hb = H + b.hb_off;
-      double s = 0.0;
-      for (int k = 0; k < db; ++k) s = fma(v[i + (int64_t)da * k], Hb[j + (int64_t)db * k], s);
-      w += s;
+      const T* Hb = H + b.hb_off;
+      Acc s = Acc(0);
+      for (int k = 0; k < db; ++k) s = Prec<T>::fma3(v[i + (int64_t)da * k], Hb[j + (int64_t)db * k], s);
+      w += (double)s;
     }
     W[b.v_off + ij] = w;
   }
 }
 
-hipError_t launch_pn_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s) {
+// ---------------------------------------------------------------- instance tables
+struct StagedInst {
+  int prec, nt, mu, mode, atomic;
+  const void* fn;
+};
+#define NSX_INST(T, P, NT, MU, MODE, AT) {P, NT, MU, MODE, AT, (const void*)k_pn_staged<T, NT, MU, MODE, AT>}
+#define NSX_INSTS(T, P, MODE, AT)                                                                    \
+  NSX_INST(T, P, 64, 1, MODE, AT), NSX_INST(T, P, 64, 2, MODE, AT), NSX_INST(T, P, 64, 4, MODE, AT), \
+      NSX_INST(T, P, 64, 7, MODE, AT), NSX_INST(T, P, 128, 1, MODE, AT),                             \
+      NSX_INST(T, P, 128, 2, MODE, AT), NSX_INST(T, P, 128, 4, MODE, AT),                            \
+      NSX_INST(T, P, 256, 1, MODE, AT), NSX_INST(T, P, 256, 2, MODE, AT)
+#define NSX_ALL(T, P) \
+  NSX_INSTS(T, P, 0, false), NSX_INSTS(T, P, 1, false), NSX_INSTS(T, P, 0, true), NSX_INSTS(T, P, 1, true)
+static const StagedInst kStaged[] = {NSX_ALL(double, 0), NSX_ALL(float, 1), NSX_ALL(Bf16x2, 2)};
+#undef NSX_ALL
+#undef NSX_INSTS
+#undef NSX_INST
+constexpr int kNumStaged = sizeof(kStaged) / sizeof(kStaged[0]);
+
+// smallest instance with the requested precision, threads and mode covering max_units
+static int pick_staged(int prec, int nt, int mode, bool atomic, int max_units) {
+  int best = -1;
+  for (int i = 0; i < kNumStaged; ++i) {
+    const StagedInst& s = kStaged[i];
+    if (s.prec != prec || s.nt != nt || s.mode != mode || s.atomic != (int)atomic ||
+        s.nt * s.mu < max_units)
+      continue;
+    if (best < 0 || s.mu < kStaged[best].mu) best = i;
+  }
+  return best;
+}
+
+int num_staged() { return kNumStaged; }
+const void* k_pn_staged_ptr(int i) { return kStaged[i].fn; }
+const void* k_pn_partial_ptr(int prec, bool atomic) {
+  switch (prec) {
+    case 0: return atomic ? (const void*)k_pn_partial<double, true> : (const void*)k_pn_partial<double, false>;
+    case 1: return atomic ? (const void*)k_pn_partial<float, true> : (const void*)k_pn_partial<float, false>;
+    default: return atomic ? (const void*)k_pn_partial<Bf16x2, true> : (const void*)k_pn_partial<Bf16x2, false>;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename T>
+static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, double* W,
+                       hipStream_t s) {
   const bool at = d.atomic != 0;
   if (d.n_items2 > 0) {
-    const int k = pick_staged(d.staged_nt, d.staged_mode, at, d.max_units);
+    const int k = pick_staged(prec, d.staged_nt, d.staged_mode, at, d.max_units);
     if (k < 0) return hipErrorInvalidConfiguration;
-    const size_t lds = sizeof(double) * (size_t)((d.staged_mode == 1 ? 2 : 1) * d.lds_buf + d.lds_y);
+    const size_t lds = sizeof(T) * (size_t)((d.staged_mode == 1 ? 2 : 1) * d.lds_buf + d.lds_y);
     void* args[] = {(void*)&d.items2, (void*)&d.phases, (void*)&d.pairs, (void*)&V,
-                    (void*)&d.Ad,     (void*)&d.Bh,     (void*)&d.part,  (void*)&W,
+                    (void*)&Ad,       (void*)&Bh,       (void*)&d.part,  (void*)&W,
                     (void*)&d.lds_buf};
     hipError_t e = hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt), args, lds, s);
     if (e != hipSuccess) return e;
   }
   if (d.n_items > 0) {
-    if (at)
-      hipLaunchKernelGGL(k_pn_partial<true>, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
-                         d.aterms, d.rterms, V, d.Ad, d.Bh, d.part, W);
-    else
-      hipLaunchKernelGGL(k_pn_partial<false>, dim3(d.n_items), dim3(kThreads), d.lds_bytes, s, d.items,
-                         d.aterms, d.rterms, V, d.Ad, d.Bh, d.part, W);
+    const size_t lds = sizeof(T) * (size_t)std::max(d.max_lds, 1);
+    void* args[] = {(void*)&d.items, (void*)&d.aterms, (void*)&d.rterms, (void*)&V,
+                    (void*)&Ad,      (void*)&Bh,       (void*)&d.part,   (void*)&W};
+    hipError_t e = hipLaunchKernel(k_pn_partial_ptr(prec, at), dim3(d.n_items), dim3(kThreads), args, lds, s);
+    if (e != hipSuccess) return e;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s) {
-  if (d.n_blocks > 0) {
-    hipLaunchKernelGGL(k_reduce, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list,
-                       d.part, V, d.H, W, d.atomic ? 0 : 1);
+template <typename T>
+static hipError_t reduce_T(const DevicePlan& d, const T* V, const T* H, double* W, hipStream_t s) {
+  if (d.n_blocks > 0)
+    hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
+                       V, H, W, d.atomic ? 0 : 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
+                     hipStream_t s) {
+  switch (prec) {
+    case 0: return pn_T<double>(d, 0, V, d.Ad, d.Bh, W, s);
+    case 1: return pn_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.Ad, (const float*)lp.Bh, W, s);
+    default:
+      return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.Ad, (const Bf16x2*)lp.Bh, W, s);
   }
+}
+
+hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
+                         hipStream_t s) {
+  switch (prec) {
+    case 0: return reduce_T<double>(d, V, d.H, W, s);
+    case 1: return reduce_T<float>(d, (const float*)lp.vin, (const float*)lp.H, W, s);
+    default: return reduce_T<Bf16x2>(d, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s);
+  }
+}
+
+hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s) {
+  if (prec == 0 || d.N == 0) return hipSuccess;
+  const dim3 g((unsigned)((d.N + 255) / 256));
+  if (prec == 1)
+    hipLaunchKernelGGL(k_convert<float>, g, dim3(256), 0, s, V, (float*)lp.vin, d.N);
+  else
+    hipLaunchKernelGGL(k_convert<Bf16x2>, g, dim3(256), 0, s, V, (Bf16x2*)lp.vin, d.N);
   return hipGetLastError();
 }
 
 // One H*v: deterministic mode runs pn partials then the reduce; atomic mode
 // writes the like-particle part first and lets the pn kernels add into W.
-hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s) {
-  if (d.atomic) {
-    hipError_t e = launch_reduce_f64(d, V, W, s);
-    if (e != hipSuccess) return e;
-    return launch_pn_f64(d, V, W, s);
-  }
-  hipError_t e = launch_pn_f64(d, V, W, s);
+hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
+                     hipStream_t s) {
+  hipError_t e = launch_convert(d, lp, prec, V, s);
   if (e != hipSuccess) return e;
-  return launch_reduce_f64(d, V, W, s);
-}
-
-const void* k_pn_partial_ptr() { return (const void*)k_pn_partial<false>; }
-const void* k_pn_partial_atomic_ptr() { return (const void*)k_pn_partial<true>; }
-// number of staged-kernel instances and their entry points (launch attributes)
-int num_staged() { return kNumStaged; }
-const void* k_pn_staged_ptr(int i) { return kStaged[i].fn; }
-
-std::string build_lowp(const HostPlan&, int, LowpPlan&) {
-  return "reduced-precision H*v is not built yet";
-}
-void free_lowp(LowpPlan& lp) { lp = LowpPlan(); }
-hipError_t launch_pn_lowp(const DevicePlan&, const LowpPlan&, const double*, double*, hipStream_t) {
-  return hipErrorNotSupported;
-}
-hipError_t launch_reduce_lowp(const DevicePlan&, const LowpPlan&, const double*, double*, hipStream_t) {
-  return hipErrorNotSupported;
-}
-hipError_t launch_hv_lowp(const DevicePlan&, const LowpPlan&, const double*, double*, hipStream_t) {
-  return hipErrorNotSupported;
+  if (d.atomic) {
+    e = launch_reduce(d, lp, prec, V, W, s);
+    if (e != hipSuccess) return e;
+    return launch_pn(d, lp, prec, V, W, s);
+  }
+  e = launch_pn(d, lp, prec, V, W, s);
+  if (e != hipSuccess) return e;
+  return launch_reduce(d, lp, prec, V, W, s);
 }
 
 }  // namespace nsx

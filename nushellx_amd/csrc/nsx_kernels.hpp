@@ -25,7 +25,7 @@ struct DevicePlan {
   int64_t* part_list = nullptr;
   double *Ad = nullptr, *Bh = nullptr, *H = nullptr, *part = nullptr;
   int n_items = 0, n_blocks = 0;
-  size_t lds_bytes = 0;
+  int max_lds = 1;           // direct kernel Y tile (elements)
   int64_t N = 0;
 };
 
@@ -33,28 +33,54 @@ struct DevicePlan {
 std::string upload_plan(const HostPlan& hp, DevicePlan& d);
 void free_plan(DevicePlan& d);
 
-hipError_t launch_hv_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
-hipError_t launch_pn_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
-hipError_t launch_reduce_f64(const DevicePlan& d, const double* V, double* W, hipStream_t s);
-const void* k_pn_partial_ptr();
-const void* k_pn_partial_atomic_ptr();
-const void* k_pn_staged_ptr(int i);
-int num_staged();
+// Operand storage of the BF16 hi/lo split: x ~ hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi), both round-to-nearest-even; hi in the upper 16 bits.
+struct Bf16x2 {
+  uint32_t bits;
+};
+__host__ __device__ inline uint32_t f32_bits(float x) {
+  union { float f; uint32_t u; } c;
+  c.f = x;
+  return c.u;
+}
+__host__ __device__ inline float bits_f32(uint32_t u) {
+  union { float f; uint32_t u; } c;
+  c.u = u;
+  return c.f;
+}
+// float -> bf16 bits, round to nearest even (finite inputs)
+__host__ __device__ inline uint32_t bf16_rne(float x) {
+  const uint32_t u = f32_bits(x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__host__ __device__ inline Bf16x2 bf16x2_split(float x) {
+  const uint32_t hi = bf16_rne(x);
+  const float rest = x - bits_f32(hi << 16);
+  return Bf16x2{(hi << 16) | bf16_rne(rest)};
+}
 
-// Reduced-precision operands of the block contractions (FP32 or BF16 hi/lo).
+// Reduced-precision copies of the operators (FP32 or BF16 hi/lo) and the
+// converted input vector; structure arrays are shared with the FP64 plan.
 struct LowpPlan {
   int precision = 0;
-  void* Ad = nullptr;   // operand copies in the working precision
+  void* Ad = nullptr;
   void* Bh = nullptr;
   void* H = nullptr;
-  void* vin = nullptr;  // input vector converted per call
+  void* vin = nullptr;
 };
 std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp);
 void free_lowp(LowpPlan& lp);
-hipError_t launch_hv_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W,
-                          hipStream_t s);
-hipError_t launch_pn_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W, hipStream_t s);
-hipError_t launch_reduce_lowp(const DevicePlan& d, const LowpPlan& lp, const double* V, double* W,
-                              hipStream_t s);
+
+// prec: 0 FP64, 1 FP32, 2 BF16 hi/lo.  launch_hv = convert + (pn, reduce) in mode order.
+hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
+                     hipStream_t s);
+hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s);
+hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
+                     hipStream_t s);
+hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
+                         hipStream_t s);
+int num_staged();
+const void* k_pn_staged_ptr(int i);
+const void* k_pn_partial_ptr(int prec, bool atomic);
 
 }  // namespace nsx
