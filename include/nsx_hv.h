@@ -105,6 +105,8 @@ typedef struct {
   int32_t max_iter;   /* cap on H*v applications */
   double tol;         /* residual |beta * s_last| bound per wanted pair (MeV) */
   uint64_t seed;      /* random start when no start vector is given */
+  int32_t check_every; /* Lanczos steps between convergence tests (0 = 8) */
+  int32_t graph;      /* 1: replay each block of steps as a captured hipGraph */
 } nsx_trl_opts;
 
 typedef struct {
@@ -132,6 +134,15 @@ int nsx_batch_time(nsx_batch *b, const double *d_in, double *d_out, int32_t warm
                    double *ms_hv, double *ms_pn, double *ms_reduce);
 /* device name and compute-unit count of the current device */
 int nsx_device_info(char *name, int32_t len, int32_t *cus);
+
+/* All instances of a batch in lockstep: one batched H*v per Lanczos step, the
+ * projected problems solved per instance.  start: optional concatenated start
+ * vectors (N_total doubles; warm starts).  evals/residuals: n*nev doubles;
+ * converged_at[i]: H*v applications when instance i first had all nev pairs
+ * within tol (-1 if never).  evecs: optional N_total*nev doubles, column q of
+ * instance i's rows = its q-th Ritz vector (same layout as the start vector). */
+int nsx_batch_trl_solve(nsx_batch *b, const nsx_trl_opts *opt, const double *start, double *evals,
+                        double *evecs, double *residuals, int32_t *converged_at, nsx_trl_stats *stats);
 
 const char *nsx_last_error(void);
 

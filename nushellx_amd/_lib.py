@@ -21,13 +21,15 @@ EXPORTS = [
     "nsx_batch_create", "nsx_batch_destroy", "nsx_batch_dims", "nsx_batch_offset",
     "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
-    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_last_error",
+    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_trl_solve",
+    "nsx_last_error",
 ]
 
 
 class TrlOpts(C.Structure):
     _fields_ = [("nev", C.c_int32), ("max_basis", C.c_int32), ("keep", C.c_int32),
-                ("max_iter", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64)]
+                ("max_iter", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64),
+                ("check_every", C.c_int32), ("graph", C.c_int32)]
 
 
 class TuneResult(C.Structure):
@@ -84,6 +86,7 @@ def lib() -> C.CDLL:
     L.nsx_hv_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_set_config.argtypes = [vp, C.POINTER(TuneResult)]
+    L.nsx_batch_trl_solve.argtypes = [vp, C.POINTER(TrlOpts), vp, vp, vp, vp, vp, C.POINTER(TrlStats)]
     L.nsx_last_error.restype = C.c_char_p
     _LIB = L
     return L

@@ -1,13 +1,16 @@
-// Thick-restart Lanczos (TRL) for the lowest eigenpairs of the pn J-scheme H.
+// Thick-restart Lanczos (TRL) for the lowest eigenpairs, for one instance or a
+// whole batch of instances advanced in lockstep (one batched H*v per step).
 //
 // Replaces TRL.Lnz_OP (src/NuTRL/NuCoreX.f90:240-542).  The reference keeps the
 // Lanczos vectors on disk in REAL*4 and re-reads them every step (lnzort,
-// :544-582); here the basis Q (N x (m+1), FP64) stays in HBM and the full
-// reorthogonalisation is classical Gram-Schmidt applied twice (two GEMV pairs
-// per step).  After m steps the projected matrix (diagonal Ritz block, arrow
-// row, tridiagonal tail; :426-438,472-481) is diagonalised on the host in FP64,
-// and the lowest `keep` Ritz vectors are formed on the device (Q S).  A pair is
-// converged when its residual |beta_m s_m,i| <= tol.
+// :544-582); here the basis Q (N_total x (m+1), FP64) stays in HBM and the full
+// reorthogonalisation is classical Gram-Schmidt applied twice with segmented
+// (per-instance) reductions.  A restart cycle of m steps runs without host
+// synchronisation: alpha/beta of every step are recorded on the device and read
+// back once per cycle.  On the host, per instance, the projected matrix
+// (diagonal Ritz block, arrow row, tridiagonal tail; :426-438,472-481) is
+// diagonalised in FP64, residuals |beta_m s_m,i| decide convergence, and the
+// lowest `keep` Ritz vectors are formed on the device for the restart.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,22 +23,21 @@
 
 #include "../../include/nsx_hv.h"
 #include "nsx_api_internal.hpp"
+#include "nsx_eig.hpp"
 
 namespace nsx {
 
-constexpr int kVT = 256;       // threads per block of the vector kernels
-constexpr int kDotChunks = 64; // row chunks per column in the dot kernels
+constexpr int kVT = 256;
 
-// partial[c * kDotChunks + chunk] = sum over the chunk's rows of Q[:, c] * w
-__global__ __launch_bounds__(kVT) void k_dot_partial(const double* __restrict__ Q, int64_t ldq,
-                                                     const double* __restrict__ w, int64_t n,
-                                                     double* __restrict__ partial) {
-  const int c = blockIdx.x, chunk = blockIdx.y;
-  const int64_t per = (n + kDotChunks - 1) / kDotChunks;
-  const int64_t r0 = chunk * per, r1 = min(n, r0 + per);
-  const double* q = Q + ldq * c;
+// c[i * ldc + col] = <Q[:, col], w> over the rows of instance i; grid (B, ncols)
+__global__ __launch_bounds__(kVT) void k_seg_dot(const double* __restrict__ Q, int64_t ldq, int col0,
+                                                 const double* __restrict__ w, const int64_t* __restrict__ seg,
+                                                 double* __restrict__ c, int ldc) {
+  const int i = blockIdx.x, col = col0 + blockIdx.y;
+  const int64_t r0 = seg[i], r1 = seg[i + 1];
+  const double* q = Q + ldq * col;
   double s = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += kVT) s = fma(q[i], w[i], s);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kVT) s = fma(q[r], w[r], s);
   __shared__ double red[kVT];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -43,280 +45,419 @@ __global__ __launch_bounds__(kVT) void k_dot_partial(const double* __restrict__ 
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[c * kDotChunks + chunk] = red[0];
+  if (threadIdx.x == 0) c[(int64_t)i * ldc + col] = red[0];
 }
 
-__global__ void k_dot_final(const double* __restrict__ partial, int ncols, double* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
+// w[r] -= sum_{col < ncols} Q[r, col] c[inst(r), col]
+__global__ __launch_bounds__(kVT) void k_seg_sub(double* __restrict__ w, const double* __restrict__ Q,
+                                                 int64_t ldq, int ncols, const double* __restrict__ c, int ldc,
+                                                 const int* __restrict__ inst, int64_t n) {
+  const int64_t r = blockIdx.x * (int64_t)kVT + threadIdx.x;
+  if (r >= n) return;
+  const double* ci = c + (int64_t)inst[r] * ldc;
   double s = 0.0;
-  for (int k = 0; k < kDotChunks; ++k) s += partial[c * kDotChunks + k];
-  out[c] = s;
+  for (int k = 0; k < ncols; ++k) s = fma(Q[r + ldq * k], ci[k], s);
+  w[r] -= s;
 }
 
-// w -= Q[:, 0:ncols] c
-__global__ __launch_bounds__(kVT) void k_sub_qc(double* __restrict__ w, const double* __restrict__ Q,
-                                                int64_t ldq, const double* __restrict__ c, int ncols,
-                                                int64_t n) {
-  const int64_t i = blockIdx.x * (int64_t)kVT + threadIdx.x;
-  if (i >= n) return;
+// per instance: alpha_j = c1[j] + c2[j], beta = ||w||; record both for step j;
+// an instance whose beta collapses (invariant subspace) stops at dimension j+1.
+__global__ __launch_bounds__(kVT) void k_seg_norm(const double* __restrict__ w, const int64_t* __restrict__ seg,
+                                                  const double* __restrict__ c1, const double* __restrict__ c2,
+                                                  int ldc, int j, double* __restrict__ alpha_h,
+                                                  double* __restrict__ beta_h, int B, double* __restrict__ scale,
+                                                  int* __restrict__ dim) {
+  const int i = blockIdx.x;
+  const int64_t r0 = seg[i], r1 = seg[i + 1];
   double s = 0.0;
-  for (int k = 0; k < ncols; ++k) s = fma(Q[i + ldq * k], c[k], s);
-  w[i] -= s;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kVT) s = fma(w[r], w[r], s);
+  __shared__ double red[kVT];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kVT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = c1[(int64_t)i * ldc + j] + c2[(int64_t)i * ldc + j];
+    const double b = sqrt(red[0]);
+    const bool live = dim[i] < 0;  // still growing
+    alpha_h[(int64_t)j * B + i] = a;
+    const bool breakdown = !(b > 1e-10 * fmax(1.0, fabs(a)));
+    if (live && breakdown) dim[i] = j + 1;
+    const bool ok = live && !breakdown;
+    beta_h[(int64_t)j * B + i] = ok ? b : 0.0;
+    scale[i] = ok ? 1.0 / b : 0.0;
+  }
 }
 
-// y = alpha * x
-__global__ __launch_bounds__(kVT) void k_scale(double* __restrict__ y, const double* __restrict__ x,
-                                               double alpha, int64_t n) {
-  const int64_t i = blockIdx.x * (int64_t)kVT + threadIdx.x;
-  if (i < n) y[i] = alpha * x[i];
+// q[r] = w[r] * scale[inst(r)]
+__global__ __launch_bounds__(kVT) void k_seg_scale(double* __restrict__ q, const double* __restrict__ w,
+                                                   const double* __restrict__ scale, const int* __restrict__ inst,
+                                                   int64_t n) {
+  const int64_t r = blockIdx.x * (int64_t)kVT + threadIdx.x;
+  if (r < n) q[r] = w[r] * scale[inst[r]];
 }
 
-// Y[:, 0:k] = Q[:, 0:m] S[0:m, 0:k] (S column-major, ld m)
-__global__ __launch_bounds__(kVT) void k_ritz(double* __restrict__ Y, const double* __restrict__ Q,
-                                              int64_t ldq, const double* __restrict__ S, int m, int k,
-                                              int64_t n) {
-  const int64_t i = blockIdx.x * (int64_t)kVT + threadIdx.x;
-  if (i >= n) return;
-  for (int c = 0; c < k; ++c) {
+// Y[r, col] = sum_j Q[r, j] S[inst(r)][j][col], S per instance (mm x kk, column-major)
+__global__ __launch_bounds__(kVT) void k_seg_ritz(double* __restrict__ Y, const double* __restrict__ Q,
+                                                  int64_t ldq, const double* __restrict__ S, int mm, int kk,
+                                                  const int* __restrict__ inst, int64_t n) {
+  const int64_t r = blockIdx.x * (int64_t)kVT + threadIdx.x;
+  if (r >= n) return;
+  const double* Si = S + (int64_t)inst[r] * mm * kk;
+  for (int c = 0; c < kk; ++c) {
     double s = 0.0;
-    for (int j = 0; j < m; ++j) s = fma(Q[i + ldq * j], S[j + (int64_t)m * c], s);
-    Y[i + ldq * c] = s;
+    for (int j = 0; j < mm; ++j) s = fma(Q[r + ldq * j], Si[j + (int64_t)mm * c], s);
+    Y[r + ldq * c] = s;
   }
 }
 
-// Cyclic Jacobi eigensolver for a small dense symmetric matrix (row-major n x n).
-// Eigenvalues ascending in w, eigenvectors as columns of V (column-major n x n).
-static void jacobi_eig(std::vector<double> A, int n, std::vector<double>& w, std::vector<double>& V) {
-  V.assign(size_t(n) * n, 0.0);
-  for (int i = 0; i < n; ++i) V[size_t(i) * n + i] = 1.0;
-  auto a = [&](int i, int j) -> double& { return A[size_t(i) * n + j]; };
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        tot += a(i, j) * a(i, j);
-        if (i != j) off += a(i, j) * a(i, j);
-      }
-    if (off <= 1e-30 * std::max(tot, 1e-300)) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a(p, q);
-        if (std::fabs(apq) < 1e-300) continue;
-        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p, q
-          const double akp = a(k, p), akq = a(k, q);
-          a(k, p) = c * akp - s * akq;
-          a(k, q) = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // rows p, q
-          const double apk = a(p, k), aqk = a(q, k);
-          a(p, k) = c * apk - s * aqk;
-          a(q, k) = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double vkp = V[size_t(p) * n + k], vkq = V[size_t(q) * n + k];
-          V[size_t(p) * n + k] = c * vkp - s * vkq;
-          V[size_t(q) * n + k] = s * vkp + c * vkq;
-        }
-      }
-  }
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
-  w.resize(n);
-  std::vector<double> V2(size_t(n) * n);
-  for (int c = 0; c < n; ++c) {
-    w[c] = a(idx[c], idx[c]);
-    std::memcpy(&V2[size_t(c) * n], &V[size_t(idx[c]) * n], sizeof(double) * n);
-  }
-  V.swap(V2);
-}
-
-struct TrlWork {
-  double *Q = nullptr, *w = nullptr, *c = nullptr, *partial = nullptr, *S = nullptr, *Y = nullptr;
-  ~TrlWork() {
-    (void)hipFree(Q); (void)hipFree(w); (void)hipFree(c); (void)hipFree(partial);
-    (void)hipFree(S); (void)hipFree(Y);
-  }
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { (void)hipFree(p); }
+  template <typename T>
+  T* as() { return static_cast<T*>(p); }
 };
+
+struct Instance {
+  int64_t off, n;
+  const std::vector<uint8_t>* lcoef;
+  int64_t nstates;
+};
+
+// Lockstep TRL over instances sharing one device plan.  Outputs per instance i:
+// evals[i*nev..], residuals[i*nev..], converged_at[i] (matvecs when all nev
+// pairs first met tol, -1 if never); evecs (optional): N x nev column-major.
+static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_t s,
+                   const std::vector<Instance>& inst, const nsx_trl_opts& opt, const double* start,
+                   double* evals, double* evecs, double* residuals, int32_t* converged_at,
+                   nsx_trl_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int B = (int)inst.size();
+  const int nev = opt.nev;
+  int64_t N = 0, min_states = INT64_MAX;
+  for (const auto& x : inst) { N = std::max(N, x.off + x.n); min_states = std::min(min_states, x.nstates); }
+  if (nev < 1 || N < 1) { set_error("nothing to solve"); return NSX_EARG; }
+  if (nev > min_states) { set_error("more eigenvalues requested than basis states of an instance"); return NSX_EARG; }
+  const int m = std::max(opt.max_basis, nev + 2);
+  const int keep = std::min(std::max(opt.keep, nev), m - 2);
+  const double tol = opt.tol > 0 ? opt.tol : 1e-6;
+  const int max_iter = opt.max_iter > 0 ? opt.max_iter : 5000;
+  const int check = opt.check_every > 0 ? opt.check_every : 8;
+  const int64_t ldq = N;
+  const int ldc = m + 1;
+
+  DevBuf Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S;
+#define TRL_MALLOC(buf, bytes)                                                 \
+  if (hipMalloc(&buf.p, (size_t)(bytes)) != hipSuccess) {                      \
+    set_error("hipMalloc failed (Lanczos workspace)");                         \
+    return NSX_EGPU;                                                           \
+  }
+  TRL_MALLOC(Q, sizeof(double) * N * (m + 1));
+  TRL_MALLOC(W, sizeof(double) * N);
+  TRL_MALLOC(Y, sizeof(double) * N * std::max(keep, nev));
+  TRL_MALLOC(inst_of, sizeof(int) * N);
+  TRL_MALLOC(seg, sizeof(int64_t) * (B + 1));
+  TRL_MALLOC(c1, sizeof(double) * B * ldc);
+  TRL_MALLOC(c2, sizeof(double) * B * ldc);
+  TRL_MALLOC(ah, sizeof(double) * (m + 1) * B);
+  TRL_MALLOC(bh, sizeof(double) * (m + 1) * B);
+  TRL_MALLOC(scale, sizeof(double) * B);
+  TRL_MALLOC(dim, sizeof(int) * B);
+  TRL_MALLOC(S, sizeof(double) * (size_t)B * m * std::max(keep, nev));
+#undef TRL_MALLOC
+#define TRL_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t _e = (call);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      set_error(std::string("HIP error: ") + hipGetErrorString(_e));           \
+      return NSX_EGPU;                                                         \
+    }                                                                          \
+  } while (0)
+
+  // segment table and row -> instance map (instances are contiguous and ordered)
+  std::vector<int64_t> hseg(B + 1);
+  std::vector<int> hinst(N, 0);
+  for (int i = 0; i < B; ++i) {
+    hseg[i] = inst[i].off;
+    for (int64_t r = inst[i].off; r < inst[i].off + inst[i].n; ++r) hinst[r] = i;
+  }
+  hseg[B] = inst[B - 1].off + inst[B - 1].n;
+  for (int i = 0; i + 1 < B; ++i)
+    if (inst[i].off + inst[i].n != inst[i + 1].off) { set_error("instances must be contiguous"); return NSX_EARG; }
+  TRL_HIP(hipMemcpy(seg.p, hseg.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice));
+  TRL_HIP(hipMemcpy(inst_of.p, hinst.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  const dim3 gv((unsigned)((N + kVT - 1) / kVT));
+
+  // start vectors (given, or 0.5 - U[0,1) on the parity mask as :299-312 with an own RNG)
+  {
+    std::vector<double> v0(N, 0.0);
+    std::mt19937_64 rng(opt.seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    for (int i = 0; i < B; ++i) {
+      double nrm = 0.0;
+      for (int64_t r = 0; r < inst[i].n; ++r) {
+        const bool on = (*inst[i].lcoef)[r] != 0;
+        double x = on ? (start ? start[inst[i].off + r] : 0.5 - U(rng)) : 0.0;
+        if (!std::isfinite(x)) x = 0.0;
+        v0[inst[i].off + r] = x;
+        nrm += x * x;
+      }
+      if (!(nrm > 1e-300)) {  // unusable warm start: fall back to a random start
+        nrm = 0.0;
+        for (int64_t r = 0; r < inst[i].n; ++r) {
+          const double x = (*inst[i].lcoef)[r] ? 0.5 - U(rng) : 0.0;
+          v0[inst[i].off + r] = x;
+          nrm += x * x;
+        }
+      }
+      const double sc = 1.0 / std::sqrt(nrm);
+      for (int64_t r = 0; r < inst[i].n; ++r) v0[inst[i].off + r] *= sc;
+    }
+    TRL_HIP(hipMemcpy(Q.p, v0.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  }
+
+  // Lanczos steps [j0, j1) on the stream: H*v, two Gram-Schmidt passes, normalise
+  auto launch_steps = [&](int j0, int j1) -> hipError_t {
+    for (int j = j0; j < j1; ++j) {
+      hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
+                         seg.as<int64_t>(), c1.as<double>(), ldc);
+      hipLaunchKernelGGL(k_seg_sub, gv, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
+                         c1.as<double>(), ldc, inst_of.as<int>(), N);
+      hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
+                         seg.as<int64_t>(), c2.as<double>(), ldc);
+      hipLaunchKernelGGL(k_seg_sub, gv, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
+                         c2.as<double>(), ldc, inst_of.as<int>(), N);
+      hipLaunchKernelGGL(k_seg_norm, dim3(B), dim3(kVT), 0, s, W.as<double>(), seg.as<int64_t>(), c1.as<double>(),
+                         c2.as<double>(), ldc, j, ah.as<double>(), bh.as<double>(), B, scale.as<double>(),
+                         dim.as<int>());
+      hipLaunchKernelGGL(k_seg_scale, gv, dim3(kVT), 0, s, Q.as<double>() + ldq * (j + 1), W.as<double>(),
+                         scale.as<double>(), inst_of.as<int>(), N);
+    }
+    return hipGetLastError();
+  };
+  // captured blocks of steps, keyed by (j0, j1); the pointers they hold stay fixed
+  struct GraphCache {
+    std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> g;
+    ~GraphCache() { for (auto& x : g) (void)hipGraphExecDestroy(x.second); }
+  } graphs;
+  auto run_steps = [&](int j0, int j1) -> hipError_t {
+    if (!opt.graph) return launch_steps(j0, j1);
+    for (auto& x : graphs.g)
+      if (x.first == std::make_pair(j0, j1)) return hipGraphLaunch(x.second, s);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    e = launch_steps(j0, j1);
+    hipError_t e2 = hipStreamEndCapture(s, &gr);
+    if (e != hipSuccess) { if (gr) (void)hipGraphDestroy(gr); return e; }
+    if (e2 != hipSuccess) return e2;
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return e;
+    graphs.g.push_back({{j0, j1}, ge});
+    return hipGraphLaunch(ge, s);
+  };
+
+  // per-instance state.  T: projected matrix ((m+1) x (m+1), row-major) of the
+  // current cycle; rows [0, kk) hold the kept Ritz values and the arrow column.
+  std::vector<std::vector<double>> T(B, std::vector<double>(size_t(m + 1) * (m + 1), 0.0));
+  // done: the Krylov space of the instance is exhausted (its pairs are exact);
+  // placed: its Ritz vectors already sit in Q[:, 0:nev] (set by a restart).
+  std::vector<int> done(B, 0), placed(B, 0), hdim(B, -1), mm_of(B, 0);
+  std::vector<double> theta_all(size_t(B) * nev, 0.0), res_all(size_t(B) * nev, 0.0);
+  std::vector<std::vector<double>> Sv(B);  // full eigenvectors where computed (mm x mm, col-major)
+  for (int i = 0; i < B; ++i) converged_at[i] = -1;
+  int matvecs = 0, restarts = 0, cycle_start = 0;
+  const int kk = std::min(keep, m - 1);
+  std::vector<double> ha((m + 1) * B), hb((m + 1) * B), lastc(m + 1);
+
+  // Rayleigh-Ritz of instance i on its current mm x mm matrix: eigenvalues into
+  // Th[i], eigenvectors into Sv[i] when full, else only their last components.
+  std::vector<std::vector<double>> Th(B, std::vector<double>(m + 1, 0.0));
+  auto solve = [&](int i, int mm, bool full) -> bool {
+    std::vector<double> Tm(size_t(mm) * mm);
+    for (int a = 0; a < mm; ++a)
+      for (int b = 0; b < mm; ++b) Tm[size_t(a) * mm + b] = T[i][size_t(a) * (m + 1) + b];
+    mm_of[i] = mm;
+    if (!full) return sym_eig(Tm.data(), mm, false, Th[i].data(), lastc.data());
+    Sv[i].assign(size_t(mm) * mm, 0.0);
+    if (!sym_eig(Tm.data(), mm, true, Th[i].data(), Sv[i].data())) return false;
+    for (int c = 0; c < mm; ++c) lastc[c] = Sv[i][size_t(c) * mm + mm - 1];
+    return true;
+  };
+  // record the wanted pairs' values and residuals |beta_last s_{mm-1,q}| after solve()
+  auto record = [&](int i, double beta_last) {
+    int conv = 0;
+    for (int q = 0; q < nev; ++q) {
+      theta_all[size_t(i) * nev + q] = Th[i][q];
+      const double r = std::fabs(beta_last * lastc[q]);
+      res_all[size_t(i) * nev + q] = r;
+      conv += r <= tol;
+    }
+    if (conv == nev && converged_at[i] < 0) converged_at[i] = matvecs;
+  };
+
+  bool finished = false;
+  while (!finished) {
+    // ---- one restart cycle: steps [cycle_start, m), tested every `check` steps
+    for (int i = 0; i < B; ++i) hdim[i] = done[i] ? 0 : -1;  // 0: frozen, -1: growing
+    TRL_HIP(hipMemcpyAsync(dim.p, hdim.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    int j1 = cycle_start;
+    bool all_conv = false;
+    while (j1 < m) {
+      const int j0 = j1;
+      j1 = std::min(m, j0 + check);
+      TRL_HIP(run_steps(j0, j1));
+      matvecs += j1 - j0;
+      TRL_HIP(hipMemcpyAsync(ha.data(), ah.p, sizeof(double) * j1 * B, hipMemcpyDeviceToHost, s));
+      TRL_HIP(hipMemcpyAsync(hb.data(), bh.p, sizeof(double) * j1 * B, hipMemcpyDeviceToHost, s));
+      TRL_HIP(hipMemcpyAsync(hdim.data(), dim.p, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+      TRL_HIP(hipStreamSynchronize(s));
+      for (int x = j0 * B; x < j1 * B; ++x)
+        if (!std::isfinite(ha[x]) || !std::isfinite(hb[x])) { set_error("non-finite Lanczos coefficient"); return NSX_EGPU; }
+      all_conv = true;
+      for (int i = 0; i < B; ++i) {
+        if (done[i]) continue;
+        auto t = [&](int a, int b) -> double& { return T[i][size_t(a) * (m + 1) + b]; };
+        const bool broke = hdim[i] > 0;
+        const int last = broke ? hdim[i] : j1;
+        // alpha_j on the diagonal; beta_{j-1} couples j-1 and j, also across the
+        // boundary of the previous block (not across the restart: the arrow does)
+        for (int j = j0; j < last; ++j) {
+          t(j, j) = ha[(size_t)j * B + i];
+          if (j > cycle_start) t(j - 1, j) = t(j, j - 1) = hb[(size_t)(j - 1) * B + i];
+        }
+        const double beta_last = broke ? 0.0 : hb[(size_t)(last - 1) * B + i];
+        if (!solve(i, last, broke)) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
+        record(i, beta_last);
+        if (broke) {  // invariant subspace: eigenpairs exact, Ritz vectors from Sv
+          done[i] = 1;
+          if (converged_at[i] < 0) converged_at[i] = matvecs;
+        }
+        all_conv = all_conv && converged_at[i] >= 0;
+      }
+      if (all_conv || matvecs >= max_iter) break;
+    }
+    if (all_conv || matvecs >= max_iter) { finished = true; break; }
+
+    // ---- thick restart of every instance: Q[:, 0:kk] = Q S, Q[:, kk] = q_m
+    std::vector<double> Sdev(size_t(B) * m * kk, 0.0);
+    for (int i = 0; i < B; ++i) {
+      double* Si = &Sdev[(size_t)i * m * kk];
+      if (placed[i]) {  // frozen instance whose vectors are already Q[:, 0:kk]
+        for (int c = 0; c < kk; ++c) Si[(size_t)c * m + c] = 1.0;
+        continue;
+      }
+      if (done[i]) {  // vectors from the solve at the breakdown; frozen from now on
+        const int mm = mm_of[i];
+        for (int c = 0; c < std::min(kk, mm); ++c)
+          for (int j = 0; j < mm; ++j) Si[(size_t)c * m + j] = Sv[i][size_t(c) * mm + j];
+        placed[i] = 1;
+        continue;
+      }
+      auto t = [&](int a, int b) -> double& { return T[i][size_t(a) * (m + 1) + b]; };
+      const double beta_last = hb[(size_t)(m - 1) * B + i];
+      if (!solve(i, m, true)) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
+      for (int c = 0; c < kk; ++c)
+        for (int j = 0; j < m; ++j) Si[(size_t)c * m + j] = Sv[i][size_t(c) * m + j];
+      std::fill(T[i].begin(), T[i].end(), 0.0);
+      for (int c = 0; c < kk; ++c) {
+        t(c, c) = Th[i][c];
+        const double b = beta_last * Sv[i][size_t(c) * m + (m - 1)];
+        t(c, kk) = b;
+        t(kk, c) = b;
+      }
+    }
+    TRL_HIP(hipMemcpyAsync(S.p, Sdev.data(), sizeof(double) * Sdev.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_seg_ritz, gv, dim3(kVT), 0, s, Y.as<double>(), Q.as<double>(), ldq, S.as<double>(), m,
+                       kk, inst_of.as<int>(), N);
+    TRL_HIP(hipGetLastError());
+    TRL_HIP(hipMemcpyAsync(Q.p, Y.p, sizeof(double) * N * kk, hipMemcpyDeviceToDevice, s));
+    TRL_HIP(hipMemcpyAsync(Q.as<double>() + ldq * kk, Q.as<double>() + ldq * m, sizeof(double) * N,
+                           hipMemcpyDeviceToDevice, s));
+    cycle_start = kk;
+    ++restarts;
+  }
+  for (int i = 0; i < B; ++i)
+    for (int q = 0; q < nev; ++q) {
+      evals[size_t(i) * nev + q] = theta_all[size_t(i) * nev + q];
+      if (residuals) residuals[size_t(i) * nev + q] = res_all[size_t(i) * nev + q];
+    }
+  if (evecs) {
+    // Ritz vectors of the wanted pairs of every instance from the current basis
+    int mmax = nev;
+    for (int i = 0; i < B; ++i) mmax = std::max(mmax, mm_of[i]);
+    std::vector<double> Sk(size_t(B) * mmax * nev, 0.0);
+    for (int i = 0; i < B; ++i) {
+      double* Si = &Sk[(size_t)i * mmax * nev];
+      if (placed[i]) {
+        for (int c = 0; c < nev; ++c) Si[(size_t)c * mmax + c] = 1.0;
+        continue;
+      }
+      const int mm = mm_of[i];
+      if (!done[i] && !solve(i, mm, true)) {  // the last test formed only the last row
+        set_error("projected eigenproblem did not converge");
+        return NSX_EGPU;
+      }
+      for (int c = 0; c < nev; ++c)
+        for (int j = 0; j < mm; ++j) Si[(size_t)c * mmax + j] = Sv[i][size_t(c) * mm + j];
+    }
+    TRL_HIP(hipMemcpyAsync(S.p, Sk.data(), sizeof(double) * Sk.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_seg_ritz, gv, dim3(kVT), 0, s, Y.as<double>(), Q.as<double>(), ldq, S.as<double>(), mmax,
+                       nev, inst_of.as<int>(), N);
+    TRL_HIP(hipGetLastError());
+    TRL_HIP(hipMemcpyAsync(evecs, Y.p, sizeof(double) * N * nev, hipMemcpyDeviceToHost, s));
+    TRL_HIP(hipStreamSynchronize(s));
+  }
+  if (stats) {
+    int conv = 0;
+    for (int i = 0; i < B; ++i) conv += converged_at[i] >= 0;
+    stats->matvecs = matvecs;
+    stats->restarts = restarts;
+    stats->converged = conv;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+#undef TRL_HIP
+  return NSX_OK;
+}
 
 }  // namespace nsx
 
 using namespace nsx;
 
-#define TRL_HIP(call)                                                     \
-  do {                                                                    \
-    hipError_t _e = (call);                                               \
-    if (_e != hipSuccess) {                                               \
-      set_error(std::string("HIP error: ") + hipGetErrorString(_e));      \
-      return NSX_EGPU;                                                    \
-    }                                                                     \
-  } while (0)
-
 extern "C" int nsx_trl_solve(nsx_hv* h, const nsx_trl_opts* opt, const double* start, double* evals,
                              double* evecs, double* residuals, nsx_trl_stats* stats) {
   if (!h || !opt || !evals) { set_error("null argument"); return NSX_EARG; }
-  const auto t0 = std::chrono::steady_clock::now();
-  const int64_t N = h->model.N;
-  const int nev = opt->nev;
-  if (nev < 1 || N < 1) { set_error("nothing to solve"); return NSX_EARG; }
-  int64_t nstates = 0;
-  for (uint8_t b : h->model.lcoef) nstates += b;
-  if (nev > nstates) { set_error("more eigenvalues requested than basis states"); return NSX_EARG; }
-  const int m = (int)std::min<int64_t>(std::max(opt->max_basis, nev + 2), nstates);
-  const int keep = std::min(std::max(opt->keep, nev), std::max(m - 2, 1));
-  const double tol = opt->tol > 0 ? opt->tol : 1e-6;
-  const int max_iter = opt->max_iter > 0 ? opt->max_iter : 5000;
-  hipStream_t s = h->stream;
-  TrlWork W;
-  const int64_t ldq = N;
-  TRL_HIP(hipMalloc(&W.Q, sizeof(double) * N * (m + 1)));
-  TRL_HIP(hipMalloc(&W.w, sizeof(double) * N));
-  TRL_HIP(hipMalloc(&W.c, sizeof(double) * (m + 1)));
-  TRL_HIP(hipMalloc(&W.partial, sizeof(double) * (m + 1) * kDotChunks));
-  TRL_HIP(hipMalloc(&W.S, sizeof(double) * (size_t)m * m));
-  TRL_HIP(hipMalloc(&W.Y, sizeof(double) * N * (m + 1)));
-  const dim3 gv((unsigned)((N + kVT - 1) / kVT));
+  int64_t nst = 0;
+  for (uint8_t b : h->model.lcoef) nst += b;
+  std::vector<Instance> inst{{0, h->model.N, &h->model.lcoef, nst}};
+  nsx_trl_opts o = *opt;
+  o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, nst);
+  int32_t conv_at = -1;
+  int rc = trl_run(h->dplan, h->lowp, h->precision, h->stream, inst, o, start, evals, evecs, residuals,
+                   &conv_at, stats);
+  if (rc == NSX_OK && stats) stats->converged = conv_at >= 0 ? opt->nev : 0;
+  return rc;
+}
 
-  // projections c = Q[:, 0:nc]^T x, returned on the host
-  std::vector<double> hc(m + 1);
-  auto project = [&](const double* x, int nc) -> hipError_t {
-    hipLaunchKernelGGL(k_dot_partial, dim3(nc, kDotChunks), dim3(kVT), 0, s, W.Q, ldq, x, N, W.partial);
-    hipLaunchKernelGGL(k_dot_final, dim3((nc + 63) / 64), dim3(64), 0, s, W.partial, nc, W.c);
-    hipError_t e = hipMemcpyAsync(hc.data(), W.c, sizeof(double) * nc, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
-  };
-  auto norm2 = [&](const double* x) -> double {
-    hipLaunchKernelGGL(k_dot_partial, dim3(1, kDotChunks), dim3(kVT), 0, s, x, ldq, x, N, W.partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, s, W.partial, 1, W.c);
-    double v = 0.0;
-    (void)hipMemcpyAsync(&v, W.c, sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    return v;
-  };
-
-  // start vector: given, or 0.5 - U[0,1) on the parity mask (as :299-312, own RNG)
-  std::vector<double> v0(N, 0.0);
-  std::mt19937_64 rng(opt->seed);
-  std::uniform_real_distribution<double> U(0.0, 1.0);
-  auto random_fill = [&](std::vector<double>& v) {
-    for (int64_t i = 0; i < N; ++i) v[i] = h->model.lcoef[i] ? 0.5 - U(rng) : 0.0;
-  };
-  if (start) {
-    for (int64_t i = 0; i < N; ++i) v0[i] = h->model.lcoef[i] ? start[i] : 0.0;
-  } else {
-    random_fill(v0);
+extern "C" int nsx_batch_trl_solve(nsx_batch* b, const nsx_trl_opts* opt, const double* start, double* evals,
+                                   double* evecs, double* residuals, int32_t* converged_at, nsx_trl_stats* stats) {
+  if (!b || !opt || !evals || !converged_at) { set_error("null argument"); return NSX_EARG; }
+  std::vector<Instance> inst;
+  int64_t min_st = INT64_MAX;
+  for (size_t i = 0; i < b->models.size(); ++i) {
+    int64_t nst = 0;
+    for (uint8_t x : b->models[i]->lcoef) nst += x;
+    min_st = std::min(min_st, nst);
+    inst.push_back({b->hplan.vbase[i], b->models[i]->N, &b->models[i]->lcoef, nst});
   }
-  TRL_HIP(hipMemcpyAsync(W.w, v0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-  double nrm = std::sqrt(norm2(W.w));
-  if (!(nrm > 0.0) || !std::isfinite(nrm)) {
-    random_fill(v0);
-    TRL_HIP(hipMemcpyAsync(W.w, v0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    nrm = std::sqrt(norm2(W.w));
-  }
-  hipLaunchKernelGGL(k_scale, gv, dim3(kVT), 0, s, W.Q, W.w, 1.0 / nrm, N);
-
-  // projected matrix T ((m+1) x (m+1), row-major) and bookkeeping
-  std::vector<double> T(size_t(m + 1) * (m + 1), 0.0);
-  auto t = [&](int i, int j) -> double& { return T[size_t(i) * (m + 1) + j]; };
-  int k0 = 0, matvecs = 0, restarts = 0, converged = 0;
-  std::vector<double> theta, Svec, res(nev, 0.0);
-  double beta_last = 0.0;
-  bool done = false;
-  while (!done) {
-    int j_end = m;
-    for (int j = k0; j < m; ++j) {
-      int rc = nsx_hv_apply_dev(h, W.Q + ldq * j, W.w, s);
-      if (rc) return rc;
-      ++matvecs;
-      // CGS2 against q_0..q_j
-      TRL_HIP(project(W.w, j + 1));
-      double alpha = hc[j];
-      std::vector<double> c1(hc.begin(), hc.begin() + j + 1);
-      TRL_HIP(hipMemcpyAsync(W.c, c1.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_sub_qc, gv, dim3(kVT), 0, s, W.w, W.Q, ldq, W.c, j + 1, N);
-      TRL_HIP(project(W.w, j + 1));
-      alpha += hc[j];
-      TRL_HIP(hipMemcpyAsync(W.c, hc.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_sub_qc, gv, dim3(kVT), 0, s, W.w, W.Q, ldq, W.c, j + 1, N);
-      t(j, j) = alpha;
-      double beta = std::sqrt(norm2(W.w));
-      if (!std::isfinite(beta) || !std::isfinite(alpha)) { set_error("non-finite Lanczos coefficient"); return NSX_EGPU; }
-      if (j + 1 >= nstates) { beta_last = 0.0; j_end = j + 1; break; }
-      if (beta < 1e-7 * std::max(1.0, std::fabs(alpha))) {
-        // invariant subspace: continue from a fresh random direction, coupling 0 (:368-382)
-        random_fill(v0);
-        TRL_HIP(hipMemcpyAsync(W.w, v0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-        for (int pass = 0; pass < 2; ++pass) {
-          TRL_HIP(project(W.w, j + 1));
-          TRL_HIP(hipMemcpyAsync(W.c, hc.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, s));
-          hipLaunchKernelGGL(k_sub_qc, gv, dim3(kVT), 0, s, W.w, W.Q, ldq, W.c, j + 1, N);
-        }
-        const double nb = std::sqrt(norm2(W.w));
-        hipLaunchKernelGGL(k_scale, gv, dim3(kVT), 0, s, W.Q + ldq * (j + 1), W.w, 1.0 / nb, N);
-        beta = 0.0;
-      } else {
-        hipLaunchKernelGGL(k_scale, gv, dim3(kVT), 0, s, W.Q + ldq * (j + 1), W.w, 1.0 / beta, N);
-      }
-      if (j + 1 < m + 1) { t(j, j + 1) = beta; t(j + 1, j) = beta; }
-      beta_last = beta;
-    }
-    const int mm = j_end;
-    std::vector<double> Tm(size_t(mm) * mm);
-    for (int i = 0; i < mm; ++i)
-      for (int j = 0; j < mm; ++j) Tm[size_t(i) * mm + j] = t(i, j);
-    jacobi_eig(Tm, mm, theta, Svec);
-    converged = 0;
-    for (int i = 0; i < nev; ++i) {
-      res[i] = std::fabs(beta_last * Svec[size_t(i) * mm + (mm - 1)]);
-      if (res[i] <= tol) ++converged;
-    }
-    if (converged == nev || mm < m || matvecs >= max_iter) {
-      done = true;
-      // Ritz vectors of the wanted pairs
-      if (evecs) {
-        std::vector<double> Sk(size_t(mm) * nev);
-        for (int c = 0; c < nev; ++c)
-          for (int j = 0; j < mm; ++j) Sk[size_t(c) * mm + j] = Svec[size_t(c) * mm + j];
-        TRL_HIP(hipMemcpyAsync(W.S, Sk.data(), sizeof(double) * Sk.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_ritz, gv, dim3(kVT), 0, s, W.Y, W.Q, ldq, W.S, mm, nev, N);
-        TRL_HIP(hipMemcpyAsync(evecs, W.Y, sizeof(double) * N * nev, hipMemcpyDeviceToHost, s));
-        TRL_HIP(hipStreamSynchronize(s));
-      }
-      break;
-    }
-    // thick restart: keep the lowest `keep` Ritz vectors and the residual direction q_m
-    const int kk = std::min(keep, mm - 1);
-    std::vector<double> Sk(size_t(mm) * kk);
-    for (int c = 0; c < kk; ++c)
-      for (int j = 0; j < mm; ++j) Sk[size_t(c) * mm + j] = Svec[size_t(c) * mm + j];
-    TRL_HIP(hipMemcpyAsync(W.S, Sk.data(), sizeof(double) * Sk.size(), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_ritz, gv, dim3(kVT), 0, s, W.Y, W.Q, ldq, W.S, mm, kk, N);
-    TRL_HIP(hipMemcpyAsync(W.Q, W.Y, sizeof(double) * N * kk, hipMemcpyDeviceToDevice, s));
-    TRL_HIP(hipMemcpyAsync(W.Q + ldq * kk, W.Q + ldq * mm, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-    std::fill(T.begin(), T.end(), 0.0);
-    for (int i = 0; i < kk; ++i) {
-      t(i, i) = theta[i];
-      const double b = beta_last * Svec[size_t(i) * mm + (mm - 1)];
-      t(i, kk) = b;
-      t(kk, i) = b;
-    }
-    k0 = kk;
-    ++restarts;
-  }
-  for (int i = 0; i < nev; ++i) {
-    evals[i] = theta[i];
-    if (residuals) residuals[i] = res[i];
-  }
-  if (stats) {
-    stats->matvecs = matvecs;
-    stats->restarts = restarts;
-    stats->converged = converged;
-    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  }
-  return NSX_OK;
+  nsx_trl_opts o = *opt;
+  o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, std::max<int64_t>(min_st, o.nev + 2));
+  return trl_run(b->dplan, b->lowp, b->precision, b->stream, inst, o, start, evals, evecs, residuals,
+                 converged_at, stats);
 }

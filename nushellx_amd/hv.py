@@ -109,12 +109,15 @@ class Hamiltonian:
     # ---- Lanczos ----------------------------------------------------------------
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,
                 max_iter: int = 5000, seed: int = 5, start: Optional[np.ndarray] = None,
-                vectors: bool = False):
-        """Thick-restart Lanczos for the lowest ``nev`` states (NuTRL's Lnz_OP)."""
-        opts = TrlOpts(nev, max_basis, keep, max_iter, tol, seed)
+                vectors: bool = False, check_every: int = 8, graph: bool = True):
+        """Thick-restart Lanczos for the lowest ``nev`` states (NuTRL's Lnz_OP).
+
+        Convergence is tested every ``check_every`` steps; with ``graph`` each
+        block of steps is captured once as a hipGraph and replayed."""
+        opts = TrlOpts(nev, max_basis, keep, max_iter, tol, seed, check_every, int(graph))
         ev = np.zeros(nev)
         res = np.zeros(nev)
-        vec = np.zeros((nev, self.N)) if vectors else None
+        vec = np.zeros((nev, self.N)) if vectors else None  # column-major N x nev == row-major nev x N
         st = TrlStats()
         s = None
         if start is not None:
@@ -169,6 +172,34 @@ class Batch:
         r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"],
                        cfg.get("atomic", 0), 0.0)
         check(lib().nsx_batch_set_config(self._b, C.byref(r)), "nsx_batch_set_config")
+
+    def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,
+                max_iter: int = 5000, seed: int = 5, start: Optional[np.ndarray] = None,
+                vectors: bool = False, check_every: int = 8, graph: bool = True):
+        """Lockstep thick-restart Lanczos over every instance of the batch.
+
+        ``evecs`` (with ``vectors``) is nev x N_total: row q holds every
+        instance's q-th Ritz vector at that instance's offset."""
+        n = len(self._hams)
+        opts = TrlOpts(nev, max_basis, keep, max_iter, tol, seed, check_every, int(graph))
+        vec = np.zeros((nev, self.N)) if vectors else None
+        ev = np.zeros((n, nev))
+        res = np.zeros((n, nev))
+        conv = np.zeros(n, dtype=np.int32)
+        st = TrlStats()
+        s = None
+        if start is not None:
+            start = np.ascontiguousarray(start, dtype=np.float64)
+            assert start.shape == (self.N,)
+            s = start.ctypes.data
+        check(lib().nsx_batch_trl_solve(self._b, C.byref(opts), s, ev.ctypes.data,
+                                        vec.ctypes.data if vectors else None, res.ctypes.data,
+                                        conv.ctypes.data, C.byref(st)), "nsx_batch_trl_solve")
+        out = {"evals": ev, "residuals": res, "converged_at": conv, "matvecs": st.matvecs,
+               "restarts": st.restarts, "converged": st.converged, "seconds": st.seconds}
+        if vectors:
+            out["evecs"] = vec
+        return out
 
     def time(self, d_in, d_out, warmup: int = 10, iters: int = 100):
         """HIP-event timing (ms per batched H*v, pn kernel, reduce kernel)."""

@@ -18,7 +18,7 @@ def lib():
     global _LIB
     if _LIB is None:
         newest = max(os.path.getmtime(s) for s in SRCS + [os.path.join(ROOT, "nushellx_amd", "csrc", h)
-                                                         for h in ("nsx_model.hpp", "nsx_plan.hpp", "nsx_io.hpp")])
+                                                         for h in ("nsx_model.hpp", "nsx_plan.hpp", "nsx_io.hpp", "nsx_eig.hpp")])
         if not os.path.exists(SO) or os.path.getmtime(SO) < newest:
             subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", SO, *SRCS], check=True)
         L = C.CDLL(SO)
@@ -27,6 +27,7 @@ def lib():
         L.emu_last_error.restype = C.c_char_p
         L.emu_sixj.argtypes = [C.c_int] * 6
         L.emu_sixj.restype = C.c_double
+        L.emu_sym_eig.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -50,3 +51,14 @@ def plan_stats(args):
 
 def sixj(*a):
     return lib().emu_sixj(*a)
+
+
+def sym_eig(A, full=True):
+    """Host eigensolver of the Lanczos driver (nsx_eig.hpp): (w, Z) or (w, last row)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    Z = np.zeros((n, n) if full else n)
+    if lib().emu_sym_eig(A.ctypes.data, n, int(full), w.ctypes.data, Z.ctypes.data):
+        raise RuntimeError("QL did not converge")
+    return w, (Z.T if full else Z)  # Z stored column-major: rows of the C array are eigenvectors

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../nushellx_amd/csrc/nsx_eig.hpp"
 #include "../nushellx_amd/csrc/nsx_model.hpp"
 #include "../nushellx_amd/csrc/nsx_plan.hpp"
 
@@ -126,3 +127,7 @@ extern "C" int emu_plan_stats(const char* nus, const char* opa, const char* opb,
 
 // the product's recoupling coefficients, for the golden-value tests
 extern "C" double emu_sixj(int a, int b, int c, int d, int e, int f) { return nsx::sixj(a, b, c, d, e, f); }
+
+extern "C" int emu_sym_eig(const double* A, int n, int full, double* w, double* Z) {
+  return nsx::sym_eig(A, n, full != 0, w, Z) ? 0 : 1;
+}
