@@ -132,6 +132,10 @@ int nsx_dev_sync(void);
  * launch.  Averages in ms: whole H*v, pn kernel, reduce kernel. */
 int nsx_batch_time(nsx_batch *b, const double *d_in, double *d_out, int32_t warmup, int32_t iters,
                    double *ms_hv, double *ms_pn, double *ms_reduce);
+/* iters launches of a plain 16-byte-per-lane copy kernel (bytes % 16 == 0) on
+ * the null stream; *ms = average per launch (HIP events).  Reference point for
+ * HBM counters and bandwidth. */
+int nsx_dev_copy(void *dst, const void *src, int64_t bytes, int32_t iters, double *ms);
 /* device name and compute-unit count of the current device */
 int nsx_device_info(char *name, int32_t len, int32_t *cus);
 
@@ -143,6 +147,30 @@ int nsx_device_info(char *name, int32_t len, int32_t *cus);
  * instance i's rows = its q-th Ritz vector (same layout as the start vector). */
 int nsx_batch_trl_solve(nsx_batch *b, const nsx_trl_opts *opt, const double *start, double *evals,
                         double *evecs, double *residuals, int32_t *converged_at, nsx_trl_stats *stats);
+
+/* ---------------------------------------------------------------- learned warm start
+ * No counterpart in the reference, whose Lanczos always starts from a random
+ * vector (src/NuTRL/NuCoreX.f90:299-312).  A multilayer perceptron
+ * n_in -> n_hidden (SiLU) x n_layers -> n_out (linear), trained and evaluated
+ * on the GPU: BF16 weights (rounded from FP32 master weights) in every
+ * product, FP32 activations and accumulation, loss 1/(2 rows) sum ||y_p - y||^2,
+ * Adam (beta 0.9 / 0.999, eps 1e-8).  Host arrays are row-major FP32.
+ * Parameter layout: per layer, W (out x in, row-major) then b (out). */
+typedef struct nsx_mlp nsx_mlp;
+int nsx_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out, uint64_t seed,
+                   nsx_mlp **out);
+void nsx_mlp_destroy(nsx_mlp *m);
+int nsx_mlp_num_params(const nsx_mlp *m, int64_t *n);
+int nsx_mlp_get_params(nsx_mlp *m, float *params);
+/* sets the master weights and clears the optimiser state */
+int nsx_mlp_set_params(nsx_mlp *m, const float *params);
+/* one forward + backward pass over rows (X: rows x n_in, Y: rows x n_out);
+ * grad: n_params floats (optional); loss: value of the loss */
+int nsx_mlp_grad(nsx_mlp *m, const float *X, const float *Y, int32_t rows, float *grad, double *loss);
+/* epochs of shuffled mini-batch Adam; epoch_loss (optional): mean loss per epoch */
+int nsx_mlp_train(nsx_mlp *m, const float *X, const float *Y, int32_t rows, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float *epoch_loss);
+int nsx_mlp_predict(nsx_mlp *m, const float *X, int32_t rows, float *Y);
 
 const char *nsx_last_error(void);
 

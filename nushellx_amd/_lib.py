@@ -21,7 +21,9 @@ EXPORTS = [
     "nsx_batch_create", "nsx_batch_destroy", "nsx_batch_dims", "nsx_batch_offset",
     "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
-    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_trl_solve",
+    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_trl_solve", "nsx_dev_copy",
+    "nsx_mlp_create", "nsx_mlp_destroy", "nsx_mlp_num_params", "nsx_mlp_get_params",
+    "nsx_mlp_set_params", "nsx_mlp_grad", "nsx_mlp_train", "nsx_mlp_predict",
     "nsx_last_error",
 ]
 
@@ -86,7 +88,17 @@ def lib() -> C.CDLL:
     L.nsx_hv_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_set_config.argtypes = [vp, C.POINTER(TuneResult)]
+    L.nsx_dev_copy.argtypes = [vp, vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.nsx_batch_trl_solve.argtypes = [vp, C.POINTER(TrlOpts), vp, vp, vp, vp, vp, C.POINTER(TrlStats)]
+    L.nsx_mlp_create.argtypes = [i32, i32, i32, i32, C.c_uint64, C.POINTER(vp)]
+    L.nsx_mlp_destroy.argtypes = [vp]
+    L.nsx_mlp_destroy.restype = None
+    L.nsx_mlp_num_params.argtypes = [vp, C.POINTER(i64)]
+    L.nsx_mlp_get_params.argtypes = [vp, vp]
+    L.nsx_mlp_set_params.argtypes = [vp, vp]
+    L.nsx_mlp_grad.argtypes = [vp, vp, vp, i32, vp, C.POINTER(C.c_double)]
+    L.nsx_mlp_train.argtypes = [vp, vp, vp, i32, i32, i32, C.c_float, C.c_uint64, vp]
+    L.nsx_mlp_predict.argtypes = [vp, vp, i32, vp]
     L.nsx_last_error.restype = C.c_char_p
     _LIB = L
     return L

@@ -332,6 +332,38 @@ int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warm
   return NSX_OK;
 }
 
+}  // extern "C"
+
+// plain streaming copy, 16 bytes per lane: the reference point that scales the
+// FETCH_SIZE / WRITE_SIZE counters to bytes, and a measured HBM rate
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     int64_t n) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+extern "C" {
+
+int nsx_dev_copy(void* dst, const void* src, int64_t bytes, int32_t iters, double* ms) {
+  if (!dst || !src || bytes <= 0 || bytes % 16 || iters <= 0) { set_error("bad copy arguments"); return NSX_EARG; }
+  const int64_t n = bytes / 16;
+  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  hipEvent_t e0, e1;
+  NSX_HIP(hipEventCreate(&e0));
+  NSX_HIP(hipEventCreate(&e1));
+  NSX_HIP(hipEventRecord(e0, nullptr));
+  for (int i = 0; i < iters; ++i)
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, nullptr, (const uint4*)src, (uint4*)dst, n);
+  NSX_HIP(hipGetLastError());
+  NSX_HIP(hipEventRecord(e1, nullptr));
+  NSX_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  NSX_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = t / iters;
+  return NSX_OK;
+}
+
 int nsx_device_info(char* name, int32_t len, int32_t* cus) {
   int dev = 0;
   NSX_HIP(hipGetDevice(&dev));

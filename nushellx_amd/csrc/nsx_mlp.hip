@@ -1,0 +1,498 @@
+// Learned warm start: a multilayer perceptron trained and evaluated on the GPU.
+//
+// The network maps interaction parameters (single-particle energies and JT
+// two-body matrix elements, standardised by the caller) to a Lanczos start
+// vector in the partition-block basis.  Layers: n_in -> n_hidden (SiLU) ->
+// ... (n_layers hidden layers, SiLU) -> n_out (linear).  Weights are kept in
+// FP32 as the optimiser's master copy and rounded to BF16 (round to nearest
+// even) for every forward and backward product; activations, gradients and all
+// accumulation are FP32.  Loss: L = 1/(2 n) sum_rows ||y_pred - y||^2.  Adam
+// updates the master weights.  There is no counterpart in the reference (its
+// Lanczos always starts from a random vector, src/NuTRL/NuCoreX.f90:299-312).
+//
+// Parameter layout (host views, get/set/grad): for each layer l in order,
+// W_l as out_l x in_l row-major, then b_l (out_l).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/nsx_hv.h"
+#include "nsx_api_internal.hpp"
+
+namespace nsx {
+
+__host__ __device__ inline float bf16_to_f32(uint16_t h) { return bits_f32(uint32_t(h) << 16); }
+
+template <typename T>
+__device__ inline float ldf(const T* p, int64_t i);
+template <>
+__device__ inline float ldf<float>(const float* p, int64_t i) { return p[i]; }
+template <>
+__device__ inline float ldf<uint16_t>(const uint16_t* p, int64_t i) { return bf16_to_f32(p[i]); }
+
+__device__ inline float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ inline float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+constexpr int kTile = 64, kKc = 16, kGT = 256;
+
+// C[m, n] (+)= sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn].
+// 64 x 64 output tile per block, 16 x 16 threads with 4 x 4 outputs each, K in
+// chunks of 16 through LDS (the load order follows whichever stride is unit).
+// Epilogue: + bias[n] (optional), store to C (ldc); with Zact, also store the
+// activation (SiLU when act) to Zact.  accumulate: C += result.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kGT) void k_gemm(int M, int N, int K, const TA* __restrict__ A, int64_t sam,
+                                              int64_t sak, const TB* __restrict__ B, int64_t sbk, int64_t sbn,
+                                              float* __restrict__ C, int64_t ldc, int accumulate,
+                                              const float* __restrict__ bias, int act, float* __restrict__ Zact) {
+  __shared__ float As[kKc][kTile + 1];
+  __shared__ float Bs[kKc][kTile + 1];
+  const int tm = blockIdx.y * kTile, tn = blockIdx.x * kTile;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+  const bool a_k_fast = sak == 1, b_n_fast = sbn == 1;
+  for (int k0 = 0; k0 < K; k0 += kKc) {
+    for (int e = threadIdx.x; e < kKc * kTile; e += kGT) {
+      int mm, kk;
+      if (a_k_fast) { mm = e / kKc; kk = e % kKc; } else { kk = e / kTile; mm = e % kTile; }
+      const int m = tm + mm, k = k0 + kk;
+      As[kk][mm] = (m < M && k < K) ? ldf(A, (int64_t)m * sam + (int64_t)k * sak) : 0.0f;
+      int nn;
+      if (b_n_fast) { kk = e / kTile; nn = e % kTile; } else { nn = e / kKc; kk = e % kKc; }
+      const int n = tn + nn, k2 = k0 + kk;
+      Bs[kk][nn] = (n < N && k2 < K) ? ldf(B, (int64_t)k2 * sbk + (int64_t)n * sbn) : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kKc; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = tm + ty + 16 * i;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = tn + tx + 16 * j;
+      if (n >= N) continue;
+      float v = acc[i][j];
+      if (bias) v += bias[n];
+      float* c = C + (int64_t)m * ldc + n;
+      if (accumulate) v += *c;
+      *c = v;
+      if (Zact) Zact[(int64_t)m * ldc + n] = act ? silu(v) : v;
+    }
+  }
+}
+
+// db[n] = sum_m dZ[m * ld + n]
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dZ, int M, int N, int64_t ld,
+                                                float* __restrict__ db) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; ++m) s += dZ[(int64_t)m * ld + n];
+  db[n] = s;
+}
+
+// dZ = dA * silu'(Z), elementwise
+__global__ __launch_bounds__(256) void k_silu_back(float* __restrict__ dZ, const float* __restrict__ Z, int64_t n) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i < n) dZ[i] *= silu_grad(Z[i]);
+}
+
+// dY = (Yp - Y) / rows; per-block partial sums of (Yp - Y)^2
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Yp, const float* __restrict__ Y,
+                                             float* __restrict__ dY, int64_t n, float inv_rows,
+                                             double* __restrict__ partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const float d = Yp[i] - Y[i];
+    dY[i] = d * inv_rows;
+    s += (double)d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adam on the master weights; refreshes the BF16 copy of weight entries
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float* __restrict__ g,
+                                              float* __restrict__ m1, float* __restrict__ m2,
+                                              uint16_t* __restrict__ wbf, int64_t n, float lr, float b1, float b2,
+                                              float c1, float c2, float eps) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  const float x = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  w[i] = x;
+  if (wbf) wbf[i] = (uint16_t)bf16_rne(x);
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ w, uint16_t* __restrict__ wbf,
+                                                 int64_t n) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i < n) wbf[i] = (uint16_t)bf16_rne(w[i]);
+}
+
+// rows idx[r] of src (ld cols) -> dst row r
+__global__ __launch_bounds__(256) void k_gather_rows(const float* __restrict__ src, const int* __restrict__ idx,
+                                                     float* __restrict__ dst, int rows, int64_t cols) {
+  const int r = blockIdx.y;
+  if (r >= rows) return;
+  const float* s = src + (int64_t)idx[r] * cols;
+  float* d = dst + (int64_t)r * cols;
+  for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < cols; c += (int64_t)gridDim.x * 256) d[c] = s[c];
+}
+
+}  // namespace nsx
+
+using namespace nsx;
+
+struct nsx_mlp {
+  std::vector<int> width;            // n_in, hidden..., n_out
+  std::vector<int64_t> w_off, b_off;  // offsets into the flat parameter array
+  int64_t n_params = 0;
+  float* p = nullptr;      // master parameters (FP32)
+  float* g = nullptr;      // gradients
+  float* m1 = nullptr;     // Adam moments
+  float* m2 = nullptr;
+  uint16_t* pbf = nullptr;  // BF16 copy (weights; bias entries unused)
+  int64_t step = 0;
+  // activations for `cap` rows: Z_l (pre-activation) and A_l (post) per layer
+  int cap = 0;
+  std::vector<float*> Z, A;
+  float* dA = nullptr;   // scratch gradients, cap x max width
+  float* dB = nullptr;
+  double* partial = nullptr;
+  float* X = nullptr;    // input rows (cap x n_in)
+  float* Y = nullptr;    // target rows (cap x n_out)
+  hipStream_t s = nullptr;
+  int layers() const { return (int)width.size() - 1; }
+};
+
+namespace {
+
+#define MLP_HIP(call)                                                            \
+  do {                                                                           \
+    hipError_t _e = (call);                                                      \
+    if (_e != hipSuccess) {                                                      \
+      set_error(std::string("HIP error: ") + hipGetErrorString(_e));             \
+      return NSX_EGPU;                                                           \
+    }                                                                            \
+  } while (0)
+
+int free_rows(nsx_mlp* h) {
+  for (float* x : h->Z) (void)hipFree(x);
+  for (float* x : h->A) (void)hipFree(x);
+  h->Z.clear();
+  h->A.clear();
+  (void)hipFree(h->dA);
+  (void)hipFree(h->dB);
+  (void)hipFree(h->partial);
+  (void)hipFree(h->X);
+  (void)hipFree(h->Y);
+  h->dA = h->dB = h->X = h->Y = nullptr;
+  h->partial = nullptr;
+  h->cap = 0;
+  return NSX_OK;
+}
+
+// make room for `rows` rows of activations
+int ensure_rows(nsx_mlp* h, int rows) {
+  if (rows <= h->cap) return NSX_OK;
+  free_rows(h);
+  const int L = h->layers();
+  const int wmax = *std::max_element(h->width.begin(), h->width.end());
+  h->Z.assign(L, nullptr);
+  h->A.assign(L, nullptr);
+  for (int l = 0; l < L; ++l) {
+    MLP_HIP(hipMalloc(&h->Z[l], sizeof(float) * (size_t)rows * h->width[l + 1]));
+    MLP_HIP(hipMalloc(&h->A[l], sizeof(float) * (size_t)rows * h->width[l + 1]));
+  }
+  MLP_HIP(hipMalloc(&h->dA, sizeof(float) * (size_t)rows * wmax));
+  MLP_HIP(hipMalloc(&h->dB, sizeof(float) * (size_t)rows * wmax));
+  MLP_HIP(hipMalloc(&h->partial, sizeof(double) * 1024));
+  MLP_HIP(hipMalloc(&h->X, sizeof(float) * (size_t)rows * h->width[0]));
+  MLP_HIP(hipMalloc(&h->Y, sizeof(float) * (size_t)rows * h->width.back()));
+  h->cap = rows;
+  return NSX_OK;
+}
+
+template <typename TA, typename TB>
+void gemm(hipStream_t s, int M, int N, int K, const TA* A, int64_t sam, int64_t sak, const TB* B, int64_t sbk,
+          int64_t sbn, float* C, int64_t ldc, int accumulate, const float* bias, int act, float* Zact) {
+  const dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
+  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(kGT), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
+                     accumulate, bias, act, Zact);
+}
+
+// forward over `rows` rows already in h->X; output in h->A[L-1]
+void forward(nsx_mlp* h, int rows) {
+  const int L = h->layers();
+  const float* in = h->X;
+  for (int l = 0; l < L; ++l) {
+    const int ni = h->width[l], no = h->width[l + 1];
+    // Z = in . W^T + b ; A = silu(Z) (hidden) or Z (output)
+    gemm<float, uint16_t>(h->s, rows, no, ni, in, ni, 1, h->pbf + h->w_off[l], 1, ni, h->Z[l], no, 0,
+                          h->p + h->b_off[l], l + 1 < L ? 1 : 0, h->A[l]);
+    in = h->A[l];
+  }
+}
+
+// backward from dY (in h->dA, rows x n_out) into h->g; needs forward() first
+void backward(nsx_mlp* h, int rows) {
+  const int L = h->layers();
+  float* d = h->dA;  // dZ of layer l
+  float* e = h->dB;
+  for (int l = L - 1; l >= 0; --l) {
+    const int ni = h->width[l], no = h->width[l + 1];
+    const float* in = l > 0 ? h->A[l - 1] : h->X;
+    // dW[o, i] = sum_r dZ[r, o] in[r, i]
+    gemm<float, float>(h->s, no, ni, rows, d, 1, no, in, ni, 1, h->g + h->w_off[l], ni, 0, nullptr, 0, nullptr);
+    hipLaunchKernelGGL(k_colsum, dim3((no + 255) / 256), dim3(256), 0, h->s, d, rows, no, (int64_t)no,
+                       h->g + h->b_off[l]);
+    if (l == 0) break;
+    // dA_prev[r, i] = sum_o dZ[r, o] W[o, i]; then dZ_prev = dA_prev * silu'(Z_prev)
+    gemm<float, uint16_t>(h->s, rows, ni, no, d, no, 1, h->pbf + h->w_off[l], ni, 1, e, ni, 0, nullptr, 0, nullptr);
+    const int64_t n = (int64_t)rows * ni;
+    hipLaunchKernelGGL(k_silu_back, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->s, e, h->Z[l - 1], n);
+    std::swap(d, e);
+  }
+}
+
+// loss and dY for rows in h->X / h->Y; returns sum of squares through *sq
+int loss_grad(nsx_mlp* h, int rows, double* sq) {
+  const int L = h->layers();
+  const int64_t n = (int64_t)rows * h->width.back();
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, h->s, h->A[L - 1], h->Y, h->dA, n, 1.0f / rows, h->partial);
+  if (sq) {
+    std::vector<double> part(blocks);
+    MLP_HIP(hipMemcpyAsync(part.data(), h->partial, sizeof(double) * blocks, hipMemcpyDeviceToHost, h->s));
+    MLP_HIP(hipStreamSynchronize(h->s));
+    double t = 0.0;
+    for (double x : part) t += x;
+    *sq = t;
+  }
+  return NSX_OK;
+}
+
+int refresh_bf16(nsx_mlp* h) {
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((h->n_params + 255) / 256)), dim3(256), 0, h->s, h->p, h->pbf,
+                     h->n_params);
+  MLP_HIP(hipGetLastError());
+  return NSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsx_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out, uint64_t seed,
+                   nsx_mlp** out) {
+  if (!out || n_in < 1 || n_hidden < 1 || n_layers < 0 || n_out < 1) { set_error("bad network shape"); return NSX_EARG; }
+  *out = nullptr;
+  auto* h = new nsx_mlp();
+  h->width.push_back(n_in);
+  for (int l = 0; l < n_layers; ++l) h->width.push_back(n_hidden);
+  h->width.push_back(n_out);
+  int64_t off = 0;
+  for (int l = 0; l < h->layers(); ++l) {
+    h->w_off.push_back(off);
+    off += (int64_t)h->width[l] * h->width[l + 1];
+    h->b_off.push_back(off);
+    off += h->width[l + 1];
+  }
+  h->n_params = off;
+  // init: W ~ N(0, 1/fan_in) for hidden layers, N(0, 0.01/fan_in) for the output layer, b = 0
+  std::vector<float> p(off, 0.0f);
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l < h->layers(); ++l) {
+    const double sd = std::sqrt((l + 1 < h->layers() ? 1.0 : 0.01) / h->width[l]);
+    std::normal_distribution<double> nd(0.0, sd);
+    const int64_t n = (int64_t)h->width[l] * h->width[l + 1];
+    for (int64_t i = 0; i < n; ++i) p[h->w_off[l] + i] = (float)nd(rng);
+  }
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("HIP error: ") + hipGetErrorString(e));
+    (void)hipFree(h->p); (void)hipFree(h->g); (void)hipFree(h->m1); (void)hipFree(h->m2); (void)hipFree(h->pbf);
+    delete h;
+    return NSX_EGPU;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreate(&h->s)) != hipSuccess) { delete h; set_error(hipGetErrorString(e)); return NSX_EGPU; }
+  if ((e = hipMalloc(&h->p, sizeof(float) * off)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&h->g, sizeof(float) * off)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&h->m1, sizeof(float) * off)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&h->m2, sizeof(float) * off)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&h->pbf, sizeof(uint16_t) * off)) != hipSuccess) return fail(e);
+  if ((e = hipMemcpy(h->p, p.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+  if ((e = hipMemset(h->m1, 0, sizeof(float) * off)) != hipSuccess) return fail(e);
+  if ((e = hipMemset(h->m2, 0, sizeof(float) * off)) != hipSuccess) return fail(e);
+  if (refresh_bf16(h) != NSX_OK) return NSX_EGPU;
+  *out = h;
+  return NSX_OK;
+}
+
+void nsx_mlp_destroy(nsx_mlp* h) {
+  if (!h) return;
+  if (h->s) (void)hipStreamSynchronize(h->s);
+  free_rows(h);
+  (void)hipFree(h->p);
+  (void)hipFree(h->g);
+  (void)hipFree(h->m1);
+  (void)hipFree(h->m2);
+  (void)hipFree(h->pbf);
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+}
+
+int nsx_mlp_num_params(const nsx_mlp* h, int64_t* n) {
+  if (!h || !n) { set_error("null argument"); return NSX_EARG; }
+  *n = h->n_params;
+  return NSX_OK;
+}
+
+int nsx_mlp_get_params(nsx_mlp* h, float* p) {
+  if (!h || !p) { set_error("null argument"); return NSX_EARG; }
+  MLP_HIP(hipMemcpyAsync(p, h->p, sizeof(float) * h->n_params, hipMemcpyDeviceToHost, h->s));
+  MLP_HIP(hipStreamSynchronize(h->s));
+  return NSX_OK;
+}
+
+int nsx_mlp_set_params(nsx_mlp* h, const float* p) {
+  if (!h || !p) { set_error("null argument"); return NSX_EARG; }
+  MLP_HIP(hipMemcpyAsync(h->p, p, sizeof(float) * h->n_params, hipMemcpyHostToDevice, h->s));
+  MLP_HIP(hipMemsetAsync(h->m1, 0, sizeof(float) * h->n_params, h->s));
+  MLP_HIP(hipMemsetAsync(h->m2, 0, sizeof(float) * h->n_params, h->s));
+  h->step = 0;
+  int rc = refresh_bf16(h);
+  if (rc) return rc;
+  MLP_HIP(hipStreamSynchronize(h->s));
+  return NSX_OK;
+}
+
+int nsx_mlp_grad(nsx_mlp* h, const float* X, const float* Y, int32_t rows, float* grad, double* loss) {
+  if (!h || !X || !Y || rows < 1) { set_error("bad arguments"); return NSX_EARG; }
+  int rc = ensure_rows(h, rows);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpyAsync(h->X, X, sizeof(float) * (size_t)rows * h->width[0], hipMemcpyHostToDevice, h->s));
+  MLP_HIP(hipMemcpyAsync(h->Y, Y, sizeof(float) * (size_t)rows * h->width.back(), hipMemcpyHostToDevice, h->s));
+  forward(h, rows);
+  double sq = 0.0;
+  rc = loss_grad(h, rows, &sq);
+  if (rc) return rc;
+  backward(h, rows);
+  MLP_HIP(hipGetLastError());
+  if (grad) MLP_HIP(hipMemcpyAsync(grad, h->g, sizeof(float) * h->n_params, hipMemcpyDeviceToHost, h->s));
+  MLP_HIP(hipStreamSynchronize(h->s));
+  if (loss) *loss = 0.5 * sq / rows;
+  return NSX_OK;
+}
+
+int nsx_mlp_train(nsx_mlp* h, const float* X, const float* Y, int32_t rows, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float* epoch_loss) {
+  if (!h || !X || !Y || rows < 1 || epochs < 0 || batch < 1) { set_error("bad arguments"); return NSX_EARG; }
+  batch = std::min(batch, rows);
+  int rc = ensure_rows(h, batch);
+  if (rc) return rc;
+  const int64_t nin = h->width[0], nout = h->width.back();
+  float *dX = nullptr, *dY = nullptr;
+  int* didx = nullptr;
+  auto cleanup = [&]() { (void)hipFree(dX); (void)hipFree(dY); (void)hipFree(didx); };
+#define MLP_TRY(call)                                                            \
+  do {                                                                           \
+    hipError_t _e = (call);                                                      \
+    if (_e != hipSuccess) {                                                      \
+      cleanup();                                                                 \
+      set_error(std::string("HIP error: ") + hipGetErrorString(_e));             \
+      return NSX_EGPU;                                                           \
+    }                                                                            \
+  } while (0)
+  MLP_TRY(hipMalloc(&dX, sizeof(float) * (size_t)rows * nin));
+  MLP_TRY(hipMalloc(&dY, sizeof(float) * (size_t)rows * nout));
+  MLP_TRY(hipMalloc(&didx, sizeof(int) * (size_t)rows));
+  MLP_TRY(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)rows * nin, hipMemcpyHostToDevice, h->s));
+  MLP_TRY(hipMemcpyAsync(dY, Y, sizeof(float) * (size_t)rows * nout, hipMemcpyHostToDevice, h->s));
+  std::vector<int> perm(rows);
+  for (int i = 0; i < rows; ++i) perm[i] = i;
+  std::mt19937_64 rng(seed);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    MLP_TRY(hipMemcpyAsync(didx, perm.data(), sizeof(int) * rows, hipMemcpyHostToDevice, h->s));
+    // the host permutation buffer is rewritten next epoch: wait for the copy
+    MLP_TRY(hipStreamSynchronize(h->s));
+    double ep_sq = 0.0;
+    for (int r0 = 0; r0 < rows; r0 += batch) {
+      const int nb = std::min(batch, rows - r0);
+      hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)std::min<int64_t>((nin + 255) / 256, 64), nb), dim3(256), 0,
+                         h->s, dX, didx + r0, h->X, nb, nin);
+      hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 64), nb), dim3(256), 0,
+                         h->s, dY, didx + r0, h->Y, nb, nout);
+      forward(h, nb);
+      double sq = 0.0;
+      rc = loss_grad(h, nb, epoch_loss ? &sq : nullptr);
+      if (rc) { cleanup(); return rc; }
+      ep_sq += sq;
+      backward(h, nb);
+      ++h->step;
+      const float c1 = 1.0f - std::pow(b1, (float)h->step), c2 = 1.0f - std::pow(b2, (float)h->step);
+      hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->n_params + 255) / 256)), dim3(256), 0, h->s, h->p, h->g, h->m1,
+                         h->m2, h->pbf, h->n_params, lr, b1, b2, c1, c2, eps);
+      MLP_TRY(hipGetLastError());
+    }
+    if (epoch_loss) epoch_loss[ep] = (float)(0.5 * ep_sq / rows);
+  }
+  MLP_TRY(hipStreamSynchronize(h->s));
+  cleanup();
+#undef MLP_TRY
+  return NSX_OK;
+}
+
+int nsx_mlp_predict(nsx_mlp* h, const float* X, int32_t rows, float* Y) {
+  if (!h || !X || !Y || rows < 1) { set_error("bad arguments"); return NSX_EARG; }
+  int rc = ensure_rows(h, rows);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpyAsync(h->X, X, sizeof(float) * (size_t)rows * h->width[0], hipMemcpyHostToDevice, h->s));
+  forward(h, rows);
+  MLP_HIP(hipGetLastError());
+  MLP_HIP(hipMemcpyAsync(Y, h->A[h->layers() - 1], sizeof(float) * (size_t)rows * h->width.back(),
+                         hipMemcpyDeviceToHost, h->s));
+  MLP_HIP(hipStreamSynchronize(h->s));
+  return NSX_OK;
+}
+
+}  // extern "C"
