@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 SEED = 2024
+HELD_OUT_SEED = 777
 
 
 def make_workload(outdir: str):
@@ -63,6 +64,105 @@ def cpu_baseline(mf, seconds: float = 10.0):
                       f"(REAL*4 build with the reference's 1e-7 product thresholds)"}
 
 
+def traffic_from_profile():
+    """HBM bytes per launch of the staged pn kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factor
+    recorded there), or None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_pn_staged_sd44_J0-8_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch"):
+                vals["t"] = float(line.split("=")[1])
+    return vals.get("t")
+
+
+def time_to_5(log, train_samples: int = 256, epochs: int = 200, repeats: int = 3):
+    """Wall time to the lowest 5 eigenvalues of every J = 0..8 instance of a
+    held-out sd 4p4n interaction, to residual 1e-4 MeV (which bounds each
+    eigenvalue's error by 1e-4 MeV), plain vs batched/mixed/warm-started.
+
+    plain: one FP64 handle per J, solved one after another from the random start.
+    fast: one batch of all J, FP32 or BF16 hi/lo H*v, warm start predicted by the
+    network (inference and the fallback test inside the timed region).
+    The FP64 oracle is the same solver at residual 1e-9."""
+    from nushellx_amd.hv import Batch, Hamiltonian
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import SD, PNSpace
+    from nushellx_amd.synth.writer import PNModel, SideCache, write_model
+    from nushellx_amd.warm import WarmStart, build_training_set
+
+    J2s = list(range(0, 17, 2))
+    space, base = PNSpace(SD), random_interaction(SD, seed=SEED)
+    work = tempfile.mkdtemp(prefix="nsx_t5_")
+    cache = SideCache()
+    t0 = time.perf_counter()
+    F, T, _, _ = build_training_set(space, 4, 4, J2s, base, range(1000, 1000 + train_samples), work,
+                                    cache=cache, log=log)
+    t_data = time.perf_counter() - t0
+    ws = WarmStart(F.shape[1], T.shape[1], seed=0)
+    t0 = time.perf_counter()
+    losses = ws.fit(F, T, epochs=epochs, batch=16, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    log(f"warm start trained on {len(F)} samples ({t_data:.0f} s solves, {t_fit:.1f} s fit)")
+
+    held = base.perturbed(HELD_OUT_SEED)
+    mf = write_model(PNModel(space, 4, 4, held), work, "ws", J2s, cache=cache, densities=False, xtra="y")
+    feats = held.parameter_vector()[None, :]
+    hams64 = [Hamiltonian(*mf.args(J2)) for J2 in J2s]
+    oracle = np.array([h.lanczos(nev=5, tol=1e-9)["evals"] for h in hams64])
+    for h in hams64:
+        h.autotune(iters=5)
+
+    def best(fn):
+        out = None
+        for _ in range(repeats):
+            t = time.perf_counter()
+            r = fn()
+            el = time.perf_counter() - t
+            if out is None or el < out[0]:
+                out = (el, r)
+        return out
+
+    t_plain, rp = best(lambda: [h.lanczos(nev=5, tol=1e-4) for h in hams64])
+    ev_plain = np.array([r["evals"] for r in rp])
+    res = {"workload": "sd 4p4n J=0..8, held-out interaction (base seed %d perturbed with seed %d)"
+                       % (SEED, HELD_OUT_SEED),
+           "criterion": "residual <= 1e-4 MeV for each of the 5 lowest pairs of every J",
+           "plain_fp64_s": t_plain, "plain_matvecs": int(sum(r["matvecs"] for r in rp)),
+           "plain_max_err_MeV": float(np.abs(ev_plain - oracle).max()),
+           "train": {"samples": int(len(F)), "epochs": epochs, "solve_s": t_data, "fit_s": t_fit,
+                     "loss_first": float(losses[0]), "loss_last": float(losses[-1])}}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        hams = [Hamiltonian(*mf.args(J2), precision=prec) for J2 in J2s]
+        b = Batch(hams, precision=prec)
+        b.autotune(iters=5)
+        t_rand, rr = best(lambda: b.lanczos(nev=5, tol=1e-4))
+
+        def warm():
+            v, fell = ws.start_vector(b, feats)
+            r = b.lanczos(nev=5, tol=1e-4, start=v)
+            r["fell"] = fell
+            return r
+        t_warm, rw = best(warm)
+        res[prec] = {"batched_random_s": t_rand, "batched_random_matvecs": rr["matvecs"],
+                     "batched_warm_s": t_warm, "batched_warm_matvecs": rw["matvecs"],
+                     "fallback_instances": rw["fell"],
+                     "max_err_random_MeV": float(np.abs(rr["evals"] - oracle).max()),
+                     "max_err_warm_MeV": float(np.abs(rw["evals"] - oracle).max()),
+                     "speedup_vs_plain": t_plain / t_warm}
+        b.close()
+        for h in hams:
+            h.close()
+    ok = {p: res[p]["max_err_warm_MeV"] <= 1e-4 for p in ("fp64", "fp32", "bf16x3")}
+    fastest = min((p for p in ok if ok[p]), key=lambda p: res[p]["batched_warm_s"], default=None)
+    res["fast"] = fastest
+    res["speedup"] = res[fastest]["speedup_vs_plain"] if fastest else None
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +170,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-t5", action="store_true", help="skip the time-to-5-eigenvalues comparison")
+    ap.add_argument("--train-samples", type=int, default=256)
+    ap.add_argument("--copy-mb", type=int, default=0, help="also run the streaming copy kernel (MiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -77,14 +180,18 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
     from nushellx_amd.hv import Batch, DeviceBuffer, Hamiltonian, device_sync
-    from nushellx_amd._lib import device_info
+    from nushellx_amd._lib import device_info, stream_copy
 
     tmp = tempfile.mkdtemp(prefix="nsx_bench_")
     mf = make_workload(tmp)
     hams = [Hamiltonian(*mf.args(J2)) for J2 in mf.J2s]
     batch = Batch(hams)
     tuned = batch.autotune(iters=10)   # staged-kernel shape, chosen before timing
+    log(f"autotuned: {tuned}")
     flops, flops_like, bytes_alg = batch.work()
     v = np.random.default_rng(1).standard_normal(batch.N)
     din = DeviceBuffer.from_array(v)
@@ -100,14 +207,17 @@ def main():
     el = time.perf_counter() - t0
     ms_step = 1e3 * el / args.steps
     value = len(hams) * args.steps / el
+    log(f"timed {args.steps} batched H*v: {value:.0f} matvecs/s")
 
     # HIP events on the batch stream around every launch of the timed loop
     ms_hv, ms_pn, ms_red = batch.time(din, dout, warmup=10, iters=min(args.steps, 1000))
     flops_pn = flops - flops_like
     achieved = flops_pn / (ms_pn * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": "k_pn_partial", "kernel_ms": ms_pn, "flops_per_launch": flops_pn}
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(),
+                "kernel": "k_pn_staged (pn two-body part, FP64)", "kernel_ms": ms_pn,
+                "flops_per_launch": flops_pn,
+                "hbm_GBs_alg": (bytes_alg / (ms_hv * 1e-3)) / 1e9}
 
     out = {
         "metric": "time-to-5-eigenvalues @1e-4 MeV & matvecs/s: mixed+warm vs plain FP64",
@@ -133,9 +243,21 @@ def main():
         out["device"] = list(device_info())
     except Exception:
         pass
+    if args.copy_mb:
+        nb = args.copy_mb << 20
+        ms = stream_copy(nb, iters=5)
+        out["copy_kernel"] = {"bytes": nb, "ms": ms, "GBs": 2 * nb / (ms * 1e-3) / 1e9}
+    din.free()
+    dout.free()
+    batch.close()
+    if not args.no_t5:
+        out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mf, args.cpu_seconds)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        if "time_to_5" in out:
+            per = out["cpu_baseline"]["value"]
+            out["cpu_baseline"]["time_to_5_est_s"] = out["time_to_5"]["plain_matvecs"] / per
     print(json.dumps(out))
 
 

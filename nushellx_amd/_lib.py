@@ -162,6 +162,18 @@ def device_sync() -> None:
     check(lib().nsx_dev_sync(), "nsx_dev_sync")
 
 
+def stream_copy(nbytes: int, iters: int = 5) -> float:
+    """Average ms of the library's streaming copy kernel over nbytes (both buffers on device)."""
+    a, b = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+    ms = C.c_double()
+    try:
+        check(lib().nsx_dev_copy(b.ptr, a.ptr, nbytes, iters, C.byref(ms)), "nsx_dev_copy")
+    finally:
+        a.free()
+        b.free()
+    return ms.value
+
+
 def device_info():
     name = C.create_string_buffer(128)
     cus = C.c_int32()
