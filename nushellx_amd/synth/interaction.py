@@ -56,6 +56,22 @@ class Interaction:
     spe: np.ndarray                      # per orbit (MeV)
     tbme: Dict[Key, float] = field(default_factory=dict)
 
+    def v_key(self, a: int, b: int, c: int, d: int, J2: int, T2: int):
+        """(canonical key, phase) with <ab;JT|V|cd;JT> = phase * tbme[key]."""
+        o = self.orbits
+        ph = 1.0
+        if a > b:
+            if ((o[a].j2 + o[b].j2 - J2 - T2) // 2) % 2:
+                ph = -ph
+            a, b = b, a
+        if c > d:
+            if ((o[c].j2 + o[d].j2 - J2 - T2) // 2) % 2:
+                ph = -ph
+            c, d = d, c
+        if (a, b) > (c, d):
+            a, b, c, d = c, d, a, b
+        return (a, b, c, d, J2, T2), ph
+
     def v(self, a: int, b: int, c: int, d: int, J2: int, T2: int) -> float:
         """<ab;JT|V|cd;JT> (normalised, antisymmetrised) for any orbit order."""
         o = self.orbits

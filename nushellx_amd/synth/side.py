@@ -369,6 +369,78 @@ class SideBasis:
                     np.add.at(H, (dst, src[free]), sg * V[x, y])
         return H
 
+    def hamiltonian_m_stack(self, M2: int, inter: Interaction) -> np.ndarray:
+        """d H / d theta_i in the M slice for every parameter of
+        ``Interaction.parameter_keys`` (dense, n_params x n x n).  The same
+        second-quantised construction as ``hamiltonian_m``, with each pair
+        matrix element kept as a vector over the parameters, so one pass over
+        the determinants serves every parameter."""
+        keys = inter.parameter_keys()
+        pos = {k: i for i, k in enumerate(keys)}
+        S = self.slice(M2)
+        n = len(S.masks)
+        H = np.zeros((len(keys), n, n))
+        if n == 0:
+            return H
+        for i in range(self.nms):
+            occ = ((S.masks >> np.uint64(i)) & np.uint64(1)).astype(float)
+            H[pos[("spe", int(self.ms_shell[i]))], np.arange(n), np.arange(n)] += occ
+        pairs = [(a, b) for a in range(self.nms) for b in range(a + 1, self.nms)]
+        by_m: Dict[int, List[Tuple[int, int]]] = {}
+        for (a, b) in pairs:
+            by_m.setdefault(int(self.ms_m2[a] + self.ms_m2[b]), []).append((a, b))
+        o = inter.orbits
+        jmax2 = 2 * max(x.j2 for x in o)
+        for Mp, plist in by_m.items():
+            npair = len(plist)
+            V = np.zeros((npair, npair, len(keys)))
+            for x, (a, b) in enumerate(plist):
+                oa, ob = self.ms_shell[a], self.ms_shell[b]
+                for y, (c, d) in enumerate(plist):
+                    oc, od = self.ms_shell[c], self.ms_shell[d]
+                    for J2 in range(0, jmax2 + 1, 2):
+                        c1 = cg(o[oa].j2, o[ob].j2, J2, int(self.ms_m2[a]), int(self.ms_m2[b]), Mp)
+                        if c1 == 0.0:
+                            continue
+                        c2 = cg(o[oc].j2, o[od].j2, J2, int(self.ms_m2[c]), int(self.ms_m2[d]), Mp)
+                        if c2 == 0.0:
+                            continue
+                        key, ph = inter.v_key(int(oa), int(ob), int(oc), int(od), J2, 2)
+                        if ("v",) + key not in pos:
+                            continue
+                        nab = np.sqrt(2.0) if oa == ob else 1.0
+                        ncd = np.sqrt(2.0) if oc == od else 1.0
+                        V[x, y, pos[("v",) + key]] += c1 * c2 * nab * ncd * ph
+            for y, (c, d) in enumerate(plist):
+                bc, bd = np.uint64(1 << c), np.uint64(1 << d)
+                has = ((S.masks & bc) != 0) & ((S.masks & bd) != 0)
+                src = np.nonzero(has)[0]
+                if len(src) == 0:
+                    continue
+                m0 = S.masks[src]
+                s1 = _popcount(m0 & _below(c))
+                m1 = m0 ^ bc
+                s2 = _popcount(m1 & _below(d))
+                m2 = m1 ^ bd
+                for x, (a, b) in enumerate(plist):
+                    nz = np.nonzero(V[x, y])[0]
+                    if len(nz) == 0:
+                        continue
+                    ba, bb = np.uint64(1 << a), np.uint64(1 << b)
+                    free = ((m2 & ba) == 0) & ((m2 & bb) == 0)
+                    if not free.any():
+                        continue
+                    mm = m2[free]
+                    s3 = _popcount(mm & _below(b))
+                    m3 = mm | bb
+                    s4 = _popcount(m3 & _below(a))
+                    m4 = m3 | ba
+                    sg = np.where(((s1[free] + s2[free] + s3 + s4) % 2) == 1, -1.0, 1.0)
+                    dst = np.array([S.index[int(v)] for v in m4])
+                    # (dst, src) pairs are distinct within one (x, y): plain indexed add
+                    H[nz[:, None], dst[None, :], src[free][None, :]] += V[x, y, nz][:, None] * sg[None, :]
+        return H
+
     def hamiltonian_basis(self, inter: Interaction):
         """Per 2J: (segment partition pairs, stack of per-parameter segment matrices)
         so that H(theta) segments = sum_i theta_i M_i (parameters of
@@ -381,14 +453,11 @@ class SideBasis:
             pairs = [(p, q) for p in range(self.no_spart) if ph[p].shape[1]
                      for q in range(p, self.no_spart)
                      if ph[q].shape[1] and self.part_parity[p] == self.part_parity[q]]
-            mats = [np.zeros((len(keys), ph[q].shape[1], ph[p].shape[1])) for (p, q) in pairs]
-            for i, k in enumerate(keys):
-                if k[0] == "v" and k[-1] == 0:
-                    continue
-                Hm = self.hamiltonian_m(J2, inter.unit(i))
-                for n, (p, q) in enumerate(pairs):
-                    blk = Hm[S.start[q]:S.start[q] + S.count[q], S.start[p]:S.start[p] + S.count[p]]
-                    mats[n][i] = ph[q].T @ blk @ ph[p]
+            Hs = self.hamiltonian_m_stack(J2, inter)
+            mats = []
+            for (p, q) in pairs:
+                blk = Hs[:, S.start[q]:S.start[q] + S.count[q], S.start[p]:S.start[p] + S.count[p]]
+                mats.append(ph[q].T[None, :, :] @ blk @ ph[p][None, :, :])
             keep = [n for n, (p, q) in enumerate(pairs) if p == q or np.abs(mats[n]).max() > 1e-13]
             out[J2] = ([(pairs[n][0] + 1, pairs[n][1] + 1) for n in keep], [mats[n] for n in keep])
         return out
