@@ -1,0 +1,57 @@
+"""pf shell, 4 protons + 4 neutrons, J = 0 (N = 41,355): block sides up to 99,
+so the direct (non-staged) pn kernel runs beside the staged one.  H*v against
+the FP64 oracle, and the device Lanczos against ARPACK (scipy eigsh) driven by
+the oracle's H*v."""
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle.refcpu import RefCPU
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pf44():
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import PF, PNSpace
+    from nushellx_amd.synth.writer import PNModel, write_model
+    print("generating pf 4p4n J=0 operator files", flush=True)
+    inter = random_interaction(PF, seed=2024, spe_range=(-10.0, -4.0))
+    return write_model(PNModel(PNSpace(PF), 4, 4, inter), tempfile.mkdtemp(prefix="nsx_pf44_"), "pf44", [0])
+
+
+def test_pf44_hv_matches_oracle(pf44):
+    from nushellx_amd.hv import Hamiltonian
+    args = pf44.args(0)
+    ref = RefCPU(*args, fp64=True)
+    assert ref.N == 41355
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(ref.N)
+    w_ref = ref.apply(v)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)):
+        print(f"case pf44 J=0 {prec}", flush=True)
+        h = Hamiltonian(*args, precision=prec)
+        assert h.N == ref.N
+        err = np.abs(h.multiply(v) - w_ref).max() / np.abs(w_ref).max()
+        print(f"{prec}: relative error {err:.2e}", flush=True)
+        assert err < tol, (prec, err)
+        h.close()
+
+
+def test_pf44_lanczos_matches_arpack(pf44):
+    from scipy.sparse.linalg import LinearOperator, eigsh
+
+    from nushellx_amd.hv import Hamiltonian
+    args = pf44.args(0)
+    ref = RefCPU(*args, fp64=True)
+    mask = ref.lcoef().astype(float)
+    # restrict to the parity-allowed states, as the solver's start vector does
+    op = LinearOperator((ref.N, ref.N), matvec=lambda x: mask * ref.apply(mask * np.ravel(x)), dtype=np.float64)
+    v0 = mask * np.random.default_rng(1).standard_normal(ref.N)
+    ev_ref = np.sort(eigsh(op, k=5, which="SA", tol=1e-11, v0=v0)[0])
+    h = Hamiltonian(*args)
+    r = h.lanczos(nev=5, tol=1e-7)
+    print("device", r["evals"], "arpack", ev_ref, r["matvecs"], flush=True)
+    assert np.abs(r["evals"] - ev_ref).max() < 1e-6
