@@ -163,6 +163,46 @@ def time_to_5(log, train_samples: int = 256, epochs: int = 200, repeats: int = 3
     return res
 
 
+def ensemble(log, samples: int = 256):
+    """Batched ensemble: `samples` perturbed interactions (V_0 + N(0, (0.1 |V_0|)^2))
+    x J = 0, 2, 4 x sd 4p4n, 5p5n, 6p6n, all instances in one lockstep batch,
+    lowest 5 states each to residual 1e-4 MeV; aggregate eigenpairs/s."""
+    from nushellx_amd.ensemble import build_instances, solve_ensemble, write_ensemble
+    from nushellx_amd.hv import DeviceBuffer
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import SD, PNSpace
+    J2s, nuclei = [0, 4, 8], [(4, 4), (5, 5), (6, 6)]
+    t0 = time.perf_counter()
+    entries = write_ensemble(PNSpace(SD), nuclei, J2s, random_interaction(SD, seed=SEED),
+                             range(1000, 1000 + samples), tempfile.mkdtemp(prefix="nsx_ens_"), log=log)
+    t_files = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hams = build_instances(entries, J2s, threads=16)
+    t_inst = time.perf_counter() - t0
+    tune_on = build_instances(entries[:len(nuclei)], J2s)
+    out = {"workload": f"{samples} samples x J=0,2,4 x sd 4p4n/5p5n/6p6n, lowest 5 each, residual 1e-4 MeV",
+           "instances": len(hams), "files_s": t_files, "instances_s": t_inst}
+    for prec in ("fp64", "fp32"):
+        r = solve_ensemble(hams, nev=5, tol=1e-4, precision=prec, tune_on=tune_on, log=log)
+        b = r["batch"]
+        flops = b.work()[0]
+        din, dout = DeviceBuffer(8 * b.N), DeviceBuffer(8 * b.N)
+        din.upload(np.random.default_rng(0).standard_normal(b.N))
+        ms_hv, ms_pn, _ = b.time(din, dout, warmup=3, iters=20)
+        din.free()
+        dout.free()
+        out["N_total"] = int(b.N)
+        out[prec] = {"eigenpairs_per_s": r["eigenpairs_per_s"], "solve_s": r["t_solve"],
+                     "matvecs": r["matvecs"], "converged": int((r["converged_at"] >= 0).sum()),
+                     "plan_s": r["t_plan"], "tune_s": r["t_tune"], "host_s": r["host_seconds"],
+                     "hv_ms": ms_hv, "hv_TFLOPs": flops / (ms_hv * 1e-3) / 1e12, "tuned": r["tuned"]}
+        log(f"ensemble {prec}: {r['eigenpairs_per_s']:.0f} eigenpairs/s")
+        b.close()
+    for h in hams + tune_on:
+        h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +213,8 @@ def main():
     ap.add_argument("--no-t5", action="store_true", help="skip the time-to-5-eigenvalues comparison")
     ap.add_argument("--train-samples", type=int, default=256)
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the streaming copy kernel (MiB)")
+    ap.add_argument("--ensemble-samples", type=int, default=256,
+                    help="interaction samples of the batched ensemble (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -252,6 +294,8 @@ def main():
     batch.close()
     if not args.no_t5:
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
+    if args.ensemble_samples > 0:
+        out["ensemble"] = ensemble(log, args.ensemble_samples)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mf, args.cpu_seconds)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -18,6 +18,7 @@ module nsx_hv_api
   integer(c_int), parameter, public :: NSX_OK = 0, NSX_EIO = -1, NSX_EFORMAT = -2, NSX_ETAG = -3, &
                                        NSX_ECAP = -4, NSX_EGPU = -5, NSX_EARG = -6
   integer(c_int), parameter, public :: NSX_FP64 = 0, NSX_FP32 = 1, NSX_BF16X3 = 2
+  integer(c_int), parameter, public :: NSX_HOST_ONLY = 256   ! or-ed into the precision
 
   ! C: nsx_trl_opts (seed is uint64_t in C; any 64-bit pattern is accepted)
   type, bind(C) :: nsx_trl_opts
@@ -29,7 +30,7 @@ module nsx_hv_api
 
   type, bind(C) :: nsx_trl_stats
     integer(c_int32_t) :: matvecs, restarts, converged
-    real(c_double) :: seconds
+    real(c_double) :: seconds, host_seconds
   end type nsx_trl_stats
 
   interface

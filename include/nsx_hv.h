@@ -30,6 +30,12 @@ enum {
   NSX_BF16X3 = 2  /* BF16 hi/lo split operands (3 products), FP32 accumulation */
 };
 
+/* Flag or-ed into the precision of nsx_hv_create: build the instance on the
+ * host only (no device plan, no device memory).  Such a handle can only be
+ * used through nsx_batch_create; its own apply/solve calls fail with NSX_EARG.
+ * For ensembles of many instances, whose operators are planned once per batch. */
+#define NSX_HOST_ONLY 0x100
+
 typedef struct nsx_hv nsx_hv;
 
 /* Reads the files program NuTRL reads for one nucleus and total J and uploads
@@ -93,6 +99,10 @@ typedef struct {
 int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);
 int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);
 int nsx_batch_set_config(nsx_batch *b, const nsx_tune_result *cfg);
+/* nsx_batch_create with a plan shape and kernel choice known in advance (for
+ * instance tuned on a small representative batch): plans the batch once. */
+int nsx_batch_create_config(nsx_hv *const *hs, int32_t n, int precision, const nsx_tune_result *cfg,
+                            nsx_batch **b);
 
 /* ---------------------------------------------------------------- Lanczos
  * Thick-restart Lanczos for the lowest eigenpairs (replaces TRL.Lnz_OP,
@@ -112,6 +122,7 @@ typedef struct {
 typedef struct {
   int32_t matvecs, restarts, converged;
   double seconds;     /* wall time of the solve */
+  double host_seconds; /* of which host work on the projected problems */
 } nsx_trl_stats;
 
 /* start: optional host start vector (N doubles, NULL = seeded random on lcoef).

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnsx.so")
 
 NSX_FP64, NSX_FP32, NSX_BF16X3 = 0, 1, 2
+NSX_HOST_ONLY = 0x100
 PRECISIONS = {"fp64": NSX_FP64, "fp32": NSX_FP32, "bf16x3": NSX_BF16X3}
 
 # every symbol include/nsx_hv.h declares
@@ -21,7 +22,7 @@ EXPORTS = [
     "nsx_batch_create", "nsx_batch_destroy", "nsx_batch_dims", "nsx_batch_offset",
     "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
-    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_trl_solve", "nsx_dev_copy",
+    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_trl_solve", "nsx_dev_copy", "nsx_batch_create_config",
     "nsx_mlp_create", "nsx_mlp_destroy", "nsx_mlp_num_params", "nsx_mlp_get_params",
     "nsx_mlp_set_params", "nsx_mlp_grad", "nsx_mlp_train", "nsx_mlp_predict",
     "nsx_last_error",
@@ -44,7 +45,7 @@ class TuneResult(C.Structure):
 
 class TrlStats(C.Structure):
     _fields_ = [("matvecs", C.c_int32), ("restarts", C.c_int32), ("converged", C.c_int32),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("host_seconds", C.c_double)]
 
 
 _LIB = None
@@ -88,6 +89,7 @@ def lib() -> C.CDLL:
     L.nsx_hv_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_set_config.argtypes = [vp, C.POINTER(TuneResult)]
+    L.nsx_batch_create_config.argtypes = [C.POINTER(vp), i32, C.c_int, C.POINTER(TuneResult), C.POINTER(vp)]
     L.nsx_dev_copy.argtypes = [vp, vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.nsx_batch_trl_solve.argtypes = [vp, C.POINTER(TrlOpts), vp, vp, vp, vp, vp, C.POINTER(TrlStats)]
     L.nsx_mlp_create.argtypes = [i32, i32, i32, i32, C.c_uint64, C.POINTER(vp)]

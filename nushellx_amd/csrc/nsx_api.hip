@@ -111,12 +111,19 @@ int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfi
                   const char* intfile, int precision, nsx_hv** out) {
   if (!out) { set_error("null output handle"); return NSX_EARG; }
   *out = nullptr;
+  const bool host_only = (precision & NSX_HOST_ONLY) != 0;
+  precision &= ~NSX_HOST_ONLY;
   if (precision < NSX_FP64 || precision > NSX_BF16X3) { set_error("unknown precision"); return NSX_EARG; }
   std::unique_ptr<nsx_hv> h(new nsx_hv());
   h->precision = precision;
+  h->host_only = host_only;
   std::string e = build_model(trimmed(nus_prefix), trimmed(opfile_a), trimmed(opfile_b),
                               trimmed(intfile), h->model);
   if (!e.empty()) { set_error(e); return code_of(e); }
+  if (host_only) {  // operators stay on the host until a batch plans them
+    *out = h.release();
+    return NSX_OK;
+  }
   plan_add(h->hplan, h->model);
   if (precision != NSX_FP64) {
     e = build_lowp(h->hplan, precision, h->lowp);
@@ -167,6 +174,12 @@ int nsx_hv_lcoef(const nsx_hv* h, uint8_t* mask) {
 
 int nsx_hv_work(const nsx_hv* h, double* flops, double* flops_like, int64_t* bytes) {
   if (!h) { set_error("null handle"); return NSX_EARG; }
+  if (h->host_only) {  // no plan: work from the model, bytes unknown until planned
+    if (flops) *flops = h->model.flops_ref;
+    if (flops_like) *flops_like = h->model.flops_like;
+    if (bytes) *bytes = 0;
+    return NSX_OK;
+  }
   if (flops) *flops = h->hplan.flops_ref;
   if (flops_like) *flops_like = h->hplan.flops_like;
   if (bytes) *bytes = h->hplan.bytes_alg;
@@ -175,6 +188,7 @@ int nsx_hv_work(const nsx_hv* h, double* flops, double* flops_like, int64_t* byt
 
 int nsx_hv_apply_dev(nsx_hv* h, const double* d_in, double* d_out, void* stream) {
   if (!h || (!d_in && h->model.N) || (!d_out && h->model.N)) { set_error("null argument"); return NSX_EARG; }
+  if (h->host_only) { set_error("host-only handle: apply it through a batch"); return NSX_EARG; }
   if (d_in == d_out && h->model.N) { set_error("in-place H*v is not supported"); return NSX_EARG; }
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
   hipError_t e = launch_hv(h->dplan, h->lowp, h->precision, d_in, d_out, s);
@@ -184,6 +198,7 @@ int nsx_hv_apply_dev(nsx_hv* h, const double* d_in, double* d_out, void* stream)
 
 int nsx_hv_apply_f64(nsx_hv* h, const double* v_in, double* v_out) {
   if (!h || (h->model.N && (!v_in || !v_out))) { set_error("null argument"); return NSX_EARG; }
+  if (h->host_only) { set_error("host-only handle: apply it through a batch"); return NSX_EARG; }
   const size_t bytes = sizeof(double) * h->model.N;
   if (!bytes) return NSX_OK;
   NSX_HIP(hipMemcpyAsync(h->d_in, v_in, bytes, hipMemcpyHostToDevice, h->stream));
@@ -204,10 +219,20 @@ int nsx_hv_apply_f32(nsx_hv* h, const float* v_in, float* v_out) {
 }
 
 int nsx_batch_create(nsx_hv* const* hs, int32_t n, int precision, nsx_batch** out) {
+  return nsx_batch_create_config(hs, n, precision, nullptr, out);
+}
+
+int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const nsx_tune_result* cfg,
+                            nsx_batch** out) {
   if (!out || !hs || n <= 0) { set_error("bad batch arguments"); return NSX_EARG; }
   *out = nullptr;
   std::unique_ptr<nsx_batch> b(new nsx_batch());
+  if (precision < NSX_FP64 || precision > NSX_BF16X3) { set_error("unknown precision"); return NSX_EARG; }
   b->precision = precision;
+  if (cfg) {
+    b->hplan.cfg.fat_cap = std::min(std::max(cfg->fat_cap, 256), kFatCap);
+    b->hplan.cfg.max_phases = std::max(cfg->max_phases, 1);
+  }
   for (int i = 0; i < n; ++i) {
     if (!hs[i]) { set_error("null handle in batch"); return NSX_EARG; }
     b->models.push_back(&hs[i]->model);
@@ -219,6 +244,11 @@ int nsx_batch_create(nsx_hv* const* hs, int32_t n, int precision, nsx_batch** ou
     return NSX_EGPU;
   }
   if (!(e = upload_plan(b->hplan, b->dplan)).empty()) { set_error(e); return NSX_EGPU; }
+  if (cfg) {
+    b->dplan.staged_nt = cfg->threads;
+    b->dplan.staged_mode = cfg->mode;
+    b->dplan.atomic = cfg->atomic;
+  }
   NSX_HIP(hipStreamCreate(&b->stream));
   *out = b.release();
   return NSX_OK;

@@ -15,6 +15,7 @@ struct nsx_hv {
   nsx::DevicePlan dplan;
   nsx::LowpPlan lowp;
   int precision = 0;
+  bool host_only = false;  // no device plan: only usable inside a batch
   hipStream_t stream = nullptr;
   double* d_in = nullptr;
   double* d_out = nullptr;

@@ -14,11 +14,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nsx_hv.h"
@@ -111,6 +113,24 @@ __global__ __launch_bounds__(kVT) void k_seg_ritz(double* __restrict__ Y, const 
     for (int j = 0; j < mm; ++j) s = fma(Q[r + ldq * j], Si[j + (int64_t)mm * c], s);
     Y[r + ldq * c] = s;
   }
+}
+
+// f(i) for i in [0, n) on up to 16 host threads (dynamic assignment)
+template <typename F>
+static void parallel_for(int n, F&& f) {
+  int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  nt = std::min(nt, std::max(1, n / 8));
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&]() {
+      for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+    });
+  for (auto& th : pool) th.join();
 }
 
 struct DevBuf {
@@ -274,7 +294,9 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   for (int i = 0; i < B; ++i) converged_at[i] = -1;
   int matvecs = 0, restarts = 0, cycle_start = 0;
   const int kk = std::min(keep, m - 1);
-  std::vector<double> ha((m + 1) * B), hb((m + 1) * B), lastc(m + 1);
+  std::vector<double> ha((m + 1) * B), hb((m + 1) * B);
+  std::vector<std::vector<double>> LC(B, std::vector<double>(m + 1, 0.0));  // last eigenvector rows
+  double host_s = 0.0;  // wall time of the host projected-problem work
 
   // Rayleigh-Ritz of instance i on its current mm x mm matrix: eigenvalues into
   // Th[i], eigenvectors into Sv[i] when full, else only their last components.
@@ -284,10 +306,10 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     for (int a = 0; a < mm; ++a)
       for (int b = 0; b < mm; ++b) Tm[size_t(a) * mm + b] = T[i][size_t(a) * (m + 1) + b];
     mm_of[i] = mm;
-    if (!full) return sym_eig(Tm.data(), mm, false, Th[i].data(), lastc.data());
+    if (!full) return sym_eig(Tm.data(), mm, false, Th[i].data(), LC[i].data());
     Sv[i].assign(size_t(mm) * mm, 0.0);
     if (!sym_eig(Tm.data(), mm, true, Th[i].data(), Sv[i].data())) return false;
-    for (int c = 0; c < mm; ++c) lastc[c] = Sv[i][size_t(c) * mm + mm - 1];
+    for (int c = 0; c < mm; ++c) LC[i][c] = Sv[i][size_t(c) * mm + mm - 1];
     return true;
   };
   // record the wanted pairs' values and residuals |beta_last s_{mm-1,q}| after solve()
@@ -295,7 +317,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     int conv = 0;
     for (int q = 0; q < nev; ++q) {
       theta_all[size_t(i) * nev + q] = Th[i][q];
-      const double r = std::fabs(beta_last * lastc[q]);
+      const double r = std::fabs(beta_last * LC[i][q]);
       res_all[size_t(i) * nev + q] = r;
       conv += r <= tol;
     }
@@ -320,9 +342,10 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       TRL_HIP(hipStreamSynchronize(s));
       for (int x = j0 * B; x < j1 * B; ++x)
         if (!std::isfinite(ha[x]) || !std::isfinite(hb[x])) { set_error("non-finite Lanczos coefficient"); return NSX_EGPU; }
-      all_conv = true;
-      for (int i = 0; i < B; ++i) {
-        if (done[i]) continue;
+      std::atomic<int> failed(0);
+      const auto th0 = std::chrono::steady_clock::now();
+      parallel_for(B, [&](int i) {
+        if (done[i]) return;
         auto t = [&](int a, int b) -> double& { return T[i][size_t(a) * (m + 1) + b]; };
         const bool broke = hdim[i] > 0;
         const int last = broke ? hdim[i] : j1;
@@ -333,36 +356,41 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
           if (j > cycle_start) t(j - 1, j) = t(j, j - 1) = hb[(size_t)(j - 1) * B + i];
         }
         const double beta_last = broke ? 0.0 : hb[(size_t)(last - 1) * B + i];
-        if (!solve(i, last, broke)) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
+        if (!solve(i, last, broke)) { failed = 1; return; }
         record(i, beta_last);
         if (broke) {  // invariant subspace: eigenpairs exact, Ritz vectors from Sv
           done[i] = 1;
           if (converged_at[i] < 0) converged_at[i] = matvecs;
         }
-        all_conv = all_conv && converged_at[i] >= 0;
-      }
+      });
+      host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
+      if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
+      all_conv = true;
+      for (int i = 0; i < B; ++i) all_conv = all_conv && converged_at[i] >= 0;
       if (all_conv || matvecs >= max_iter) break;
     }
     if (all_conv || matvecs >= max_iter) { finished = true; break; }
 
     // ---- thick restart of every instance: Q[:, 0:kk] = Q S, Q[:, kk] = q_m
     std::vector<double> Sdev(size_t(B) * m * kk, 0.0);
-    for (int i = 0; i < B; ++i) {
+    std::atomic<int> failed(0);
+    const auto th0 = std::chrono::steady_clock::now();
+    parallel_for(B, [&](int i) {
       double* Si = &Sdev[(size_t)i * m * kk];
       if (placed[i]) {  // frozen instance whose vectors are already Q[:, 0:kk]
         for (int c = 0; c < kk; ++c) Si[(size_t)c * m + c] = 1.0;
-        continue;
+        return;
       }
       if (done[i]) {  // vectors from the solve at the breakdown; frozen from now on
         const int mm = mm_of[i];
         for (int c = 0; c < std::min(kk, mm); ++c)
           for (int j = 0; j < mm; ++j) Si[(size_t)c * m + j] = Sv[i][size_t(c) * mm + j];
         placed[i] = 1;
-        continue;
+        return;
       }
       auto t = [&](int a, int b) -> double& { return T[i][size_t(a) * (m + 1) + b]; };
       const double beta_last = hb[(size_t)(m - 1) * B + i];
-      if (!solve(i, m, true)) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
+      if (!solve(i, m, true)) { failed = 1; return; }
       for (int c = 0; c < kk; ++c)
         for (int j = 0; j < m; ++j) Si[(size_t)c * m + j] = Sv[i][size_t(c) * m + j];
       std::fill(T[i].begin(), T[i].end(), 0.0);
@@ -372,7 +400,9 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
         t(c, kk) = b;
         t(kk, c) = b;
       }
-    }
+    });
+    host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
+    if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
     TRL_HIP(hipMemcpyAsync(S.p, Sdev.data(), sizeof(double) * Sdev.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_seg_ritz, gv, dim3(kVT), 0, s, Y.as<double>(), Q.as<double>(), ldq, S.as<double>(), m,
                        kk, inst_of.as<int>(), N);
@@ -393,20 +423,19 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     int mmax = nev;
     for (int i = 0; i < B; ++i) mmax = std::max(mmax, mm_of[i]);
     std::vector<double> Sk(size_t(B) * mmax * nev, 0.0);
-    for (int i = 0; i < B; ++i) {
+    std::atomic<int> failed(0);
+    parallel_for(B, [&](int i) {
       double* Si = &Sk[(size_t)i * mmax * nev];
       if (placed[i]) {
         for (int c = 0; c < nev; ++c) Si[(size_t)c * mmax + c] = 1.0;
-        continue;
+        return;
       }
       const int mm = mm_of[i];
-      if (!done[i] && !solve(i, mm, true)) {  // the last test formed only the last row
-        set_error("projected eigenproblem did not converge");
-        return NSX_EGPU;
-      }
+      if (!done[i] && !solve(i, mm, true)) { failed = 1; return; }  // the last test formed only the last row
       for (int c = 0; c < nev; ++c)
         for (int j = 0; j < mm; ++j) Si[(size_t)c * mmax + j] = Sv[i][size_t(c) * mm + j];
-    }
+    });
+    if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
     TRL_HIP(hipMemcpyAsync(S.p, Sk.data(), sizeof(double) * Sk.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_seg_ritz, gv, dim3(kVT), 0, s, Y.as<double>(), Q.as<double>(), ldq, S.as<double>(), mmax,
                        nev, inst_of.as<int>(), N);
@@ -421,6 +450,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     stats->restarts = restarts;
     stats->converged = conv;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats->host_seconds = host_s;
   }
 #undef TRL_HIP
   return NSX_OK;
@@ -433,6 +463,7 @@ using namespace nsx;
 extern "C" int nsx_trl_solve(nsx_hv* h, const nsx_trl_opts* opt, const double* start, double* evals,
                              double* evecs, double* residuals, nsx_trl_stats* stats) {
   if (!h || !opt || !evals) { set_error("null argument"); return NSX_EARG; }
+  if (h->host_only) { set_error("host-only handle: solve it through a batch"); return NSX_EARG; }
   int64_t nst = 0;
   for (uint8_t b : h->model.lcoef) nst += b;
   std::vector<Instance> inst{{0, h->model.N, &h->model.lcoef, nst}};

@@ -14,18 +14,22 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import (NSX_FP64, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, TuneResult, check,
+from ._lib import (NSX_FP64, NSX_HOST_ONLY, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, TuneResult, check,
                    device_sync, lib)
 
 
 class Hamiltonian:
     def __init__(self, nus_prefix: str, opfile_a: str, opfile_b: str, intfile: str,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", device: bool = True):
+        """``device=False`` builds a host-only instance (no device memory), usable
+        only as a member of a Batch."""
         L = lib()
         self.precision = precision
+        self.device = device
         h = C.c_void_p()
+        flags = PRECISIONS[precision] | (0 if device else NSX_HOST_ONLY)
         check(L.nsx_hv_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
-                              intfile.encode(), PRECISIONS[precision], C.byref(h)), "nsx_hv_create")
+                              intfile.encode(), flags, C.byref(h)), "nsx_hv_create")
         self._h = h
         n, b = C.c_int64(), C.c_int32()
         check(L.nsx_hv_dims(h, C.byref(n), C.byref(b)), "nsx_hv_dims")
@@ -127,7 +131,8 @@ class Hamiltonian:
                                   vec.ctypes.data if vectors else None, res.ctypes.data, C.byref(st)),
               "nsx_trl_solve")
         out = {"evals": ev, "residuals": res, "matvecs": st.matvecs, "restarts": st.restarts,
-               "converged": st.converged, "seconds": st.seconds}
+               "converged": st.converged, "seconds": st.seconds,
+               "host_seconds": st.host_seconds}
         if vectors:
             out["evecs"] = vec
         return out
@@ -136,10 +141,17 @@ class Hamiltonian:
 class Batch:
     """Many instances in one launch pair (vectors concatenated in instance order)."""
 
-    def __init__(self, hams: Sequence[Hamiltonian], precision: str = "fp64"):
+    def __init__(self, hams: Sequence[Hamiltonian], precision: str = "fp64", config: Optional[dict] = None):
+        """``config``: a result of ``autotune`` (plan shape and kernel choice) to
+        plan with from the start instead of the defaults."""
         arr = (C.c_void_p * len(hams))(*[h.handle for h in hams])
         b = C.c_void_p()
-        check(lib().nsx_batch_create(arr, len(hams), PRECISIONS[precision], C.byref(b)), "nsx_batch_create")
+        if config is None:
+            check(lib().nsx_batch_create(arr, len(hams), PRECISIONS[precision], C.byref(b)), "nsx_batch_create")
+        else:
+            cfg = TuneResult(**{k: config[k] for k, _ in TuneResult._fields_})
+            check(lib().nsx_batch_create_config(arr, len(hams), PRECISIONS[precision], C.byref(cfg), C.byref(b)),
+                  "nsx_batch_create_config")
         self._b = b
         self._hams = list(hams)
         n = C.c_int64()
@@ -196,7 +208,8 @@ class Batch:
                                         vec.ctypes.data if vectors else None, res.ctypes.data,
                                         conv.ctypes.data, C.byref(st)), "nsx_batch_trl_solve")
         out = {"evals": ev, "residuals": res, "converged_at": conv, "matvecs": st.matvecs,
-               "restarts": st.restarts, "converged": st.converged, "seconds": st.seconds}
+               "restarts": st.restarts, "converged": st.converged, "seconds": st.seconds,
+               "host_seconds": st.host_seconds}
         if vectors:
             out["evecs"] = vec
         return out
