@@ -130,6 +130,18 @@ typedef struct {
 int nsx_trl_solve(nsx_hv *h, const nsx_trl_opts *opt, const double *start, double *evals,
                   double *evecs, double *residuals, nsx_trl_stats *stats);
 
+/* Lanczos-vector files for NuVec and the other downstream programs: writes
+ * <prefix>.lnz (direct access, one REAL*4 record of N values per vector) and
+ * <prefix>.abz (sequential: isave, alpha, beta / imax, no_level, tt, no_state,
+ * basd / bas), the layout Lnzs writes (src/NuTRL/NuCoreX.f90:738-743) and NuVec
+ * reads (src/NuVec/NuVec.f90:113-131).  The nvec Ritz vectors (vecs: N x nvec,
+ * column-major) are the records, alpha their Ritz values and beta zero, so
+ * NuVec's tridiagonal step is diagonal and it rebuilds exactly these states.
+ * prefix is NuVec's Nucleus // file_code(J, nA, nB) (with its directory).
+ * Works on host-only handles too. */
+int nsx_hv_write_vectors(const nsx_hv *h, const char *prefix, int32_t nvec, const double *evals,
+                         const double *vecs, int32_t no_level, float tt);
+
 /* ---------------------------------------------------------------- device helpers
  * For hosts without their own HIP binding (Fortran, ctypes): device buffers,
  * copies on the default stream, and HIP-event timing of the H*v kernels.

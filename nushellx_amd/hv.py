@@ -75,6 +75,14 @@ class Hamiltonian:
         check(lib().nsx_hv_lcoef(self._h, m.ctypes.data), "nsx_hv_lcoef")
         return m[:self.N].astype(bool)
 
+    def write_vectors(self, prefix: str, evals, vecs, no_level: int = 5, tt: float = 0.0) -> None:
+        """<prefix>.lnz / <prefix>.abz in the layout NuVec reads (vecs: nvec x N)."""
+        vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+        evals = np.ascontiguousarray(evals, dtype=np.float64)
+        assert vecs.shape == (len(evals), self.N)
+        check(lib().nsx_hv_write_vectors(self._h, prefix.encode(), len(evals), evals.ctypes.data,
+                                         vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_vectors")
+
     def work(self) -> Tuple[float, float, int]:
         """(flops, like-particle flops, FP64 bytes) of one H*v (algorithmic counts)."""
         f, fl, b = C.c_double(), C.c_double(), C.c_int64()
@@ -163,6 +171,14 @@ class Batch:
             check(lib().nsx_batch_offset(b, i, C.byref(o)), "nsx_batch_offset")
             self.offsets.append(int(o.value))
 
+    def write_vectors(self, prefix: str, evals, vecs, no_level: int = 5, tt: float = 0.0) -> None:
+        """<prefix>.lnz / <prefix>.abz in the layout NuVec reads (vecs: nvec x N)."""
+        vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+        evals = np.ascontiguousarray(evals, dtype=np.float64)
+        assert vecs.shape == (len(evals), self.N)
+        check(lib().nsx_hv_write_vectors(self._h, prefix.encode(), len(evals), evals.ctypes.data,
+                                         vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_vectors")
+
     def work(self) -> Tuple[float, float, int]:
         f, fl, by = C.c_double(), C.c_double(), C.c_int64()
         check(lib().nsx_batch_work(self._b, C.byref(f), C.byref(fl), C.byref(by)), "nsx_batch_work")
@@ -213,6 +229,12 @@ class Batch:
         if vectors:
             out["evecs"] = vec
         return out
+
+    def write_vectors(self, prefixes: Sequence[str], result: dict, no_level: int = 5) -> None:
+        """Per instance, <prefix>.lnz/.abz (NuVec layout) from ``lanczos(vectors=True)``."""
+        for k, (h, prefix) in enumerate(zip(self._hams, prefixes)):
+            o = self.offsets[k]
+            h.write_vectors(prefix, result["evals"][k], result["evecs"][:, o:o + h.N], no_level=no_level)
 
     def time(self, d_in, d_out, warmup: int = 10, iters: int = 100):
         """HIP-event timing (ms per batched H*v, pn kernel, reduce kernel)."""

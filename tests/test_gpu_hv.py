@@ -154,3 +154,22 @@ def test_warm_start_converges_faster(models):
     warm = h.lanczos(nev=1, tol=1e-8, start=cold["evecs"][0])
     assert abs(warm["evals"][0] - cold["evals"][0]) < 1e-8
     assert warm["matvecs"] <= 8 < cold["matvecs"]
+
+
+def test_solver_vectors_through_nuvec_files(models, tmp_path):
+    """Device Ritz vectors written as .lnz/.abz: the states NuVec would form
+    from them are eigenvectors of the oracle's H to REAL*4 precision."""
+    from oracle.nuvec import nuvec_states
+    from nushellx_amd.hv import Hamiltonian
+    args = models["sd44"].args(0)
+    h = Hamiltonian(*args)
+    r = h.lanczos(nev=5, tol=1e-9, vectors=True)
+    prefix = str(tmp_path / "sd440")
+    h.write_vectors(prefix, r["evals"], r["evecs"], no_level=5)
+    E, X = nuvec_states(prefix)
+    ref = RefCPU(*args, fp64=True)
+    assert np.abs(E - r["evals"]).max() < 1e-4   # REAL*4 energies
+    for q in range(5):
+        x = X[q] / np.linalg.norm(X[q])
+        res = np.linalg.norm(ref.apply(x) - E[q] * x)
+        assert res < 1e-4 * max(1.0, abs(E[q])), (q, res)
