@@ -11,6 +11,7 @@
 
 #include "../../include/nsx_hv.h"
 #include "nsx_api_internal.hpp"
+#include "nsx_parallel.hpp"
 
 namespace nsx {
 
@@ -236,8 +237,8 @@ int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const n
   for (int i = 0; i < n; ++i) {
     if (!hs[i]) { set_error("null handle in batch"); return NSX_EARG; }
     b->models.push_back(&hs[i]->model);
-    plan_add(b->hplan, hs[i]->model);
   }
+  plan_models(b->models, b->hplan);
   std::string e;
   if (precision != NSX_FP64 && !(e = build_lowp(b->hplan, precision, b->lowp)).empty()) {
     set_error(e);
@@ -411,6 +412,22 @@ int nsx_device_info(char* name, int32_t len, int32_t* cus) {
 
 namespace nsx {
 
+// Plans every model on its own in parallel, then merges the plans in order
+// (same result as plan_add of each model into hp in turn).
+void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
+  const int n = (int)models.size();
+  if (n == 1) { plan_add(hp, *models[0]); return; }
+  std::vector<HostPlan> local(n);
+  parallel_for(n, [&](int i) {
+    local[i].cfg = hp.cfg;
+    plan_add(local[i], *models[i]);
+  }, 1);
+  for (int i = 0; i < n; ++i) {
+    plan_merge(hp, local[i]);
+    local[i] = HostPlan();  // release as we go
+  }
+}
+
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
                    HostPlan& hp, DevicePlan& d, LowpPlan& lp) {
   const int nt = d.staged_nt, mode = d.staged_mode, at = d.atomic;
@@ -418,7 +435,7 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
   free_lowp(lp);
   hp = HostPlan();
   hp.cfg = cfg;
-  for (const Model* m : models) plan_add(hp, *m);
+  plan_models(models, hp);
   std::string e;
   if (precision != NSX_FP64 && !(e = build_lowp(hp, precision, lp)).empty()) return e;
   if (!(e = upload_plan(hp, d)).empty()) return e;

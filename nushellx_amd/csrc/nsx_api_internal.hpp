@@ -32,6 +32,7 @@ struct nsx_batch {
 
 namespace nsx {
 void set_error(const std::string& e);
+void plan_models(const std::vector<const Model*>& models, HostPlan& hp);
 // Rebuilds and uploads the plan of `models` with `cfg`; keeps kernel choices of `d`.
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
                    HostPlan& hp, DevicePlan& d, LowpPlan& lp);

@@ -1,21 +1,28 @@
 #include "nsx_plan.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 
 namespace nsx {
 
-// FNV-1a over the bytes of a matrix
+// Hash of a matrix's bit patterns, one 64-bit word per step (multiply-xorshift
+// mixing); equality is still decided by an exact comparison.
 static uint64_t content_hash(const double* v, size_t n) {
-  uint64_t h = 1469598103934665603ull;
-  const unsigned char* b = reinterpret_cast<const unsigned char*>(v);
-  for (size_t i = 0; i < n * sizeof(double); ++i) { h ^= b[i]; h *= 1099511628211ull; }
-  return h ^ (uint64_t)n;
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t)n;
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t w;
+    std::memcpy(&w, v + i, 8);
+    h = (h ^ w) * 0xff51afd7ed558ccdull;
+    h ^= h >> 32;
+  }
+  h *= 0xc4ceb9fe1a85ec53ull;
+  return h ^ (h >> 29);
 }
 
 // Append a matrix to `store` unless an identical one is already there; returns its offset.
 static int64_t intern(std::vector<double>& store, std::unordered_map<uint64_t, std::vector<int64_t>>& idx,
-                      const double* v, size_t n) {
+                      const double* v, size_t n, std::vector<std::pair<int64_t, int64_t>>& blocks) {
   const uint64_t h = content_hash(v, n);
   auto& cands = idx[h];
   for (int64_t off : cands)
@@ -23,6 +30,7 @@ static int64_t intern(std::vector<double>& store, std::unordered_map<uint64_t, s
   const int64_t off = (int64_t)store.size();
   store.insert(store.end(), v, v + n);
   cands.push_back(off);
+  blocks.push_back({off, (int64_t)n});
   return off;
 }
 
@@ -38,7 +46,7 @@ void plan_add(HostPlan& p, const Model& m) {
   // operator values of this instance
   std::vector<int64_t> a_off(m.adense.size()), b_off(m.bhat.size());
   for (size_t k = 0; k < m.adense.size(); ++k)
-    a_off[k] = intern(p.Ad, p.dedup_a, m.adense[k].data(), m.adense[k].size());
+    a_off[k] = intern(p.Ad, p.dedup_a, m.adense[k].data(), m.adense[k].size(), p.a_blocks);
   // Bhat^T blocks grouped by (Jb, lam, sa, sa') with every Jb' of the group's
   // range adjacent (zeros where a Jb' has no operator), so that the blocks one
   // a-term needs form a single contiguous run.
@@ -70,7 +78,7 @@ void plan_add(HostPlan& p, const Model& m) {
           for (int i = 0; i < dbr; ++i)
             run.push_back(it == slot_of.end() ? 0.0 : m.bhat[it->second][size_t(i) * db + c]);
       }
-      const int64_t off = intern(p.Bh, p.dedup_b, run.data(), run.size());
+      const int64_t off = intern(p.Bh, p.dedup_b, run.data(), run.size(), p.b_blocks);
       for (int k : g.second) b_off[k] = off + within[m.bhat_key[k][1]];
     }
   }
@@ -78,12 +86,12 @@ void plan_add(HostPlan& p, const Model& m) {
   int64_t nnz_like = 0;
   for (int j = 0; j < m.A.nJ(); ++j)
     if (!m.A.H.empty() && !m.A.H[j].empty()) {
-      ha_off[j] = intern(p.H, p.dedup_h, m.A.H[j].data(), m.A.H[j].size());
+      ha_off[j] = intern(p.H, p.dedup_h, m.A.H[j].data(), m.A.H[j].size(), p.h_blocks);
       for (double v : m.A.H[j]) nnz_like += v != 0.0;
     }
   for (int j = 0; j < m.B.nJ(); ++j)
     if (!m.B.H.empty() && !m.B.H[j].empty()) {
-      hb_off[j] = intern(p.H, p.dedup_h, m.B.H[j].data(), m.B.H[j].size());
+      hb_off[j] = intern(p.H, p.dedup_h, m.B.H[j].data(), m.B.H[j].size(), p.h_blocks);
       for (double v : m.B.H[j]) nnz_like += v != 0.0;
     }
   // items: partial tiles
@@ -224,6 +232,92 @@ void plan_add(HostPlan& p, const Model& m) {
   for (const auto& d : m.A.dens) nnzA += d.count;
   for (const auto& d : m.B.dens) nnzB += d.count;
   p.bytes_alg += 16 * m.N + 8 * (nnzA + nnzB + nnz_like + m.n_bterms);
+}
+
+// Maps offsets of src's operator store into dst's, interning src's blocks.
+struct StoreMap {
+  std::vector<int64_t> start, shift;  // sorted block starts in src, and the offset change
+  void add(const std::vector<double>& src_store, const std::vector<std::pair<int64_t, int64_t>>& blocks,
+           std::vector<double>& dst_store, std::unordered_map<uint64_t, std::vector<int64_t>>& idx,
+           std::vector<std::pair<int64_t, int64_t>>& dst_blocks) {
+    auto sorted = blocks;
+    std::sort(sorted.begin(), sorted.end());
+    for (auto& b : sorted) {
+      const int64_t off = intern(dst_store, idx, src_store.data() + b.first, (size_t)b.second, dst_blocks);
+      start.push_back(b.first);
+      shift.push_back(off - b.first);
+    }
+  }
+  // any offset inside a block moves with it
+  int64_t operator()(int64_t off) const {
+    if (off < 0) return off;
+    const auto it = std::upper_bound(start.begin(), start.end(), off);
+    return off + shift[size_t(it - start.begin()) - 1];
+  }
+};
+
+void plan_merge(HostPlan& p, const HostPlan& q) {
+  const int64_t sv = p.N, sp = p.part_size;
+  StoreMap ma, mb, mh;
+  ma.add(q.Ad, q.a_blocks, p.Ad, p.dedup_a, p.a_blocks);
+  mb.add(q.Bh, q.b_blocks, p.Bh, p.dedup_b, p.b_blocks);
+  mh.add(q.H, q.h_blocks, p.H, p.dedup_h, p.h_blocks);
+  const int i_pairs = (int)p.pairs.size(), i_phases = (int)p.phases.size();
+  const int i_aterms = (int)p.aterms.size(), i_rterms = (int)p.rterms.size();
+  const int i_plist = (int)p.part_list.size();
+  for (int64_t v : q.vbase) p.vbase.push_back(v + sv);
+  p.pairs.insert(p.pairs.end(), q.pairs.begin(), q.pairs.end());
+  for (DPhase ph : q.phases) {
+    ph.v_off += sv;
+    ph.b_off = mb(ph.b_off);
+    ph.a_off = ma(ph.a_off);
+    ph.pair_begin += i_pairs;
+    ph.pair_end += i_pairs;
+    p.phases.push_back(ph);
+  }
+  for (DItem2 it : q.items2) {
+    it.part_off += sp;
+    it.w_off += sv;
+    it.ph_begin += i_phases;
+    it.ph_end += i_phases;
+    p.items2.push_back(it);
+  }
+  for (DRTerm r : q.rterms) {
+    r.v_off += sv;
+    r.b_off = mb(r.b_off);
+    p.rterms.push_back(r);
+  }
+  for (DATerm t : q.aterms) {
+    t.a_off = ma(t.a_off);
+    t.r_begin += i_rterms;
+    t.r_end += i_rterms;
+    p.aterms.push_back(t);
+  }
+  for (DItem it : q.items) {
+    it.part_off += sp;
+    it.w_off += sv;
+    it.a_begin += i_aterms;
+    it.a_end += i_aterms;
+    p.items.push_back(it);
+  }
+  for (int64_t o : q.part_list) p.part_list.push_back(o + sp);
+  for (DBlock b : q.blocks) {
+    b.v_off += sv;
+    b.ha_off = mh(b.ha_off);
+    b.hb_off = mh(b.hb_off);
+    b.p_begin += i_plist;
+    b.p_end += i_plist;
+    p.blocks.push_back(b);
+  }
+  p.max_units = std::max(p.max_units, q.max_units);
+  p.lds_y = std::max(p.lds_y, q.lds_y);
+  p.lds_buf = std::max(p.lds_buf, q.lds_buf);
+  p.max_lds = std::max(p.max_lds, q.max_lds);
+  p.N += q.N;
+  p.part_size += q.part_size;
+  p.flops_ref += q.flops_ref;
+  p.flops_like += q.flops_like;
+  p.bytes_alg += q.bytes_alg;
 }
 
 }  // namespace nsx

@@ -98,6 +98,8 @@ struct HostPlan {
   std::vector<int64_t> vbase;       // per instance
   // identical operator matrices are stored once (content hash -> offsets)
   std::unordered_map<uint64_t, std::vector<int64_t>> dedup_a, dedup_b, dedup_h;
+  // (offset, length) of every block stored in Ad, Bh, H, in insertion order
+  std::vector<std::pair<int64_t, int64_t>> a_blocks, b_blocks, h_blocks;
   int64_t N = 0, part_size = 0;
   int max_lds = 0;                  // doubles of LDS the pn kernel needs
   double flops_ref = 0.0;           // algorithmic work of one batched H*v
@@ -107,5 +109,10 @@ struct HostPlan {
 
 // Appends one instance to a plan.
 void plan_add(HostPlan& p, const Model& m);
+// Appends a plan built on its own (same PlanConfig) to p: offsets are moved and
+// q's operator blocks are interned into p's stores.  plan_add of instances
+// into separate plans (in parallel) followed by plan_merge in order gives the
+// same H*v as plan_add of all of them into one plan.
+void plan_merge(HostPlan& p, const HostPlan& q);
 
 }  // namespace nsx

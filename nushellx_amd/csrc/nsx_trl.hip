@@ -26,6 +26,7 @@
 #include "../../include/nsx_hv.h"
 #include "nsx_api_internal.hpp"
 #include "nsx_eig.hpp"
+#include "nsx_parallel.hpp"
 
 namespace nsx {
 
@@ -113,24 +114,6 @@ __global__ __launch_bounds__(kVT) void k_seg_ritz(double* __restrict__ Y, const 
     for (int j = 0; j < mm; ++j) s = fma(Q[r + ldq * j], Si[j + (int64_t)mm * c], s);
     Y[r + ldq * c] = s;
   }
-}
-
-// f(i) for i in [0, n) on up to 16 host threads (dynamic assignment)
-template <typename F>
-static void parallel_for(int n, F&& f) {
-  int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-  nt = std::min(nt, std::max(1, n / 8));
-  if (nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int> next(0);
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nt; ++t)
-    pool.emplace_back([&]() {
-      for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
-    });
-  for (auto& th : pool) th.join();
 }
 
 struct DevBuf {

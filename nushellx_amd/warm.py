@@ -134,9 +134,13 @@ class WarmStart:
 
     def start_vector(self, batch, features: np.ndarray, seed: int = 99) -> tuple:
         """Predicted start vector for a Batch, with the random fallback applied
-        per instance.  Returns (vector, list of instances that fell back)."""
+        per instance.  ``features``: one row per sample; the batch holds the
+        samples' instances sample-major, in the order the network was trained
+        on.  Returns (vector, list of instances that fell back)."""
         from .hv import DeviceBuffer, device_sync
-        v = self.predict(features)[0]
+        v = self.predict(features).reshape(-1)
+        if v.size != batch.N:
+            raise ValueError(f"prediction has {v.size} amplitudes, the batch {batch.N}")
         rng = np.random.default_rng(seed)
         r = np.zeros_like(v)
         fell = []
@@ -215,4 +219,47 @@ def build_training_set(space, nprot: int, nneut: int, J2s: Sequence[int], base_i
         targs.append(align_and_sum(r["evecs"], ref["evecs"], r["offsets"], r["dims"]))
         if log and (k + 1) % 32 == 0:
             log(f"training set: {k + 1}/{len(seeds)} samples solved")
+    return np.array(feats), np.array(targs), ref["evecs"], ref
+
+
+def build_training_set_batched(space, nprot: int, nneut: int, J2s: Sequence[int], base_inter,
+                               seeds: Sequence[int], workdir: str, nev: int = 8, tol: float = 1e-6,
+                               rel_sigma: float = 0.1, chunk: int = 64, threads: int = 16, log=None):
+    """As ``build_training_set``, with the samples solved ``chunk`` at a time
+    as one lockstep batch of host-only instances (for large model spaces)."""
+    from .ensemble import build_instances, solve_ensemble, write_ensemble
+    from .synth.writer import SideCache
+    cache = SideCache()
+    ref_entries = write_ensemble(space, [(nprot, nneut)], J2s, base_inter, [0], workdir + "/ref",
+                                 rel_sigma=0.0, cache=cache)
+    ref_hams = build_instances(ref_entries, J2s, threads=threads)
+    ref = solve_ensemble(ref_hams, nev=nev, tol=tol, vectors=True, tune_on=None)
+    ref["batch"].close()
+    feats, targs = [], []
+    seeds = list(seeds)
+    for c0 in range(0, len(seeds), chunk):
+        part = seeds[c0:c0 + chunk]
+        entries = write_ensemble(space, [(nprot, nneut)], J2s, base_inter, part, f"{workdir}/c{c0}",
+                                 rel_sigma=rel_sigma, cache=cache)
+        hams = build_instances(entries, J2s, threads=threads)
+        r = solve_ensemble(hams, nev=nev, tol=tol, vectors=True)
+        b = r["batch"]
+        nJ = len(J2s)
+        for k, s in enumerate(part):
+            sl = slice(k * nJ, (k + 1) * nJ)
+            if (r["converged_at"][sl] < 0).any():
+                continue
+            o0 = b.offsets[k * nJ]
+            n_tot = sum(h.N for h in hams[sl])
+            vec = r["evecs"][:, o0:o0 + n_tot]
+            offs = [b.offsets[k * nJ + j] - o0 for j in range(nJ)]
+            feats.append(base_inter.perturbed(int(s), rel_sigma).parameter_vector())
+            targs.append(align_and_sum(vec, ref["evecs"], offs, [h.N for h in hams[sl]]))
+        b.close()
+        for h in hams:
+            h.close()
+        if log:
+            log(f"training set: {min(c0 + chunk, len(seeds))}/{len(seeds)} samples solved")
+    for h in ref_hams:
+        h.close()
     return np.array(feats), np.array(targs), ref["evecs"], ref

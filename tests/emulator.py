@@ -27,6 +27,8 @@ def lib():
         L.emu_last_error.restype = C.c_char_p
         L.emu_sixj.argtypes = [C.c_int] * 6
         L.emu_sixj.restype = C.c_double
+        L.emu_hv_batch.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_void_p, C.c_void_p, C.c_int64,
+                                   C.c_void_p]
         L.emu_sym_eig.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
@@ -62,3 +64,18 @@ def sym_eig(A, full=True):
     if lib().emu_sym_eig(A.ctypes.data, n, int(full), w.ctypes.data, Z.ctypes.data):
         raise RuntimeError("QL did not converge")
     return w, (Z.T if full else Z)  # Z stored column-major: rows of the C array are eigenvectors
+
+
+def emu_hv_batch(arg_lists, v, merge):
+    """Emulated H*v of a batch planned in one pass (merge=False) or per
+    instance and merged (merge=True); returns (w, store sizes Ad/Bh/H)."""
+    flat = [a.encode() for args in arg_lists for a in args]
+    arr = (C.c_char_p * len(flat))(*flat)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    sizes = np.zeros(3, dtype=np.int64)
+    rc = lib().emu_hv_batch(len(arg_lists), arr, int(merge), v.ctypes.data, out.ctypes.data, len(v),
+                            sizes.ctypes.data)
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out, sizes

@@ -16,14 +16,8 @@ static thread_local std::string g_err;
 
 extern "C" const char* emu_last_error() { return g_err.c_str(); }
 
-extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const char* intf,
-                      const double* vin, double* vout, int64_t n) {
-  Model m;
-  std::string e = build_model(nus, opa, opb, intf, m);
-  if (!e.empty()) { g_err = e; return -1; }
-  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
-  HostPlan p;
-  plan_add(p, m);
+// H*v of every instance of a plan, computed on the CPU from the plan arrays
+static void emulate(const HostPlan& p, const double* vin, double* vout) {
   std::vector<double> part(p.part_size, 0.0), buf;
   // staged items
   for (const DItem2& it : p.items2) {
@@ -103,6 +97,45 @@ extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const c
       }
     }
   }
+}
+
+extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const char* intf,
+                      const double* vin, double* vout, int64_t n) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  plan_add(p, m);
+  emulate(p, vin, vout);
+  return 0;
+}
+
+// A batch of instances planned into one plan (merge = 0) or planned one by one
+// and merged (merge = 1); args holds 4 file names per instance.
+extern "C" int emu_hv_batch(int n, const char* const* args, int merge, const double* vin, double* vout,
+                            int64_t ntot, int64_t* sizes) {
+  std::vector<Model> ms(n);
+  for (int i = 0; i < n; ++i) {
+    std::string e = build_model(args[4 * i], args[4 * i + 1], args[4 * i + 2], args[4 * i + 3], ms[i]);
+    if (!e.empty()) { g_err = e; return -1; }
+  }
+  HostPlan p;
+  for (int i = 0; i < n; ++i) {
+    if (merge) {
+      HostPlan q;
+      q.cfg = p.cfg;
+      plan_add(q, ms[i]);
+      plan_merge(p, q);
+    } else {
+      plan_add(p, ms[i]);
+    }
+  }
+  if (p.N != ntot) { g_err = "dimension mismatch"; return -2; }
+  emulate(p, vin, vout);
+  sizes[0] = (int64_t)p.Ad.size();
+  sizes[1] = (int64_t)p.Bh.size();
+  sizes[2] = (int64_t)p.H.size();
   return 0;
 }
 

@@ -23,3 +23,22 @@ def test_plan_stats_sd44(models):
     st = plan_stats(models["sd44"].args(8))
     assert st["items2"] > 0 and st["items"] == 0  # every sd block takes the staged path
     assert st["lds_buf"] <= 5120
+
+
+def test_plan_merge_equals_single_pass(models):
+    """Instances planned separately and merged give bit-identical H*v, and the
+    merge deduplicates operator blocks shared between instances."""
+    from tests.emulator import emu_hv_batch
+    args = [models["sd44"].args(0), models["sd44"].args(4), models["sd22"].args(2), models["sd44"].args(8)]
+    n = sum(RefCPU(*a, fp64=True).N for a in args)
+    v = np.random.default_rng(9).standard_normal(n)
+    w1, s1 = emu_hv_batch(args, v, merge=False)
+    w2, s2 = emu_hv_batch(args, v, merge=True)
+    assert np.array_equal(w1, w2)
+    assert (s1 == s2).all()
+    off = 0
+    for a in args:
+        r = RefCPU(*a, fp64=True)
+        ref = r.apply(v[off:off + r.N])
+        assert np.abs(w2[off:off + r.N] - ref).max() <= 1e-12 * np.abs(ref).max()
+        off += r.N
