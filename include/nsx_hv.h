@@ -95,6 +95,10 @@ typedef struct {
   int32_t atomic;      /* 0: deterministic partials + reduce; 1: FP64 atomic accumulation
                           (summation order varies run to run at the 1e-16 relative level) */
   double ms;           /* measured time of one H*v with this shape */
+  int32_t stacked;     /* 1: two-stage stacked path (X = V Bhat^T shared across left
+                          blocks, then W += A sum r X; atomic accumulation) */
+  int32_t stk_cap;     /* stacked path: doubles a chunk stages in LDS (0 = default) */
+  int32_t stk_work;    /* stacked path: work units per chunk (0 = default) */
 } nsx_tune_result;
 int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);
 int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);

@@ -53,6 +53,29 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
   if (const char* v = std::getenv("NSX_STAGED_NT")) d.staged_nt = std::atoi(v);
   if (const char* v = std::getenv("NSX_STAGED_MODE")) d.staged_mode = std::atoi(v);
   if (const char* v = std::getenv("NSX_ATOMIC")) d.atomic = std::atoi(v);
+  d.stacked = hp.cfg.stacked;
+  if (d.stacked) {
+    if (!(e = up(d.s_chunk1, hp.s_chunk1)).empty()) return e;
+    if (!(e = up(d.s_unit1, hp.s_unit1)).empty()) return e;
+    if (!(e = up(d.s_chunk2, hp.s_chunk2)).empty()) return e;
+    if (!(e = up(d.s_term2, hp.s_term2)).empty()) return e;
+    if (!(e = up(d.s_xref, hp.s_xref)).empty()) return e;
+    d.n_chunk1 = (int)hp.s_chunk1.size();
+    d.n_chunk2 = (int)hp.s_chunk2.size();
+    d.s_lds1 = hp.s_lds1;
+    d.s_lds2 = hp.s_lds2;
+    if (hp.x_size > 0 && hipMalloc(&d.X, sizeof(double) * (size_t)hp.x_size) != hipSuccess)
+      return "hipMalloc failed (stacked X buffer)";
+    const size_t lds = sizeof(double) * (size_t)std::max(hp.s_lds1, hp.s_lds2);
+    if (lds > 64 * 1024 - 4096)
+      for (int prec = 0; prec < 3; ++prec)
+        for (int nt : {64, 128, 256})
+          for (int st = 1; st <= 2; ++st)
+            if (hipFuncSetAttribute(k_stk_ptr(st, prec, nt), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+              return "LDS request too large (stacked kernels)";
+    d.atomic = 1;
+  }
   if (!(e = up(d.items, hp.items)).empty()) return e;
   if (!(e = up(d.aterms, hp.aterms)).empty()) return e;
   if (!(e = up(d.rterms, hp.rterms)).empty()) return e;
@@ -80,6 +103,8 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.items2); (void)hipFree(d.phases); (void)hipFree(d.pairs);
   (void)hipFree(d.items); (void)hipFree(d.aterms); (void)hipFree(d.rterms); (void)hipFree(d.blocks);
   (void)hipFree(d.part_list); (void)hipFree(d.Ad); (void)hipFree(d.Bh); (void)hipFree(d.H); (void)hipFree(d.part);
+  (void)hipFree(d.s_chunk1); (void)hipFree(d.s_unit1); (void)hipFree(d.s_chunk2); (void)hipFree(d.s_term2);
+  (void)hipFree(d.s_xref); (void)hipFree(d.X);
   d = DevicePlan();
 }
 
@@ -230,10 +255,7 @@ int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const n
   std::unique_ptr<nsx_batch> b(new nsx_batch());
   if (precision < NSX_FP64 || precision > NSX_BF16X3) { set_error("unknown precision"); return NSX_EARG; }
   b->precision = precision;
-  if (cfg) {
-    b->hplan.cfg.fat_cap = std::min(std::max(cfg->fat_cap, 256), kFatCap);
-    b->hplan.cfg.max_phases = std::max(cfg->max_phases, 1);
-  }
+  if (cfg) b->hplan.cfg = plan_config(*cfg);
   for (int i = 0; i < n; ++i) {
     if (!hs[i]) { set_error("null handle in batch"); return NSX_EARG; }
     b->models.push_back(&hs[i]->model);
@@ -248,7 +270,8 @@ int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const n
   if (cfg) {
     b->dplan.staged_nt = cfg->threads;
     b->dplan.staged_mode = cfg->mode;
-    b->dplan.atomic = cfg->atomic;
+    b->dplan.atomic = cfg->stacked ? 1 : cfg->atomic;
+    b->dplan.stk_nt = cfg->threads;
   }
   NSX_HIP(hipStreamCreate(&b->stream));
   *out = b.release();
@@ -412,11 +435,25 @@ int nsx_device_info(char* name, int32_t len, int32_t* cus) {
 
 namespace nsx {
 
+PlanConfig plan_config(const nsx_tune_result& r) {
+  PlanConfig pc;
+  pc.fat_cap = std::min(std::max(r.fat_cap, 256), kFatCap);
+  pc.max_phases = std::max(r.max_phases, 1);
+  pc.stacked = r.stacked ? 1 : 0;
+  if (r.stk_cap > 0) pc.stk_cap = r.stk_cap;
+  if (r.stk_work > 0) pc.stk_work = r.stk_work;
+  return pc;
+}
+
 // Plans every model on its own in parallel, then merges the plans in order
 // (same result as plan_add of each model into hp in turn).
 void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
   const int n = (int)models.size();
-  if (n == 1) { plan_add(hp, *models[0]); return; }
+  if (n == 1) {
+    plan_add(hp, *models[0]);
+    if (hp.cfg.stacked) build_stacked(hp, models);
+    return;
+  }
   std::vector<HostPlan> local(n);
   parallel_for(n, [&](int i) {
     local[i].cfg = hp.cfg;
@@ -426,11 +463,12 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
     plan_merge(hp, local[i]);
     local[i] = HostPlan();  // release as we go
   }
+  if (hp.cfg.stacked) build_stacked(hp, models);
 }
 
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
                    HostPlan& hp, DevicePlan& d, LowpPlan& lp) {
-  const int nt = d.staged_nt, mode = d.staged_mode, at = d.atomic;
+  const int nt = d.staged_nt, mode = d.staged_mode, at = d.atomic, snt = d.stk_nt;
   free_plan(d);
   free_lowp(lp);
   hp = HostPlan();
@@ -441,7 +479,8 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
   if (!(e = upload_plan(hp, d)).empty()) return e;
   d.staged_nt = nt;
   d.staged_mode = mode;
-  d.atomic = at;
+  d.atomic = cfg.stacked ? 1 : at;
+  d.stk_nt = snt;
   return "";
 }
 
@@ -479,7 +518,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
   std::vector<double> h(N);
   for (int64_t i = 0; i < N; ++i) h[i] = std::sin(0.37 * double(i + 1));
   (void)hipMemcpy(in, h.data(), sizeof(double) * N, hipMemcpyHostToDevice);
-  nsx_tune_result best{0, 0, 0, 0, 0, 1e300};
+  nsx_tune_result best{0, 0, 0, 0, 0, 1e300, 0, 0, 0};
   const int caps[] = {768, 1536, 2560, kFatCap};
   const int chunks[] = {1, 2, 4, 8};
   const int nts[] = {64, 128, 256};
@@ -497,18 +536,33 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
             d.staged_mode = mode;
             d.atomic = at;
             const double ms = time_hv(d, lp, precision, in, out, s, iters);
-            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms};
+            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms, 0, 0, 0};
           }
+    }
+  // stacked two-stage path: chunk capacity (LDS doubles) x work units x threads
+  for (int scap : {2048, 4096})
+    for (int swork : {256, 512, 1024}) {
+      PlanConfig cfg;
+      cfg.stacked = 1;
+      cfg.stk_cap = scap;
+      cfg.stk_work = swork;
+      std::string e = replan(models, cfg, precision, hp, d, lp);
+      if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+      for (int nt : {128, 256}) {
+        d.stk_nt = nt;
+        d.atomic = 1;
+        const double ms = time_hv(d, lp, precision, in, out, s, iters);
+        if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, nt, 0, 1, ms, 1, scap, swork};
+      }
     }
   (void)hipFree(in);
   (void)hipFree(out);
   if (best.ms >= 1e300) { set_error("autotune: no configuration ran"); return NSX_EGPU; }
-  PlanConfig cfg;
-  cfg.fat_cap = best.fat_cap;
-  cfg.max_phases = best.max_phases;
+  const PlanConfig cfg = plan_config(best);
   d.staged_nt = best.threads;
   d.staged_mode = best.mode;
   d.atomic = best.atomic;
+  d.stk_nt = best.threads;
   std::string e = replan(models, cfg, precision, hp, d, lp);
   if (!e.empty()) { set_error(e); return NSX_EGPU; }
   if (best_out) *best_out = best;
@@ -532,12 +586,11 @@ int nsx_batch_autotune(nsx_batch* b, int32_t iters, nsx_tune_result* best) {
 
 int nsx_batch_set_config(nsx_batch* b, const nsx_tune_result* cfg) {
   if (!b || !cfg) { set_error("null argument"); return NSX_EARG; }
-  PlanConfig pc;
-  pc.fat_cap = std::min(std::max(cfg->fat_cap, 256), kFatCap);
-  pc.max_phases = std::max(cfg->max_phases, 1);
+  const PlanConfig pc = plan_config(*cfg);
   b->dplan.staged_nt = cfg->threads;
   b->dplan.staged_mode = cfg->mode;
-  b->dplan.atomic = cfg->atomic;
+  b->dplan.atomic = cfg->stacked ? 1 : cfg->atomic;
+  b->dplan.stk_nt = cfg->threads;
   std::string e = replan(b->models, pc, b->precision, b->hplan, b->dplan, b->lowp);
   if (!e.empty()) { set_error(e); return NSX_EGPU; }
   return NSX_OK;

@@ -8,6 +8,7 @@
 #include "nsx_kernels.hpp"
 #include "nsx_model.hpp"
 #include "nsx_plan.hpp"
+#include "../../include/nsx_hv.h"
 
 struct nsx_hv {
   nsx::Model model;
@@ -33,6 +34,7 @@ struct nsx_batch {
 namespace nsx {
 void set_error(const std::string& e);
 void plan_models(const std::vector<const Model*>& models, HostPlan& hp);
+PlanConfig plan_config(const nsx_tune_result& r);
 // Rebuilds and uploads the plan of `models` with `cfg`; keeps kernel choices of `d`.
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
                    HostPlan& hp, DevicePlan& d, LowpPlan& lp);

@@ -347,6 +347,217 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
   }
 }
 
+// ---------------------------------------------------------------- stacked two-stage path
+// Stage 1: one workgroup per chunk of (V_R, Bhat^T) pairs sharing one Bhat^T
+// block.  Bhat^T and the chunk's V_R blocks are staged in LDS with one round of
+// loads; each work unit is (row i, 4 columns) of one pair's X = V_R Bhat^T,
+// written to the X buffer in the accumulation type.
+// Index of the last segment whose start (read through `start`) is <= e, by
+// binary search over n sorted starts.
+template <typename F>
+__device__ __forceinline__ int seg_of(int e, int n, F start) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start(mid) <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+constexpr int kStkBatch = 8;  // loads each thread keeps in flight when staging
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_stk1(const SChunk1* __restrict__ chunks, const SUnit1* __restrict__ units,
+                                             const T* __restrict__ V, const T* __restrict__ Bt,
+                                             typename Prec<T>::acc* __restrict__ X) {
+  using Acc = typename Prec<T>::acc;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  __shared__ SUnit1 su[kStkMeta];
+  const SChunk1 c = chunks[blockIdx.x];
+  const int nu = c.u_end - c.u_begin;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < nu; t += NT) su[t] = units[c.u_begin + t];
+  __syncthreads();
+  // stage Bhat^T then every V_R block: LDS holds their concatenation, so the
+  // LDS index is the element index; kStkBatch independent loads per thread
+  const int nb = c.dbr * c.db, ntot = c.lds;
+  for (int e0 = tid; e0 < ntot; e0 += NT * kStkBatch) {
+    T r[kStkBatch];
+#pragma unroll
+    for (int q = 0; q < kStkBatch; ++q) {
+      const int e = e0 + q * NT;
+      if (e < ntot) {
+        if (e < nb) {
+          r[q] = Bt[c.b_off + e];
+        } else {
+          const int u = seg_of(e, nu, [&](int x) { return su[x].lds; });
+          r[q] = V[su[u].v_off + (e - su[u].lds)];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kStkBatch; ++q)
+      if (e0 + q * NT < ntot) sm[e0 + q * NT] = r[q];
+  }
+  __syncthreads();
+  const int dbr = c.dbr, db = c.db;
+  int u = 0;
+  for (int w = tid; w < c.work; w += NT) {
+    while (u + 1 < nu && su[u + 1].work0 <= w) ++u;
+    const int dar = su[u].dar, lw = w - su[u].work0;
+    const int i = lw % dar, j0 = (lw / dar) * kColBlk;
+    const T* sV = sm + su[u].lds;
+    const T* bc[kColBlk];
+#pragma unroll
+    for (int q = 0; q < kColBlk; ++q) bc[q] = sm + dbr * min(j0 + q, db - 1);
+    Acc s0[kColBlk], s1[kColBlk];
+#pragma unroll
+    for (int q = 0; q < kColBlk; ++q) { s0[q] = Acc(0); s1[q] = Acc(0); }
+    int k = 0;
+    for (; k + 1 < dbr; k += 2) {
+      const T v0 = sV[i + dar * k], v1 = sV[i + dar * (k + 1)];
+#pragma unroll
+      for (int q = 0; q < kColBlk; ++q) {
+        s0[q] = Prec<T>::fma3(v0, bc[q][k], s0[q]);
+        s1[q] = Prec<T>::fma3(v1, bc[q][k + 1], s1[q]);
+      }
+    }
+    if (k < dbr) {
+      const T v0 = sV[i + dar * k];
+#pragma unroll
+      for (int q = 0; q < kColBlk; ++q) s0[q] = Prec<T>::fma3(v0, bc[q][k], s0[q]);
+    }
+    Acc* x = X + su[u].x_off + i;
+#pragma unroll
+    for (int q = 0; q < kColBlk; ++q)
+      if (j0 + q < db) x[(int64_t)dar * (j0 + q)] = s0[q] + s1[q];
+  }
+}
+
+// Stage 2: one workgroup per chunk of the a-terms of one left block L.  Every
+// term's A and Y = sum_u r_u X_u are staged in LDS in one round (X references
+// staged first); each work unit (row i, 4 columns) of W_L sums A_k Y_k over the
+// chunk's terms in registers and is added into W once, with FP64 atomics.
+static_assert(kStkXref * sizeof(SXRef) >= 256 * kColBlk * sizeof(double),
+              "stage-2 split-K partials reuse the X-reference table");
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks, const STerm2* __restrict__ terms,
+                                             const SXRef* __restrict__ xrefs, const T* __restrict__ Ad,
+                                             const typename Prec<T>::acc* __restrict__ X, double* __restrict__ W) {
+  using Acc = typename Prec<T>::acc;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  __shared__ STerm2 st[kStkMeta];
+  __shared__ SXRef sx[kStkXref];
+  const SChunk2 c = chunks[blockIdx.x];
+  const int nt = c.t_end - c.t_begin;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < nt; t += NT) st[t] = terms[c.t_begin + t];
+  const int x0 = terms[c.t_begin].x_begin, nx = terms[c.t_end - 1].x_end - x0;
+  for (int x = tid; x < nx; x += NT) sx[x] = xrefs[x0 + x];
+  __syncthreads();
+  const int da = c.da, db = c.db, ntot = c.lds;
+  for (int e0 = tid; e0 < ntot; e0 += NT * kStkBatch) {
+    Acc y[kStkBatch];
+    T a[kStkBatch];
+    bool isa[kStkBatch];
+#pragma unroll
+    for (int q = 0; q < kStkBatch; ++q) {
+      const int e = e0 + q * NT;
+      y[q] = Acc(0);
+      isa[q] = false;
+      if (e < ntot) {
+        const int t = seg_of(e, nt, [&](int x) { return st[x].lds; });
+        const int le = e - st[t].lds, na = da * st[t].dar;
+        if (le < na) {
+          isa[q] = true;
+          a[q] = Ad[st[t].a_off + le];
+        } else {
+          for (int x = st[t].x_begin - x0; x < st[t].x_end - x0; ++x)
+            y[q] += Acc(sx[x].r) * X[sx[x].x_off + (le - na)];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kStkBatch; ++q)
+      if (e0 + q * NT < ntot) sm[e0 + q * NT] = isa[q] ? a[q] : Prec<T>::from(y[q]);
+  }
+  __syncthreads();
+  // W_L = [A_1 .. A_T] [Y_1; ..; Y_T]: a long inner dimension over few outputs,
+  // so the terms are split over `nsplit` thread groups (split-K) and the group
+  // partials are summed through LDS (the X-reference table is free by now)
+  const int ncb = (db + kColBlk - 1) / kColBlk, units = da * ncb;
+  int nsplit = 1;
+  while (nsplit < 8 && nsplit * 2 * units <= NT && nsplit * 2 <= nt) nsplit *= 2;
+  Acc* red = reinterpret_cast<Acc*>(sx);  // NT * kColBlk Acc fit in sx
+  for (int w = tid; w < units * nsplit; w += NT) {
+    const int unit = w % units, sp = w / units;
+    const int i = unit % da, j0 = (unit / da) * kColBlk;
+    Acc s0[kColBlk], s1[kColBlk];
+#pragma unroll
+    for (int q = 0; q < kColBlk; ++q) { s0[q] = Acc(0); s1[q] = Acc(0); }
+    for (int t = sp; t < nt; t += nsplit) {
+      const int dar = st[t].dar;
+      const T* sA = sm + st[t].lds;
+      const T* sY = sA + da * dar;
+      const T* yc[kColBlk];
+#pragma unroll
+      for (int q = 0; q < kColBlk; ++q) yc[q] = sY + dar * min(j0 + q, db - 1);
+      int k = 0;
+      for (; k + 1 < dar; k += 2) {
+        const T a0 = sA[i + da * k], a1 = sA[i + da * (k + 1)];
+#pragma unroll
+        for (int q = 0; q < kColBlk; ++q) {
+          s0[q] = Prec<T>::fma3(a0, yc[q][k], s0[q]);
+          s1[q] = Prec<T>::fma3(a1, yc[q][k + 1], s1[q]);
+        }
+      }
+      if (k < dar) {
+        const T a0 = sA[i + da * k];
+#pragma unroll
+        for (int q = 0; q < kColBlk; ++q) s0[q] = Prec<T>::fma3(a0, yc[q][k], s0[q]);
+      }
+    }
+    if (nsplit == 1) {
+      double* wp = W + c.w_off + i;
+#pragma unroll
+      for (int q = 0; q < kColBlk; ++q)
+        if (j0 + q < db) unsafeAtomicAdd(wp + (int64_t)da * (j0 + q), (double)(s0[q] + s1[q]));
+    } else {
+#pragma unroll
+      for (int q = 0; q < kColBlk; ++q) red[(size_t)w * kColBlk + q] = s0[q] + s1[q];
+    }
+  }
+  if (nsplit > 1) {
+    __syncthreads();
+    for (int unit = tid; unit < units; unit += NT) {
+      const int i = unit % da, j0 = (unit / da) * kColBlk;
+      double* wp = W + c.w_off + i;
+#pragma unroll
+      for (int q = 0; q < kColBlk; ++q) {
+        if (j0 + q >= db) continue;
+        Acc v = Acc(0);
+        for (int sp = 0; sp < nsplit; ++sp) v += red[(size_t)(sp * units + unit) * kColBlk + q];
+        unsafeAtomicAdd(wp + (int64_t)da * (j0 + q), (double)v);
+      }
+    }
+  }
+}
+
+const void* k_stk_ptr(int stage, int prec, int nt) {
+#define NSX_STK(T)                                                                                  \
+  if (stage == 1) return nt == 64 ? (const void*)k_stk1<T, 64> : nt == 128 ? (const void*)k_stk1<T, 128> \
+                                                               : (const void*)k_stk1<T, 256>;         \
+  return nt == 64 ? (const void*)k_stk2<T, 64> : nt == 128 ? (const void*)k_stk2<T, 128> : (const void*)k_stk2<T, 256>;
+  switch (prec) {
+    case 0: { NSX_STK(double) }
+    case 1: { NSX_STK(float) }
+    default: { NSX_STK(Bf16x2) }
+  }
+#undef NSX_STK
+}
+
 // ---------------------------------------------------------------- instance tables
 struct StagedInst {
   int prec, nt, mu, mode, atomic;
@@ -394,7 +605,24 @@ template <typename T>
 static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, double* W,
                        hipStream_t s) {
   const bool at = d.atomic != 0;
-  if (d.n_items2 > 0) {
+  if (d.stacked) {  // two-stage path for the small blocks (always accumulates into W)
+    if (!at) return hipErrorInvalidConfiguration;
+    using Acc = typename Prec<T>::acc;
+    Acc* X = reinterpret_cast<Acc*>(d.X);
+    if (d.n_chunk1 > 0) {
+      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X};
+      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3(d.n_chunk1), dim3(d.stk_nt), a1,
+                                     sizeof(T) * (size_t)d.s_lds1, s);
+      if (e != hipSuccess) return e;
+    }
+    if (d.n_chunk2 > 0) {
+      const Acc* Xc = X;
+      void* a2[] = {(void*)&d.s_chunk2, (void*)&d.s_term2, (void*)&d.s_xref, (void*)&Ad, (void*)&Xc, (void*)&W};
+      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(d.n_chunk2), dim3(d.stk_nt), a2,
+                                     sizeof(T) * (size_t)d.s_lds2, s);
+      if (e != hipSuccess) return e;
+    }
+  } else if (d.n_items2 > 0) {
     const int k = pick_staged(prec, d.staged_nt, d.staged_mode, at, d.max_units);
     if (k < 0) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(T) * (size_t)((d.staged_mode == 1 ? 2 : 1) * d.lds_buf + d.lds_y);

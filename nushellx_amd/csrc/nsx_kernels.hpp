@@ -27,6 +27,15 @@ struct DevicePlan {
   int n_items = 0, n_blocks = 0;
   int max_lds = 1;           // direct kernel Y tile (elements)
   int64_t N = 0;
+  // stacked two-stage path (replaces the staged kernel when `stacked`)
+  int stacked = 0, stk_nt = 256;
+  SChunk1* s_chunk1 = nullptr;
+  SUnit1* s_unit1 = nullptr;
+  SChunk2* s_chunk2 = nullptr;
+  STerm2* s_term2 = nullptr;
+  SXRef* s_xref = nullptr;
+  int n_chunk1 = 0, n_chunk2 = 0, s_lds1 = 1, s_lds2 = 1;
+  void* X = nullptr;         // x_size elements of the accumulation type (8 bytes reserved each)
 };
 
 // Uploads a host plan; returns "" or an error message.
@@ -82,5 +91,6 @@ hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, cons
 int num_staged();
 const void* k_pn_staged_ptr(int i);
 const void* k_pn_partial_ptr(int prec, bool atomic);
+const void* k_stk_ptr(int stage, int prec, int nt);
 
 }  // namespace nsx

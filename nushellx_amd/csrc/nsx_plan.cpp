@@ -82,6 +82,8 @@ void plan_add(HostPlan& p, const Model& m) {
       for (int k : g.second) b_off[k] = off + within[m.bhat_key[k][1]];
     }
   }
+  p.inst_a_off.push_back(a_off);
+  p.inst_b_off.push_back(b_off);
   std::vector<int64_t> ha_off(m.A.nJ(), -1), hb_off(m.B.nJ(), -1);
   int64_t nnz_like = 0;
   for (int j = 0; j < m.A.nJ(); ++j)
@@ -234,6 +236,123 @@ void plan_add(HostPlan& p, const Model& m) {
   p.bytes_alg += 16 * m.N + 8 * (nnzA + nnzB + nnz_like + m.n_bterms);
 }
 
+void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
+  p.s_chunk1.clear(); p.s_unit1.clear(); p.s_chunk2.clear(); p.s_term2.clear(); p.s_xref.clear();
+  p.x_size = 0;
+  p.s_lds1 = p.s_lds2 = 1;
+  struct U1 { int64_t v_off; int dar, dbr, db; int64_t x_off; };
+  std::map<std::pair<int64_t, int64_t>, int64_t> x_of;        // (b_off, v_off) -> x_off
+  std::map<std::array<int64_t, 3>, std::vector<U1>> by_b;     // (b_off, dbr, db) -> pairs
+  struct T2 { int64_t a_off; int dar; std::vector<std::pair<int64_t, double>> x; };
+  std::map<std::array<int64_t, 3>, std::vector<T2>> by_w;     // (w_off, da, db) -> a-terms
+  for (size_t mi = 0; mi < models.size(); ++mi) {
+    const Model& m = *models[mi];
+    const int64_t vb = p.vbase[mi];
+    const auto& a_off = p.inst_a_off[mi];
+    const auto& b_off = p.inst_b_off[mi];
+    for (const Item& it : m.items) {
+      const int da = m.da[it.L], db = m.db[it.L], dar = m.A.d(it.ajr);
+      int maxdbr = 0;
+      for (const ATerm& at : it.at)
+        for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
+      if (std::max(std::max(da, dar), std::max(db, maxdbr)) > kSmallMax) continue;  // direct kernel
+      for (const ATerm& at : it.at) {
+        T2 t;
+        t.a_off = a_off[at.adens];
+        t.dar = dar;
+        for (const RTerm& rt : at.rt) {
+          const int64_t v = vb + m.posnv[rt.R], b = b_off[rt.bhat];
+          auto f = x_of.find({b, v});
+          int64_t x;
+          if (f == x_of.end()) {
+            x = p.x_size;
+            p.x_size += int64_t(dar) * db;
+            x_of[{b, v}] = x;
+            by_b[{b, m.db[rt.R], db}].push_back({v, dar, m.db[rt.R], db, x});
+          } else {
+            x = f->second;
+          }
+          t.x.push_back({x, rt.r});
+        }
+        by_w[{vb + m.posnv[it.L], da, db}].push_back(std::move(t));
+      }
+    }
+  }
+  // an a-term's references must fit one chunk's table; otherwise keep the staged path
+  for (auto& g : by_w)
+    for (const T2& t : g.second)
+      if ((int)t.x.size() > kStkXref) {
+        p.cfg.stacked = 0;
+        p.x_size = 0;
+        return;
+      }
+  // a chunk holds at least one pair (term); the cap grows to fit the largest single one
+  int cap = p.cfg.stk_cap;
+  for (auto& g : by_b)
+    for (const U1& u : g.second) cap = std::max(cap, u.dbr * u.db + u.dar * u.dbr);
+  for (auto& g : by_w)
+    for (const T2& t : g.second) cap = std::max(cap, (int)(g.first[1] * t.dar + t.dar * g.first[2]));
+  const int wcap = std::max(p.cfg.stk_work, 1);
+  for (auto& g : by_b) {
+    const int dbr = (int)g.first[1], db = (int)g.first[2];
+    const int ncb = (db + kColBlk - 1) / kColBlk;
+    SChunk1 c{};
+    auto open = [&]() {
+      c = SChunk1{};
+      c.b_off = g.first[0]; c.dbr = dbr; c.db = db;
+      c.u_begin = c.u_end = (int)p.s_unit1.size();
+      c.lds = dbr * db;
+    };
+    open();
+    for (const U1& u : g.second) {
+      const int need = u.dar * dbr, w = u.dar * ncb;
+      if (c.u_end > c.u_begin && (c.lds + need > cap || c.work + w > wcap || c.u_end - c.u_begin >= kStkMeta)) {
+        p.s_chunk1.push_back(c);
+        p.s_lds1 = std::max(p.s_lds1, c.lds);
+        open();
+      }
+      SUnit1 s{};
+      s.v_off = u.v_off; s.x_off = u.x_off; s.dar = u.dar; s.lds = c.lds; s.work0 = c.work;
+      p.s_unit1.push_back(s);
+      c.u_end++;
+      c.lds += need;
+      c.work += w;
+    }
+    p.s_chunk1.push_back(c);
+    p.s_lds1 = std::max(p.s_lds1, c.lds);
+  }
+  for (auto& g : by_w) {
+    const int da = (int)g.first[1], db = (int)g.first[2];
+    SChunk2 c{};
+    auto open = [&]() {
+      c = SChunk2{};
+      c.w_off = g.first[0]; c.da = da; c.db = db;
+      c.t_begin = c.t_end = (int)p.s_term2.size();
+    };
+    open();
+    for (const T2& t : g.second) {
+      const int need = da * t.dar + t.dar * db;
+      const int nx = (int)p.s_xref.size() - (c.t_end > c.t_begin ? p.s_term2[c.t_begin].x_begin : 0);
+      if (c.t_end > c.t_begin && (c.lds + need > cap || c.t_end - c.t_begin >= kStkMeta ||
+                                  nx + (int)t.x.size() > kStkXref)) {
+        p.s_chunk2.push_back(c);
+        p.s_lds2 = std::max(p.s_lds2, c.lds);
+        open();
+      }
+      STerm2 s{};
+      s.a_off = t.a_off; s.dar = t.dar; s.lds = c.lds;
+      s.x_begin = (int)p.s_xref.size();
+      for (auto& x : t.x) p.s_xref.push_back({x.first, x.second});
+      s.x_end = (int)p.s_xref.size();
+      p.s_term2.push_back(s);
+      c.t_end++;
+      c.lds += need;
+    }
+    p.s_chunk2.push_back(c);
+    p.s_lds2 = std::max(p.s_lds2, c.lds);
+  }
+}
+
 // Maps offsets of src's operator store into dst's, interning src's blocks.
 struct StoreMap {
   std::vector<int64_t> start, shift;  // sorted block starts in src, and the offset change
@@ -266,6 +385,14 @@ void plan_merge(HostPlan& p, const HostPlan& q) {
   const int i_aterms = (int)p.aterms.size(), i_rterms = (int)p.rterms.size();
   const int i_plist = (int)p.part_list.size();
   for (int64_t v : q.vbase) p.vbase.push_back(v + sv);
+  for (auto a : q.inst_a_off) {
+    for (auto& x : a) x = ma(x);
+    p.inst_a_off.push_back(std::move(a));
+  }
+  for (auto b : q.inst_b_off) {
+    for (auto& x : b) x = mb(x);
+    p.inst_b_off.push_back(std::move(b));
+  }
   p.pairs.insert(p.pairs.end(), q.pairs.begin(), q.pairs.end());
   for (DPhase ph : q.phases) {
     ph.v_off += sv;

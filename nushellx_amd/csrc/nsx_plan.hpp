@@ -27,6 +27,8 @@ constexpr int kMaxU = (kSmallMax * ((kSmallMax + kColBlk - 1) / kColBlk) + kThre
 constexpr int kMaxPhases = 6;            // phases per staged workgroup (long items are split)
 constexpr int kFatCap = 5120;            // doubles one phase stages in LDS (V run + Bhat run + A)
 constexpr int kFatPF = kFatCap / kThreads;
+constexpr int kStkMeta = 64;             // stacked path: pairs / terms per chunk
+constexpr int kStkXref = 512;            // stacked path: X references per stage-2 chunk
 
 struct DItem {       // one (L, Ja') partial, one column tile
   int64_t part_off;  // partial tile base (full da x db layout of the item)
@@ -75,10 +77,50 @@ struct DBlock {      // one output block L, one column tile
   int32_t p_begin, p_end;  // range in part_list of the item partials of L
 };
 
+// Stacked two-stage pn path (small blocks):
+//   stage 1: X_u = V_R Bhat^T for every distinct (V block, Bhat^T block) pair u,
+//            computed once and shared by all left blocks that use it; a
+//            workgroup takes a chunk of pairs that share one Bhat^T block;
+//   stage 2: W_L += sum_k A_k (sum_u r_u X_u); a workgroup takes a chunk of
+//            the a-terms of one left block L, accumulates W_L in registers and
+//            adds it into W once.
+struct SChunk1 {
+  int64_t b_off;            // Bhat^T (dbr x db, column-major) in Bh
+  int32_t dbr, db;
+  int32_t u_begin, u_end;   // SUnit1 range
+  int32_t work;             // sum over units of dar * ceil(db / 4)
+  int32_t lds;              // doubles staged: dbr*db + sum dar*dbr
+};
+struct SUnit1 {
+  int64_t v_off;            // V_R (dar x dbr, column-major) in the batched vector
+  int64_t x_off;            // X (dar x db, column-major) in the X buffer
+  int32_t dar, lds;         // rows; offset of V_R in the chunk's LDS buffer
+  int32_t work0, pad;       // first work unit of this pair in the chunk
+};
+struct SChunk2 {
+  int64_t w_off;            // W_L (da x db, column-major) in the batched vector
+  int32_t da, db;
+  int32_t t_begin, t_end;   // STerm2 range (a-terms of this left block)
+  int32_t lds;              // doubles staged: sum over terms of da*dar + dar*db
+  int32_t pad;
+};
+struct STerm2 {
+  int64_t a_off;            // A (da x dar, column-major) in Ad
+  int32_t dar, lds;         // inner size; offset of A in the chunk's LDS (Y follows)
+  int32_t x_begin, x_end;   // SXRef range
+};
+struct SXRef {
+  int64_t x_off;
+  double r;
+};
+
 // Tunable plan shape (chosen by the autotuner, see nsx_tune.cpp).
 struct PlanConfig {
   int fat_cap = kFatCap;     // doubles one staged phase may load (<= kFatCap)
   int max_phases = kMaxPhases;  // phases per staged workgroup
+  int stacked = 0;           // 1: small blocks take the two-stage stacked path
+  int stk_cap = 2048;        // stacked path: doubles one chunk may stage in LDS
+  int stk_work = 512;        // stacked path: work units per chunk
 };
 
 struct HostPlan {
@@ -100,6 +142,16 @@ struct HostPlan {
   std::unordered_map<uint64_t, std::vector<int64_t>> dedup_a, dedup_b, dedup_h;
   // (offset, length) of every block stored in Ad, Bh, H, in insertion order
   std::vector<std::pair<int64_t, int64_t>> a_blocks, b_blocks, h_blocks;
+  // per instance: store offset of each adense / bhat slot of its model
+  std::vector<std::vector<int64_t>> inst_a_off, inst_b_off;
+  // stacked path (filled by build_stacked)
+  std::vector<SChunk1> s_chunk1;
+  std::vector<SUnit1> s_unit1;
+  std::vector<SChunk2> s_chunk2;
+  std::vector<STerm2> s_term2;
+  std::vector<SXRef> s_xref;
+  int64_t x_size = 0;          // X buffer elements
+  int s_lds1 = 1, s_lds2 = 1;  // largest chunk LDS (doubles)
   int64_t N = 0, part_size = 0;
   int max_lds = 0;                  // doubles of LDS the pn kernel needs
   double flops_ref = 0.0;           // algorithmic work of one batched H*v
@@ -114,5 +166,9 @@ void plan_add(HostPlan& p, const Model& m);
 // into separate plans (in parallel) followed by plan_merge in order gives the
 // same H*v as plan_add of all of them into one plan.
 void plan_merge(HostPlan& p, const HostPlan& q);
+// Fills the stacked-path arrays of p for the small-block items of `models`
+// (the instances planned into p, in order).  Clears p.cfg.stacked (the staged
+// kernel then runs) if an a-term has more references than a chunk can hold.
+void build_stacked(HostPlan& p, const std::vector<const Model*>& models);
 
 }  // namespace nsx

@@ -157,7 +157,7 @@ class Batch:
         if config is None:
             check(lib().nsx_batch_create(arr, len(hams), PRECISIONS[precision], C.byref(b)), "nsx_batch_create")
         else:
-            cfg = TuneResult(**{k: config[k] for k, _ in TuneResult._fields_})
+            cfg = TuneResult(**{k: config.get(k, 0) for k, _ in TuneResult._fields_})
             check(lib().nsx_batch_create_config(arr, len(hams), PRECISIONS[precision], C.byref(cfg), C.byref(b)),
                   "nsx_batch_create_config")
         self._b = b
@@ -198,7 +198,8 @@ class Batch:
 
     def set_config(self, cfg: dict) -> None:
         r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"],
-                       cfg.get("atomic", 0), 0.0)
+                       cfg.get("atomic", 0), 0.0, cfg.get("stacked", 0), cfg.get("stk_cap", 0),
+                       cfg.get("stk_work", 0))
         check(lib().nsx_batch_set_config(self._b, C.byref(r)), "nsx_batch_set_config")
 
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,

@@ -29,6 +29,7 @@ def lib():
         L.emu_sixj.restype = C.c_double
         L.emu_hv_batch.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.c_void_p]
+        L.emu_hv_stacked.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int]
         L.emu_sym_eig.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
@@ -79,3 +80,13 @@ def emu_hv_batch(arg_lists, v, merge):
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out, sizes
+
+
+def emu_hv_stacked(args, v, cap=2048, work=512):
+    """Emulated H*v through the stacked two-stage plan."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    rc = lib().emu_hv_stacked(*[a.encode() for a in args], v.ctypes.data, out.ctypes.data, len(v), cap, work)
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out

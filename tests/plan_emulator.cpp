@@ -164,3 +164,90 @@ extern "C" double emu_sixj(int a, int b, int c, int d, int e, int f) { return ns
 extern "C" int emu_sym_eig(const double* A, int n, int full, double* w, double* Z) {
   return nsx::sym_eig(A, n, full != 0, w, Z) ? 0 : 1;
 }
+
+// Stacked two-stage path: X = V Bhat^T per distinct pair, then W += A sum r X,
+// plus the like-particle part and the direct (large-block) items, all from the
+// plan arrays the stacked kernels read.
+extern "C" int emu_hv_stacked(const char* nus, const char* opa, const char* opb, const char* intf,
+                              const double* vin, double* vout, int64_t n, int cap, int work) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  p.cfg.stacked = 1;
+  p.cfg.stk_cap = cap;
+  p.cfg.stk_work = work;
+  plan_add(p, m);
+  std::vector<const Model*> ms{&m};
+  build_stacked(p, ms);
+  std::vector<double> X(p.x_size, 0.0);
+  for (const SChunk1& c : p.s_chunk1) {
+    if (c.lds > p.s_lds1) { g_err = "chunk LDS above the plan maximum"; return -3; }
+    for (int u = c.u_begin; u < c.u_end; ++u) {
+      const SUnit1& U = p.s_unit1[u];
+      for (int j = 0; j < c.db; ++j)
+        for (int i = 0; i < U.dar; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < c.dbr; ++k) s += vin[U.v_off + i + int64_t(U.dar) * k] * p.Bh[c.b_off + k + int64_t(c.dbr) * j];
+          X[U.x_off + i + int64_t(U.dar) * j] = s;
+        }
+    }
+  }
+  std::fill(vout, vout + n, 0.0);
+  // like-particle part (reduce kernel without partials)
+  for (const DBlock& b : p.blocks)
+    for (int jj = 0; jj < b.nc; ++jj) {
+      const int j = b.j0 + jj;
+      for (int i = 0; i < b.da; ++i) {
+        double w = 0.0;
+        if (b.ha_off >= 0)
+          for (int k = 0; k < b.da; ++k) w += p.H[b.ha_off + i + int64_t(b.da) * k] * vin[b.v_off + k + int64_t(b.da) * j];
+        if (b.hb_off >= 0)
+          for (int k = 0; k < b.db; ++k) w += vin[b.v_off + i + int64_t(b.da) * k] * p.H[b.hb_off + j + int64_t(b.db) * k];
+        vout[b.v_off + i + int64_t(b.da) * j] = w;
+      }
+    }
+  for (const SChunk2& c : p.s_chunk2) {
+    if (c.lds > p.s_lds2) { g_err = "chunk LDS above the plan maximum"; return -3; }
+    std::vector<double> Wc(size_t(c.da) * c.db, 0.0);
+    for (int t = c.t_begin; t < c.t_end; ++t) {
+      const STerm2& T = p.s_term2[t];
+      std::vector<double> Y(size_t(T.dar) * c.db, 0.0);
+      for (int x = T.x_begin; x < T.x_end; ++x)
+        for (size_t q = 0; q < Y.size(); ++q) Y[q] += p.s_xref[x].r * X[p.s_xref[x].x_off + q];
+      for (int j = 0; j < c.db; ++j)
+        for (int i = 0; i < c.da; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < T.dar; ++k) s += p.Ad[T.a_off + i + int64_t(c.da) * k] * Y[k + size_t(T.dar) * j];
+          Wc[i + size_t(c.da) * j] += s;
+        }
+    }
+    for (size_t q = 0; q < Wc.size(); ++q) vout[c.w_off + q] += Wc[q];
+  }
+  // direct items (large blocks) add into W
+  for (const DItem& it : p.items)
+    for (int jj = 0; jj < it.nc; ++jj) {
+      const int j = it.j0 + jj;
+      std::vector<double> w(it.da, 0.0);
+      for (int a = it.a_begin; a < it.a_end; ++a) {
+        const DATerm& T = p.aterms[a];
+        std::vector<double> y(it.dar, 0.0);
+        for (int u = T.r_begin; u < T.r_end; ++u) {
+          const DRTerm& R = p.rterms[u];
+          for (int i = 0; i < it.dar; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < R.dbr; ++k) s += vin[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
+            y[i] += R.r * s;
+          }
+        }
+        for (int i = 0; i < it.da; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < it.dar; ++k) s += p.Ad[T.a_off + i + int64_t(it.da) * k] * y[k];
+          w[i] += s;
+        }
+      }
+      for (int i = 0; i < it.da; ++i) vout[it.w_off + i + int64_t(it.da) * j] += w[i];
+    }
+  return 0;
+}

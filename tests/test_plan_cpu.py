@@ -42,3 +42,19 @@ def test_plan_merge_equals_single_pass(models):
         ref = r.apply(v[off:off + r.N])
         assert np.abs(w2[off:off + r.N] - ref).max() <= 1e-12 * np.abs(ref).max()
         off += r.N
+
+
+def test_stacked_plan_matches_refcpu(models):
+    """The stacked two-stage plan (shared X = V Bhat^T, then W += A sum r X)
+    gives the oracle's H*v, for several chunk capacities."""
+    from tests.emulator import emu_hv_stacked
+    rng = np.random.default_rng(11)
+    for name, J2s in (("sd22", [0, 4]), ("sd44", [0, 8]), ("sd23", [3])):
+        for J2 in J2s:
+            args = models[name].args(J2)
+            r = RefCPU(*args, fp64=True)
+            v = rng.standard_normal(r.N)
+            ref = r.apply(v)
+            for cap, work in ((512, 64), (2048, 512), (8192, 4096)):
+                w = emu_hv_stacked(args, v, cap, work)
+                assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, cap)
