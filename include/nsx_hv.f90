@@ -26,6 +26,7 @@ module nsx_hv_api
     real(c_double) :: tol = 1.0d-6
     integer(c_int64_t) :: seed = 5
     integer(c_int32_t) :: check_every = 8, graph = 1
+    type(c_ptr) :: checkpoint = c_null_ptr   ! c_loc of a NUL-terminated file name, or null
   end type nsx_trl_opts
 
   type, bind(C) :: nsx_trl_stats

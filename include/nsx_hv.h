@@ -121,6 +121,10 @@ typedef struct {
   uint64_t seed;      /* random start when no start vector is given */
   int32_t check_every; /* Lanczos steps between convergence tests (0 = 8) */
   int32_t graph;      /* 1: replay each block of steps as a captured hipGraph */
+  const char *checkpoint; /* NULL, or a file: the restart state is written there at
+                             every thick restart, and a solve that finds a matching
+                             file resumes from it (NuTRL's .trl resume,
+                             src/NuTRL/NuCoreX.f90:266-292, in FP64) */
 } nsx_trl_opts;
 
 typedef struct {

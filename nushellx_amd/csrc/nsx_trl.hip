@@ -17,6 +17,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <string>
@@ -277,6 +278,59 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   for (int i = 0; i < B; ++i) converged_at[i] = -1;
   int matvecs = 0, restarts = 0, cycle_start = 0;
   const int kk = std::min(keep, m - 1);
+
+  // checkpoint at every thick restart: the restart state (kept Ritz vectors and
+  // residual vector, projected matrices, per-instance flags), FP64, so that a
+  // later solve with the same options resumes exactly where this one stopped
+  const std::string ckpt = opt.checkpoint ? std::string(opt.checkpoint) : std::string();
+  const char kMagic[8] = {'N', 'S', 'X', 'T', 'R', 'L', '0', '1'};
+  auto save_ckpt = [&]() -> int {
+    std::vector<double> hq(size_t(N) * (kk + 1));
+    TRL_HIP(hipMemcpyAsync(hq.data(), Q.p, sizeof(double) * hq.size(), hipMemcpyDeviceToHost, s));
+    TRL_HIP(hipStreamSynchronize(s));
+    const std::string tmp = ckpt + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) { set_error("cannot open " + tmp + " for writing"); return NSX_EIO; }
+    const int32_t hdr[6] = {B, m, kk, nev, matvecs, restarts};
+    const int64_t n64 = N;
+    bool ok = std::fwrite(kMagic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 6, f) == 6 && std::fwrite(&n64, 8, 1, f) == 1 &&
+              std::fwrite(done.data(), 4, B, f) == (size_t)B && std::fwrite(placed.data(), 4, B, f) == (size_t)B &&
+              std::fwrite(converged_at, 4, B, f) == (size_t)B &&
+              std::fwrite(theta_all.data(), 8, theta_all.size(), f) == theta_all.size() &&
+              std::fwrite(res_all.data(), 8, res_all.size(), f) == res_all.size();
+    for (int i = 0; ok && i < B; ++i) ok = std::fwrite(T[i].data(), 8, T[i].size(), f) == T[i].size();
+    ok = ok && std::fwrite(hq.data(), 8, hq.size(), f) == hq.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), ckpt.c_str()) != 0) { set_error("write failed: " + ckpt); return NSX_EIO; }
+    return NSX_OK;
+  };
+  if (!ckpt.empty()) {
+    if (FILE* f = std::fopen(ckpt.c_str(), "rb")) {
+      char mg[8];
+      int32_t hdr[6];
+      int64_t n64 = 0;
+      bool ok = std::fread(mg, 1, 8, f) == 8 && std::memcmp(mg, kMagic, 8) == 0 && std::fread(hdr, 4, 6, f) == 6 &&
+                std::fread(&n64, 8, 1, f) == 1;
+      if (!ok || hdr[0] != B || hdr[1] != m || hdr[2] != kk || hdr[3] != nev || n64 != N) {
+        std::fclose(f);
+        set_error("checkpoint " + ckpt + " does not match this solve (instances, basis, keep, nev or size)");
+        return NSX_EFORMAT;
+      }
+      std::vector<double> hq(size_t(N) * (kk + 1));
+      ok = std::fread(done.data(), 4, B, f) == (size_t)B && std::fread(placed.data(), 4, B, f) == (size_t)B &&
+           std::fread(converged_at, 4, B, f) == (size_t)B &&
+           std::fread(theta_all.data(), 8, theta_all.size(), f) == theta_all.size() &&
+           std::fread(res_all.data(), 8, res_all.size(), f) == res_all.size();
+      for (int i = 0; ok && i < B; ++i) ok = std::fread(T[i].data(), 8, T[i].size(), f) == T[i].size();
+      ok = ok && std::fread(hq.data(), 8, hq.size(), f) == hq.size();
+      std::fclose(f);
+      if (!ok) { set_error("checkpoint " + ckpt + " is truncated"); return NSX_EFORMAT; }
+      TRL_HIP(hipMemcpy(Q.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice));
+      matvecs = hdr[4];
+      restarts = hdr[5];
+      cycle_start = kk;
+    }
+  }
   std::vector<double> ha((m + 1) * B), hb((m + 1) * B);
   std::vector<std::vector<double>> LC(B, std::vector<double>(m + 1, 0.0));  // last eigenvector rows
   double host_s = 0.0;  // wall time of the host projected-problem work
@@ -395,6 +449,10 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
                            hipMemcpyDeviceToDevice, s));
     cycle_start = kk;
     ++restarts;
+    if (!ckpt.empty()) {
+      const int rc = save_ckpt();
+      if (rc != NSX_OK) return rc;
+    }
   }
   for (int i = 0; i < B; ++i)
     for (int q = 0; q < nev; ++q) {

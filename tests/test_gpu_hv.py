@@ -203,3 +203,31 @@ def test_stacked_path_matches_refcpu(models, prec, tol):
         din.free()
         dout.free()
         b.close()
+
+
+def test_trl_checkpoint_resume(models, tmp_path):
+    """A solve cut short by max_iter leaves its last thick-restart state in the
+    checkpoint file; a second solve resumes from it and ends with the eigenpairs
+    and matvec count of an uninterrupted solve.  A checkpoint from another
+    configuration is refused."""
+    from nushellx_amd.hv import Batch, Hamiltonian, NsxError
+    h = Hamiltonian(*models["sd44"].args(0))
+    full = h.lanczos(nev=5, tol=1e-8)
+    ck = str(tmp_path / "sd440.ckpt")
+    cut = h.lanczos(nev=5, tol=1e-8, max_iter=60, checkpoint=ck)
+    assert cut["converged"] == 0 and cut["matvecs"] < full["matvecs"]
+    res = h.lanczos(nev=5, tol=1e-8, checkpoint=ck)
+    assert res["converged"] == 5
+    assert np.abs(res["evals"] - full["evals"]).max() < 1e-8
+    assert abs(res["matvecs"] - full["matvecs"]) <= 8
+    with pytest.raises(NsxError):
+        h.lanczos(nev=5, keep=12, tol=1e-8, checkpoint=ck)
+    # batches checkpoint every instance
+    hams = [Hamiltonian(*models["sd44"].args(J2)) for J2 in (0, 4, 8)]
+    b = Batch(hams)
+    fb = b.lanczos(nev=5, tol=1e-8)
+    ckb = str(tmp_path / "batch.ckpt")
+    b.lanczos(nev=5, tol=1e-8, max_iter=60, checkpoint=ckb)
+    rb = b.lanczos(nev=5, tol=1e-8, checkpoint=ckb)
+    assert (rb["converged_at"] >= 0).all()
+    assert np.abs(rb["evals"] - fb["evals"]).max() < 1e-8
