@@ -27,6 +27,7 @@ module nsx_hv_api
     integer(c_int64_t) :: seed = 5
     integer(c_int32_t) :: check_every = 8, graph = 1
     type(c_ptr) :: checkpoint = c_null_ptr   ! c_loc of a NUL-terminated file name, or null
+    integer(c_int32_t) :: standard = 0       ! 1: standard Lanczos (Lnzs)
   end type nsx_trl_opts
 
   type, bind(C) :: nsx_trl_stats

@@ -125,6 +125,9 @@ typedef struct {
                              every thick restart, and a solve that finds a matching
                              file resumes from it (NuTRL's .trl resume,
                              src/NuTRL/NuCoreX.f90:266-292, in FP64) */
+  int32_t standard;   /* 1: standard Lanczos as Lnzs (src/NuTRL/NuCoreX.f90:595-751):
+                         no restarts (max_basis caps the basis), converged when the
+                         lowest nev+1 Ritz values move by <= tol between tests */
 } nsx_trl_opts;
 
 typedef struct {

@@ -32,7 +32,8 @@ EXPORTS = [
 class TrlOpts(C.Structure):
     _fields_ = [("nev", C.c_int32), ("max_basis", C.c_int32), ("keep", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64),
-                ("check_every", C.c_int32), ("graph", C.c_int32), ("checkpoint", C.c_char_p)]
+                ("check_every", C.c_int32), ("graph", C.c_int32), ("checkpoint", C.c_char_p),
+                ("standard", C.c_int32)]
 
 
 class TuneResult(C.Structure):

@@ -350,6 +350,12 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     return true;
   };
   // record the wanted pairs' values and residuals |beta_last s_{mm-1,q}| after solve()
+  // standard Lanczos (Lnzs, src/NuTRL/NuCoreX.f90:595-751): no restarts, and a
+  // pair counts as converged when its Ritz value moved by <= tol since the
+  // previous test; the lowest min(nev + 1, dimension) values are watched (:719-737)
+  const bool standard = opt.standard != 0;
+  const int nwatch = nev + 1;
+  std::vector<std::vector<double>> prev(B, std::vector<double>(nwatch, NAN));
   auto record = [&](int i, double beta_last) {
     int conv = 0;
     for (int q = 0; q < nev; ++q) {
@@ -357,6 +363,15 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       const double r = std::fabs(beta_last * LC[i][q]);
       res_all[size_t(i) * nev + q] = r;
       conv += r <= tol;
+    }
+    if (standard) {
+      const int nw = std::min(nwatch, mm_of[i]);
+      bool stable = true;
+      for (int q = 0; q < nw; ++q) {
+        stable = stable && std::fabs(Th[i][q] - prev[i][q]) <= tol;  // NaN (no previous test) fails
+        prev[i][q] = Th[i][q];
+      }
+      conv = stable ? nev : 0;
     }
     if (conv == nev && converged_at[i] < 0) converged_at[i] = matvecs;
   };
@@ -406,7 +421,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       for (int i = 0; i < B; ++i) all_conv = all_conv && converged_at[i] >= 0;
       if (all_conv || matvecs >= max_iter) break;
     }
-    if (all_conv || matvecs >= max_iter) { finished = true; break; }
+    if (all_conv || matvecs >= max_iter || standard) { finished = true; break; }  // standard: basis full
 
     // ---- thick restart of every instance: Q[:, 0:kk] = Q S, Q[:, kk] = q_m
     std::vector<double> Sdev(size_t(B) * m * kk, 0.0);

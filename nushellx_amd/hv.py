@@ -122,15 +122,17 @@ class Hamiltonian:
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,
                 max_iter: int = 5000, seed: int = 5, start: Optional[np.ndarray] = None,
                 vectors: bool = False, check_every: int = 8, graph: bool = True,
-                checkpoint: Optional[str] = None):
-        """Thick-restart Lanczos for the lowest ``nev`` states (NuTRL's Lnz_OP).
+                checkpoint: Optional[str] = None, standard: bool = False):
+        """Thick-restart Lanczos for the lowest ``nev`` states (NuTRL's Lnz_OP);
+        ``standard=True``: standard Lanczos without restarts (Lnzs), converged
+        when the lowest nev+1 Ritz values move by <= tol between tests.
 
         Convergence is tested every ``check_every`` steps; with ``graph`` each
         block of steps is captured once as a hipGraph and replayed.  With
         ``checkpoint`` the restart state is saved to that file at every thick
         restart, and a matching file found at the start is resumed."""
         opts = TrlOpts(nev, max_basis, keep, max_iter, tol, seed, check_every, int(graph),
-                       checkpoint.encode() if checkpoint else None)
+                       checkpoint.encode() if checkpoint else None, int(standard))
         ev = np.zeros(nev)
         res = np.zeros(nev)
         vec = np.zeros((nev, self.N)) if vectors else None  # column-major N x nev == row-major nev x N
@@ -209,14 +211,14 @@ class Batch:
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,
                 max_iter: int = 5000, seed: int = 5, start: Optional[np.ndarray] = None,
                 vectors: bool = False, check_every: int = 8, graph: bool = True,
-                checkpoint: Optional[str] = None):
+                checkpoint: Optional[str] = None, standard: bool = False):
         """Lockstep thick-restart Lanczos over every instance of the batch.
 
         ``evecs`` (with ``vectors``) is nev x N_total: row q holds every
         instance's q-th Ritz vector at that instance's offset."""
         n = len(self._hams)
         opts = TrlOpts(nev, max_basis, keep, max_iter, tol, seed, check_every, int(graph),
-                       checkpoint.encode() if checkpoint else None)
+                       checkpoint.encode() if checkpoint else None, int(standard))
         vec = np.zeros((nev, self.N)) if vectors else None
         ev = np.zeros((n, nev))
         res = np.zeros((n, nev))

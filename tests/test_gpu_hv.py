@@ -231,3 +231,17 @@ def test_trl_checkpoint_resume(models, tmp_path):
     rb = b.lanczos(nev=5, tol=1e-8, checkpoint=ckb)
     assert (rb["converged_at"] >= 0).all()
     assert np.abs(rb["evals"] - fb["evals"]).max() < 1e-8
+
+
+def test_standard_lanczos_matches_dense(models):
+    """Standard Lanczos (no restarts, eigenvalue-stability test every 3 steps as
+    Lnzs) reaches the dense eigenvalues; small spaces end exactly."""
+    from nushellx_amd.hv import Hamiltonian
+    for name, J2 in (("sd44", 0), ("sd22", 0)):
+        args = models[name].args(J2)
+        H = RefCPU(*args, fp64=True).dense()
+        ev = np.linalg.eigvalsh(0.5 * (H + H.T))[:5]
+        r = Hamiltonian(*args).lanczos(nev=5, max_basis=300, tol=1e-9, check_every=3, standard=True)
+        print(name, r["matvecs"], r["evals"] - ev, flush=True)
+        assert r["converged"] == 5
+        assert np.abs(r["evals"] - ev).max() < 1e-7
