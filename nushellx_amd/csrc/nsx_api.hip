@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -36,6 +37,8 @@ static std::string up(T*& dst, const std::vector<T>& src) {
 }
 
 std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
+  static std::atomic<uint64_t> generation{0};
+  d.gen = ++generation;
   std::string e;
   if (!(e = up(d.items2, hp.items2)).empty()) return e;
   if (!(e = up(d.phases, hp.phases)).empty()) return e;
@@ -168,6 +171,7 @@ int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfi
 void nsx_hv_destroy(nsx_hv* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_trl_work(h->trl);
   free_plan(h->dplan);
   free_lowp(h->lowp);
   (void)hipFree(h->d_in);
@@ -281,6 +285,7 @@ int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const n
 void nsx_batch_destroy(nsx_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_trl_work(b->trl);
   free_plan(b->dplan);
   free_lowp(b->lowp);
   if (b->stream) (void)hipStreamDestroy(b->stream);

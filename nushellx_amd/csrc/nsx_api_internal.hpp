@@ -10,6 +10,11 @@
 #include "nsx_plan.hpp"
 #include "../../include/nsx_hv.h"
 
+namespace nsx {
+struct TrlWork;  // Lanczos workspace kept between solves (nsx_trl.hip)
+void free_trl_work(TrlWork* w);
+}  // namespace nsx
+
 struct nsx_hv {
   nsx::Model model;
   nsx::HostPlan hplan;
@@ -17,6 +22,7 @@ struct nsx_hv {
   nsx::LowpPlan lowp;
   int precision = 0;
   bool host_only = false;  // no device plan: only usable inside a batch
+  nsx::TrlWork* trl = nullptr;
   hipStream_t stream = nullptr;
   double* d_in = nullptr;
   double* d_out = nullptr;
@@ -28,6 +34,7 @@ struct nsx_batch {
   nsx::DevicePlan dplan;
   nsx::LowpPlan lowp;
   int precision = 0;
+  nsx::TrlWork* trl = nullptr;
   hipStream_t stream = nullptr;
 };
 

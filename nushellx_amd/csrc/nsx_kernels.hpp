@@ -36,6 +36,7 @@ struct DevicePlan {
   SXRef* s_xref = nullptr;
   int n_chunk1 = 0, n_chunk2 = 0, s_lds1 = 1, s_lds2 = 1;
   void* X = nullptr;         // x_size elements of the accumulation type (8 bytes reserved each)
+  uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
 };
 
 // Uploads a host plan; returns "" or an error message.

@@ -117,12 +117,48 @@ __global__ __launch_bounds__(kVT) void k_seg_ritz(double* __restrict__ Y, const 
   }
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() { (void)hipFree(p); }
+// untyped device pointer with typed views
+struct DPtr {
+  void* p;
   template <typename T>
-  T* as() { return static_cast<T*>(p); }
+  T* as() const { return static_cast<T*>(p); }
 };
+
+// Lanczos workspace kept by a handle or batch between solves, so that the
+// captured graphs (which hold its addresses) are replayed by later solves too.
+struct TrlWork {
+  int64_t N = 0;
+  int B = 0, m = 0, keep = 0, nev = 0, prec = -1;
+  uint64_t plan_gen = 0;
+  void *Q = nullptr, *W = nullptr, *Y = nullptr, *inst_of = nullptr, *seg = nullptr, *c1 = nullptr,
+       *c2 = nullptr, *ah = nullptr, *bh = nullptr, *scale = nullptr, *dim = nullptr, *S = nullptr;
+  double *hha = nullptr, *hhb = nullptr;  // pinned alpha / beta histories
+  int* hdim[2] = {nullptr, nullptr};      // pinned dimension flags, one per block parity
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> graphs;
+  void clear_graphs() {
+    for (auto& x : graphs) (void)hipGraphExecDestroy(x.second);
+    graphs.clear();
+  }
+  void release() {
+    clear_graphs();
+    for (void* p : {Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S}) (void)hipFree(p);
+    Q = W = Y = inst_of = seg = c1 = c2 = ah = bh = scale = dim = S = nullptr;
+    (void)hipHostFree(hha);
+    (void)hipHostFree(hhb);
+    (void)hipHostFree(hdim[0]);
+    (void)hipHostFree(hdim[1]);
+    hha = hhb = nullptr;
+    hdim[0] = hdim[1] = nullptr;
+    for (auto& e : ev)
+      if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    N = 0;
+    B = m = keep = nev = 0;
+  }
+  ~TrlWork() { release(); }
+};
+
+void free_trl_work(TrlWork* w) { delete w; }
 
 struct Instance {
   int64_t off, n;
@@ -133,7 +169,7 @@ struct Instance {
 // Lockstep TRL over instances sharing one device plan.  Outputs per instance i:
 // evals[i*nev..], residuals[i*nev..], converged_at[i] (matvecs when all nev
 // pairs first met tol, -1 if never); evecs (optional): N x nev column-major.
-static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_t s,
+static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_t s, TrlWork*& work,
                    const std::vector<Instance>& inst, const nsx_trl_opts& opt, const double* start,
                    double* evals, double* evecs, double* residuals, int32_t* converged_at,
                    nsx_trl_stats* stats) {
@@ -152,25 +188,38 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   const int64_t ldq = N;
   const int ldc = m + 1;
 
-  DevBuf Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S;
-#define TRL_MALLOC(buf, bytes)                                                 \
-  if (hipMalloc(&buf.p, (size_t)(bytes)) != hipSuccess) {                      \
-    set_error("hipMalloc failed (Lanczos workspace)");                         \
-    return NSX_EGPU;                                                           \
+  if (!work) work = new TrlWork();
+  TrlWork& ws = *work;
+  const bool fresh = !(ws.N == N && ws.B == B && ws.m == m && ws.keep == keep && ws.nev == nev);
+  if (fresh) {
+    ws.release();
+    auto dm = [&](void*& p, size_t bytes) { return hipMalloc(&p, bytes) == hipSuccess; };
+    bool ok = dm(ws.Q, sizeof(double) * N * (m + 1)) && dm(ws.W, sizeof(double) * N) &&
+              dm(ws.Y, sizeof(double) * N * std::max(keep, nev)) && dm(ws.inst_of, sizeof(int) * N) &&
+              dm(ws.seg, sizeof(int64_t) * (B + 1)) && dm(ws.c1, sizeof(double) * B * ldc) &&
+              dm(ws.c2, sizeof(double) * B * ldc) && dm(ws.ah, sizeof(double) * (m + 1) * B) &&
+              dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * B) &&
+              dm(ws.dim, sizeof(int) * B) && dm(ws.S, sizeof(double) * (size_t)B * m * std::max(keep, nev)) &&
+              hipHostMalloc((void**)&ws.hha, sizeof(double) * (m + 1) * B) == hipSuccess &&
+              hipHostMalloc((void**)&ws.hhb, sizeof(double) * (m + 1) * B) == hipSuccess &&
+              hipHostMalloc((void**)&ws.hdim[0], sizeof(int) * B) == hipSuccess &&
+              hipHostMalloc((void**)&ws.hdim[1], sizeof(int) * B) == hipSuccess &&
+              hipEventCreateWithFlags(&ws.ev[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ws.ev[1], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      ws.release();
+      set_error("allocation failed (Lanczos workspace)");
+      return NSX_EGPU;
+    }
+    ws.N = N; ws.B = B; ws.m = m; ws.keep = keep; ws.nev = nev;
   }
-  TRL_MALLOC(Q, sizeof(double) * N * (m + 1));
-  TRL_MALLOC(W, sizeof(double) * N);
-  TRL_MALLOC(Y, sizeof(double) * N * std::max(keep, nev));
-  TRL_MALLOC(inst_of, sizeof(int) * N);
-  TRL_MALLOC(seg, sizeof(int64_t) * (B + 1));
-  TRL_MALLOC(c1, sizeof(double) * B * ldc);
-  TRL_MALLOC(c2, sizeof(double) * B * ldc);
-  TRL_MALLOC(ah, sizeof(double) * (m + 1) * B);
-  TRL_MALLOC(bh, sizeof(double) * (m + 1) * B);
-  TRL_MALLOC(scale, sizeof(double) * B);
-  TRL_MALLOC(dim, sizeof(int) * B);
-  TRL_MALLOC(S, sizeof(double) * (size_t)B * m * std::max(keep, nev));
-#undef TRL_MALLOC
+  if (ws.plan_gen != d.gen || ws.prec != prec) {  // graphs hold plan pointers and kernel choices
+    ws.clear_graphs();
+    ws.plan_gen = d.gen;
+    ws.prec = prec;
+  }
+  const DPtr Q{ws.Q}, W{ws.W}, Y{ws.Y}, inst_of{ws.inst_of}, seg{ws.seg}, c1{ws.c1}, c2{ws.c2}, ah{ws.ah},
+      bh{ws.bh}, scale{ws.scale}, dim{ws.dim}, S{ws.S};
 #define TRL_HIP(call)                                                          \
   do {                                                                         \
     hipError_t _e = (call);                                                    \
@@ -243,14 +292,11 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     }
     return hipGetLastError();
   };
-  // captured blocks of steps, keyed by (j0, j1); the pointers they hold stay fixed
-  struct GraphCache {
-    std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> g;
-    ~GraphCache() { for (auto& x : g) (void)hipGraphExecDestroy(x.second); }
-  } graphs;
+  // captured blocks of steps, keyed by (j0, j1), kept in the workspace: the
+  // addresses they hold stay fixed while the workspace and plan do
   auto run_steps = [&](int j0, int j1) -> hipError_t {
     if (!opt.graph) return launch_steps(j0, j1);
-    for (auto& x : graphs.g)
+    for (auto& x : ws.graphs)
       if (x.first == std::make_pair(j0, j1)) return hipGraphLaunch(x.second, s);
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -263,7 +309,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
     if (e != hipSuccess) return e;
-    graphs.g.push_back({{j0, j1}, ge});
+    ws.graphs.push_back({{j0, j1}, ge});
     return hipGraphLaunch(ge, s);
   };
 
@@ -277,6 +323,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   std::vector<std::vector<double>> Sv(B);  // full eigenvectors where computed (mm x mm, col-major)
   for (int i = 0; i < B; ++i) converged_at[i] = -1;
   int matvecs = 0, restarts = 0, cycle_start = 0;
+  int matvecs_run = 0;  // H*v launched, including a block queued past the final test
   const int kk = std::min(keep, m - 1);
 
   // checkpoint at every thick restart: the restart state (kept Ritz vectors and
@@ -326,12 +373,13 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       std::fclose(f);
       if (!ok) { set_error("checkpoint " + ckpt + " is truncated"); return NSX_EFORMAT; }
       TRL_HIP(hipMemcpy(Q.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice));
-      matvecs = hdr[4];
+      matvecs = matvecs_run = hdr[4];
       restarts = hdr[5];
       cycle_start = kk;
     }
   }
-  std::vector<double> ha((m + 1) * B), hb((m + 1) * B);
+  double* const ha = ws.hha;  // pinned: filled by asynchronous copies
+  double* const hb = ws.hhb;
   std::vector<std::vector<double>> LC(B, std::vector<double>(m + 1, 0.0));  // last eigenvector rows
   double host_s = 0.0;  // wall time of the host projected-problem work
 
@@ -381,17 +429,31 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     // ---- one restart cycle: steps [cycle_start, m), tested every `check` steps
     for (int i = 0; i < B; ++i) hdim[i] = done[i] ? 0 : -1;  // 0: frozen, -1: growing
     TRL_HIP(hipMemcpyAsync(dim.p, hdim.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
-    int j1 = cycle_start;
+    // Blocks of `check` steps.  The next block of the cycle is queued before the
+    // host tests the current one, so the projected-problem work on the host
+    // overlaps the GPU; its results only ever extend the basis.
+    auto queue_block = [&](int b0, int b1, int slot) -> int {
+      TRL_HIP(run_steps(b0, b1));
+      TRL_HIP(hipMemcpyAsync(ha + (size_t)b0 * B, ah.as<double>() + (size_t)b0 * B,
+                             sizeof(double) * (b1 - b0) * B, hipMemcpyDeviceToHost, s));
+      TRL_HIP(hipMemcpyAsync(hb + (size_t)b0 * B, bh.as<double>() + (size_t)b0 * B,
+                             sizeof(double) * (b1 - b0) * B, hipMemcpyDeviceToHost, s));
+      TRL_HIP(hipMemcpyAsync(ws.hdim[slot], dim.p, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+      TRL_HIP(hipEventRecord(ws.ev[slot], s));
+      matvecs_run += b1 - b0;
+      return NSX_OK;
+    };
     bool all_conv = false;
-    while (j1 < m) {
-      const int j0 = j1;
-      j1 = std::min(m, j0 + check);
-      TRL_HIP(run_steps(j0, j1));
+    int j0 = cycle_start, j1 = std::min(m, cycle_start + check), slot = 0;
+    if (int rc = queue_block(j0, j1, slot)) return rc;
+    while (true) {
+      const bool spec = j1 < m && matvecs_run < max_iter;
+      const int k0 = j1, k1 = std::min(m, j1 + check);
+      if (spec)
+        if (int rc = queue_block(k0, k1, slot ^ 1)) return rc;
+      TRL_HIP(hipEventSynchronize(ws.ev[slot]));
       matvecs += j1 - j0;
-      TRL_HIP(hipMemcpyAsync(ha.data(), ah.p, sizeof(double) * j1 * B, hipMemcpyDeviceToHost, s));
-      TRL_HIP(hipMemcpyAsync(hb.data(), bh.p, sizeof(double) * j1 * B, hipMemcpyDeviceToHost, s));
-      TRL_HIP(hipMemcpyAsync(hdim.data(), dim.p, sizeof(int) * B, hipMemcpyDeviceToHost, s));
-      TRL_HIP(hipStreamSynchronize(s));
+      const int* hd = ws.hdim[slot];
       for (int x = j0 * B; x < j1 * B; ++x)
         if (!std::isfinite(ha[x]) || !std::isfinite(hb[x])) { set_error("non-finite Lanczos coefficient"); return NSX_EGPU; }
       std::atomic<int> failed(0);
@@ -399,8 +461,8 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       parallel_for(B, [&](int i) {
         if (done[i]) return;
         auto t = [&](int a, int b) -> double& { return T[i][size_t(a) * (m + 1) + b]; };
-        const bool broke = hdim[i] > 0;
-        const int last = broke ? hdim[i] : j1;
+        const bool broke = hd[i] > 0;
+        const int last = broke ? hd[i] : j1;
         // alpha_j on the diagonal; beta_{j-1} couples j-1 and j, also across the
         // boundary of the previous block (not across the restart: the arrow does)
         for (int j = j0; j < last; ++j) {
@@ -419,7 +481,10 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
       all_conv = true;
       for (int i = 0; i < B; ++i) all_conv = all_conv && converged_at[i] >= 0;
-      if (all_conv || matvecs >= max_iter) break;
+      if (all_conv || matvecs >= max_iter || !spec) break;
+      j0 = k0;
+      j1 = k1;
+      slot ^= 1;
     }
     if (all_conv || matvecs >= max_iter || standard) { finished = true; break; }  // standard: basis full
 
@@ -502,7 +567,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   if (stats) {
     int conv = 0;
     for (int i = 0; i < B; ++i) conv += converged_at[i] >= 0;
-    stats->matvecs = matvecs;
+    stats->matvecs = matvecs;  // H*v up to the final test (one queued block may have run past it)
     stats->restarts = restarts;
     stats->converged = conv;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -526,7 +591,7 @@ extern "C" int nsx_trl_solve(nsx_hv* h, const nsx_trl_opts* opt, const double* s
   nsx_trl_opts o = *opt;
   o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, nst);
   int32_t conv_at = -1;
-  int rc = trl_run(h->dplan, h->lowp, h->precision, h->stream, inst, o, start, evals, evecs, residuals,
+  int rc = trl_run(h->dplan, h->lowp, h->precision, h->stream, h->trl, inst, o, start, evals, evecs, residuals,
                    &conv_at, stats);
   if (rc == NSX_OK && stats) stats->converged = conv_at >= 0 ? opt->nev : 0;
   return rc;
@@ -545,6 +610,6 @@ extern "C" int nsx_batch_trl_solve(nsx_batch* b, const nsx_trl_opts* opt, const 
   }
   nsx_trl_opts o = *opt;
   o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, std::max<int64_t>(min_st, o.nev + 2));
-  return trl_run(b->dplan, b->lowp, b->precision, b->stream, inst, o, start, evals, evecs, residuals,
+  return trl_run(b->dplan, b->lowp, b->precision, b->stream, b->trl, inst, o, start, evals, evecs, residuals,
                  converged_at, stats);
 }
