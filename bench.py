@@ -64,11 +64,12 @@ def cpu_baseline(mf, seconds: float = 10.0):
                       f"(REAL*4 build with the reference's 1e-7 product thresholds)"}
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the staged pn kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factor
-    recorded there), or None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_pn_staged_sd44_J0-8_pmc.txt")
+def traffic_from_profile(stacked: bool):
+    """HBM-side bytes per launch of the pn kernels of the tuned path from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
+    copy-kernel factor recorded there), or None when that summary is absent."""
+    name = "k_stk1_k_stk2" if stacked else "k_pn_staged"
+    path = os.path.join(ROOT, "profiles", f"{name}_sd44_J0-8_pmc.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -255,9 +256,11 @@ def main():
     ms_hv, ms_pn, ms_red = batch.time(din, dout, warmup=10, iters=min(args.steps, 1000))
     flops_pn = flops - flops_like
     achieved = flops_pn / (ms_pn * 1e-3) / 1e12
+    stacked = bool(tuned.get("stacked"))
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(),
-                "kernel": "k_pn_staged (pn two-body part, FP64)", "kernel_ms": ms_pn,
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(stacked),
+                "kernel": ("k_stk1 + k_stk2 (stacked two-stage pn path, FP64)" if stacked
+                           else "k_pn_staged (pn two-body part, FP64)"), "kernel_ms": ms_pn,
                 "flops_per_launch": flops_pn,
                 "hbm_GBs_alg": (bytes_alg / (ms_hv * 1e-3)) / 1e9}
 

@@ -366,15 +366,26 @@ __device__ __forceinline__ int seg_of(int e, int n, F start) {
 
 constexpr int kStkBatch = 8;  // loads each thread keeps in flight when staging
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Map them so that XCD x takes the contiguous chunk range [x*per, (x+1)*per):
+// neighbouring chunks share operands, which then stay in one XCD's L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_chunk(int b, int n) {
+  const int per = (n + kXcds - 1) / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_stk1(const SChunk1* __restrict__ chunks, const SUnit1* __restrict__ units,
-                                             const T* __restrict__ V, const T* __restrict__ Bt,
-                                             typename Prec<T>::acc* __restrict__ X) {
+__global__ __launch_bounds__(NT) void k_stk1(const SChunk1* __restrict__ chunks, int n_chunks,
+                                             const SUnit1* __restrict__ units, const T* __restrict__ V,
+                                             const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   __shared__ SUnit1 su[kStkMeta];
-  const SChunk1 c = chunks[blockIdx.x];
+  const int ci = xcd_chunk(blockIdx.x, n_chunks);
+  if (ci >= n_chunks) return;
+  const SChunk1 c = chunks[ci];
   const int nu = c.u_end - c.u_begin;
   const int tid = threadIdx.x;
   for (int t = tid; t < nu; t += NT) su[t] = units[c.u_begin + t];
@@ -442,15 +453,18 @@ __global__ __launch_bounds__(NT) void k_stk1(const SChunk1* __restrict__ chunks,
 static_assert(kStkXref * sizeof(SXRef) >= 256 * kColBlk * sizeof(double),
               "stage-2 split-K partials reuse the X-reference table");
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks, const STerm2* __restrict__ terms,
-                                             const SXRef* __restrict__ xrefs, const T* __restrict__ Ad,
-                                             const typename Prec<T>::acc* __restrict__ X, double* __restrict__ W) {
+__global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks, int n_chunks,
+                                             const STerm2* __restrict__ terms, const SXRef* __restrict__ xrefs,
+                                             const T* __restrict__ Ad, const typename Prec<T>::acc* __restrict__ X,
+                                             double* __restrict__ W) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   __shared__ STerm2 st[kStkMeta];
   __shared__ SXRef sx[kStkXref];
-  const SChunk2 c = chunks[blockIdx.x];
+  const int ci = xcd_chunk(blockIdx.x, n_chunks);
+  if (ci >= n_chunks) return;
+  const SChunk2 c = chunks[ci];
   const int nt = c.t_end - c.t_begin;
   const int tid = threadIdx.x;
   for (int t = tid; t < nt; t += NT) st[t] = terms[c.t_begin + t];
@@ -610,15 +624,18 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     using Acc = typename Prec<T>::acc;
     Acc* X = reinterpret_cast<Acc*>(d.X);
     if (d.n_chunk1 > 0) {
-      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X};
-      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3(d.n_chunk1), dim3(d.stk_nt), a1,
+      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X};
+      const unsigned g1 = (unsigned)(((d.n_chunk1 + kXcds - 1) / kXcds) * kXcds);
+      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3(g1), dim3(d.stk_nt), a1,
                                      sizeof(T) * (size_t)d.s_lds1, s);
       if (e != hipSuccess) return e;
     }
     if (d.n_chunk2 > 0) {
       const Acc* Xc = X;
-      void* a2[] = {(void*)&d.s_chunk2, (void*)&d.s_term2, (void*)&d.s_xref, (void*)&Ad, (void*)&Xc, (void*)&W};
-      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(d.n_chunk2), dim3(d.stk_nt), a2,
+      void* a2[] = {(void*)&d.s_chunk2, (void*)&d.n_chunk2, (void*)&d.s_term2, (void*)&d.s_xref,
+                    (void*)&Ad,         (void*)&Xc,         (void*)&W};
+      const unsigned g2 = (unsigned)(((d.n_chunk2 + kXcds - 1) / kXcds) * kXcds);
+      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2), dim3(d.stk_nt), a2,
                                      sizeof(T) * (size_t)d.s_lds2, s);
       if (e != hipSuccess) return e;
     }

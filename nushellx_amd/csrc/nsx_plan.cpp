@@ -244,7 +244,9 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
   std::map<std::pair<int64_t, int64_t>, int64_t> x_of;        // (b_off, v_off) -> x_off
   std::map<std::array<int64_t, 3>, std::vector<U1>> by_b;     // (b_off, dbr, db) -> pairs
   struct T2 { int64_t a_off; int dar; std::vector<std::pair<int64_t, double>> x; };
-  std::map<std::array<int64_t, 3>, std::vector<T2>> by_w;     // (w_off, da, db) -> a-terms
+  // (instance, 2Jb, 2Ja, w_off, da, db) -> a-terms: left blocks of one instance
+  // with the same Jb read the same X products, so their chunks are kept adjacent
+  std::map<std::array<int64_t, 6>, std::vector<T2>> by_w;
   for (size_t mi = 0; mi < models.size(); ++mi) {
     const Model& m = *models[mi];
     const int64_t vb = p.vbase[mi];
@@ -274,7 +276,7 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
           }
           t.x.push_back({x, rt.r});
         }
-        by_w[{vb + m.posnv[it.L], da, db}].push_back(std::move(t));
+        by_w[{(int64_t)mi, m.bas_b[it.L], m.bas_a[it.L], vb + m.posnv[it.L], da, db}].push_back(std::move(t));
       }
     }
   }
@@ -291,7 +293,7 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
   for (auto& g : by_b)
     for (const U1& u : g.second) cap = std::max(cap, u.dbr * u.db + u.dar * u.dbr);
   for (auto& g : by_w)
-    for (const T2& t : g.second) cap = std::max(cap, (int)(g.first[1] * t.dar + t.dar * g.first[2]));
+    for (const T2& t : g.second) cap = std::max(cap, (int)(g.first[4] * t.dar + t.dar * g.first[5]));
   const int wcap = std::max(p.cfg.stk_work, 1);
   for (auto& g : by_b) {
     const int dbr = (int)g.first[1], db = (int)g.first[2];
@@ -322,11 +324,11 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
     p.s_lds1 = std::max(p.s_lds1, c.lds);
   }
   for (auto& g : by_w) {
-    const int da = (int)g.first[1], db = (int)g.first[2];
+    const int da = (int)g.first[4], db = (int)g.first[5];
     SChunk2 c{};
     auto open = [&]() {
       c = SChunk2{};
-      c.w_off = g.first[0]; c.da = da; c.db = db;
+      c.w_off = g.first[3]; c.da = da; c.db = db;
       c.t_begin = c.t_end = (int)p.s_term2.size();
     };
     open();
