@@ -12,7 +12,8 @@ module nsx_hv_api
   implicit none
   private
   public :: nsx_hv_create, nsx_hv_destroy, nsx_hv_dims, nsx_hv_lcoef, nsx_hv_apply_f32, &
-            nsx_hv_apply_f64, nsx_trl_solve, nsx_last_error, nsx_cstr, nsx_error_message, &
+            nsx_hv_apply_f64, nsx_trl_solve, nsx_hv_write_vectors, nsx_last_error, nsx_cstr, &
+            nsx_error_message, &
             nsx_trl_opts, nsx_trl_stats
 
   integer(c_int), parameter, public :: NSX_OK = 0, NSX_EIO = -1, NSX_EFORMAT = -2, NSX_ETAG = -3, &
@@ -88,6 +89,17 @@ module nsx_hv_api
       type(c_ptr), value :: evecs, residuals
       type(nsx_trl_stats), intent(out) :: stats
     end function nsx_trl_solve
+
+    ! <prefix>.lnz / <prefix>.abz in the layout NuVec reads (vecs: N x nvec, column-major)
+    integer(c_int) function nsx_hv_write_vectors(h, prefix, nvec, evals, vecs, no_level, tt) &
+        bind(C, name='nsx_hv_write_vectors')
+      import :: c_int, c_ptr, c_char, c_int32_t, c_double, c_float
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: prefix
+      integer(c_int32_t), value :: nvec, no_level
+      real(c_double), dimension(*), intent(in) :: evals, vecs
+      real(c_float), value :: tt
+    end function nsx_hv_write_vectors
 
     type(c_ptr) function nsx_last_error() bind(C, name='nsx_last_error')
       import :: c_ptr
