@@ -204,6 +204,49 @@ def ensemble(log, samples: int = 256):
     return out
 
 
+def pf_warm(log, train_samples: int):
+    """Learned warm start on pf 4p4n J=0 (N = 41,355; 195 TBMEs ~ N(0, 1.5 MeV),
+    SPEs ~ U[-10, -4] MeV): network trained on the lowest-8 Lanczos eigenvectors
+    of `train_samples` perturbed interactions, then 8 held-out interactions
+    solved in one lockstep batch from the random and from the predicted start
+    (lowest 5, residual 1e-4 MeV)."""
+    from nushellx_amd.ensemble import build_instances, solve_ensemble, write_ensemble
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import PF, PNSpace
+    from nushellx_amd.warm import WarmStart, build_training_set_batched
+    base = random_interaction(PF, seed=SEED, spe_range=(-10.0, -4.0))
+    space, J2s = PNSpace(PF), [0]
+    work = tempfile.mkdtemp(prefix="nsx_pfw_")
+    t0 = time.perf_counter()
+    F, T, _, _ = build_training_set_batched(space, 4, 4, J2s, base, range(1000, 1000 + train_samples), work,
+                                            log=log)
+    t_data = time.perf_counter() - t0
+    ws = WarmStart(F.shape[1], T.shape[1], seed=0)
+    t0 = time.perf_counter()
+    losses = ws.fit(F, T, epochs=200, batch=16, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    held = list(range(9001, 9009))
+    hams = build_instances(write_ensemble(space, [(4, 4)], J2s, base, held, work + "/held"), J2s)
+    feats = np.array([base.perturbed(s).parameter_vector() for s in held])
+    r0 = solve_ensemble(hams, nev=5, tol=1e-4)
+    b = r0["batch"]
+    t0 = time.perf_counter()
+    v, fell = ws.start_vector(b, feats)
+    rw = b.lanczos(nev=5, tol=1e-4, start=v)
+    t_warm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rr = b.lanczos(nev=5, tol=1e-4)
+    t_rand = time.perf_counter() - t0
+    out = {"workload": f"pf 4p4n J=0, {len(F)} training samples, 8 held-out interactions in one batch",
+           "train_solve_s": t_data, "fit_s": t_fit, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "random": {"s": t_rand, "matvecs": rr["matvecs"], "converged_at": rr["converged_at"].tolist()},
+           "warm": {"s": t_warm, "matvecs": rw["matvecs"], "converged_at": rw["converged_at"].tolist(),
+                    "fallback": fell},
+           "max_eval_diff_MeV": float(np.abs(rw["evals"] - rr["evals"]).max())}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +259,8 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the streaming copy kernel (MiB)")
     ap.add_argument("--ensemble-samples", type=int, default=256,
                     help="interaction samples of the batched ensemble (0 skips it)")
+    ap.add_argument("--pf-warm-samples", type=int, default=0,
+                    help="training samples of the pf 4p4n warm-start study (0 skips it; 512 takes ~8 min)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -299,6 +344,8 @@ def main():
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
     if args.ensemble_samples > 0:
         out["ensemble"] = ensemble(log, args.ensemble_samples)
+    if args.pf_warm_samples > 0:
+        out["pf_warm"] = pf_warm(log, args.pf_warm_samples)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mf, args.cpu_seconds)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

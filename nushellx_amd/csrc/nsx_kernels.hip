@@ -471,30 +471,57 @@ __global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks,
   const int x0 = terms[c.t_begin].x_begin, nx = terms[c.t_end - 1].x_end - x0;
   for (int x = tid; x < nx; x += NT) sx[x] = xrefs[x0 + x];
   __syncthreads();
+  // staging: A elements are plain copies; a Y element sums r X over its term's
+  // references.  The X loads of all kStkBatch elements are issued in rounds of
+  // kStkRefs references before any is used, so one round costs one latency.
+  constexpr int kStkRefs = 4, kStk2Batch = 4;  // register budget: 16 X loads in flight
   const int da = c.da, db = c.db, ntot = c.lds;
-  for (int e0 = tid; e0 < ntot; e0 += NT * kStkBatch) {
-    Acc y[kStkBatch];
-    T a[kStkBatch];
-    bool isa[kStkBatch];
+  for (int e0 = tid; e0 < ntot; e0 += NT * kStk2Batch) {
+    Acc y[kStk2Batch];
+    T a[kStk2Batch];
+    int le[kStk2Batch], xb[kStk2Batch], xe[kStk2Batch];
+    bool isa[kStk2Batch];
+    int nref = 0;
 #pragma unroll
-    for (int q = 0; q < kStkBatch; ++q) {
+    for (int q = 0; q < kStk2Batch; ++q) {
       const int e = e0 + q * NT;
       y[q] = Acc(0);
       isa[q] = false;
+      le[q] = 0;
+      xb[q] = xe[q] = 0;
       if (e < ntot) {
         const int t = seg_of(e, nt, [&](int x) { return st[x].lds; });
-        const int le = e - st[t].lds, na = da * st[t].dar;
-        if (le < na) {
+        const int l = e - st[t].lds, na = da * st[t].dar;
+        if (l < na) {
           isa[q] = true;
-          a[q] = Ad[st[t].a_off + le];
+          a[q] = Ad[st[t].a_off + l];
         } else {
-          for (int x = st[t].x_begin - x0; x < st[t].x_end - x0; ++x)
-            y[q] += Acc(sx[x].r) * X[sx[x].x_off + (le - na)];
+          le[q] = l - na;
+          xb[q] = st[t].x_begin - x0;
+          xe[q] = st[t].x_end - x0;
+          nref = max(nref, xe[q] - xb[q]);
         }
       }
     }
+    for (int r0 = 0; r0 < nref; r0 += kStkRefs) {
+      Acc xv[kStk2Batch][kStkRefs];
 #pragma unroll
-    for (int q = 0; q < kStkBatch; ++q)
+      for (int q = 0; q < kStk2Batch; ++q)
+#pragma unroll
+        for (int u = 0; u < kStkRefs; ++u) {
+          const int x = xb[q] + r0 + u;
+          xv[q][u] = x < xe[q] ? X[sx[x].x_off + le[q]] : Acc(0);
+        }
+#pragma unroll
+      for (int q = 0; q < kStk2Batch; ++q)
+#pragma unroll
+        for (int u = 0; u < kStkRefs; ++u) {
+          const int x = xb[q] + r0 + u;
+          if (x < xe[q]) y[q] += Acc(sx[x].r) * xv[q][u];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kStk2Batch; ++q)
       if (e0 + q * NT < ntot) sm[e0 + q * NT] = isa[q] ? a[q] : Prec<T>::from(y[q]);
   }
   __syncthreads();

@@ -64,43 +64,74 @@ __global__ __launch_bounds__(kVT) void k_seg_sub(double* __restrict__ w, const d
   w[r] -= s;
 }
 
-// per instance: alpha_j = c1[j] + c2[j], beta = ||w||; record both for step j;
-// an instance whose beta collapses (invariant subspace) stops at dimension j+1.
-__global__ __launch_bounds__(kVT) void k_seg_norm(const double* __restrict__ w, const int64_t* __restrict__ seg,
-                                                  const double* __restrict__ c1, const double* __restrict__ c2,
-                                                  int ldc, int j, double* __restrict__ alpha_h,
-                                                  double* __restrict__ beta_h, int B, double* __restrict__ scale,
-                                                  int* __restrict__ dim) {
-  const int i = blockIdx.x;
-  const int64_t r0 = seg[i], r1 = seg[i + 1];
-  double s = 0.0;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kVT) s = fma(w[r], w[r], s);
+// Second Gram-Schmidt pass fused with the norm: w[r] -= sum_k Q[r, k] c[inst(r), k],
+// and per instance sum w^2 accumulated into nrm[inst] (block partial sums, one
+// FP64 atomic add per instance present in the block).
+__global__ __launch_bounds__(kVT) void k_seg_sub_norm(double* __restrict__ w, const double* __restrict__ Q,
+                                                      int64_t ldq, int ncols, const double* __restrict__ c, int ldc,
+                                                      const int* __restrict__ inst, int64_t n,
+                                                      double* __restrict__ nrm) {
   __shared__ double red[kVT];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = kVT / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
+  const int64_t r0 = blockIdx.x * (int64_t)kVT;
+  const int64_t r = r0 + threadIdx.x;
+  double sq = 0.0;
+  int my = -1;
+  if (r < n) {
+    my = inst[r];
+    const double* ci = c + (int64_t)my * ldc;
+    double s = 0.0;
+    for (int k = 0; k < ncols; ++k) s = fma(Q[r + ldq * k], ci[k], s);
+    const double x = w[r] - s;
+    w[r] = x;
+    sq = x * x;
   }
-  if (threadIdx.x == 0) {
-    const double a = c1[(int64_t)i * ldc + j] + c2[(int64_t)i * ldc + j];
-    const double b = sqrt(red[0]);
-    const bool live = dim[i] < 0;  // still growing
-    alpha_h[(int64_t)j * B + i] = a;
-    const bool breakdown = !(b > 1e-10 * fmax(1.0, fabs(a)));
-    if (live && breakdown) dim[i] = j + 1;
-    const bool ok = live && !breakdown;
-    beta_h[(int64_t)j * B + i] = ok ? b : 0.0;
-    scale[i] = ok ? 1.0 / b : 0.0;
+  const int i_lo = inst[r0];
+  const int i_hi = inst[min(r0 + kVT, n) - 1];
+  for (int i = i_lo; i <= i_hi; ++i) {  // instances are contiguous: usually one or two per block
+    red[threadIdx.x] = (my == i) ? sq : 0.0;
+    __syncthreads();
+    for (int o = kVT / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) unsafeAtomicAdd(nrm + i, red[0]);
+    __syncthreads();
   }
 }
 
-// q[r] = w[r] * scale[inst(r)]
-__global__ __launch_bounds__(kVT) void k_seg_scale(double* __restrict__ q, const double* __restrict__ w,
-                                                   const double* __restrict__ scale, const int* __restrict__ inst,
-                                                   int64_t n) {
+// Finishes step j: beta = sqrt(nrm[inst]), alpha = c1[j] + c2[j]; the row that
+// starts an instance records alpha/beta, marks a breakdown (dim[i] = j + 1: an
+// invariant subspace was reached) and clears the other norm slot for step
+// j + 1; every row writes q_{j+1} = w / beta (0 once the instance stopped).
+// A row reading dim after the starting row updated it computes the same zero
+// scale, since the update happens only on breakdown.
+__global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, const double* __restrict__ w,
+                                                    const int64_t* __restrict__ seg, const int* __restrict__ inst,
+                                                    int64_t n, const double* __restrict__ c1,
+                                                    const double* __restrict__ c2, int ldc, int j,
+                                                    const double* __restrict__ nrm, double* __restrict__ nrm_next,
+                                                    double* __restrict__ alpha_h, double* __restrict__ beta_h, int B,
+                                                    int* __restrict__ dim) {
   const int64_t r = blockIdx.x * (int64_t)kVT + threadIdx.x;
-  if (r < n) q[r] = w[r] * scale[inst[r]];
+  if (r >= n) return;
+  const int i = inst[r];
+  const double a = c1[(int64_t)i * ldc + j] + c2[(int64_t)i * ldc + j];
+  const double b = sqrt(nrm[i]);
+  const bool live = dim[i] < 0;
+  const bool breakdown = !(b > 1e-10 * fmax(1.0, fabs(a)));
+  const bool ok = live && !breakdown;
+  if (r == seg[i]) {
+    alpha_h[(int64_t)j * B + i] = a;
+    beta_h[(int64_t)j * B + i] = ok ? b : 0.0;
+    if (live && breakdown) dim[i] = j + 1;
+    nrm_next[i] = 0.0;
+  }
+  q[r] = ok ? w[r] / b : 0.0;
+}
+
+__global__ __launch_bounds__(kVT) void k_fill0(double* __restrict__ x, int n) {
+  const int i = blockIdx.x * kVT + threadIdx.x;
+  if (i < n) x[i] = 0.0;
 }
 
 // Y[r, col] = sum_j Q[r, j] S[inst(r)][j][col], S per instance (mm x kk, column-major)
@@ -198,7 +229,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
               dm(ws.Y, sizeof(double) * N * std::max(keep, nev)) && dm(ws.inst_of, sizeof(int) * N) &&
               dm(ws.seg, sizeof(int64_t) * (B + 1)) && dm(ws.c1, sizeof(double) * B * ldc) &&
               dm(ws.c2, sizeof(double) * B * ldc) && dm(ws.ah, sizeof(double) * (m + 1) * B) &&
-              dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * B) &&
+              dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * 2 * B) &&
               dm(ws.dim, sizeof(int) * B) && dm(ws.S, sizeof(double) * (size_t)B * m * std::max(keep, nev)) &&
               hipHostMalloc((void**)&ws.hha, sizeof(double) * (m + 1) * B) == hipSuccess &&
               hipHostMalloc((void**)&ws.hhb, sizeof(double) * (m + 1) * B) == hipSuccess &&
@@ -273,6 +304,8 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
 
   // Lanczos steps [j0, j1) on the stream: H*v, two Gram-Schmidt passes, normalise
   auto launch_steps = [&](int j0, int j1) -> hipError_t {
+    // norm accumulator of the block's first step (later steps are cleared by k_seg_finish)
+    hipLaunchKernelGGL(k_fill0, dim3((B + kVT - 1) / kVT), dim3(kVT), 0, s, scale.as<double>() + (size_t)(j0 & 1) * B, B);
     for (int j = j0; j < j1; ++j) {
       hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s);
       if (e != hipSuccess) return e;
@@ -282,13 +315,13 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
                          c1.as<double>(), ldc, inst_of.as<int>(), N);
       hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
                          seg.as<int64_t>(), c2.as<double>(), ldc);
-      hipLaunchKernelGGL(k_seg_sub, gv, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
-                         c2.as<double>(), ldc, inst_of.as<int>(), N);
-      hipLaunchKernelGGL(k_seg_norm, dim3(B), dim3(kVT), 0, s, W.as<double>(), seg.as<int64_t>(), c1.as<double>(),
-                         c2.as<double>(), ldc, j, ah.as<double>(), bh.as<double>(), B, scale.as<double>(),
-                         dim.as<int>());
-      hipLaunchKernelGGL(k_seg_scale, gv, dim3(kVT), 0, s, Q.as<double>() + ldq * (j + 1), W.as<double>(),
-                         scale.as<double>(), inst_of.as<int>(), N);
+      double* nrm_j = scale.as<double>() + (size_t)(j & 1) * B;
+      double* nrm_n = scale.as<double>() + (size_t)((j + 1) & 1) * B;
+      hipLaunchKernelGGL(k_seg_sub_norm, gv, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
+                         c2.as<double>(), ldc, inst_of.as<int>(), N, nrm_j);
+      hipLaunchKernelGGL(k_seg_finish, gv, dim3(kVT), 0, s, Q.as<double>() + ldq * (j + 1), W.as<double>(),
+                         seg.as<int64_t>(), inst_of.as<int>(), N, c1.as<double>(), c2.as<double>(), ldc, j, nrm_j,
+                         nrm_n, ah.as<double>(), bh.as<double>(), B, dim.as<int>());
     }
     return hipGetLastError();
   };
