@@ -11,8 +11,9 @@ module nsx_hv_api
   use iso_c_binding
   implicit none
   private
-  public :: nsx_hv_create, nsx_hv_destroy, nsx_hv_dims, nsx_hv_lcoef, nsx_hv_apply_f32, &
-            nsx_hv_apply_f64, nsx_trl_solve, nsx_hv_write_vectors, nsx_last_error, nsx_cstr, &
+  public :: nsx_hv_create, nsx_hv_create_sep, nsx_hv_destroy, nsx_hv_dims, nsx_hv_lcoef, &
+            nsx_hv_apply_f32, nsx_hv_apply_f64, nsx_trl_solve, nsx_hv_write_vectors, &
+            nsx_hv_write_xvc, nsx_hv_expect, nsx_hv_xpct_files, nsx_last_error, nsx_cstr, &
             nsx_error_message, &
             nsx_trl_opts, nsx_trl_stats
 
@@ -44,6 +45,15 @@ module nsx_hv_api
       integer(c_int), value :: precision
       type(c_ptr), intent(out) :: h
     end function nsx_hv_create
+
+    ! NuTRLZ / NuXpct: the pn part from the separable operator <dopfile>.dop
+    integer(c_int) function nsx_hv_create_sep(nus, opfile_a, opfile_b, intfile, dopfile, precision, h) &
+        bind(C, name='nsx_hv_create_sep')
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: nus, opfile_a, opfile_b, intfile, dopfile
+      integer(c_int), value :: precision
+      type(c_ptr), intent(out) :: h
+    end function nsx_hv_create_sep
 
     subroutine nsx_hv_destroy(h) bind(C, name='nsx_hv_destroy')
       import :: c_ptr
@@ -100,6 +110,32 @@ module nsx_hv_api
       real(c_double), dimension(*), intent(in) :: evals, vecs
       real(c_float), value :: tt
     end function nsx_hv_write_vectors
+
+    ! <prefix>.xvc as NuVec writes it (vecs: N x nvec, column-major)
+    integer(c_int) function nsx_hv_write_xvc(h, prefix, nvec, evals, vecs) bind(C, name='nsx_hv_write_xvc')
+      import :: c_int, c_ptr, c_char, c_int32_t, c_double
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: prefix
+      integer(c_int32_t), value :: nvec
+      real(c_double), dimension(*), intent(in) :: evals, vecs
+    end function nsx_hv_write_xvc
+
+    ! M(i,j) = (H v_i) . v_j (NuXpct's matrixs), M: nvec x nvec
+    integer(c_int) function nsx_hv_expect(h, nvec, vecs, m) bind(C, name='nsx_hv_expect')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nvec
+      real(c_double), dimension(*), intent(in) :: vecs
+      real(c_double), dimension(*), intent(out) :: m
+    end function nsx_hv_expect
+
+    ! NuXpct's file step: <prefix>.xvc in, <prefix><operat>.ovl/.ovx out
+    integer(c_int) function nsx_hv_xpct_files(h, prefix, operat, nlev) bind(C, name='nsx_hv_xpct_files')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: prefix, operat
+      integer(c_int32_t), intent(out) :: nlev
+    end function nsx_hv_xpct_files
 
     type(c_ptr) function nsx_last_error() bind(C, name='nsx_last_error')
       import :: c_ptr

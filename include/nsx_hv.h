@@ -48,6 +48,17 @@ typedef struct nsx_hv nsx_hv;
  * Names are used as given after trimming blanks (Fortran blank-padded names). */
 int nsx_hv_create(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
                   const char *intfile, int precision, nsx_hv **h);
+/* The same for program NuTRLZ (src/NuTRLZ/NuCoreXZ.f90), whose pn part is the
+ * separable operator of <dopfile>.dop (the solver's interaction argument, read by
+ * prd() at :1637; written by NuDiop, src/NuDiop/NuDiop.f90:170-280).  The
+ * densities are merged along its eigen-terms (prd, :1613-1887) and the term
+ * tables carry the Racah factor alone (drdbh, :1536-1595); the NuTRLZ zero
+ * limits apply (1e-6 on densities, |W| > 1e-6, :1130,1205,1237).  The like-particle
+ * part and <intfile>.oph/.spe are read as by nsx_hv_create.  The handle works
+ * with every other call (apply, solve, batch, expect).  NuXpct
+ * (src/NuXpct/NuCoreXE.f90) builds its operator the same way. */
+int nsx_hv_create_sep(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
+                      const char *intfile, const char *dopfile, int precision, nsx_hv **h);
 void nsx_hv_destroy(nsx_hv *h);
 
 /* N = no_state, basd = number of (2Ja,2Jb) blocks (src/NuTRL/NuCoreX.f90:975-988). */
@@ -152,6 +163,21 @@ int nsx_trl_solve(nsx_hv *h, const nsx_trl_opts *opt, const double *start, doubl
  * Works on host-only handles too. */
 int nsx_hv_write_vectors(const nsx_hv *h, const char *prefix, int32_t nvec, const double *evals,
                          const double *vecs, int32_t no_level, float tt);
+/* <prefix>.xvc, the state file NuVec writes (src/NuVec/NuVec.f90:166,200,291) and
+ * NuXpct reads: record (no_state, nvec), then per state (energy, vector), REAL*4. */
+int nsx_hv_write_xvc(const nsx_hv *h, const char *prefix, int32_t nvec, const double *evals,
+                     const double *vecs);
+
+/* Matrix elements of the handle's operator between nvec states (vecs: N x nvec,
+ * column-major, FP64): M(i,j) = (H v_i) . v_j, stored M[i + j*nvec], the matrix
+ * NuXpct's matrixs forms (src/NuXpct/NuCoreXE.f90:1441-1448).  H v runs on the
+ * device in the handle's precision; the dots are FP64. */
+int nsx_hv_expect(nsx_hv *h, int32_t nvec, const double *vecs, double *M);
+/* NuXpct's file step (:1431-1470): reads the states of <prefix>.xvc, forms M and
+ * writes <prefix><operat>.ovl (text) and <prefix><operat>.ovx (record nlev, then
+ * one REAL*4 record per column j).  nlev_out (may be NULL) receives the number of
+ * states used: those present, at most the count in the file's header. */
+int nsx_hv_xpct_files(nsx_hv *h, const char *prefix, const char *operat, int32_t *nlev_out);
 
 /* ---------------------------------------------------------------- device helpers
  * For hosts without their own HIP binding (Fortran, ctypes): device buffers,

@@ -136,9 +136,12 @@ extern "C" {
 
 const char* nsx_last_error(void) { return nsx::g_err.c_str(); }
 
-int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfile_b,
-                  const char* intfile, int precision, nsx_hv** out) {
+}  // extern "C"
+
+static int create_impl(const char* nus_prefix, const char* opfile_a, const char* opfile_b,
+                       const char* intfile, const char* dopfile, int precision, nsx_hv** out) {
   if (!out) { set_error("null output handle"); return NSX_EARG; }
+  if (!nus_prefix || !opfile_a || !opfile_b || !intfile) { set_error("null file name"); return NSX_EARG; }
   *out = nullptr;
   const bool host_only = (precision & NSX_HOST_ONLY) != 0;
   precision &= ~NSX_HOST_ONLY;
@@ -147,7 +150,7 @@ int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfi
   h->precision = precision;
   h->host_only = host_only;
   std::string e = build_model(trimmed(nus_prefix), trimmed(opfile_a), trimmed(opfile_b),
-                              trimmed(intfile), h->model);
+                              trimmed(intfile), h->model, dopfile ? trimmed(dopfile) : std::string());
   if (!e.empty()) { set_error(e); return code_of(e); }
   if (host_only) {  // operators stay on the host until a batch plans them
     *out = h.release();
@@ -166,6 +169,23 @@ int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfi
   NSX_HIP(hipMalloc(&h->d_out, sizeof(double) * N));
   *out = h.release();
   return NSX_OK;
+}
+
+extern "C" {
+
+int nsx_hv_create(const char* nus_prefix, const char* opfile_a, const char* opfile_b,
+                  const char* intfile, int precision, nsx_hv** out) {
+  return create_impl(nus_prefix, opfile_a, opfile_b, intfile, nullptr, precision, out);
+}
+
+int nsx_hv_create_sep(const char* nus_prefix, const char* opfile_a, const char* opfile_b,
+                      const char* intfile, const char* dopfile, int precision, nsx_hv** out) {
+  if (!dopfile || trimmed(dopfile).empty()) {
+    if (out) *out = nullptr;
+    set_error("empty .dop name");
+    return NSX_EARG;
+  }
+  return create_impl(nus_prefix, opfile_a, opfile_b, intfile, dopfile, precision, out);
 }
 
 void nsx_hv_destroy(nsx_hv* h) {

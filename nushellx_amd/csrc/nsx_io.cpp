@@ -2,6 +2,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -262,6 +263,98 @@ std::string read_oph(const std::string& intfile, std::vector<OphEntry>& out, int
     if (!r.next(p, len) || len < 16) return intfile + ".oph: truncated";
     std::memcpy(out[k].key, p, 8);
     std::memcpy(&out[k].me, p + 8, 8);
+  }
+  return "";
+}
+
+namespace {
+
+// Fortran list-directed input over a text file held as lines of tokens.
+class ListInput {
+ public:
+  bool open(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    while (std::getline(f, line)) {
+      for (char& c : line)
+        if (c == ',' || c == '\t' || c == '\r') c = ' ';
+      std::istringstream is(line);
+      std::vector<std::string> toks;
+      std::string t;
+      while (is >> t) toks.push_back(t);
+      lines_.push_back(std::move(toks));
+    }
+    return true;
+  }
+  // one READ statement of n values; false at end of file or on a token that is not a number
+  bool read(int n, double* v) {
+    int got = 0;
+    while (got < n) {
+      if (ln_ >= lines_.size()) return false;
+      const auto& toks = lines_[ln_++];
+      for (size_t k = 0; k < toks.size() && got < n; ++k) {
+        std::string t = toks[k];
+        for (char& c : t)
+          if (c == 'd' || c == 'D') c = 'e';
+        char* end = nullptr;
+        v[got++] = std::strtod(t.c_str(), &end);
+        if (end == t.c_str() || *end != 0) return false;
+      }
+    }
+    return true;
+  }
+  size_t line() const { return ln_; }
+
+ private:
+  std::vector<std::vector<std::string>> lines_;
+  size_t ln_ = 0;
+};
+
+}  // namespace
+
+std::string read_dop(const std::string& dopfile, int& max_lmo, std::vector<DopSection>& out) {
+  const std::string path = dopfile + ".dop";
+  ListInput in;
+  if (!in.open(path)) return "cannot open " + path;
+  auto bad = [&](const char* what) { return path + ": " + what + " near line " + std::to_string(in.line()); };
+  double v[5];
+  if (!in.read(1, v)) return bad("no header");
+  max_lmo = (int)v[0];
+  if (max_lmo < 0 || max_lmo > 254) return bad("bad maximum multipole");
+  out.clear();
+  // the reader loops over 2 * (max_lmo/2 + 1) sections (src/NuTRLZ/NuCoreXZ.f90:1638-1641)
+  const int nsec = 2 * (max_lmo / 2 + 1);
+  for (int s = 0; s < nsec; ++s) {
+    DopSection sec;
+    if (!in.read(2, v)) return bad("missing section header");
+    sec.lm = (int)v[0];
+    sec.par = (int)v[1];
+    if (sec.lm < 0 || sec.lm > max_lmo || (sec.lm & 1) || (sec.par != 1 && sec.par != -1))
+      return bad("bad section header");
+    if (!in.read(1, v)) return bad("missing term count");
+    const int nelm = (int)v[0];
+    if (nelm < 0 || nelm > 255) return bad("bad term count");
+    for (int n = 0; n < nelm; ++n) {
+      DopTerm t;
+      if (!in.read(1, v)) return bad("missing eigenvalue");
+      t.e = (double)(float)v[0];  // REAL*4 in the reader (:1617)
+      if (!in.read(1, v)) return bad("missing component count");
+      const int nle = (int)v[0];
+      if (nle < 0) return bad("bad component count");
+      for (int c = 0; c < nle; ++c) {
+        if (!in.read(5, v)) return bad("missing component");
+        DopLine l;
+        for (int q = 0; q < 4; ++q) {
+          l.s[q] = (int)v[q];
+          if (l.s[q] < 1 || l.s[q] > 255) return bad("bad shell index");
+        }
+        l.me = v[4];
+        t.lines.push_back(l);
+      }
+      sec.terms.push_back(std::move(t));
+    }
+    out.push_back(std::move(sec));
   }
   return "";
 }

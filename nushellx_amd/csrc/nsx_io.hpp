@@ -43,12 +43,24 @@ struct SideFiles {
 
 struct OphEntry { uint8_t key[8]; double me; };
 
+// Separable (Dufour-Zuker) pn operator, <name>.dop as NuDiop writes it
+// (src/NuDiop/NuDiop.f90:170-280) and NuTRLZ reads it (src/NuTRLZ/NuCoreXZ.f90:1637-1660):
+// per (2lam, parity) section, eigen-terms e_n with components (proton pair,
+// neutron pair, amplitude), so that the pn multipole interaction is
+// V(pp', nn'; lam) = sum_n e_n u_n(pp') u_n(nn').
+struct DopLine { int s[4]; double me; };        // shells p, p', n, n' (as written), amplitude
+struct DopTerm { double e = 0.0; std::vector<DopLine> lines; };  // e rounded to REAL*4
+struct DopSection { int lm = 0, par = 0; std::vector<DopTerm> terms; };
+
 // Returns an empty string on success, else the error message.
 std::string read_nus(const std::string& prefix, NusInput& out);
 std::string read_side(const std::string& opfile, const NusInput& nus, SideFiles& out);
 std::string read_mats(const std::string& opfile, const NusInput& nus, int side_nmp, char xtra,
                       SideFiles& s);
 std::string read_oph(const std::string& intfile, std::vector<OphEntry>& out, int& max_lmo);
+// Reads <dopfile>.dop with Fortran list-directed rules: every READ starts on a
+// new line, may continue onto the next lines, and skips the rest of its last line.
+std::string read_dop(const std::string& dopfile, int& max_lmo, std::vector<DopSection>& out);
 std::string read_spe(const std::string& intfile, double& renorm, int& no_level);
 
 std::string file_code(int i, int j, int k);

@@ -86,8 +86,174 @@ static std::vector<double> densify(const SideFiles& s, const Density& dd, int J2
   return D;
 }
 
+// Union-mask sum of densities: the dense form of NuTRLZ's coefor merge
+// (src/NuXpct/NuCoreXE.f90:70-138 is the same routine), which adds the values of
+// partition blocks present in both masks and copies the others.
+struct Merged {
+  std::vector<double> D;         // d(Jl) x d(Jr), column-major
+  std::vector<uint32_t> mask;    // union of the contributing masks
+  int64_t count = 0;             // stored values in the union layout
+};
+
+static void merge_into(Merged& g, const SideFiles& s, const Density& dd, int J2l, int J2r, double scale) {
+  const std::vector<double> D = densify(s, dd, J2l, J2r);
+  if (g.D.empty()) {
+    g.D.assign(D.size(), 0.0);
+    g.mask.assign(dd.mask.size(), 0u);
+    for (size_t q = 0; q < D.size(); ++q) g.D[q] = scale * D[q];
+  } else {
+    for (size_t q = 0; q < D.size(); ++q) g.D[q] = g.D[q] + scale * D[q];
+  }
+  for (size_t w = 0; w < dd.mask.size(); ++w) g.mask[w] |= dd.mask[w];
+  g.count = 0;
+  for (int j = 0; j < s.nsp; ++j)
+    for (int i = 0; i < s.nsp; ++i)
+      if (bit(g.mask, s.words, j, i)) g.count += int64_t(s.n(J2r, j)) * s.n(J2l, i);
+}
+
+// pn part of NuTRLZ: prd() merges the densities along the .dop eigen-terms
+// (src/NuTRLZ/NuCoreXZ.f90:1613-1887) and drdbh() builds the term tables over
+// (lam, parity, n) with the Racah factor alone (:1536-1595).
+static std::string build_separable(Model& m, const std::string& dopfile, char typea,
+                                   const std::map<std::array<int, 2>, int>& bidx) {
+  const SideFiles& A = m.A;
+  const SideFiles& B = m.B;
+  int max_lmo = 0;
+  std::vector<DopSection> secs;
+  std::string err = read_dop(dopfile, max_lmo, secs);
+  if (!err.empty()) return err;
+  const int totlms = max_lmo / 2 + 1;
+  int max_elm = 0;
+  for (auto& sc : secs) max_elm = std::max(max_elm, (int)sc.terms.size());
+  // lmel(lam index, n, parity) = 2lam of every eigen-term present (:1778)
+  std::vector<int> lmel(size_t(totlms) * (max_elm + 1) * 2, -1);
+  auto lmel_at = [&](int t, int n, int pp) -> int& { return lmel[(size_t(t) * (max_elm + 1) + n) * 2 + pp]; };
+  for (auto& sc : secs)
+    for (int n = 1; n <= (int)sc.terms.size(); ++n) lmel_at(sc.lm / 2, n, (sc.par + 3) / 2 - 1) = sc.lm;
+
+  using Key = std::array<int, 5>;  // (2Jl, 2Jr, n, parity slot, 2lam)
+  std::map<Key, Merged> amerg, bmerg;
+  for (int pass = 0; pass < 2; ++pass) {
+    const SideFiles& S = pass ? B : A;
+    auto& out = pass ? bmerg : amerg;
+    for (int jl = S.minJ2; jl <= S.maxJ2; jl += S.delJ2)
+      for (int jr = S.minJ2; jr <= S.maxJ2; jr += S.delJ2)
+        for (auto& sc : secs) {
+          const int lm = sc.lm;
+          if (!(jr >= std::abs(jl - lm) && jr <= jl + lm)) continue;
+          const int pp = (sc.par + 3) / 2 - 1;
+          for (int n = 1; n <= (int)sc.terms.size(); ++n) {
+            const DopTerm& t = sc.terms[n - 1];
+            for (const DopLine& l : t.lines) {
+              // proton pair first on a line (:1645-1646)
+              const bool a_first = typea == 'p';
+              const int s1 = (a_first != (pass == 1)) ? l.s[0] : l.s[2];
+              const int s2 = (a_first != (pass == 1)) ? l.s[1] : l.s[3];
+              const uint8_t k[8] = {(uint8_t)jl, (uint8_t)jr, (uint8_t)s1, (uint8_t)s2, (uint8_t)lm, 0, 0, 0};
+              const int ds = find_key(S.dens, k);
+              if (ds < 0) continue;
+              // a side: e*renorm*me*trdav (:1678); b side: me*trdbv (:1800)
+              const double scale = pass ? l.me : (t.e * m.renorm) * l.me;
+              merge_into(out[{jl, jr, n, pp, lm}], S, S.dens[ds], jl, jr, scale);
+            }
+          }
+        }
+  }
+  for (auto& kv : amerg) m.nnz_a += kv.second.count;
+  for (auto& kv : bmerg) m.nnz_b += kv.second.count;
+
+  int max_sj2u = 0;
+  for (int s : m.nus.sj2) max_sj2u = std::max(max_sj2u, s);
+  std::map<Key, int> adense_of, bhat_of;
+  std::vector<uint32_t> lbr(B.words);
+  for (int L = 0; L < m.basd; ++L) {
+    const int ajl = m.bas_a[L], bjl = m.bas_b[L];
+    for (int ajr = A.minJ2; ajr <= A.maxJ2; ajr += A.delJ2) {
+      Item it;
+      it.L = L;
+      it.ajr = ajr;
+      for (int t = 0; t < totlms; ++t)
+        for (int pp = 0; pp < 2; ++pp)
+          for (int n = 1; n <= max_elm; ++n) {
+            const int lm = lmel_at(t, n, pp);
+            if (lm < 0) continue;
+            if (!(ajr >= std::abs(ajl - lm) && ajr <= ajl + lm)) continue;
+            auto fa = amerg.find({ajl, ajr, n, pp, lm});
+            if (fa == amerg.end()) continue;
+            ATerm at;
+            at.adens = -1;
+            std::fill(lbr.begin(), lbr.end(), 0u);
+            int bjrn = std::max(std::abs(bjl - lm), std::abs(ajr - m.J2));
+            const int bjrx = std::min(std::min(bjl + lm, ajr + m.J2), B.maxJ2);
+            bjrn += (bjrn - B.minJ2) % B.delJ2;
+            for (int bjr = bjrn; bjr <= bjrx; bjr += B.delJ2) {
+              // the Racah table holds |W| > 1e-6 for 2lam <= 2 max(2j) and Jb' on the b grid (:1115-1133)
+              if (bjr < B.minJ2 || lm > 2 * max_sj2u) continue;
+              const double w = racah_w(ajl, bjl, ajr, bjr, m.J2, lm);
+              if (!(std::fabs(w) > 1e-6)) continue;
+              const double ph = (((ajl + bjr - m.J2 - lm) / 2) & 1) ? -1.0 : 1.0;
+              const double r = ph * w * std::sqrt(double(lm + 1));
+              auto bi = bidx.find({ajr, bjr});
+              if (bi == bidx.end()) return "find error bh tables";
+              if (!(bjr >= std::abs(bjl - lm) && bjr <= bjl + lm)) continue;
+              const Key bk = {bjl, bjr, n, pp, lm};
+              auto fb = bmerg.find(bk);
+              if (fb == bmerg.end()) continue;
+              int slot;
+              auto f = bhat_of.find(bk);
+              if (f == bhat_of.end()) {
+                slot = (int)m.bhat.size();
+                m.bhat.push_back(fb->second.D);
+                m.bhat_J.push_back({bjl, bjr});
+                m.bhat_key.push_back({bjl, bjr, lm, n, pp});
+                bhat_of[bk] = slot;
+              } else {
+                slot = f->second;
+              }
+              at.rt.push_back({bi->second, slot, r});
+              const Merged& mb = fb->second;
+              for (int c = 0; c < B.nsp; ++c)
+                for (int w2 = 0; w2 < B.words; ++w2) lbr[w2] |= mb.mask[size_t(c) * B.words + w2];
+              int dar = 0;
+              const Merged& ma = fa->second;
+              for (int c = 0; c < A.nsp; ++c) {
+                bool nz = false;
+                for (int w2 = 0; w2 < A.words; ++w2) nz |= ma.mask[size_t(c) * A.words + w2] != 0u;
+                if (nz) dar += A.n(ajr, c);
+              }
+              m.flops_ref += 2.0 * dar * mb.count;
+            }
+            if (at.rt.empty()) continue;
+            {
+              int dbl = 0;
+              for (int i = 0; i < B.nsp; ++i)
+                if ((lbr[i >> 5] >> (i & 31)) & 1u) dbl += B.n(bjl, i);
+              m.flops_ref += 2.0 * dbl * fa->second.count;
+            }
+            const Key ak = {ajl, ajr, n, pp, lm};
+            auto fd = adense_of.find(ak);
+            if (fd == adense_of.end()) {
+              at.adens = (int)m.adense.size();
+              m.adense.push_back(fa->second.D);
+              m.adense_J.push_back({ajl, ajr});
+              adense_of[ak] = at.adens;
+            } else {
+              at.adens = fd->second;
+            }
+            m.n_aterms++;
+            m.n_rterms += (int64_t)at.rt.size();
+            m.n_bterms += (int64_t)at.rt.size();
+            it.at.push_back(std::move(at));
+          }
+      if (!it.at.empty()) m.items.push_back(std::move(it));
+    }
+  }
+  return "";
+}
+
 std::string build_model(const std::string& nus_prefix, const std::string& opfile_a,
-                        const std::string& opfile_b, const std::string& intfile, Model& m) {
+                        const std::string& opfile_b, const std::string& intfile, Model& m,
+                        const std::string& dopfile) {
   std::string err;
   if (!(err = read_nus(nus_prefix, m.nus)).empty()) return err;
   const NusInput& nus = m.nus;
@@ -95,9 +261,12 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
     return "truncated (N-hbar-omega window) bases are not supported";
   if (!(err = read_side(opfile_a, nus, m.A)).empty()) return err;
   if (!(err = read_side(opfile_b, nus, m.B)).empty()) return err;
-  // load-time zero limits of the reference (src/NuTRL/NuCoreX.f90:1163,1195)
-  for (double& v : m.A.vals) if (std::fabs(v) < 1e-11) v = 0.0;
-  for (double& v : m.B.vals) if (std::fabs(v) < 1e-10) v = 0.0;
+  // load-time zero limits of the reference (src/NuTRL/NuCoreX.f90:1163,1195;
+  // the separable build uses 1e-6 on both sides, src/NuTRLZ/NuCoreXZ.f90:1205,1237)
+  m.separable = !dopfile.empty();
+  const double lim_a = m.separable ? 1e-6 : 1e-11, lim_b = m.separable ? 1e-6 : 1e-10;
+  for (double& v : m.A.vals) if (std::fabs(v) < lim_a) v = 0.0;
+  for (double& v : m.B.vals) if (std::fabs(v) < lim_b) v = 0.0;
   char typea = m.A.proton ? 'p' : 'n', typeb = m.B.proton ? 'p' : 'n';
   if (typea == typeb) { typea = 'n'; typeb = 'p'; }
   int sa0, sa1, sb0, sb1;
@@ -153,6 +322,9 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
   std::vector<OphEntry> oph;
   int max_lmo = 0;
   if (!(err = read_oph(intfile, oph, max_lmo)).empty()) return err;
+  if (m.separable) return build_separable(m, dopfile, typea, bidx);
+  for (const auto& d : A.dens) m.nnz_a += d.count;
+  for (const auto& d : B.dens) m.nnz_b += d.count;
   std::map<std::array<uint8_t, 8>, double> opn;
   for (auto& e : oph) {
     std::array<uint8_t, 8> k;

@@ -11,6 +11,10 @@
 //
 // with Bhat = sum_{sb,sb'} opmeV(sa,sa',sb,sb',lam) B(Jb,Jb',sb,sb',lam) folded once
 // (the b-density list of a drdbh entry depends only on (Jb,Jb',lam,sa,sa')).
+//
+// The separable variant of NuTRLZ (src/NuTRLZ/NuCoreXZ.f90:1495-1887) has the same
+// shape with k = (Ja,Ja',lam,parity,n) running over the eigen-terms of a .dop file:
+// A_k = sum_lines e_n renorm u_n(sa,sa') A(sa,sa') and Bhat = sum_lines u_n(sb,sb') B(sb,sb').
 #pragma once
 #include <array>
 #include <cstdint>
@@ -44,6 +48,7 @@ struct Model {
   double renorm = 1.0;
   int64_t N = 0;
   bool azero = false, bzero = false;
+  bool separable = false;                  // pn part from a .dop file (NuTRLZ)
   std::vector<int> bas_a, bas_b, da, db;   // per block: 2Ja, 2Jb, d(Ja), d(Jb)
   std::vector<int64_t> posnv;
   std::vector<uint8_t> lcoef;              // parity mask of the basis states
@@ -54,7 +59,8 @@ struct Model {
   // folded b-operators d(Jb) x d(Jb'), column-major
   std::vector<std::vector<double>> bhat;
   std::vector<std::array<int, 2>> bhat_J;    // (2Jb, 2Jb')
-  std::vector<std::array<int, 5>> bhat_key;  // (2Jb, 2Jb', 2lam, sa, sa')
+  std::vector<std::array<int, 5>> bhat_key;  // (2Jb, 2Jb', 2lam, sa, sa') or (2Jb, 2Jb', 2lam, n, parity)
+  int64_t nnz_a = 0, nnz_b = 0;              // operator values the reference stores per side
   // algorithmic (reference sparse) work of one H*v, and term counts
   double flops_ref = 0.0;               // all terms
   double flops_like = 0.0;              // H_a V + V H_b^T part of flops_ref
@@ -62,8 +68,10 @@ struct Model {
 };
 
 // Reads every file of one NuTRL run and builds the model.  Returns "" on success.
+// With a non-empty `dopfile` the pn part comes from <dopfile>.dop as in NuTRLZ.
 std::string build_model(const std::string& nus_prefix, const std::string& opfile_a,
-                        const std::string& opfile_b, const std::string& intfile, Model& m);
+                        const std::string& opfile_b, const std::string& intfile, Model& m,
+                        const std::string& dopfile = "");
 
 // Restatement of the reference's float-bisection key search
 // (src/NuCommonX/NuCommonX.f90:147-183); 1-based index or 0.

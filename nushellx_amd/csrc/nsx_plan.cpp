@@ -230,10 +230,7 @@ void plan_add(HostPlan& p, const Model& m) {
   p.flops_ref += m.flops_ref;
   p.flops_like += m.flops_like;
   // algorithmic bytes (FP64): read v, write w, read every operator value once
-  int64_t nnzA = 0, nnzB = 0;
-  for (const auto& d : m.A.dens) nnzA += d.count;
-  for (const auto& d : m.B.dens) nnzB += d.count;
-  p.bytes_alg += 16 * m.N + 8 * (nnzA + nnzB + nnz_like + m.n_bterms);
+  p.bytes_alg += 16 * m.N + 8 * (m.nnz_a + m.nnz_b + nnz_like + m.n_bterms);
 }
 
 void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {

@@ -20,16 +20,23 @@ from ._lib import (NSX_FP64, NSX_HOST_ONLY, PRECISIONS, DeviceBuffer, NsxError, 
 
 class Hamiltonian:
     def __init__(self, nus_prefix: str, opfile_a: str, opfile_b: str, intfile: str,
-                 precision: str = "fp64", device: bool = True):
+                 precision: str = "fp64", device: bool = True, dopfile: Optional[str] = None):
         """``device=False`` builds a host-only instance (no device memory), usable
-        only as a member of a Batch."""
+        only as a member of a Batch.  ``dopfile`` takes the pn part from the
+        separable operator ``<dopfile>.dop`` (program NuTRLZ / NuXpct)."""
         L = lib()
         self.precision = precision
         self.device = device
+        self.dopfile = dopfile
         h = C.c_void_p()
         flags = PRECISIONS[precision] | (0 if device else NSX_HOST_ONLY)
-        check(L.nsx_hv_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
-                              intfile.encode(), flags, C.byref(h)), "nsx_hv_create")
+        if dopfile is None:
+            check(L.nsx_hv_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
+                                  intfile.encode(), flags, C.byref(h)), "nsx_hv_create")
+        else:
+            check(L.nsx_hv_create_sep(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
+                                      intfile.encode(), dopfile.encode(), flags, C.byref(h)),
+                  "nsx_hv_create_sep")
         self._h = h
         n, b = C.c_int64(), C.c_int32()
         check(L.nsx_hv_dims(h, C.byref(n), C.byref(b)), "nsx_hv_dims")
@@ -82,6 +89,30 @@ class Hamiltonian:
         assert vecs.shape == (len(evals), self.N)
         check(lib().nsx_hv_write_vectors(self._h, prefix.encode(), len(evals), evals.ctypes.data,
                                          vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_vectors")
+
+    def write_xvc(self, prefix: str, evals, vecs) -> None:
+        """<prefix>.xvc, NuVec's state file (vecs: nvec x N)."""
+        vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+        evals = np.ascontiguousarray(evals, dtype=np.float64)
+        assert vecs.shape == (len(evals), self.N)
+        check(lib().nsx_hv_write_xvc(self._h, prefix.encode(), len(evals), evals.ctypes.data,
+                                     vecs.ctypes.data), "nsx_hv_write_xvc")
+
+    def expect(self, vecs) -> np.ndarray:
+        """M[i, j] = (H v_i) . v_j for the rows v_i of ``vecs`` (NuXpct's matrixs)."""
+        vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+        n = vecs.shape[0]
+        assert vecs.shape[1] == self.N
+        M = np.empty((n, n), dtype=np.float64)  # C order: M[j, i] holds the (i, j) element
+        check(lib().nsx_hv_expect(self._h, n, vecs.ctypes.data, M.ctypes.data), "nsx_hv_expect")
+        return M.T.copy()
+
+    def xpct_files(self, prefix: str, operat: str) -> int:
+        """NuXpct's file step: <prefix>.xvc in, <prefix><operat>.ovl/.ovx out; returns the level count."""
+        n = C.c_int32()
+        check(lib().nsx_hv_xpct_files(self._h, prefix.encode(), operat.encode(), C.byref(n)),
+              "nsx_hv_xpct_files")
+        return int(n.value)
 
     def work(self) -> Tuple[float, float, int]:
         """(flops, like-particle flops, FP64 bytes) of one H*v (algorithmic counts)."""

@@ -181,11 +181,15 @@ def write_nus(path: str, nucleus: str, space: PNSpace, nval: int, nmp: int, J2mi
 
 
 def write_oph(path: str, space: PNSpace, inter: Interaction) -> None:
+    """Records (key, meV, pl, pr) with the pair parities pl = p1 p3 of the a-side
+    pair and pr = p2 p4 of the b-side pair (``src/NuOp/NuOp.f90:378-379``)."""
     items, max_lmo = multipole_pn(space, inter)
     with RecordWriter(path) as w:
         w.write(len(items), max_lmo)
         for key, me in items:
-            w.write(key, np.float64(me), 1, 1)
+            pl = space.parity(key[0]) * space.parity(key[1])
+            pr = space.parity(key[2]) * space.parity(key[3])
+            w.write(key, np.float64(me), pl, pr)
 
 
 def write_spe(path: str, inter: Interaction, renorm: float, no_level: int,

@@ -407,6 +407,335 @@ static int read_nus(nus_t *n, const char *prefix) {
   return 0;
 }
 
+
+/* ---------------------------------------------------------------- separable operator (NuTRLZ) */
+/* .dop reader, list-directed: each READ starts on a new line and skips the rest
+   of its last line (read by prd(), src/NuTRLZ/NuCoreXZ.f90:1637-1660). */
+typedef struct { char **tok; int *ntok; int nl, ln; } ldin_t;
+
+static int ld_open(ldin_t *r, const char *path) {
+  FILE *fp = fopen(path, "r");
+  if (!fp) { set_err("cannot open %s", path); return -1; }
+  int cap = 256;
+  r->tok = (char **)malloc(sizeof(char *) * cap);
+  r->ntok = (int *)malloc(sizeof(int) * cap);
+  r->nl = r->ln = 0;
+  char line[4096];
+  while (fgets(line, sizeof line, fp)) {
+    if (r->nl == cap) {
+      cap *= 2;
+      r->tok = (char **)realloc(r->tok, sizeof(char *) * cap);
+      r->ntok = (int *)realloc(r->ntok, sizeof(int) * cap);
+    }
+    for (char *c = line; *c; c++) if (*c == ',' || *c == '\t' || *c == '\r' || *c == '\n') *c = ' ';
+    r->tok[r->nl] = strdup(line);
+    r->ntok[r->nl] = 0;
+    r->nl++;
+  }
+  fclose(fp);
+  return 0;
+}
+static void ld_close(ldin_t *r) {
+  for (int i = 0; i < r->nl; i++) free(r->tok[i]);
+  free(r->tok); free(r->ntok);
+}
+static int ld_read(ldin_t *r, int n, double *v) {
+  int got = 0;
+  while (got < n) {
+    if (r->ln >= r->nl) return -1;
+    char *p = r->tok[r->ln++];
+    while (got < n) {
+      while (*p == ' ') p++;
+      if (!*p) break;
+      char buf[128];
+      int k = 0;
+      while (*p && *p != ' ' && k < 127) { buf[k++] = (*p == 'd' || *p == 'D') ? 'e' : *p; p++; }
+      buf[k] = 0;
+      char *end;
+      v[got++] = strtod(buf, &end);
+      if (end == buf || *end) return -1;
+    }
+  }
+  return 0;
+}
+
+typedef struct { int s[4]; double me; } dline_t;
+typedef struct { double e; int nl; dline_t *l; } dterm_t;
+typedef struct { int lm, par, nt; dterm_t *t; } dsec_t;
+
+static int read_dop(const char *prefix, int *max_lmo, dsec_t **secs_out, int *nsec_out) {
+  char path[1100];
+  snprintf(path, sizeof path, "%s.dop", prefix);
+  ldin_t r;
+  if (ld_open(&r, path)) return -1;
+  double v[5];
+  int rc = -2;
+  dsec_t *secs = NULL;
+  int nsec = 0;
+  if (ld_read(&r, 1, v)) goto bad;
+  *max_lmo = (int)v[0];
+  nsec = 2 * (*max_lmo / 2 + 1);
+  secs = (dsec_t *)calloc(nsec, sizeof(dsec_t));
+  for (int s = 0; s < nsec; s++) {
+    if (ld_read(&r, 2, v)) goto bad;
+    secs[s].lm = (int)v[0];
+    secs[s].par = (int)v[1];
+    if (ld_read(&r, 1, v)) goto bad;
+    secs[s].nt = (int)v[0];
+    if (secs[s].nt < 0 || secs[s].nt > 255) goto bad;
+    secs[s].t = (dterm_t *)calloc(secs[s].nt + 1, sizeof(dterm_t));
+    for (int n = 0; n < secs[s].nt; n++) {
+      dterm_t *t = &secs[s].t[n];
+      if (ld_read(&r, 1, v)) goto bad;
+      t->e = (double)(float)v[0]; /* real(kind=4):: e (:1617) */
+      if (ld_read(&r, 1, v)) goto bad;
+      t->nl = (int)v[0];
+      if (t->nl < 0) goto bad;
+      t->l = (dline_t *)calloc(t->nl + 1, sizeof(dline_t));
+      for (int c = 0; c < t->nl; c++) {
+        if (ld_read(&r, 5, v)) goto bad;
+        for (int q = 0; q < 4; q++) t->l[c].s[q] = (int)v[q];
+        t->l[c].me = v[4];
+      }
+    }
+  }
+  ld_close(&r);
+  *secs_out = secs;
+  *nsec_out = nsec;
+  return 0;
+bad:
+  set_err("%s: malformed near line %d", path, r.ln);
+  ld_close(&r);
+  if (secs) {
+    for (int s = 0; s < nsec; s++) {
+      if (secs[s].t) for (int n = 0; n < secs[s].nt; n++) free(secs[s].t[n].l);
+      free(secs[s].t);
+    }
+    free(secs);
+  }
+  return rc;
+}
+static void free_dop(dsec_t *secs, int nsec) {
+  for (int s = 0; s < nsec; s++) {
+    for (int n = 0; n < secs[s].nt; n++) free(secs[s].t[n].l);
+    free(secs[s].t);
+  }
+  free(secs);
+}
+
+/* one merged density in the union-mask layout */
+typedef struct {
+  uint8_t key[8];
+  uint32_t *mask;
+  int dim;
+  double *v64;
+  float *v32;
+} mdens_t;
+
+/* coefor (src/NuXpct/NuCoreXE.f90:70-138; NuTRLZ has the same routine): walk the
+   union of the old and new masks in next_ij order (column partition outer, row
+   partition inner) and add, copy new, or copy old, block by block. */
+static void coefor(const side_t *s, int J2l, int J2r, mdens_t *g, const uint32_t *nmask, const double *n64,
+                   const float *n32) {
+  const size_t mw = (size_t)s->nsp * s->words;
+  uint32_t *um = (uint32_t *)calloc(mw, sizeof(uint32_t));
+  for (size_t w = 0; w < mw; w++) um[w] = g->mask[w] | nmask[w];
+  int dim = 0;
+  for (int j = 0; j < s->nsp; j++)
+    for (int i = 0; i < s->nsp; i++)
+      if (bit_of(um + (size_t)j * s->words, i)) dim += nJTs(s, J2r, j) * nJTs(s, J2l, i);
+  double *t64 = (double *)calloc(dim ? dim : 1, sizeof(double));
+  float *t32 = (float *)calloc(dim ? dim : 1, sizeof(float));
+  int po = 0, pn = 0, pon = 0;
+  for (int j = 0; j < s->nsp; j++)
+    for (int i = 0; i < s->nsp; i++) {
+      if (!bit_of(um + (size_t)j * s->words, i)) continue;
+      int d = nJTs(s, J2r, j) * nJTs(s, J2l, i);
+      int in_o = bit_of(g->mask + (size_t)j * s->words, i), in_n = bit_of(nmask + (size_t)j * s->words, i);
+      for (int q = 0; q < d; q++) {
+        if (in_o && in_n) { t64[pon + q] = g->v64[po + q] + n64[pn + q]; t32[pon + q] = g->v32[po + q] + n32[pn + q]; }
+        else if (in_n) { t64[pon + q] = n64[pn + q]; t32[pon + q] = n32[pn + q]; }
+        else { t64[pon + q] = g->v64[po + q]; t32[pon + q] = g->v32[po + q]; }
+      }
+      if (in_o) po += d;
+      if (in_n) pn += d;
+      pon += d;
+    }
+  free(g->mask); free(g->v64); free(g->v32);
+  g->mask = um; g->v64 = t64; g->v32 = t32; g->dim = dim;
+}
+
+static int mkey_cmp(const void *x, const void *y) {
+  return memcmp(((const mdens_t *)x)->key, ((const mdens_t *)y)->key, 8);
+}
+
+/* prd() for one side (a: :1637-1695, b: :1759-1810).  Keys of the merged
+   densities: (2Jl, 2Jr, n, 200+parity slot, 2lam); a side scaled by
+   e*renorm*me, b side by me.  The side's density arrays are replaced. */
+static int prd_side(side_t *s, const dsec_t *secs, int nsec, int first_pair, int scale_e, double renorm) {
+  int cap = 64, nm = 0;
+  mdens_t *md = (mdens_t *)calloc(cap, sizeof(mdens_t));
+  const size_t mw = (size_t)s->nsp * s->words;
+  const float renorm32 = (float)renorm;
+  for (int jl = s->minJ2; jl <= s->maxJ2; jl += s->delJ2)
+    for (int jr = s->minJ2; jr <= s->maxJ2; jr += s->delJ2)
+      for (int sc = 0; sc < nsec; sc++) {
+        const int lm = secs[sc].lm;
+        if (!(jr >= abs(jl - lm) && jr <= jl + lm)) continue;
+        for (int n = 0; n < secs[sc].nt; n++) {
+          const dterm_t *t = &secs[sc].t[n];
+          for (int c = 0; c < t->nl; c++) {
+            const dline_t *l = &t->l[c];
+            int s1 = first_pair ? l->s[0] : l->s[2], s2 = first_pair ? l->s[1] : l->s[3];
+            uint8_t k[8] = {(uint8_t)jl, (uint8_t)jr, (uint8_t)s1, (uint8_t)s2, (uint8_t)lm, 0, 0, 0};
+            int ds = kfind(s->keys, s->ndens, k);
+            if (!ds) continue;
+            ds--;
+            const int dim = s->dim[ds];
+            double *n64 = (double *)malloc(sizeof(double) * (dim ? dim : 1));
+            float *n32 = (float *)malloc(sizeof(float) * (dim ? dim : 1));
+            const double f64 = scale_e ? (t->e * renorm) * l->me : l->me;
+            const float f32 = scale_e ? ((float)t->e * renorm32) * (float)l->me : (float)l->me;
+            for (int q = 0; q < dim; q++) {
+              n64[q] = f64 * s->v64[s->pos[ds] + q];
+              n32[q] = f32 * s->v32[s->pos[ds] + q];
+            }
+            uint8_t mk[8] = {(uint8_t)jl, (uint8_t)jr, (uint8_t)(n + 1), (uint8_t)(200 + (secs[sc].par + 3) / 2),
+                             (uint8_t)lm, 0, 0, 0};
+            int g = -1;
+            for (int q = nm - 1; q >= 0; q--)
+              if (!memcmp(md[q].key, mk, 8)) { g = q; break; }
+            if (g < 0) {
+              if (nm == cap) { cap *= 2; md = (mdens_t *)realloc(md, sizeof(mdens_t) * cap); }
+              g = nm++;
+              memcpy(md[g].key, mk, 8);
+              md[g].mask = (uint32_t *)calloc(mw, sizeof(uint32_t));
+              memcpy(md[g].mask, s->mask + (size_t)ds * mw, sizeof(uint32_t) * mw);
+              md[g].dim = dim;
+              md[g].v64 = n64;
+              md[g].v32 = n32;
+            } else {
+              coefor(s, jl, jr, &md[g], s->mask + (size_t)ds * mw, n64, n32);
+              free(n64);
+              free(n32);
+            }
+          }
+        }
+      }
+  qsort(md, nm, sizeof(mdens_t), mkey_cmp);
+  int64_t total = 0;
+  for (int q = 0; q < nm; q++) total += md[q].dim;
+  free(s->keys); free(s->pos); free(s->dim); free(s->mask); free(s->v64); free(s->v32);
+  s->ndens = nm;
+  s->keys = (uint8_t *)malloc(8 * (size_t)(nm ? nm : 1));
+  s->pos = (int64_t *)calloc(nm + 1, sizeof(int64_t));
+  s->dim = (int *)calloc(nm + 1, sizeof(int));
+  s->mask = (uint32_t *)calloc((size_t)(nm ? nm : 1) * mw, sizeof(uint32_t));
+  s->v64 = (double *)calloc(total ? total : 1, sizeof(double));
+  s->v32 = (float *)calloc(total ? total : 1, sizeof(float));
+  int64_t off = 0;
+  for (int q = 0; q < nm; q++) {
+    memcpy(s->keys + 8 * q, md[q].key, 8);
+    s->pos[q] = off;
+    s->dim[q] = md[q].dim;
+    memcpy(s->mask + (size_t)q * mw, md[q].mask, sizeof(uint32_t) * mw);
+    memcpy(s->v64 + off, md[q].v64, sizeof(double) * md[q].dim);
+    memcpy(s->v32 + off, md[q].v32, sizeof(float) * md[q].dim);
+    off += md[q].dim;
+    free(md[q].mask); free(md[q].v64); free(md[q].v32);
+  }
+  free(md);
+  return 0;
+}
+
+/* drdbh() of NuTRLZ (src/NuTRLZ/NuCoreXZ.f90:1536-1595): per (L, Ja') the merged
+   a-densities over (2lam index, parity, n), each with (merged b-density, R, racah). */
+static int drdbh_sep(refcpu_t *h, const dsec_t *secs, int nsec, int max_lmo, int max_sj2u, int limits) {
+  const int totlms = max_lmo / 2 + 1;
+  int max_elm = 0;
+  for (int s = 0; s < nsec; s++) if (secs[s].nt > max_elm) max_elm = secs[s].nt;
+  int *lmel = (int *)malloc(sizeof(int) * (size_t)totlms * (max_elm + 1) * 2);
+  for (size_t q = 0; q < (size_t)totlms * (max_elm + 1) * 2; q++) lmel[q] = -1;
+#define LMEL(t, n, p) lmel[((size_t)(t) * (max_elm + 1) + (n)) * 2 + (p)]
+  for (int s = 0; s < nsec; s++)
+    for (int n = 1; n <= secs[s].nt; n++) LMEL(secs[s].lm / 2, n, (secs[s].par + 3) / 2 - 1) = secs[s].lm;
+  h->nslot = h->A.maxJ2 / 2 + 1;
+  h->tab = (tab_t *)calloc((size_t)h->basd * h->nslot, sizeof(tab_t));
+  size_t bcap = 1024;
+  h->tb_b = (int *)malloc(sizeof(int) * bcap);
+  h->tb_h = (int *)malloc(sizeof(int) * bcap);
+  h->tb_me64 = (double *)malloc(sizeof(double) * bcap);
+  int64_t nt = 0;
+  for (int bsc = 0; bsc < h->basd; bsc++) {
+    int ajl = h->bas_a[bsc], bjl = h->bas_b[bsc];
+    for (int ajr = h->A.minJ2; ajr <= h->A.maxJ2; ajr += h->A.delJ2) {
+      tab_t *t = &h->tab[bsc * h->nslot + ajr / 2];
+      int acap = 16;
+      t->a = (int *)malloc(sizeof(int) * acap);
+      t->boff = (int *)malloc(sizeof(int) * (acap + 1));
+      t->na = 0;
+      for (int tl = 0; tl < totlms; tl++)
+        for (int pp = 0; pp < 2; pp++)
+          for (int n = 1; n <= max_elm; n++) {
+            int lm = LMEL(tl, n, pp);
+            if (lm == -1) continue;
+            if (!(ajr >= abs(ajl - lm) && ajr <= ajl + lm)) continue;
+            uint8_t ta[8] = {(uint8_t)ajl, (uint8_t)ajr, (uint8_t)n, (uint8_t)(201 + pp), (uint8_t)lm, 0, 0, 0};
+            int atsc = kfind(h->A.keys, h->A.ndens, ta);
+            if (!atsc) continue;
+            if (t->na + 1 >= acap) {
+              acap *= 2;
+              t->a = (int *)realloc(t->a, sizeof(int) * acap);
+              t->boff = (int *)realloc(t->boff, sizeof(int) * (acap + 1));
+            }
+            t->a[t->na] = atsc - 1;
+            t->boff[t->na] = (int)nt;
+            int bjrn = abs(bjl - lm) > abs(ajr - h->J2) ? abs(bjl - lm) : abs(ajr - h->J2);
+            int bjrx = bjl + lm;
+            if (ajr + h->J2 < bjrx) bjrx = ajr + h->J2;
+            if (h->B.maxJ2 < bjrx) bjrx = h->B.maxJ2;
+            bjrn = bjrn + (bjrn - h->B.minJ2) % h->B.delJ2;
+            for (int bjr = bjrn; bjr <= bjrx; bjr += h->B.delJ2) {
+              /* racah table entries: Jb' on the b grid, 2lam <= 2 max(2j), |W| above the limit (:1115-1133) */
+              if (bjr < h->B.minJ2 || lm > 2 * max_sj2u) continue;
+              double rcw = racahW(ajl, bjl, ajr, bjr, h->J2, lm);
+              if (limits ? !(fabs(rcw) > 1e-6) : fabs(rcw) == 0.0) continue;
+              double ph = ((((ajl + bjr - h->J2 - lm) / 2)) & 1) ? -1.0 : 1.0;
+              double rcv = ph * rcw * sqrt((double)(lm + 1));
+              if (!h->fp64) rcv = (double)(float)rcv;
+              int R = -1;
+              for (int q = 0; q < h->basd; q++)
+                if (h->bas_a[q] == ajr && h->bas_b[q] == bjr) { R = q; break; }
+              if (R < 0) { set_err("find error bh tables"); free(lmel); return -5; }
+              if (!(bjr >= abs(bjl - lm) && bjr <= bjl + lm)) continue;
+              uint8_t tb[8] = {(uint8_t)bjl, (uint8_t)bjr, (uint8_t)n, (uint8_t)(201 + pp), (uint8_t)lm, 0, 0, 0};
+              int btsc = kfind(h->B.keys, h->B.ndens, tb);
+              if (!btsc) continue;
+              if ((size_t)nt + 1 >= bcap) {
+                bcap *= 2;
+                h->tb_b = (int *)realloc(h->tb_b, sizeof(int) * bcap);
+                h->tb_h = (int *)realloc(h->tb_h, sizeof(int) * bcap);
+                h->tb_me64 = (double *)realloc(h->tb_me64, sizeof(double) * bcap);
+              }
+              h->tb_b[nt] = btsc - 1;
+              h->tb_h[nt] = R;
+              h->tb_me64[nt] = rcv;
+              nt++;
+            }
+            t->na++;
+            t->boff[t->na] = (int)nt;
+          }
+      if (t->na == 0) t->boff[0] = (int)nt;
+    }
+  }
+#undef LMEL
+  free(lmel);
+  h->ntb = nt;
+  h->tb_me32 = (float *)malloc(sizeof(float) * (nt ? nt : 1));
+  for (int64_t i = 0; i < nt; i++) h->tb_me32[i] = (float)h->tb_me64[i];
+  return 0;
+}
+
 /* ---------------------------------------------------------------- create */
 static int cmp2(const void *x, const void *y) {
   const int *a = (const int *)x, *b = (const int *)y;
@@ -414,8 +743,9 @@ static int cmp2(const void *x, const void *y) {
   return a[1] - b[1];
 }
 
-int refcpu_create(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
-                  const char *intfile, int fp64, int thresholds, refcpu_t **out) {
+static int create_impl(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
+                       const char *intfile, const char *dopfile, int fp64, int thresholds, int limits,
+                       refcpu_t **out) {
   refcpu_t *h = (refcpu_t *)calloc(1, sizeof(refcpu_t));
   nus_t nus;
   int rc;
@@ -431,8 +761,10 @@ int refcpu_create(const char *nus_prefix, const char *opfile_a, const char *opfi
   memcpy(h->sj2, nus.sj2, sizeof(int) * 64);
   if ((rc = read_side(&h->A, opfile_a, nus.sp, nus.no_shells))) goto fail;
   if ((rc = read_side(&h->B, opfile_b, nus.sp, nus.no_shells))) goto fail;
-  zero_small(&h->A, 1e-11);
-  zero_small(&h->B, 1e-10);
+  /* zero limits: NuTRL 1e-11 / 1e-10 (:1163, :1195); NuTRLZ 1e-6 on both sides
+     (src/NuTRLZ/NuCoreXZ.f90:1205,1237), applied when `limits` is set */
+  zero_small(&h->A, dopfile && limits ? 1e-6 : 1e-11);
+  zero_small(&h->B, dopfile && limits ? 1e-6 : 1e-10);
   {
     char typea = h->A.proton ? 'p' : 'n', typeb = h->B.proton ? 'p' : 'n';
     if (typea == typeb) { typea = 'n'; typeb = 'p'; }
@@ -526,6 +858,22 @@ int refcpu_create(const char *nus_prefix, const char *opfile_a, const char *opfi
       opmeV[k] = renorm * v;
     }
     ff_close(&fo);
+    if (dopfile) {
+      /* NuTRLZ: the .oph is read but the pn part comes from the .dop (prd, drdbh) */
+      free(opn);
+      free(opmeV);
+      int dmax = 0, nsec = 0;
+      dsec_t *secs = NULL;
+      if ((rc = read_dop(dopfile, &dmax, &secs, &nsec))) goto fail;
+      int a_first = h->sh_a0 == 1; /* typea == 'p': the proton pair is first on a line */
+      prd_side(&h->A, secs, nsec, a_first, 1, renorm);
+      prd_side(&h->B, secs, nsec, !a_first, 0, renorm);
+      rc = drdbh_sep(h, secs, nsec, dmax, max_sj2u, limits);
+      free_dop(secs, nsec);
+      if (rc) goto fail;
+      *out = h;
+      return 0;
+    }
     /* max_lm (:1320-1326) */
     int max_lambda = nus.del_cJ2;
     if (max_lambda <= 2 || (max_lambda & 1)) max_lambda = 2 * max_sj2u;
@@ -632,6 +980,17 @@ int refcpu_create(const char *nus_prefix, const char *opfile_a, const char *opfi
 fail:
   refcpu_destroy(h);
   return rc ? rc : -1;
+}
+
+int refcpu_create(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
+                  const char *intfile, int fp64, int thresholds, refcpu_t **out) {
+  return create_impl(nus_prefix, opfile_a, opfile_b, intfile, NULL, fp64, thresholds, 1, out);
+}
+
+int refcpu_create_sep(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
+                      const char *intfile, const char *dopfile, int fp64, int thresholds, int limits,
+                      refcpu_t **out) {
+  return create_impl(nus_prefix, opfile_a, opfile_b, intfile, dopfile, fp64, thresholds, limits, out);
 }
 
 int64_t refcpu_dim(const refcpu_t *h) { return h->N; }

@@ -27,6 +27,14 @@ typedef struct refcpu refcpu_t;
 
 int refcpu_create(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
                   const char *intfile, int fp64, int thresholds, refcpu_t **out);
+/* NuTRLZ's separable pn part from <dopfile>.dop: prd() merges the densities along
+ * the eigen-terms (src/NuTRLZ/NuCoreXZ.f90:1613-1887, coefor layout merge) and
+ * drdbh() tabulates them with the Racah factor alone (:1536-1595).  limits = 1
+ * applies NuTRLZ's zero limits (densities 1e-6, |W| > 1e-6); limits = 0 keeps
+ * NuTRL's (1e-11 / 1e-10, W != 0), which isolates the algebra in tests. */
+int refcpu_create_sep(const char *nus_prefix, const char *opfile_a, const char *opfile_b,
+                      const char *intfile, const char *dopfile, int fp64, int thresholds, int limits,
+                      refcpu_t **out);
 void refcpu_destroy(refcpu_t *h);
 int64_t refcpu_dim(const refcpu_t *h);
 int refcpu_basd(const refcpu_t *h);

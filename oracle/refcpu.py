@@ -28,6 +28,7 @@ def lib():
             build()
         L = C.CDLL(path)
         L.refcpu_create.argtypes = [C.c_char_p] * 4 + [C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.refcpu_create_sep.argtypes = [C.c_char_p] * 5 + [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.refcpu_destroy.argtypes = [C.c_void_p]
         L.refcpu_dim.argtypes = [C.c_void_p]
         L.refcpu_dim.restype = C.c_int64
@@ -49,11 +50,19 @@ def lib():
 class RefCPU:
     """The reference H*v restated in C; ``fp64=False`` is the REAL*4 build."""
 
-    def __init__(self, nus_prefix, opfile_a, opfile_b, intfile, fp64=True, thresholds=False):
+    def __init__(self, nus_prefix, opfile_a, opfile_b, intfile, fp64=True, thresholds=False,
+                 dopfile=None, limits=True):
+        """``dopfile``: NuTRLZ's separable pn part from ``<dopfile>.dop``; ``limits``
+        applies its zero limits (False keeps NuTRL's, to compare the algebra)."""
         L = lib()
         h = C.c_void_p()
-        rc = L.refcpu_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
-                             intfile.encode(), int(fp64), int(thresholds), C.byref(h))
+        if dopfile is None:
+            rc = L.refcpu_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
+                                 intfile.encode(), int(fp64), int(thresholds), C.byref(h))
+        else:
+            rc = L.refcpu_create_sep(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
+                                     intfile.encode(), dopfile.encode(), int(fp64), int(thresholds),
+                                     int(limits), C.byref(h))
         if rc != 0:
             raise RuntimeError(f"refcpu_create failed ({rc}): {L.refcpu_last_error().decode()}")
         self._h = h

@@ -17,3 +17,10 @@ def models(tmp_path_factory):
     """Seeded synthetic sd-shell models in NuShellX file format (generated once per session)."""
     from tests.modelgen import make_models
     return make_models(str(tmp_path_factory.mktemp("models")))
+
+
+@pytest.fixture(scope="session")
+def dops(models):
+    """Separable (.dop) operators of the session's models."""
+    from tests.modelgen import make_dops
+    return make_dops(models)

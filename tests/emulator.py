@@ -23,6 +23,7 @@ def lib():
             subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", SO, *SRCS], check=True)
         L = C.CDLL(SO)
         L.emu_hv.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64]
+        L.emu_hv_sep.argtypes = [C.c_char_p] * 5 + [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double)]
         L.emu_plan_stats.argtypes = [C.c_char_p] * 4 + [C.c_void_p]
         L.emu_last_error.restype = C.c_char_p
         L.emu_sixj.argtypes = [C.c_int] * 6
@@ -42,6 +43,18 @@ def emu_hv(args, v):
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out
+
+
+def emu_hv_sep(args, dop, v):
+    """H*v of the separable model (<dop>.dop) through the device plan; returns (w, flops)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    f = C.c_double()
+    rc = lib().emu_hv_sep(*[a.encode() for a in args], dop.encode(), v.ctypes.data, out.ctypes.data, len(v),
+                          C.byref(f))
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out, f.value
 
 
 def plan_stats(args):

@@ -17,4 +17,20 @@ def make_models(outdir: str):
     out["sd44"] = write_model(PNModel(sd, 4, 4, inter), outdir + "/sd44", "sd44", list(range(0, 17, 2)), cache=cache)
     out["sd66"] = write_model(PNModel(sd, 6, 6, inter), outdir + "/sd66", "sd66", [0, 4, 8], cache=cache)
     out["sd23"] = write_model(PNModel(sd, 2, 3, inter), outdir + "/sd23", "sd23", [1, 3, 5], cache=cache)
+    out["sd32"] = write_model(PNModel(sd, 3, 2, inter), outdir + "/sd32", "sd32", [1, 3], cache=cache)
+    return out
+
+
+def make_dops(models):
+    """Separable operators (.dop) of every model's interaction, as NuDiop writes
+    them: "exact" (all terms, 17 digits), "ref" (all terms, NuDiop's f11.7) and
+    "cut" (|e|/sqrt(2lam+1) > 0.3 MeV for lam > 0, amplitudes > 0.05)."""
+    from nushellx_amd.synth.diop import write_model_dop
+    out = {}
+    for name, mf in models.items():
+        out[name] = {
+            "exact": write_model_dop(mf.intfile, mf.intfile + "e", exact=True),
+            "ref": write_model_dop(mf.intfile, mf.intfile + "r"),
+            "cut": write_model_dop(mf.intfile, mf.intfile + "c", cut=0.3, mcut=0.05),
+        }
     return out

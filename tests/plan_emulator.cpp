@@ -111,6 +111,20 @@ extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const c
   return 0;
 }
 
+// The separable (NuTRLZ) model of <dop>.dop through the same plan.
+extern "C" int emu_hv_sep(const char* nus, const char* opa, const char* opb, const char* intf, const char* dop,
+                          const double* vin, double* vout, int64_t n, double* flops) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m, dop);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  plan_add(p, m);
+  emulate(p, vin, vout);
+  if (flops) *flops = m.flops_ref;
+  return 0;
+}
+
 // A batch of instances planned into one plan (merge = 0) or planned one by one
 // and merged (merge = 1); args holds 4 file names per instance.
 extern "C" int emu_hv_batch(int n, const char* const* args, int merge, const double* vin, double* vout,
