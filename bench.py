@@ -247,6 +247,41 @@ def pf_warm(log, train_samples: int):
     return out
 
 
+def separable(log, mf, cuts=((0.0, 0.0), (0.3, 0.05), (1.0, 0.1))):
+    """NuTRLZ's separable pn operator on the same workload: the .dop files NuDiop
+    would write at several cuts (eigenvalue / sqrt(2lam+1) in MeV, amplitude), the
+    batched H*v time of each (autotuned like the headline batch), and how far the
+    lowest five J=0 eigenvalues move from the full operator's."""
+    from nushellx_amd.hv import Batch, DeviceBuffer, Hamiltonian
+    from nushellx_amd.synth.diop import diop_sections, read_oph, write_dop
+    items, max_lmo = read_oph(mf.intfile + ".oph")
+    ref = Hamiltonian(*mf.args(0)).lanczos(nev=5, tol=1e-8)["evals"]
+    out = []
+    for k, (cut, mcut) in enumerate(cuts):
+        secs = diop_sections(items, max_lmo, cut, mcut)
+        dop = mf.intfile + f"d{k}"
+        write_dop(dop + ".dop", secs, max_lmo)
+        hams = [Hamiltonian(*mf.args(J2), dopfile=dop) for J2 in mf.J2s]
+        b = Batch(hams)
+        tuned = b.autotune(iters=5)
+        flops, flops_like, _ = b.work()
+        v = np.random.default_rng(2).standard_normal(b.N)
+        din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+        ms_hv, ms_pn, _ = b.time(din, dout, warmup=10, iters=500)
+        ev = hams[0].lanczos(nev=5, tol=1e-8)["evals"]
+        row = {"cut_MeV": cut, "mcut": mcut, "terms": sum(len(t) for _, _, t in secs),
+               "hv_ms": ms_hv, "pn_ms": ms_pn, "matvecs_per_s": len(hams) / (ms_hv * 1e-3),
+               "flops_pn": flops - flops_like, "pn_TFLOPs": (flops - flops_like) / (ms_pn * 1e-3) / 1e12,
+               "J0_eval_shift_MeV": float(np.abs(ev - ref).max()), "stacked": int(tuned.get("stacked", 0))}
+        log(f"separable cut {cut}/{mcut}: {row['terms']} terms, H*v {ms_hv * 1e3:.1f} us, "
+            f"J=0 eigenvalue shift {row['J0_eval_shift_MeV']:.3g} MeV")
+        out.append(row)
+        din.free()
+        dout.free()
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +294,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the streaming copy kernel (MiB)")
     ap.add_argument("--ensemble-samples", type=int, default=256,
                     help="interaction samples of the batched ensemble (0 skips it)")
+    ap.add_argument("--no-sep", action="store_true", help="skip the separable-operator (NuTRLZ) timings")
     ap.add_argument("--pf-warm-samples", type=int, default=0,
                     help="training samples of the pf 4p4n warm-start study (0 skips it; 512 takes ~8 min)")
     args = ap.parse_args()
@@ -342,6 +378,8 @@ def main():
     batch.close()
     if not args.no_t5:
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
+    if not args.no_sep:
+        out["separable"] = separable(log, mf)
     if args.ensemble_samples > 0:
         out["ensemble"] = ensemble(log, args.ensemble_samples)
     if args.pf_warm_samples > 0:
