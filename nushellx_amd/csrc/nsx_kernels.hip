@@ -280,27 +280,30 @@ __global__ __launch_bounds__(NT) void k_pn_staged(
 // ---------------------------------------------------------------- like-particle / reduce
 // W_L = (sum of the pn partials of L, when use_partials) + H_a V_L + V_L H_b^T.
 // In atomic mode this runs first (use_partials = 0) and the pn kernels add to W.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_reduce(
-    const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
-    const double* __restrict__ part, const T* __restrict__ V, const T* __restrict__ H,
-    double* __restrict__ W, int use_partials) {
+// A whole block whose V_L, H_a and H_b fit in `cap` elements of LDS is staged
+// there (packed: V_L, then H_a, then H_b) and contracted from LDS; otherwise
+// the operands are read from global memory.
+template <typename T, int NT>
+__device__ __forceinline__ void reduce_block(DBlock b, const int64_t* __restrict__ part_list,
+                                             const double* __restrict__ part, const T* __restrict__ V,
+                                             const T* __restrict__ H, double* __restrict__ W, int use_partials,
+                                             T* sm, int cap) {
   using Acc = typename Prec<T>::acc;
-  DBlock b = blocks[blockIdx.x];
   if (!use_partials) b.p_end = b.p_begin;
   const int da = b.da, db = b.db, nW = da * b.nc;
   const T* v = V + b.v_off;
-  if (da <= kSmallMax && db <= kSmallMax && b.j0 == 0 && b.nc == db) {
-    // small block: stage V_L, H_a, H_b in LDS, then contract from LDS
-    __shared__ T sv[kSmallMax * kSmallMax], sha[kSmallMax * kSmallMax], shb[kSmallMax * kSmallMax];
-    const int nv = da * db;
-    for (int e = threadIdx.x; e < nv; e += kThreads) sv[e] = v[e];
+  const int nv = da * db;
+  if (b.j0 == 0 && b.nc == db && nv + da * da + db * db <= cap) {
+    T* sv = sm;
+    T* sha = sm + nv;
+    T* shb = sha + da * da;
+    for (int e = threadIdx.x; e < nv; e += NT) sv[e] = v[e];
     if (b.ha_off >= 0)
-      for (int e = threadIdx.x; e < da * da; e += kThreads) sha[e] = H[b.ha_off + e];
+      for (int e = threadIdx.x; e < da * da; e += NT) sha[e] = H[b.ha_off + e];
     if (b.hb_off >= 0)
-      for (int e = threadIdx.x; e < db * db; e += kThreads) shb[e] = H[b.hb_off + e];
+      for (int e = threadIdx.x; e < db * db; e += NT) shb[e] = H[b.hb_off + e];
     __syncthreads();
-    for (int e = threadIdx.x; e < nv; e += kThreads) {
+    for (int e = threadIdx.x; e < nv; e += NT) {
       const int i = e % da, j = e / da;
       double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
       int p = b.p_begin;
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
     }
     return;
   }
-  for (int e = threadIdx.x; e < nW; e += kThreads) {
+  for (int e = threadIdx.x; e < nW; e += NT) {
     const int i = e % da, j = b.j0 + e / da;
     const int64_t ij = i + (int64_t)da * j;
     double w = 0.0;
@@ -345,6 +348,17 @@ __global__ __launch_bounds__(kThreads) void k_reduce(
     }
     W[b.v_off + ij] = w;
   }
+}
+
+constexpr int kReduceLds = 3 * kSmallMax * kSmallMax;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_reduce(
+    const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
+    const double* __restrict__ part, const T* __restrict__ V, const T* __restrict__ H,
+    double* __restrict__ W, int use_partials) {
+  __shared__ T sm[kReduceLds];
+  reduce_block<T, kThreads>(blocks[blockIdx.x], part_list, part, V, H, W, use_partials, sm, kReduceLds);
 }
 
 // ---------------------------------------------------------------- stacked two-stage path
@@ -375,15 +389,28 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
   return (b % kXcds) * per + b / kXcds;
 }
 
+// Occupancy targets: both stages fit 8 workgroups of LDS per CU (2048-double
+// staging buffers) and, told so, the compiler keeps them within 64 VGPRs
+// without spills (stage 1: 60 instead of 114).
+// The like-particle blocks (W_L = H_a V_L + V_L H_b^T, which initialise W for
+// stage 2's atomics) ride in the same launch: workgroups [0, g_red) take one
+// block each, the rest take stage-1 chunks.  g_red is a multiple of 8, so the
+// XCD mapping of the chunks is unchanged.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_stk1(const SChunk1* __restrict__ chunks, int n_chunks,
-                                             const SUnit1* __restrict__ units, const T* __restrict__ V,
-                                             const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_stk1(
+    const SChunk1* __restrict__ chunks, int n_chunks, const SUnit1* __restrict__ units, const T* __restrict__ V,
+    const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X, const DBlock* __restrict__ blocks,
+    int n_blocks, int g_red, const T* __restrict__ H, double* __restrict__ W, int lds_cap) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   __shared__ SUnit1 su[kStkMeta];
-  const int ci = xcd_chunk(blockIdx.x, n_chunks);
+  if ((int)blockIdx.x < g_red) {
+    if ((int)blockIdx.x < n_blocks)
+      reduce_block<T, NT>(blocks[blockIdx.x], nullptr, nullptr, V, H, W, 0, sm, lds_cap);
+    return;
+  }
+  const int ci = xcd_chunk((int)blockIdx.x - g_red, n_chunks);
   if (ci >= n_chunks) return;
   const SChunk1 c = chunks[ci];
   const int nu = c.u_end - c.u_begin;
@@ -450,17 +477,15 @@ __global__ __launch_bounds__(NT) void k_stk1(const SChunk1* __restrict__ chunks,
 // term's A and Y = sum_u r_u X_u are staged in LDS in one round (X references
 // staged first); each work unit (row i, 4 columns) of W_L sums A_k Y_k over the
 // chunk's terms in registers and is added into W once, with FP64 atomics.
-static_assert(kStkXref * sizeof(SXRef) >= 256 * kColBlk * sizeof(double),
-              "stage-2 split-K partials reuse the X-reference table");
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks, int n_chunks,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_stk2(const SChunk2* __restrict__ chunks, int n_chunks,
                                              const STerm2* __restrict__ terms, const SXRef* __restrict__ xrefs,
                                              const T* __restrict__ Ad, const typename Prec<T>::acc* __restrict__ X,
                                              double* __restrict__ W) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
-  __shared__ STerm2 st[kStkMeta];
+  __shared__ STerm2 st[kStkTerms];
   __shared__ SXRef sx[kStkXref];
   const int ci = xcd_chunk(blockIdx.x, n_chunks);
   if (ci >= n_chunks) return;
@@ -474,7 +499,7 @@ __global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks,
   // staging: A elements are plain copies; a Y element sums r X over its term's
   // references.  The X loads of all kStkBatch elements are issued in rounds of
   // kStkRefs references before any is used, so one round costs one latency.
-  constexpr int kStkRefs = 4, kStk2Batch = 4;  // register budget: 16 X loads in flight
+  constexpr int kStkRefs = 2, kStk2Batch = 4;  // 8 X loads in flight: 60 VGPRs, 8 waves per SIMD
   const int da = c.da, db = c.db, ntot = c.lds;
   for (int e0 = tid; e0 < ntot; e0 += NT * kStk2Batch) {
     Acc y[kStk2Batch];
@@ -526,18 +551,19 @@ __global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks,
   }
   __syncthreads();
   // W_L = [A_1 .. A_T] [Y_1; ..; Y_T]: a long inner dimension over few outputs,
-  // so the terms are split over `nsplit` thread groups (split-K) and the group
-  // partials are summed through LDS (the X-reference table is free by now)
+  // so the terms are split over `nsplit` thread groups (split-K).  With
+  // nsplit > 1 every thread has at most one work unit: its partial waits in
+  // registers until all threads are done with the staged operands, then the
+  // partials take over the staging buffer (which holds NT * kColBlk Acc) and
+  // are summed per unit.
   const int ncb = (db + kColBlk - 1) / kColBlk, units = da * ncb;
   int nsplit = 1;
   while (nsplit < 8 && nsplit * 2 * units <= NT && nsplit * 2 <= nt) nsplit *= 2;
-  Acc* red = reinterpret_cast<Acc*>(sx);  // NT * kColBlk Acc fit in sx
-  for (int w = tid; w < units * nsplit; w += NT) {
-    const int unit = w % units, sp = w / units;
+  // sum over the terms t = sp, sp + nsplit, ... of A_t(i, :) Y_t(:, j0 .. j0+3)
+  auto contract = [&](int unit, int sp, Acc (&acc)[kColBlk]) {
     const int i = unit % da, j0 = (unit / da) * kColBlk;
-    Acc s0[kColBlk], s1[kColBlk];
 #pragma unroll
-    for (int q = 0; q < kColBlk; ++q) { s0[q] = Acc(0); s1[q] = Acc(0); }
+    for (int q = 0; q < kColBlk; ++q) acc[q] = Acc(0);
     for (int t = sp; t < nt; t += nsplit) {
       const int dar = st[t].dar;
       const T* sA = sm + st[t].lds;
@@ -545,43 +571,44 @@ __global__ __launch_bounds__(NT) void k_stk2(const SChunk2* __restrict__ chunks,
       const T* yc[kColBlk];
 #pragma unroll
       for (int q = 0; q < kColBlk; ++q) yc[q] = sY + dar * min(j0 + q, db - 1);
-      int k = 0;
-      for (; k + 1 < dar; k += 2) {
-        const T a0 = sA[i + da * k], a1 = sA[i + da * (k + 1)];
-#pragma unroll
-        for (int q = 0; q < kColBlk; ++q) {
-          s0[q] = Prec<T>::fma3(a0, yc[q][k], s0[q]);
-          s1[q] = Prec<T>::fma3(a1, yc[q][k + 1], s1[q]);
-        }
-      }
-      if (k < dar) {
+      for (int k = 0; k < dar; ++k) {
         const T a0 = sA[i + da * k];
 #pragma unroll
-        for (int q = 0; q < kColBlk; ++q) s0[q] = Prec<T>::fma3(a0, yc[q][k], s0[q]);
+        for (int q = 0; q < kColBlk; ++q) acc[q] = Prec<T>::fma3(a0, yc[q][k], acc[q]);
       }
     }
-    if (nsplit == 1) {
-      double* wp = W + c.w_off + i;
-#pragma unroll
-      for (int q = 0; q < kColBlk; ++q)
-        if (j0 + q < db) unsafeAtomicAdd(wp + (int64_t)da * (j0 + q), (double)(s0[q] + s1[q]));
-    } else {
-#pragma unroll
-      for (int q = 0; q < kColBlk; ++q) red[(size_t)w * kColBlk + q] = s0[q] + s1[q];
-    }
-  }
-  if (nsplit > 1) {
-    __syncthreads();
+  };
+  if (nsplit == 1) {
     for (int unit = tid; unit < units; unit += NT) {
+      Acc acc[kColBlk];
+      contract(unit, 0, acc);
       const int i = unit % da, j0 = (unit / da) * kColBlk;
       double* wp = W + c.w_off + i;
 #pragma unroll
-      for (int q = 0; q < kColBlk; ++q) {
-        if (j0 + q >= db) continue;
-        Acc v = Acc(0);
-        for (int sp = 0; sp < nsplit; ++sp) v += red[(size_t)(sp * units + unit) * kColBlk + q];
-        unsafeAtomicAdd(wp + (int64_t)da * (j0 + q), (double)v);
-      }
+      for (int q = 0; q < kColBlk; ++q)
+        if (j0 + q < db) unsafeAtomicAdd(wp + (int64_t)da * (j0 + q), (double)acc[q]);
+    }
+    return;
+  }
+  const bool active = tid < units * nsplit;
+  Acc acc[kColBlk];
+  if (active) contract(tid % units, tid / units, acc);
+  Acc* red = reinterpret_cast<Acc*>(smem_raw);
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < kColBlk; ++q) red[(size_t)tid * kColBlk + q] = acc[q];
+  }
+  __syncthreads();
+  for (int unit = tid; unit < units; unit += NT) {
+    const int i = unit % da, j0 = (unit / da) * kColBlk;
+    double* wp = W + c.w_off + i;
+#pragma unroll
+    for (int q = 0; q < kColBlk; ++q) {
+      if (j0 + q >= db) continue;
+      Acc v = Acc(0);
+      for (int sp = 0; sp < nsplit; ++sp) v += red[(size_t)(sp * units + unit) * kColBlk + q];
+      unsafeAtomicAdd(wp + (int64_t)da * (j0 + q), (double)v);
     }
   }
 }
@@ -643,18 +670,24 @@ const void* k_pn_partial_ptr(int prec, bool atomic) {
 
 // ---------------------------------------------------------------- launchers
 template <typename T>
-static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, double* W,
+static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H, double* W,
                        hipStream_t s) {
   const bool at = d.atomic != 0;
   if (d.stacked) {  // two-stage path for the small blocks (always accumulates into W)
     if (!at) return hipErrorInvalidConfiguration;
     using Acc = typename Prec<T>::acc;
     Acc* X = reinterpret_cast<Acc*>(d.X);
-    if (d.n_chunk1 > 0) {
-      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X};
-      const unsigned g1 = (unsigned)(((d.n_chunk1 + kXcds - 1) / kXcds) * kXcds);
-      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3(g1), dim3(d.stk_nt), a1,
-                                     sizeof(T) * (size_t)d.s_lds1, s);
+    // stage 1 with the like-particle blocks (the reduce of this H*v)
+    const int g_red = ((d.n_blocks + kXcds - 1) / kXcds) * kXcds;
+    const int g1 = ((d.n_chunk1 + kXcds - 1) / kXcds) * kXcds;
+    if (g_red + g1 > 0) {
+      const T* Hs = H;
+      const int cap = d.s_lds1;
+      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X,
+                    (void*)&d.blocks,   (void*)&d.n_blocks, (void*)&g_red,    (void*)&Hs, (void*)&W,
+                    (void*)&cap};
+      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3((unsigned)(g_red + g1)), dim3(d.stk_nt),
+                                     a1, sizeof(T) * (size_t)d.s_lds1, s);
       if (e != hipSuccess) return e;
     }
     if (d.n_chunk2 > 0) {
@@ -662,8 +695,8 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
       void* a2[] = {(void*)&d.s_chunk2, (void*)&d.n_chunk2, (void*)&d.s_term2, (void*)&d.s_xref,
                     (void*)&Ad,         (void*)&Xc,         (void*)&W};
       const unsigned g2 = (unsigned)(((d.n_chunk2 + kXcds - 1) / kXcds) * kXcds);
-      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2), dim3(d.stk_nt), a2,
-                                     sizeof(T) * (size_t)d.s_lds2, s);
+      const size_t lds2 = std::max(sizeof(T) * (size_t)d.s_lds2, sizeof(Acc) * (size_t)d.stk_nt * kColBlk);
+      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2), dim3(d.stk_nt), a2, lds2, s);
       if (e != hipSuccess) return e;
     }
   } else if (d.n_items2 > 0) {
@@ -688,6 +721,7 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
 
 template <typename T>
 static hipError_t reduce_T(const DevicePlan& d, const T* V, const T* H, double* W, hipStream_t s) {
+  if (d.stacked) return hipSuccess;  // the stacked stage-1 launch carries the like-particle blocks
   if (d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
                        V, H, W, d.atomic ? 0 : 1);
@@ -697,10 +731,13 @@ static hipError_t reduce_T(const DevicePlan& d, const T* V, const T* H, double* 
 hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s) {
   switch (prec) {
-    case 0: return pn_T<double>(d, 0, V, d.Ad, d.Bh, W, s);
-    case 1: return pn_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.Ad, (const float*)lp.Bh, W, s);
+    case 0: return pn_T<double>(d, 0, V, d.Ad, d.Bh, d.H, W, s);
+    case 1:
+      return pn_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.Ad, (const float*)lp.Bh, (const float*)lp.H,
+                         W, s);
     default:
-      return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.Ad, (const Bf16x2*)lp.Bh, W, s);
+      return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.Ad, (const Bf16x2*)lp.Bh,
+                          (const Bf16x2*)lp.H, W, s);
   }
 }
 

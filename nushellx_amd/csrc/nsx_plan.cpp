@@ -332,7 +332,7 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
     for (const T2& t : g.second) {
       const int need = da * t.dar + t.dar * db;
       const int nx = (int)p.s_xref.size() - (c.t_end > c.t_begin ? p.s_term2[c.t_begin].x_begin : 0);
-      if (c.t_end > c.t_begin && (c.lds + need > cap || c.t_end - c.t_begin >= kStkMeta ||
+      if (c.t_end > c.t_begin && (c.lds + need > cap || c.t_end - c.t_begin >= kStkTerms ||
                                   nx + (int)t.x.size() > kStkXref)) {
         p.s_chunk2.push_back(c);
         p.s_lds2 = std::max(p.s_lds2, c.lds);

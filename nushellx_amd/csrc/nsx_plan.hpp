@@ -27,8 +27,9 @@ constexpr int kMaxU = (kSmallMax * ((kSmallMax + kColBlk - 1) / kColBlk) + kThre
 constexpr int kMaxPhases = 6;            // phases per staged workgroup (long items are split)
 constexpr int kFatCap = 5120;            // doubles one phase stages in LDS (V run + Bhat run + A)
 constexpr int kFatPF = kFatCap / kThreads;
-constexpr int kStkMeta = 64;             // stacked path: pairs / terms per chunk
-constexpr int kStkXref = 512;            // stacked path: X references per stage-2 chunk
+constexpr int kStkMeta = 64;             // stacked path: pairs per stage-1 chunk
+constexpr int kStkTerms = 32;            // stacked path: a-terms per stage-2 chunk
+constexpr int kStkXref = 128;            // stacked path: X references per stage-2 chunk
 
 struct DItem {       // one (L, Ja') partial, one column tile
   int64_t part_off;  // partial tile base (full da x db layout of the item)
