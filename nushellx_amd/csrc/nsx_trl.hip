@@ -509,7 +509,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
           done[i] = 1;
           if (converged_at[i] < 0) converged_at[i] = matvecs;
         }
-      });
+      }, 2);
       host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
       if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
       all_conv = true;
@@ -550,7 +550,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
         t(c, kk) = b;
         t(kk, c) = b;
       }
-    });
+    }, 1);  // full eigenvectors of every instance: one thread each
     host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
     if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
     TRL_HIP(hipMemcpyAsync(S.p, Sdev.data(), sizeof(double) * Sdev.size(), hipMemcpyHostToDevice, s));
@@ -588,7 +588,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       if (!done[i] && !solve(i, mm, true)) { failed = 1; return; }  // the last test formed only the last row
       for (int c = 0; c < nev; ++c)
         for (int j = 0; j < mm; ++j) Si[(size_t)c * mmax + j] = Sv[i][size_t(c) * mm + j];
-    });
+    }, 1);
     if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
     TRL_HIP(hipMemcpyAsync(S.p, Sk.data(), sizeof(double) * Sk.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_seg_ritz, gv, dim3(kVT), 0, s, Y.as<double>(), Q.as<double>(), ldq, S.as<double>(), mmax,
