@@ -40,62 +40,87 @@ __global__ __launch_bounds__(kVT) void k_seg_dot(const double* __restrict__ Q, i
   const int i = blockIdx.x, col = col0 + blockIdx.y;
   const int64_t r0 = seg[i], r1 = seg[i + 1];
   const double* q = Q + ldq * col;
-  double s = 0.0;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kVT) s = fma(q[r], w[r], s);
-  __shared__ double red[kVT];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = kVT / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
+  // four independent chains so that four loads of each operand are in flight
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t r = r0 + threadIdx.x;
+  for (; r + 3 * kVT < r1; r += 4 * kVT) {
+    s0 = fma(q[r], w[r], s0);
+    s1 = fma(q[r + kVT], w[r + kVT], s1);
+    s2 = fma(q[r + 2 * kVT], w[r + 2 * kVT], s2);
+    s3 = fma(q[r + 3 * kVT], w[r + 3 * kVT], s3);
   }
-  if (threadIdx.x == 0) c[(int64_t)i * ldc + col] = red[0];
+  for (; r < r1; r += kVT) s0 = fma(q[r], w[r], s0);
+  double s = (s0 + s1) + (s2 + s3);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double red[kVT / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kVT / 64; ++k) t += red[k];
+    c[(int64_t)i * ldc + col] = t;
+  }
 }
 
-// w[r] -= sum_{col < ncols} Q[r, col] c[inst(r), col]
+// Subtraction passes: a workgroup takes 64 rows and its four wavefronts split
+// the columns (wave p sums columns p, p+4, ...), so a thread's chain of loads
+// is a quarter as long and four times as many workgroups share the GPU; the
+// four partial sums of a row meet in LDS.
+constexpr int kSubRows = kVT / 4;
+
+// sum_{col < ncols, col = p mod 4} Q[r, col] c[inst(r), col] of this thread's
+// (row, column class); every thread of the block calls it
+__device__ __forceinline__ double sub_partial(const double* __restrict__ Q, int64_t ldq, int ncols,
+                                              const double* __restrict__ c, int ldc, const int* __restrict__ inst,
+                                              int64_t n, int64_t r, int p) {
+  double s = 0.0;
+  if (r < n) {
+    const double* ci = c + (int64_t)inst[r] * ldc;
+    for (int k = p; k < ncols; k += 4) s = fma(Q[r + ldq * k], ci[k], s);
+  }
+  return s;
+}
+
+// w[r] -= sum_{col < ncols} Q[r, col] c[inst(r), col]; grid ceil(n / 64)
 __global__ __launch_bounds__(kVT) void k_seg_sub(double* __restrict__ w, const double* __restrict__ Q,
                                                  int64_t ldq, int ncols, const double* __restrict__ c, int ldc,
                                                  const int* __restrict__ inst, int64_t n) {
-  const int64_t r = blockIdx.x * (int64_t)kVT + threadIdx.x;
-  if (r >= n) return;
-  const double* ci = c + (int64_t)inst[r] * ldc;
-  double s = 0.0;
-  for (int k = 0; k < ncols; ++k) s = fma(Q[r + ldq * k], ci[k], s);
-  w[r] -= s;
+  __shared__ double part[4][kSubRows];
+  const int lane = threadIdx.x % kSubRows, p = threadIdx.x / kSubRows;
+  const int64_t r = blockIdx.x * (int64_t)kSubRows + lane;
+  part[p][lane] = sub_partial(Q, ldq, ncols, c, ldc, inst, n, r, p);
+  __syncthreads();
+  if (p == 0 && r < n) w[r] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // Second Gram-Schmidt pass fused with the norm: w[r] -= sum_k Q[r, k] c[inst(r), k],
-// and per instance sum w^2 accumulated into nrm[inst] (block partial sums, one
-// FP64 atomic add per instance present in the block).
+// and per instance sum w^2 accumulated into nrm[inst] (a wavefront reduction per
+// instance present in the block's rows, one FP64 atomic add each).
 __global__ __launch_bounds__(kVT) void k_seg_sub_norm(double* __restrict__ w, const double* __restrict__ Q,
                                                       int64_t ldq, int ncols, const double* __restrict__ c, int ldc,
                                                       const int* __restrict__ inst, int64_t n,
                                                       double* __restrict__ nrm) {
-  __shared__ double red[kVT];
-  const int64_t r0 = blockIdx.x * (int64_t)kVT;
-  const int64_t r = r0 + threadIdx.x;
+  __shared__ double part[4][kSubRows];
+  const int lane = threadIdx.x % kSubRows, p = threadIdx.x / kSubRows;
+  const int64_t r0 = blockIdx.x * (int64_t)kSubRows;
+  const int64_t r = r0 + lane;
+  part[p][lane] = sub_partial(Q, ldq, ncols, c, ldc, inst, n, r, p);
+  __syncthreads();
+  if (p != 0) return;  // wavefront 0 finishes the rows (kSubRows = 64 lanes)
   double sq = 0.0;
   int my = -1;
   if (r < n) {
     my = inst[r];
-    const double* ci = c + (int64_t)my * ldc;
-    double s = 0.0;
-    for (int k = 0; k < ncols; ++k) s = fma(Q[r + ldq * k], ci[k], s);
-    const double x = w[r] - s;
+    const double x = w[r] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
     w[r] = x;
     sq = x * x;
   }
   const int i_lo = inst[r0];
-  const int i_hi = inst[min(r0 + kVT, n) - 1];
-  for (int i = i_lo; i <= i_hi; ++i) {  // instances are contiguous: usually one or two per block
-    red[threadIdx.x] = (my == i) ? sq : 0.0;
-    __syncthreads();
-    for (int o = kVT / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) unsafeAtomicAdd(nrm + i, red[0]);
-    __syncthreads();
+  const int i_hi = inst[min(r0 + kSubRows, n) - 1];
+  for (int i = i_lo; i <= i_hi; ++i) {  // instances are contiguous: usually one per block
+    double v = (my == i) ? sq : 0.0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) unsafeAtomicAdd(nrm + i, v);
   }
 }
 
@@ -273,6 +298,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   TRL_HIP(hipMemcpy(seg.p, hseg.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice));
   TRL_HIP(hipMemcpy(inst_of.p, hinst.data(), sizeof(int) * N, hipMemcpyHostToDevice));
   const dim3 gv((unsigned)((N + kVT - 1) / kVT));
+  const dim3 gsub((unsigned)((N + kSubRows - 1) / kSubRows));
 
   // start vectors (given, or 0.5 - U[0,1) on the parity mask as :299-312 with an own RNG)
   {
@@ -311,13 +337,13 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
                          seg.as<int64_t>(), c1.as<double>(), ldc);
-      hipLaunchKernelGGL(k_seg_sub, gv, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
+      hipLaunchKernelGGL(k_seg_sub, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
                          c1.as<double>(), ldc, inst_of.as<int>(), N);
       hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
                          seg.as<int64_t>(), c2.as<double>(), ldc);
       double* nrm_j = scale.as<double>() + (size_t)(j & 1) * B;
       double* nrm_n = scale.as<double>() + (size_t)((j + 1) & 1) * B;
-      hipLaunchKernelGGL(k_seg_sub_norm, gv, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
+      hipLaunchKernelGGL(k_seg_sub_norm, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
                          c2.as<double>(), ldc, inst_of.as<int>(), N, nrm_j);
       hipLaunchKernelGGL(k_seg_finish, gv, dim3(kVT), 0, s, Q.as<double>() + ldq * (j + 1), W.as<double>(),
                          seg.as<int64_t>(), inst_of.as<int>(), N, c1.as<double>(), c2.as<double>(), ldc, j, nrm_j,
