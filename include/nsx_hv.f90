@@ -11,7 +11,7 @@ module nsx_hv_api
   use iso_c_binding
   implicit none
   private
-  public :: nsx_hv_create, nsx_hv_create_sep, nsx_hv_destroy, nsx_hv_dims, nsx_hv_lcoef, &
+  public :: nsx_hv_create, nsx_hv_create_sep, nsx_hv_destroy, nsx_hv_dims, nsx_hv_lcoef, nsx_hv_trunc, &
             nsx_hv_apply_f32, nsx_hv_apply_f64, nsx_trl_solve, nsx_hv_write_vectors, &
             nsx_hv_write_xvc, nsx_hv_expect, nsx_hv_xpct_files, nsx_last_error, nsx_cstr, &
             nsx_error_message, &
@@ -72,6 +72,13 @@ module nsx_hv_api
       type(c_ptr), value :: h
       integer(c_int8_t), dimension(*), intent(out) :: mask
     end function nsx_hv_lcoef
+
+    ! states a solve spans: the N-hbar-omega window mask (returns 1) or lcoef (returns 0)
+    integer(c_int) function nsx_hv_trunc(h, mask) bind(C, name='nsx_hv_trunc')
+      import :: c_int, c_ptr, c_int8_t
+      type(c_ptr), value :: h
+      integer(c_int8_t), dimension(*), intent(out) :: mask
+    end function nsx_hv_trunc
 
     ! coefL = H coefR (REAL*4 vectors of no_state elements; coefL is overwritten)
     integer(c_int) function nsx_hv_apply_f32(h, v_in, v_out) bind(C, name='nsx_hv_apply_f32')

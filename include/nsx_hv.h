@@ -17,7 +17,7 @@ enum {
   NSX_EIO = -1,     /* a file could not be opened or read */
   NSX_EFORMAT = -2, /* a record has an unexpected size or content */
   NSX_ETAG = -3,    /* .sgt x tag differs from the .tri tag */
-  NSX_ECAP = -4,    /* unsupported input (e.g. truncated basis) or size limit */
+  NSX_ECAP = -4,    /* unsupported input or size limit */
   NSX_EGPU = -5,    /* HIP runtime error */
   NSX_EARG = -6     /* invalid argument */
 };
@@ -67,6 +67,14 @@ int nsx_hv_dims(const nsx_hv *h, int64_t *N, int32_t *basd);
 int nsx_hv_block(const nsx_hv *h, int32_t b, int32_t *ja2, int32_t *jb2, int64_t *pos, int64_t *dim);
 /* lcoef parity mask of the basis (1 = state of the requested parity), N bytes. */
 int nsx_hv_lcoef(const nsx_hv *h, uint8_t *mask);
+/* The states a Lanczos solve spans, N bytes.  With an N-hbar-omega window in the
+ * .nus (min_MSon/max_MSon, src/NuTRL/NuCoreX.f90:813) this is trunc: the states
+ * of the requested parity whose partitions lie inside the window (:990-1037),
+ * and the call returns 1; otherwise it is lcoef and the call returns 0.
+ * On a truncated basis H*v leaves the blocks with no partition pair inside the
+ * window at zero (bas_log, :86-88, :155-157) and computes the others in full;
+ * the solvers zero W outside trunc after every H*v, as Lnz_OP does (:307-344). */
+int nsx_hv_trunc(const nsx_hv *h, uint8_t *mask);
 /* Algorithmic work of one H*v: the reference loop nest's sparse flops (all terms,
  * and the like-particle H_a V + V H_b^T part of them) and FP64 bytes
  * (v read, w written, every operator value read once). */

@@ -222,6 +222,13 @@ int nsx_hv_lcoef(const nsx_hv* h, uint8_t* mask) {
   return NSX_OK;
 }
 
+int nsx_hv_trunc(const nsx_hv* h, uint8_t* mask) {
+  if (!h || !mask) { set_error("null argument"); return NSX_EARG; }
+  const std::vector<uint8_t>& m = h->model.truncate ? h->model.trunc : h->model.lcoef;
+  std::memcpy(mask, m.data(), m.size());
+  return h->model.truncate ? 1 : 0;
+}
+
 int nsx_hv_work(const nsx_hv* h, double* flops, double* flops_like, int64_t* bytes) {
   if (!h) { set_error("null handle"); return NSX_EARG; }
   if (h->host_only) {  // no plan: work from the model, bytes unknown until planned

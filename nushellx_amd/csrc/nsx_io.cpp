@@ -90,6 +90,8 @@ std::string read_nus(const std::string& prefix, NusInput& n) {
   for (int k = 0; k < n.no_shells && b + k < nuc.size() && nuc[b + k] == 'p'; ++k) n.np_shells++;
   n.sj2.assign(n.no_shells, 0);
   n.sp.assign(n.no_shells, 1);
+  n.sn.assign(n.no_shells, 0);
+  n.sl.assign(n.no_shells, 0);
   for (int k = 0; k < n.no_shells; ++k) {
     if (!need(i)) return prefix + ".nus: shells";
     int a, nn, l, j2, t2, par;
@@ -97,6 +99,8 @@ std::string read_nus(const std::string& prefix, NusInput& n) {
     if (!(ss >> a >> nn >> l >> j2 >> t2 >> par)) return prefix + ".nus: bad shell line";
     n.sj2[k] = j2;
     n.sp[k] = par;
+    n.sn[k] = nn;
+    n.sl[k] = l;
   }
   if (!need(i + 2)) return prefix + ".nus: truncated";
   std::istringstream(lines[i++]) >> n.min_cJ2 >> n.max_cJ2;
@@ -136,14 +140,17 @@ std::string read_side(const std::string& opfile, const NusInput& nus, SideFiles&
   if (!r.next(p, len)) return opfile + ".tri: no partition record";
   const int ptdim = len / (2 * s.nsp);
   s.parity.assign(s.nsp, 1);
+  s.nq.assign(s.nsp, 0);
   for (int q = 0; q < s.nsp; ++q) {  // cParity, src/NuCommon/NuCommon.f90:2918-2937
-    int par = 1;
+    int par = 1, nq = 0;
     for (int k = 0; k < nus.no_shells && k < ptdim; ++k) {
       int16_t occ;
       std::memcpy(&occ, p + 2 * (size_t(q) * ptdim + k), 2);
       for (int t = 0; t < occ; ++t) par *= nus.sp[k];
+      nq += (2 * nus.sn[k] + nus.sl[k]) * occ;  // cNN, src/NuTRL/NuCoreX.f90:814-817
     }
     s.parity[q] = par;
+    s.nq[q] = nq;
   }
 
   Records t;

@@ -14,6 +14,7 @@ namespace nsx {
 struct NusInput {                     // src/NuCommon/NuCommon.f90:2546-2736
   int no_cI = 0, no_cN = 0, tparity = 0, no_shells = 0, np_shells = 0;
   std::vector<int> sj2, sp;           // per shell: 2j, parity
+  std::vector<int> sn, sl;            // per shell: radial n and l (oscillator quanta 2n + l)
   int min_cJ2 = 0, max_cJ2 = 0, min_cT2 = 0, max_cT2 = 0, nmp = 0;
   int min_MSon = 0, max_MSon = 999, del_cJ2 = 2, del_cT2 = 2;
 };
@@ -31,6 +32,7 @@ struct SideFiles {
   float xtag = 0.f;
   std::vector<int> nJT;               // [(J2/2) * nsp + p]
   std::vector<int> parity;            // per partition
+  std::vector<int> nq;                // per partition: sum over shells of (2n + l) * occupation
   std::vector<Density> dens;          // sorted by key
   std::vector<double> vals;           // density values (file REAL*4 or REAL*8, widened)
   // like-particle Hamiltonian per J: dense d(J) x d(J), symmetric, empty if absent

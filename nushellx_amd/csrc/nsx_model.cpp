@@ -167,6 +167,7 @@ static std::string build_separable(Model& m, const std::string& dopfile, char ty
   std::map<Key, int> adense_of, bhat_of;
   std::vector<uint32_t> lbr(B.words);
   for (int L = 0; L < m.basd; ++L) {
+    if (!m.bas_active[L]) continue;  // truncated basis: no output for this block
     const int ajl = m.bas_a[L], bjl = m.bas_b[L];
     for (int ajr = A.minJ2; ajr <= A.maxJ2; ajr += A.delJ2) {
       Item it;
@@ -257,8 +258,6 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
   std::string err;
   if (!(err = read_nus(nus_prefix, m.nus)).empty()) return err;
   const NusInput& nus = m.nus;
-  if (nus.min_MSon != 0 || nus.max_MSon != 999)
-    return "truncated (N-hbar-omega window) bases are not supported";
   if (!(err = read_side(opfile_a, nus, m.A)).empty()) return err;
   if (!(err = read_side(opfile_b, nus, m.B)).empty()) return err;
   // load-time zero limits of the reference (src/NuTRL/NuCoreX.f90:1163,1195;
@@ -302,16 +301,32 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
     m.N += int64_t(m.da.back()) * m.db.back();
     bidx[blk[b]] = b;
   }
-  // parity mask, state order b-partition, b-state, a-partition, a-state (:998-1030)
+  // parity mask, state order b-partition, b-state, a-partition, a-state (:998-1030);
+  // with an N-hbar-omega window also the window mask and the active blocks
+  // (:990-1037: trunc = lcoef inside [min_MSon, max_MSon]; a block is active
+  // when any partition pair of it lies inside, whatever its states' parity)
+  m.truncate = nus.min_MSon != 0 || nus.max_MSon != 999;  // :813
   m.lcoef.assign(size_t(m.N), 0);
+  m.trunc.assign(m.truncate ? size_t(m.N) : 0, 0);
+  m.bas_active.assign(m.basd, 1);
   {
     int64_t k = 0;
-    for (int b = 0; b < m.basd; ++b)
+    for (int b = 0; b < m.basd; ++b) {
+      bool any = false;
       for (int bp = 0; bp < B.nsp; ++bp)
         for (int bs = 0; bs < B.n(m.bas_b[b], bp); ++bs)
-          for (int ap = 0; ap < A.nsp; ++ap)
-            for (int as = 0; as < A.n(m.bas_a[b], ap); ++as)
-              m.lcoef[k++] = (nus.tparity == 0 || A.parity[ap] * B.parity[bp] == nus.tparity) ? 1 : 0;
+          for (int ap = 0; ap < A.nsp; ++ap) {
+            const int ntot = A.nq[ap] + B.nq[bp];
+            const bool in = ntot >= nus.min_MSon && ntot <= nus.max_MSon;
+            any = any || in;
+            for (int as = 0; as < A.n(m.bas_a[b], ap); ++as) {
+              const uint8_t lc = (nus.tparity == 0 || A.parity[ap] * B.parity[bp] == nus.tparity) ? 1 : 0;
+              if (m.truncate) m.trunc[k] = in ? lc : 0;
+              m.lcoef[k++] = lc;
+            }
+          }
+      if (m.truncate) m.bas_active[b] = any ? 1 : 0;
+    }
   }
   const char xtra = intfile.empty() ? 'x' : intfile.back();
   if (!(err = read_mats(opfile_a, nus, typea == 'n' ? no_cA : -no_cA, xtra, m.A)).empty()) return err;
@@ -346,6 +361,7 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
 
   // drdbh loop order (:1486-1565), regrouped per (L, Ja')
   for (int L = 0; L < m.basd; ++L) {
+    if (!m.bas_active[L]) continue;  // truncated basis: no output for this block (:155-157)
     const int ajl = m.bas_a[L], bjl = m.bas_b[L];
     for (int ajr = A.minJ2; ajr <= A.maxJ2; ajr += A.delJ2) {
       Item it;
@@ -454,6 +470,7 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
   }
   // like-particle work: every stored element touches a column of the block
   for (int L = 0; L < m.basd; ++L) {
+    if (!m.bas_active[L]) continue;
     const auto& Ha = m.A.H[m.bas_a[L] / 2];
     const auto& Hb = m.B.H[m.bas_b[L] / 2];
     for (double v : Ha) if (v != 0.0) m.flops_like += 2.0 * m.db[L];

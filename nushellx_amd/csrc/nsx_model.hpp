@@ -52,6 +52,11 @@ struct Model {
   std::vector<int> bas_a, bas_b, da, db;   // per block: 2Ja, 2Jb, d(Ja), d(Jb)
   std::vector<int64_t> posnv;
   std::vector<uint8_t> lcoef;              // parity mask of the basis states
+  // N-hbar-omega window (min_MSon/max_MSon of the .nus): trunc marks the states
+  // of the requested parity inside it, bas_active the blocks with any partition
+  // pair inside it; inactive blocks get no H*v output, as in the reference
+  bool truncate = false;
+  std::vector<uint8_t> trunc, bas_active;
   std::vector<Item> items;
   // dense a-densities d(Jl) x d(Jr), column-major, and their keys
   std::vector<std::vector<double>> adense;

@@ -219,8 +219,9 @@ void plan_add(HostPlan& p, const Model& m) {
     for (int j0 = 0; j0 < db; j0 += nc) {
       DBlock b;
       b.v_off = vb + m.posnv[L];
-      b.ha_off = m.azero ? -1 : ha_off[m.bas_a[L] / 2];
-      b.hb_off = m.bzero ? -1 : hb_off[m.bas_b[L] / 2];
+      // an inactive block of a truncated basis gets W_L = 0 (multiplya/b skip it, :86-88)
+      b.ha_off = (m.azero || !m.bas_active[L]) ? -1 : ha_off[m.bas_a[L] / 2];
+      b.hb_off = (m.bzero || !m.bas_active[L]) ? -1 : hb_off[m.bas_b[L] / 2];
       b.da = da; b.db = db; b.j0 = j0; b.nc = std::min(nc, db - j0);
       b.p_begin = p_begin; b.p_end = p_end;
       p.blocks.push_back(b);

@@ -154,6 +154,13 @@ __global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, cons
   q[r] = ok ? w[r] / b : 0.0;
 }
 
+// truncated bases: W = 0 outside the N-hbar-omega window after every H*v, as
+// Lnz_OP's `where (.not.trunc) coefL = 0` (src/NuTRL/NuCoreX.f90:307-308, 320-321, 343-344)
+__global__ __launch_bounds__(kVT) void k_mask(double* __restrict__ w, const uint8_t* __restrict__ keep, int64_t n) {
+  const int64_t r = blockIdx.x * (int64_t)kVT + threadIdx.x;
+  if (r < n && !keep[r]) w[r] = 0.0;
+}
+
 __global__ __launch_bounds__(kVT) void k_fill0(double* __restrict__ x, int n) {
   const int i = blockIdx.x * kVT + threadIdx.x;
   if (i < n) x[i] = 0.0;
@@ -187,7 +194,8 @@ struct TrlWork {
   int B = 0, m = 0, keep = 0, nev = 0, prec = -1;
   uint64_t plan_gen = 0;
   void *Q = nullptr, *W = nullptr, *Y = nullptr, *inst_of = nullptr, *seg = nullptr, *c1 = nullptr,
-       *c2 = nullptr, *ah = nullptr, *bh = nullptr, *scale = nullptr, *dim = nullptr, *S = nullptr;
+       *c2 = nullptr, *ah = nullptr, *bh = nullptr, *scale = nullptr, *dim = nullptr, *S = nullptr,
+       *wmask = nullptr;
   double *hha = nullptr, *hhb = nullptr;  // pinned alpha / beta histories
   int* hdim[2] = {nullptr, nullptr};      // pinned dimension flags, one per block parity
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -198,8 +206,8 @@ struct TrlWork {
   }
   void release() {
     clear_graphs();
-    for (void* p : {Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S}) (void)hipFree(p);
-    Q = W = Y = inst_of = seg = c1 = c2 = ah = bh = scale = dim = S = nullptr;
+    for (void* p : {Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S, wmask}) (void)hipFree(p);
+    Q = W = Y = inst_of = seg = c1 = c2 = ah = bh = scale = dim = S = wmask = nullptr;
     (void)hipHostFree(hha);
     (void)hipHostFree(hhb);
     (void)hipHostFree(hdim[0]);
@@ -218,9 +226,18 @@ void free_trl_work(TrlWork* w) { delete w; }
 
 struct Instance {
   int64_t off, n;
-  const std::vector<uint8_t>* lcoef;
+  const std::vector<uint8_t>* keep;  // states the solve spans: the parity mask, or the window mask
   int64_t nstates;
+  bool masked;                       // truncated basis: W is masked after every H*v
 };
+
+// The instance of one model: its kept states and their count.
+static Instance instance_of(const Model& m, int64_t off) {
+  const std::vector<uint8_t>* keep = m.truncate ? &m.trunc : &m.lcoef;
+  int64_t nst = 0;
+  for (uint8_t b : *keep) nst += b;
+  return Instance{off, m.N, keep, nst, m.truncate};
+}
 
 // Lockstep TRL over instances sharing one device plan.  Outputs per instance i:
 // evals[i*nev..], residuals[i*nev..], converged_at[i] (matvecs when all nev
@@ -256,6 +273,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
               dm(ws.c2, sizeof(double) * B * ldc) && dm(ws.ah, sizeof(double) * (m + 1) * B) &&
               dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * 2 * B) &&
               dm(ws.dim, sizeof(int) * B) && dm(ws.S, sizeof(double) * (size_t)B * m * std::max(keep, nev)) &&
+              dm(ws.wmask, (size_t)N) &&
               hipHostMalloc((void**)&ws.hha, sizeof(double) * (m + 1) * B) == hipSuccess &&
               hipHostMalloc((void**)&ws.hhb, sizeof(double) * (m + 1) * B) == hipSuccess &&
               hipHostMalloc((void**)&ws.hdim[0], sizeof(int) * B) == hipSuccess &&
@@ -275,7 +293,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     ws.prec = prec;
   }
   const DPtr Q{ws.Q}, W{ws.W}, Y{ws.Y}, inst_of{ws.inst_of}, seg{ws.seg}, c1{ws.c1}, c2{ws.c2}, ah{ws.ah},
-      bh{ws.bh}, scale{ws.scale}, dim{ws.dim}, S{ws.S};
+      bh{ws.bh}, scale{ws.scale}, dim{ws.dim}, S{ws.S}, kmask{ws.wmask};
 #define TRL_HIP(call)                                                          \
   do {                                                                         \
     hipError_t _e = (call);                                                    \
@@ -297,6 +315,16 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     if (inst[i].off + inst[i].n != inst[i + 1].off) { set_error("instances must be contiguous"); return NSX_EARG; }
   TRL_HIP(hipMemcpy(seg.p, hseg.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice));
   TRL_HIP(hipMemcpy(inst_of.p, hinst.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  // window masks of truncated instances (others keep every row: H conserves parity)
+  bool any_masked = false;
+  for (const auto& x : inst) any_masked = any_masked || x.masked;
+  if (any_masked) {
+    std::vector<uint8_t> hk(N, 1);
+    for (const auto& x : inst)
+      if (x.masked)
+        for (int64_t r = 0; r < x.n; ++r) hk[x.off + r] = (*x.keep)[r];
+    TRL_HIP(hipMemcpy(kmask.p, hk.data(), (size_t)N, hipMemcpyHostToDevice));
+  }
   const dim3 gv((unsigned)((N + kVT - 1) / kVT));
   const dim3 gsub((unsigned)((N + kSubRows - 1) / kSubRows));
 
@@ -308,7 +336,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     for (int i = 0; i < B; ++i) {
       double nrm = 0.0;
       for (int64_t r = 0; r < inst[i].n; ++r) {
-        const bool on = (*inst[i].lcoef)[r] != 0;
+        const bool on = (*inst[i].keep)[r] != 0;
         double x = on ? (start ? start[inst[i].off + r] : 0.5 - U(rng)) : 0.0;
         if (!std::isfinite(x)) x = 0.0;
         v0[inst[i].off + r] = x;
@@ -317,7 +345,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       if (!(nrm > 1e-300)) {  // unusable warm start: fall back to a random start
         nrm = 0.0;
         for (int64_t r = 0; r < inst[i].n; ++r) {
-          const double x = (*inst[i].lcoef)[r] ? 0.5 - U(rng) : 0.0;
+          const double x = (*inst[i].keep)[r] ? 0.5 - U(rng) : 0.0;
           v0[inst[i].off + r] = x;
           nrm += x * x;
         }
@@ -335,6 +363,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     for (int j = j0; j < j1; ++j) {
       hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s);
       if (e != hipSuccess) return e;
+      if (any_masked) hipLaunchKernelGGL(k_mask, gv, dim3(kVT), 0, s, W.as<double>(), kmask.as<uint8_t>(), N);
       hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
                          seg.as<int64_t>(), c1.as<double>(), ldc);
       hipLaunchKernelGGL(k_seg_sub, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
@@ -644,11 +673,9 @@ extern "C" int nsx_trl_solve(nsx_hv* h, const nsx_trl_opts* opt, const double* s
                              double* evecs, double* residuals, nsx_trl_stats* stats) {
   if (!h || !opt || !evals) { set_error("null argument"); return NSX_EARG; }
   if (h->host_only) { set_error("host-only handle: solve it through a batch"); return NSX_EARG; }
-  int64_t nst = 0;
-  for (uint8_t b : h->model.lcoef) nst += b;
-  std::vector<Instance> inst{{0, h->model.N, &h->model.lcoef, nst}};
+  std::vector<Instance> inst{instance_of(h->model, 0)};
   nsx_trl_opts o = *opt;
-  o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, nst);
+  o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, inst[0].nstates);
   int32_t conv_at = -1;
   int rc = trl_run(h->dplan, h->lowp, h->precision, h->stream, h->trl, inst, o, start, evals, evecs, residuals,
                    &conv_at, stats);
@@ -662,10 +689,8 @@ extern "C" int nsx_batch_trl_solve(nsx_batch* b, const nsx_trl_opts* opt, const 
   std::vector<Instance> inst;
   int64_t min_st = INT64_MAX;
   for (size_t i = 0; i < b->models.size(); ++i) {
-    int64_t nst = 0;
-    for (uint8_t x : b->models[i]->lcoef) nst += x;
-    min_st = std::min(min_st, nst);
-    inst.push_back({b->hplan.vbase[i], b->models[i]->N, &b->models[i]->lcoef, nst});
+    inst.push_back(instance_of(*b->models[i], b->hplan.vbase[i]));
+    min_st = std::min(min_st, inst.back().nstates);
   }
   nsx_trl_opts o = *opt;
   o.max_basis = (int32_t)std::min<int64_t>(o.max_basis, std::max<int64_t>(min_st, o.nev + 2));

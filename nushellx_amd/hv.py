@@ -90,6 +90,15 @@ class Hamiltonian:
         check(lib().nsx_hv_write_vectors(self._h, prefix.encode(), len(evals), evals.ctypes.data,
                                          vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_vectors")
 
+    def trunc(self) -> Tuple[bool, np.ndarray]:
+        """(truncated, mask): the states a solve spans (the N-hbar-omega window
+        mask on a truncated basis, else the parity mask)."""
+        m = np.zeros(self.N, dtype=np.uint8)
+        rc = lib().nsx_hv_trunc(self._h, m.ctypes.data)
+        if rc < 0:
+            check(rc, "nsx_hv_trunc")
+        return bool(rc), m.astype(bool)
+
     def write_xvc(self, prefix: str, evals, vecs) -> None:
         """<prefix>.xvc, NuVec's state file (vecs: nvec x N)."""
         vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)

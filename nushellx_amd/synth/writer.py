@@ -155,7 +155,8 @@ def write_side_matrices(side: SideBasis, side_nucleus: str, outdir: str, inter: 
 
 
 def write_nus(path: str, nucleus: str, space: PNSpace, nval: int, nmp: int, J2min: int,
-              J2max: int, T2: int, tparity: int = 0) -> None:
+              J2max: int, T2: int, tparity: int = 0, window: Tuple[int, int] = (0, 999)) -> None:
+    """``window``: the N-hbar-omega limits (min_MSon, max_MSon); (0, 999) is none."""
     with open(path, "w") as f:
         f.write(f"{nucleus:>6s}\n")
         f.write("Synthetic pn model written by nushellx_amd\n")
@@ -170,7 +171,7 @@ def write_nus(path: str, nucleus: str, space: PNSpace, nval: int, nmp: int, J2mi
             f.write(f"{s:4d}{o.n:4d}{o.l:4d}{o.j2:4d}{1:4d}{o.parity:4d}\n")
         f.write(f"{J2min:4d}{J2max:4d}\n")
         f.write(f"{T2:4d}{T2:4d}{nmp:4d}\n")
-        f.write(f"{0:4d}{999:4d}\n")
+        f.write(f"{window[0]:4d}{window[1]:4d}\n")
         f.write("".join(f"{0:4d}" for _ in range(space.no_shells)) + "\n")
         f.write("".join(f"{space.j2(s) + 1:4d}" for s in range(1, space.no_shells + 1)) + "\n")
         f.write("".join(f"{0:4d}" for _ in range(space.no_shells)) + "\n")
@@ -267,8 +268,10 @@ class SideCache:
 
 def write_model(model: PNModel, outdir: str, base: str, J2s: Sequence[int], xtra: str = "x",
                 no_level: int = 5, cache: Optional[SideCache] = None, seed_tag: int = 12345,
-                densities: bool = True, real_kind: int = 4) -> ModelFiles:
-    """Write every file NuTRL needs for the listed total 2J values."""
+                densities: bool = True, real_kind: int = 4, tparity: int = 0,
+                window: Tuple[int, int] = (0, 999)) -> ModelFiles:
+    """Write every file NuTRL needs for the listed total 2J values.  ``tparity``
+    (+1/-1, 0 for both) and ``window`` (N-hbar-omega limits) go into the .nus."""
     assert len(base) <= 5, "NuShellX nucleus names are at most 5 characters + J code"
     os.makedirs(outdir, exist_ok=True)
     cache = cache or SideCache()
@@ -296,7 +299,7 @@ def write_model(model: PNModel, outdir: str, base: str, J2s: Sequence[int], xtra
                             real_kind=real_kind, segments=cache.segments(sp_, kind, n, model.inter))
     for J2 in J2s:
         write_nus(os.path.join(outdir, base + an_code(J2 // 2)) + ".nus", base + an_code(J2 // 2),
-                  sp_, nval, nmp, J2, J2, T2)
+                  sp_, nval, nmp, J2, J2, T2, tparity=tparity, window=window)
     write_oph(os.path.join(outdir, base + xtra) + ".oph", sp_, model.inter)
     write_spe(os.path.join(outdir, base + xtra) + ".spe", model.inter, model.renorm, no_level)
     model.inter.write_int(os.path.join(outdir, base + xtra) + ".int")

@@ -142,6 +142,7 @@ typedef struct {
   float xtag;
   int *nJT;      /* [nJ][nsp] : nJT[(J2/2)*nsp + p] */
   int *par;      /* parity per partition */
+  int *nq;       /* oscillator quanta per partition: sum of cNN * occupation */
   int *d0;       /* d(J) per J2/2 */
   int *snJT0;    /* partition offsets per J: [(J2/2)*nsp + p] */
   int ndens, max_lm;
@@ -171,6 +172,8 @@ struct refcpu {
   int *bas_a, *bas_b;
   int64_t *posnv, *dimnv;
   uint8_t *lcoef;
+  uint8_t *trunc;   /* N-hbar-omega window mask (NULL when not truncated) */
+  uint8_t *bas_log; /* blocks with a partition pair in the window (NULL when not truncated) */
   side_t A, B;
   int azero, bzero, nslot, maxda, maxdb;
   tab_t *tab;      /* [basd][nslot], slot = ajr/2 */
@@ -183,7 +186,7 @@ struct refcpu {
 };
 
 static void side_free(side_t *s) {
-  free(s->nJT); free(s->par); free(s->d0); free(s->snJT0); free(s->keys); free(s->pos);
+  free(s->nJT); free(s->par); free(s->nq); free(s->d0); free(s->snJT0); free(s->keys); free(s->pos);
   free(s->dim); free(s->mask); free(s->v32); free(s->v64); free(s->has_mat); free(s->posm);
   free(s->mmask); free(s->m32); free(s->m64);
 }
@@ -198,6 +201,7 @@ void refcpu_destroy(refcpu_t *h) {
   }
   free(h->tb_b); free(h->tb_h); free(h->tb_me64); free(h->tb_me32);
   free(h->bas_a); free(h->bas_b); free(h->posnv); free(h->dimnv); free(h->lcoef); free(h->sj2);
+  free(h->trunc); free(h->bas_log);
   free(h);
 }
 
@@ -205,7 +209,7 @@ static inline int nJTs(const side_t *s, int J2, int p) { return s->nJT[(J2 / 2) 
 static inline int bit_of(const uint32_t *col, int i) { return (col[i >> 5] >> (i & 31)) & 1u; }
 
 /* .tri + .trd of one side: src/NuTRL/NuCoreX.f90:838-867,1145-1206 */
-static int read_side(side_t *s, const char *opfile, const int *shell_par, int nshells) {
+static int read_side(side_t *s, const char *opfile, const int *shell_par, const int *cNN, int nshells) {
   char path[1024];
   ffile_t f;
   int len;
@@ -229,6 +233,7 @@ static int read_side(side_t *s, const char *opfile, const int *shell_par, int ns
   if (!r) { set_err("%s: no partitions", path); ff_close(&f); return -2; }
   int ptdim = len / (2 * s->nsp);
   s->par = (int *)malloc(sizeof(int) * s->nsp);
+  s->nq = (int *)calloc(s->nsp, sizeof(int));
   for (int p = 0; p < s->nsp; p++) {
     int par = 1;
     for (int i = 0; i < nshells && i < ptdim; i++) {
@@ -236,6 +241,7 @@ static int read_side(side_t *s, const char *opfile, const int *shell_par, int ns
       memcpy(&occ, r + 2 * (p * ptdim + i), 2);
       if (occ > 0)
         for (int k = 0; k < occ; k++) par *= shell_par[i];
+      s->nq[p] += cNN[i] * occ; /* psum(cNN * spart), src/NuTRL/NuCoreX.f90:1010 */
     }
     s->par[p] = par;
   }
@@ -358,7 +364,7 @@ static int read_mats(side_t *s, const char *side_nuc, int minT2, int maxT2, int 
 
 /* ---------------------------------------------------------------- .nus */
 typedef struct {
-  int no_cI, no_cN, tparity, no_shells, sj2[64], sp[64], np_shells;
+  int no_cI, no_cN, tparity, no_shells, sj2[64], sp[64], sn[64], sl[64], np_shells;
   int min_cJ2, max_cJ2, min_cT2, max_cT2, nmp, min_MSon, max_MSon, del_cJ2, del_cT2;
 } nus_t;
 
@@ -391,6 +397,8 @@ static int read_nus(nus_t *n, const char *prefix) {
     if (sscanf(lines[i++], "%d %d %d %d %d %d", &a, &b, &c, &d, &e, &f) != 6) { set_err("%s: shells", path); return -2; }
     n->sj2[k] = d;
     n->sp[k] = f;
+    n->sn[k] = b;
+    n->sl[k] = c;
   }
   sscanf(lines[i++], "%d %d", &n->min_cJ2, &n->max_cJ2);
   sscanf(lines[i++], "%d %d %d", &n->min_cT2, &n->max_cT2, &n->nmp);
@@ -754,13 +762,14 @@ static int create_impl(const char *nus_prefix, const char *opfile_a, const char 
   h->thr_on = thresholds;
   h->nthr = omp_get_max_threads();
   if ((rc = read_nus(&nus, nus_prefix))) goto fail;
-  if (nus.min_MSon != 0 || nus.max_MSon != 999) { set_err("truncation is not supported"); rc = -4; goto fail; }
   h->J2 = nus.min_cJ2;
   h->tparity = nus.tparity;
   h->sj2 = (int *)malloc(sizeof(int) * 64);
   memcpy(h->sj2, nus.sj2, sizeof(int) * 64);
-  if ((rc = read_side(&h->A, opfile_a, nus.sp, nus.no_shells))) goto fail;
-  if ((rc = read_side(&h->B, opfile_b, nus.sp, nus.no_shells))) goto fail;
+  int cNN[64] = {0}; /* 2n + l per shell (:814-817) */
+  for (int k = 0; k < nus.no_shells; k++) cNN[k] = 2 * nus.sn[k] + nus.sl[k];
+  if ((rc = read_side(&h->A, opfile_a, nus.sp, cNN, nus.no_shells))) goto fail;
+  if ((rc = read_side(&h->B, opfile_b, nus.sp, cNN, nus.no_shells))) goto fail;
   /* zero limits: NuTRL 1e-11 / 1e-10 (:1163, :1195); NuTRLZ 1e-6 on both sides
      (src/NuTRLZ/NuCoreXZ.f90:1205,1237), applied when `limits` is set */
   zero_small(&h->A, dopfile && limits ? 1e-6 : 1e-11);
@@ -808,17 +817,28 @@ static int create_impl(const char *nus_prefix, const char *opfile_a, const char 
     for (int J = h->B.minJ2; J <= h->B.maxJ2; J += 2) if (h->B.d0[J / 2] > h->maxdb) h->maxdb = h->B.d0[J / 2];
     free(pairs);
     h->N = N;
-    /* lcoef: state order b-part, b-state, a-part, a-state (:998-1030) */
+    /* lcoef: state order b-part, b-state, a-part, a-state (:998-1030); with a
+       window also trunc and bas_log (:990-1037) */
+    int truncate = nus.min_MSon != 0 || nus.max_MSon != 999; /* :813 */
     h->lcoef = (uint8_t *)malloc(N ? N : 1);
+    if (truncate) {
+      h->trunc = (uint8_t *)calloc(N ? N : 1, 1);
+      h->bas_log = (uint8_t *)calloc(nb ? nb : 1, 1);
+    }
     int64_t nsc = 0;
     for (int b = 0; b < nb; b++)
       for (int bp = 0; bp < h->B.nsp; bp++)
         for (int bs = 0; bs < nJTs(&h->B, h->bas_b[b], bp); bs++)
-          for (int ap = 0; ap < h->A.nsp; ap++)
+          for (int ap = 0; ap < h->A.nsp; ap++) {
+            int ntot = h->A.nq[ap] + h->B.nq[bp];
+            int in = ntot >= nus.min_MSon && ntot <= nus.max_MSon;
+            if (truncate && in) h->bas_log[b] = 1;
             for (int as = 0; as < nJTs(&h->A, h->bas_a[b], ap); as++) {
               int ok = h->tparity == 0 || h->A.par[ap] * h->B.par[bp] == h->tparity;
+              if (truncate) h->trunc[nsc] = (uint8_t)(in && ok);
               h->lcoef[nsc++] = (uint8_t)ok;
             }
+          }
     /* like-particle matrices */
     char xtra = intfile[strlen(intfile) - 1];
     int nmp_a = typea == 'n' ? no_cA : -no_cA;
@@ -1001,6 +1021,11 @@ int refcpu_block(const refcpu_t *h, int b, int *ja2, int *jb2, int64_t *pos, int
   return 0;
 }
 int refcpu_lcoef(const refcpu_t *h, uint8_t *mask) { memcpy(mask, h->lcoef, h->N); return 0; }
+int refcpu_trunc(const refcpu_t *h, uint8_t *mask) {
+  if (!h->trunc) { memcpy(mask, h->lcoef, h->N); return 0; }
+  memcpy(mask, h->trunc, h->N);
+  return 1;
+}
 int refcpu_set_threads(refcpu_t *h, int n) { h->nthr = n > 0 ? n : 1; return 0; }
 
 /* ---------------------------------------------------------------- mask helpers */

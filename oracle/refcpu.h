@@ -42,6 +42,10 @@ int refcpu_basd(const refcpu_t *h);
 int refcpu_block(const refcpu_t *h, int b, int *ja2, int *jb2, int64_t *pos, int64_t *dim);
 /* lcoef parity mask (1 = state of the requested total parity) */
 int refcpu_lcoef(const refcpu_t *h, uint8_t *mask);
+/* N-hbar-omega window mask trunc (:990-1037), returning 1, or lcoef and 0 when
+   the .nus sets no window.  On a truncated basis refcpu_apply skips the blocks
+   without a partition pair in the window (bas_log, :86-88, :155-157). */
+int refcpu_trunc(const refcpu_t *h, uint8_t *mask);
 /* v_out = H v_in (double in/out; converted to float in mode 0) */
 int refcpu_apply(refcpu_t *h, const double *v_in, double *v_out);
 /* algorithmic work of one H*v: sparse flops of the reference loop nest,

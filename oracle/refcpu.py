@@ -36,6 +36,7 @@ def lib():
         L.refcpu_block.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.refcpu_lcoef.argtypes = [C.c_void_p, C.c_void_p]
+        L.refcpu_trunc.argtypes = [C.c_void_p, C.c_void_p]
         L.refcpu_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.refcpu_work.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
         L.refcpu_set_threads.argtypes = [C.c_void_p, C.c_int]
@@ -102,6 +103,12 @@ class RefCPU:
         m = np.zeros(self.N, dtype=np.uint8)
         lib().refcpu_lcoef(self._h, m.ctypes.data)
         return m.astype(bool)
+
+    def trunc(self):
+        """(truncated, mask) of the N-hbar-omega window (mask = lcoef when there is none)."""
+        m = np.zeros(self.N, dtype=np.uint8)
+        t = lib().refcpu_trunc(self._h, m.ctypes.data)
+        return bool(t), m.astype(bool)
 
     def work(self):
         f = C.c_double()

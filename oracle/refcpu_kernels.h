@@ -222,6 +222,7 @@ static int SFX(hv)(refcpu_t *h, const double *v_in, double *v_out) {
       for (int lp = 0; lp < loop; lp++) {
         int bsc = lp / delj2;
         int dj2 = lp - bsc * delj2;
+        if (h->bas_log && !h->bas_log[bsc]) continue; /* truncated basis (:155-157) */
         int ajl = h->bas_a[bsc], bjl = h->bas_b[bsc];
         int ajr = A->minJ2 + A->delJ2 * dj2;
         const tab_t *t = &h->tab[bsc * h->nslot + ajr / 2];
@@ -298,6 +299,7 @@ static int SFX(hv)(refcpu_t *h, const double *v_in, double *v_out) {
 #pragma omp for schedule(dynamic, 1)
       for (int bsc = 0; bsc < h->basd; bsc++) {
         int ja = h->bas_a[bsc], jb = h->bas_b[bsc];
+        if (h->bas_log && !h->bas_log[bsc]) continue; /* :86-88 */
         if (!A->has_mat[ja / 2]) continue;
         SFX(mulmtx)(h, coefL, coefR, h->posnv[bsc], 1, ja, jb, B1, B3);
       }
@@ -309,6 +311,7 @@ static int SFX(hv)(refcpu_t *h, const double *v_in, double *v_out) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthr)
     for (int bsc = 0; bsc < h->basd; bsc++) {
       int ja = h->bas_a[bsc], jb = h->bas_b[bsc];
+      if (h->bas_log && !h->bas_log[bsc]) continue; /* :112-114 */
       if (!B->has_mat[jb / 2]) continue;
       SFX(mulmtx)(h, coefL, coefR, h->posnv[bsc], 0, ja, jb, NULL, NULL);
     }

@@ -2,7 +2,9 @@
 from __future__ import annotations
 
 from nushellx_amd.synth.interaction import random_interaction
-from nushellx_amd.synth.mspace import PF, SD, PNSpace
+from nushellx_amd.synth.mspace import PF, SD, Orbit, PNSpace
+
+PSD = (Orbit(0, 1, 1), Orbit(0, 2, 5), Orbit(1, 0, 1))
 from nushellx_amd.synth.writer import PNModel, SideCache, write_model
 
 SEED = 2024
@@ -18,6 +20,16 @@ def make_models(outdir: str):
     out["sd66"] = write_model(PNModel(sd, 6, 6, inter), outdir + "/sd66", "sd66", [0, 4, 8], cache=cache)
     out["sd23"] = write_model(PNModel(sd, 2, 3, inter), outdir + "/sd23", "sd23", [1, 3, 5], cache=cache)
     out["sd32"] = write_model(PNModel(sd, 3, 2, inter), outdir + "/sd32", "sd32", [1, 3], cache=cache)
+    # a cross-shell space (0p1/2 with N = 1, 0d5/2 and 1s1/2 with N = 2) for the
+    # N-hbar-omega window: positive parity, with and without the window 9..11
+    psd = PNSpace(PSD)
+    pint = random_interaction(PSD, seed=SEED + 1)
+    out["psd33"] = write_model(PNModel(psd, 3, 3, pint), outdir + "/psd33", "psd33", [0, 2, 4], cache=cache,
+                               tparity=1)
+    out["psd33t"] = write_model(PNModel(psd, 3, 3, pint), outdir + "/psd33t", "pst33", [0, 2, 4], cache=cache,
+                                tparity=1, window=(9, 11))
+    out["psd33w8"] = write_model(PNModel(psd, 3, 3, pint), outdir + "/psd33w8", "psw33", [0], cache=cache,
+                                 tparity=1, window=(8, 8))
     return out
 
 
