@@ -79,6 +79,18 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
               return "LDS request too large (stacked kernels)";
     d.atomic = 1;
   }
+  if (!hp.g_tile2.empty()) {  // large blocks: tiled GEMM path, which accumulates into W atomically
+    if (!(e = up(d.g_task1, hp.g_task1)).empty()) return e;
+    if (!(e = up(d.g_tile1, hp.g_tile1)).empty()) return e;
+    if (!(e = up(d.g_tile2, hp.g_tile2)).empty()) return e;
+    if (!(e = up(d.g_term2, hp.g_term2)).empty()) return e;
+    if (!(e = up(d.g_xref, hp.g_xref)).empty()) return e;
+    d.n_gtile1 = (int)hp.g_tile1.size();
+    d.n_gtile2 = (int)hp.g_tile2.size();
+    if (hp.xg_size > 0 && hipMalloc(&d.XG, sizeof(double) * (size_t)hp.xg_size) != hipSuccess)
+      return "hipMalloc failed (large-block X buffer)";
+    d.atomic = 1;
+  }
   if (!(e = up(d.items, hp.items)).empty()) return e;
   if (!(e = up(d.aterms, hp.aterms)).empty()) return e;
   if (!(e = up(d.rterms, hp.rterms)).empty()) return e;
@@ -108,6 +120,8 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.part_list); (void)hipFree(d.Ad); (void)hipFree(d.Bh); (void)hipFree(d.H); (void)hipFree(d.part);
   (void)hipFree(d.s_chunk1); (void)hipFree(d.s_unit1); (void)hipFree(d.s_chunk2); (void)hipFree(d.s_term2);
   (void)hipFree(d.s_xref); (void)hipFree(d.X);
+  (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
+  (void)hipFree(d.g_xref); (void)hipFree(d.XG);
   d = DevicePlan();
 }
 
@@ -156,7 +170,7 @@ static int create_impl(const char* nus_prefix, const char* opfile_a, const char*
     *out = h.release();
     return NSX_OK;
   }
-  plan_add(h->hplan, h->model);
+  plan_models({&h->model}, h->hplan);
   if (precision != NSX_FP64) {
     e = build_lowp(h->hplan, precision, h->lowp);
     if (!e.empty()) { set_error(e); return NSX_EGPU; }
@@ -484,6 +498,7 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
   if (n == 1) {
     plan_add(hp, *models[0]);
     if (hp.cfg.stacked) build_stacked(hp, models);
+    build_big(hp, models);
     return;
   }
   std::vector<HostPlan> local(n);
@@ -496,6 +511,7 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
     local[i] = HostPlan();  // release as we go
   }
   if (hp.cfg.stacked) build_stacked(hp, models);
+  build_big(hp, models);
 }
 
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,

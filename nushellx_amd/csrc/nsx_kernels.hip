@@ -613,6 +613,155 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   }
 }
 
+// ---------------------------------------------------------------- large-block GEMM path (FP64)
+// 64 x 64 output tiles on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l
+// supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds D(l / 16 + 4 v, l % 16)).
+// Four wavefronts take 32 x 32 quarters as 2 x 2 MFMA blocks.  The product is
+// formed transposed, D = C^T, so that a lane's outputs run down a column of the
+// column-major result and stores (or atomic adds) coalesce.  K advances 16 at
+// a time: both operand tiles go through LDS (k-major rows of 64, padded) and the
+// next tiles are loaded into registers while the current ones are multiplied.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kBigK = 16;
+constexpr int kBigLd = kBigTile + 1;  // padded LDS row
+
+// one K step of the tile: sP holds the row operand of D (the columns j of C) as
+// [k][j], sQ the column operand (the rows i of C) as [k][i]
+__device__ __forceinline__ void big_mma(const double* sP, const double* sQ, int wj, int wi, int lane,
+                                        f64x4 (&acc)[2][2]) {
+#pragma unroll
+  for (int ks = 0; ks < kBigK; ks += 4) {
+    double a[2], b[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) a[r] = sP[(ks + lane / 16) * kBigLd + wj + r * 16 + lane % 16];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) b[c] = sQ[(ks + lane / 16) * kBigLd + wi + c * 16 + lane % 16];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+  }
+}
+
+// G1: X = V_R Bhat^T for one 64 x 64 tile of one pair (V_R m x k, Bhat^T k x n)
+__global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles, const GTask1* __restrict__ tasks,
+                                              const double* __restrict__ V, const double* __restrict__ Bt,
+                                              double* __restrict__ X) {
+  __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+  const GTile1 t = tiles[blockIdx.x];
+  const GTask1 g = tasks[t.task];
+  const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+  // staging: V tile element (i = tid % 64, kk = tid / 64 + 4q), Bhat^T tile
+  // element (kk = tid % 16, j = tid / 16 + 16q), zero outside the matrices
+  double rv[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+      rv[q] = (i < m && kk < K) ? V[g.v_off + i + (int64_t)m * kk] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = k0 + tid % 16, j = j0 + tid / 16 + 16 * q;
+      rb[q] = (kk < K && j < n) ? Bt[g.b_off + kk + (int64_t)K * j] : 0.0;
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += kBigK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = rv[q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sP[(tid % 16) * kBigLd + tid / 16 + 16 * q] = rb[q];
+    __syncthreads();
+    if (k0 + kBigK < K) load(k0 + kBigK);
+    big_mma(sP, sQ, wj, wi, lane, acc);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + lane / 16 + 4 * v;
+        if (i < m && j < n) X[g.x_off + i + (int64_t)m * j] = acc[r][c][v];
+      }
+}
+
+// G2: one 64 x 64 tile of W_L += sum_k A_k Y_k over a chunk of a-terms, with
+// Y_k = sum_u r_u X_u formed while staging; the tile is added into W atomically
+__global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, const STerm2* __restrict__ terms,
+                                              const SXRef* __restrict__ xrefs, const double* __restrict__ Ad,
+                                              const double* __restrict__ X, double* __restrict__ W) {
+  __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+  const GTile2 t = tiles[blockIdx.x];
+  const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double ra[4], ry[4];
+  // staging: A tile element (i = tid % 64, kk = tid / 64 + 4q), Y tile element
+  // (kk = tid % 16, j = tid / 16 + 16q)
+  auto load = [&](const STerm2& tm, int k0) {
+    const int dar = tm.dar;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+      ra[q] = (i < da && kk < dar) ? Ad[tm.a_off + i + (int64_t)da * kk] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ry[q] = 0.0;
+    for (int x = tm.x_begin; x < tm.x_end; ++x) {
+      const SXRef xr = xrefs[x];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = k0 + tid % 16, j = j0 + tid / 16 + 16 * q;
+        if (kk < dar && j < db) ry[q] = fma(xr.r, X[xr.x_off + kk + (int64_t)dar * j], ry[q]);
+      }
+    }
+  };
+  int tc = t.t_begin, k0 = 0;
+  STerm2 tm = terms[tc];
+  load(tm, 0);
+  while (tc < t.t_end) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sP[(tid % 16) * kBigLd + tid / 16 + 16 * q] = ry[q];
+    __syncthreads();
+    // advance to the next K step (next term when this one is done) and prefetch it
+    k0 += kBigK;
+    if (k0 >= tm.dar) {
+      k0 = 0;
+      ++tc;
+      if (tc < t.t_end) tm = terms[tc];
+    }
+    if (tc < t.t_end) load(tm, k0);
+    big_mma(sP, sQ, wj, wi, lane, acc);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + lane / 16 + 4 * v;
+        if (i < da && j < db) unsafeAtomicAdd(W + t.w_off + i + (int64_t)da * j, acc[r][c][v]);
+      }
+}
+
 const void* k_stk_ptr(int stage, int prec, int nt) {
 #define NSX_STK(T)                                                                                  \
   if (stage == 1) return nt == 64 ? (const void*)k_stk1<T, 64> : nt == 128 ? (const void*)k_stk1<T, 128> \
@@ -708,6 +857,16 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
                     (void*)&d.lds_buf};
     hipError_t e = hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt), args, lds, s);
     if (e != hipSuccess) return e;
+  }
+  if (prec == 0 && at && d.n_gtile2 > 0) {  // large blocks on the matrix cores (FP64)
+    const double* Vd = reinterpret_cast<const double*>(V);
+    const double* Bd = reinterpret_cast<const double*>(Bh);
+    const double* Adp = reinterpret_cast<const double*>(Ad);
+    if (d.n_gtile1 > 0)
+      hipLaunchKernelGGL(k_big1, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1, Vd, Bd, d.XG);
+    hipLaunchKernelGGL(k_big2, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2, d.g_xref, Adp,
+                       (const double*)d.XG, W);
+    return hipGetLastError();
   }
   if (d.n_items > 0) {
     const size_t lds = sizeof(T) * (size_t)std::max(d.max_lds, 1);

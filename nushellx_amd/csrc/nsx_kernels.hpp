@@ -36,6 +36,14 @@ struct DevicePlan {
   SXRef* s_xref = nullptr;
   int n_chunk1 = 0, n_chunk2 = 0, s_lds1 = 1, s_lds2 = 1;
   void* X = nullptr;         // x_size elements of the accumulation type (8 bytes reserved each)
+  // large-block GEMM path (FP64, atomic mode)
+  GTask1* g_task1 = nullptr;
+  GTile1* g_tile1 = nullptr;
+  GTile2* g_tile2 = nullptr;
+  STerm2* g_term2 = nullptr;
+  SXRef* g_xref = nullptr;
+  int n_gtile1 = 0, n_gtile2 = 0;
+  double* XG = nullptr;      // xg_size doubles
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
 };
 

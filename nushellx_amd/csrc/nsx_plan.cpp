@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <array>
 #include <map>
 
 namespace nsx {
@@ -350,6 +351,74 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
     }
     p.s_chunk2.push_back(c);
     p.s_lds2 = std::max(p.s_lds2, c.lds);
+  }
+}
+
+void build_big(HostPlan& p, const std::vector<const Model*>& models) {
+  p.g_task1.clear(); p.g_tile1.clear(); p.g_tile2.clear(); p.g_term2.clear(); p.g_xref.clear();
+  p.xg_size = 0;
+  std::map<std::pair<int64_t, int64_t>, int64_t> x_of;  // (b_off, v_off) -> x_off
+  // terms of every left block (instance, L), in item order
+  std::map<std::pair<size_t, int>, std::vector<STerm2>> terms_of;
+  std::map<std::pair<size_t, int>, std::array<int64_t, 3>> block_of;  // -> (w_off, da, db)
+  for (size_t mi = 0; mi < models.size(); ++mi) {
+    const Model& m = *models[mi];
+    const int64_t vb = p.vbase[mi];
+    const auto& a_off = p.inst_a_off[mi];
+    const auto& b_off = p.inst_b_off[mi];
+    for (const Item& it : m.items) {
+      const int da = m.da[it.L], db = m.db[it.L], dar = m.A.d(it.ajr);
+      int maxdbr = 0;
+      for (const ATerm& at : it.at)
+        for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
+      if (std::max(std::max(da, dar), std::max(db, maxdbr)) <= kSmallMax) continue;  // small-block paths
+      block_of[{mi, it.L}] = {vb + m.posnv[it.L], da, db};
+      auto& tl = terms_of[{mi, it.L}];
+      for (const ATerm& at : it.at) {
+        STerm2 t{};
+        t.a_off = a_off[at.adens];
+        t.dar = dar;
+        t.x_begin = (int)p.g_xref.size();
+        for (const RTerm& rt : at.rt) {
+          const int64_t v = vb + m.posnv[rt.R], b = b_off[rt.bhat];
+          const int dbr = m.db[rt.R];
+          auto f = x_of.find({b, v});
+          int64_t x;
+          if (f == x_of.end()) {
+            x = p.xg_size;
+            p.xg_size += int64_t(dar) * db;
+            x_of[{b, v}] = x;
+            const int task = (int)p.g_task1.size();
+            p.g_task1.push_back(GTask1{v, b, x, dar, dbr, db, 0});
+            for (int j0 = 0; j0 < db; j0 += kBigTile)
+              for (int i0 = 0; i0 < dar; i0 += kBigTile) p.g_tile1.push_back(GTile1{task, i0, j0, 0});
+          } else {
+            x = f->second;
+          }
+          p.g_xref.push_back({x, rt.r});
+        }
+        t.x_end = (int)p.g_xref.size();
+        tl.push_back(t);
+      }
+    }
+  }
+  for (auto& kv : terms_of) {
+    const auto& blk = block_of[kv.first];
+    const int da = (int)blk[1], db = (int)blk[2];
+    const auto& tl = kv.second;
+    size_t t = 0;
+    while (t < tl.size()) {  // split-K: chunks of terms covering about kBigKChunk
+      const int t_begin = (int)p.g_term2.size();
+      int kk = 0;
+      while (t < tl.size() && (kk == 0 || kk + tl[t].dar <= kBigKChunk)) {
+        kk += tl[t].dar;
+        p.g_term2.push_back(tl[t++]);
+      }
+      const int t_end = (int)p.g_term2.size();
+      for (int j0 = 0; j0 < db; j0 += kBigTile)
+        for (int i0 = 0; i0 < da; i0 += kBigTile)
+          p.g_tile2.push_back(GTile2{blk[0], da, db, i0, j0, t_begin, t_end});
+    }
   }
 }
 

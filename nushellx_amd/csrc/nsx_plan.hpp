@@ -115,6 +115,26 @@ struct SXRef {
   double r;
 };
 
+// Large-block path (blocks with a side above kSmallMax, FP64): the same two
+// stages as tiled GEMMs on the matrix cores, 64 x 64 output tiles.
+//   G1: X_u = V_R Bhat^T for every distinct (V block, Bhat block) pair;
+//   G2: W_L += sum_k A_k Y_k, Y_k = sum_u r_u X_u, tiles of W_L over chunks of
+//       its a-terms (split-K), added into W with FP64 atomics.
+constexpr int kBigTile = 64;     // output tile side
+constexpr int kBigKChunk = 2048; // inner dimension one stage-2 tile covers (split-K)
+struct GTask1 {
+  int64_t v_off, b_off, x_off;   // V_R (m x k), Bhat^T (k x n), X (m x n), column-major
+  int32_t m, k, n, pad;
+};
+struct GTile1 {
+  int32_t task, i0, j0, pad;
+};
+struct GTile2 {
+  int64_t w_off;                 // W_L (da x db, column-major)
+  int32_t da, db, i0, j0;
+  int32_t t_begin, t_end;        // a-terms (STerm2 with lds unused) of this tile's K chunk
+};
+
 // Tunable plan shape (chosen by the autotuner, see nsx_tune.cpp).
 struct PlanConfig {
   int fat_cap = kFatCap;     // doubles one staged phase may load (<= kFatCap)
@@ -153,6 +173,13 @@ struct HostPlan {
   std::vector<SXRef> s_xref;
   int64_t x_size = 0;          // X buffer elements
   int s_lds1 = 1, s_lds2 = 1;  // largest chunk LDS (doubles)
+  // large-block path (filled by build_big)
+  std::vector<GTask1> g_task1;
+  std::vector<GTile1> g_tile1;
+  std::vector<GTile2> g_tile2;
+  std::vector<STerm2> g_term2;
+  std::vector<SXRef> g_xref;
+  int64_t xg_size = 0;         // large-block X buffer (doubles)
   int64_t N = 0, part_size = 0;
   int max_lds = 0;                  // doubles of LDS the pn kernel needs
   double flops_ref = 0.0;           // algorithmic work of one batched H*v
@@ -171,5 +198,8 @@ void plan_merge(HostPlan& p, const HostPlan& q);
 // (the instances planned into p, in order).  Clears p.cfg.stacked (the staged
 // kernel then runs) if an a-term has more references than a chunk can hold.
 void build_stacked(HostPlan& p, const std::vector<const Model*>& models);
+// Fills the large-block GEMM arrays of p for the items the direct kernel would
+// take (a side above kSmallMax).
+void build_big(HostPlan& p, const std::vector<const Model*>& models);
 
 }  // namespace nsx

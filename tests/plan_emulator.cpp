@@ -50,8 +50,10 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
     }
     for (size_t q = 0; q < W.size(); ++q) part[it.part_off + q] = W[q];
   }
-  // direct items (large blocks)
+  // direct items (large blocks), unless the large-block GEMM path takes them
+  const bool big = !p.g_tile2.empty();
   for (const DItem& it : p.items) {
+    if (big) break;
     for (int jj = 0; jj < it.nc; ++jj) {
       const int j = it.j0 + jj;
       std::vector<double> w(it.da, 0.0);
@@ -97,6 +99,34 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
       }
     }
   }
+  if (!big) return;
+  // large-block path: X_u = V_R Bhat^T per task, then W tiles += sum_k A_k sum_u r_u X_u
+  std::vector<double> XG(size_t(p.xg_size), 0.0);
+  std::vector<char> tile_done(p.g_tile1.size(), 0);
+  for (const GTile1& t : p.g_tile1) {
+    const GTask1& g = p.g_task1[t.task];
+    for (int j = t.j0; j < std::min(t.j0 + kBigTile, g.n); ++j)
+      for (int i = t.i0; i < std::min(t.i0 + kBigTile, g.m); ++i) {
+        double s = 0.0;
+        for (int k = 0; k < g.k; ++k) s += vin[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
+        XG[g.x_off + i + int64_t(g.m) * j] = s;
+      }
+  }
+  for (const GTile2& t : p.g_tile2)
+    for (int j = t.j0; j < std::min(t.j0 + kBigTile, t.db); ++j)
+      for (int i = t.i0; i < std::min(t.i0 + kBigTile, t.da); ++i) {
+        double s = 0.0;
+        for (int q = t.t_begin; q < t.t_end; ++q) {
+          const STerm2& tm = p.g_term2[q];
+          for (int k = 0; k < tm.dar; ++k) {
+            double y = 0.0;
+            for (int x = tm.x_begin; x < tm.x_end; ++x)
+              y += p.g_xref[x].r * XG[p.g_xref[x].x_off + k + int64_t(tm.dar) * j];
+            s += p.Ad[tm.a_off + i + int64_t(t.da) * k] * y;
+          }
+        }
+        vout[t.w_off + i + int64_t(t.da) * j] += s;
+      }
 }
 
 extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const char* intf,
@@ -107,6 +137,7 @@ extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const c
   if (m.N != n) { g_err = "dimension mismatch"; return -2; }
   HostPlan p;
   plan_add(p, m);
+  build_big(p, {&m});
   emulate(p, vin, vout);
   return 0;
 }
