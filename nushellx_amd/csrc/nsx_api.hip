@@ -83,10 +83,12 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
     if (!(e = up(d.g_task1, hp.g_task1)).empty()) return e;
     if (!(e = up(d.g_tile1, hp.g_tile1)).empty()) return e;
     if (!(e = up(d.g_tile2, hp.g_tile2)).empty()) return e;
+    if (!(e = up(d.g_tile3, hp.g_tile3)).empty()) return e;
     if (!(e = up(d.g_term2, hp.g_term2)).empty()) return e;
     if (!(e = up(d.g_xref, hp.g_xref)).empty()) return e;
     d.n_gtile1 = (int)hp.g_tile1.size();
     d.n_gtile2 = (int)hp.g_tile2.size();
+    d.n_gtile3 = (int)hp.g_tile3.size();
     if (hp.xg_size > 0 && hipMalloc(&d.XG, sizeof(double) * (size_t)hp.xg_size) != hipSuccess)
       return "hipMalloc failed (large-block X buffer)";
     d.atomic = 1;
@@ -121,6 +123,7 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.s_chunk1); (void)hipFree(d.s_unit1); (void)hipFree(d.s_chunk2); (void)hipFree(d.s_term2);
   (void)hipFree(d.s_xref); (void)hipFree(d.X);
   (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
+  (void)hipFree(d.g_tile3);
   (void)hipFree(d.g_xref); (void)hipFree(d.XG);
   d = DevicePlan();
 }

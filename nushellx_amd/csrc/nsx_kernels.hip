@@ -287,8 +287,9 @@ template <typename T, int NT>
 __device__ __forceinline__ void reduce_block(DBlock b, const int64_t* __restrict__ part_list,
                                              const double* __restrict__ part, const T* __restrict__ V,
                                              const T* __restrict__ H, double* __restrict__ W, int use_partials,
-                                             T* sm, int cap) {
+                                             T* sm, int cap, int skip_large) {
   using Acc = typename Prec<T>::acc;
+  if (skip_large && (b.da > kSmallMax || b.db > kSmallMax)) return;  // the large-block GEMM tiles do it
   if (!use_partials) b.p_end = b.p_begin;
   const int da = b.da, db = b.db, nW = da * b.nc;
   const T* v = V + b.v_off;
@@ -356,9 +357,10 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_reduce(
     const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
     const double* __restrict__ part, const T* __restrict__ V, const T* __restrict__ H,
-    double* __restrict__ W, int use_partials) {
+    double* __restrict__ W, int use_partials, int skip_large) {
   __shared__ T sm[kReduceLds];
-  reduce_block<T, kThreads>(blocks[blockIdx.x], part_list, part, V, H, W, use_partials, sm, kReduceLds);
+  reduce_block<T, kThreads>(blocks[blockIdx.x], part_list, part, V, H, W, use_partials, sm, kReduceLds,
+                            skip_large);
 }
 
 // ---------------------------------------------------------------- stacked two-stage path
@@ -400,14 +402,14 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_stk1(
     const SChunk1* __restrict__ chunks, int n_chunks, const SUnit1* __restrict__ units, const T* __restrict__ V,
     const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X, const DBlock* __restrict__ blocks,
-    int n_blocks, int g_red, const T* __restrict__ H, double* __restrict__ W, int lds_cap) {
+    int n_blocks, int g_red, const T* __restrict__ H, double* __restrict__ W, int lds_cap, int skip_large) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   __shared__ SUnit1 su[kStkMeta];
   if ((int)blockIdx.x < g_red) {
     if ((int)blockIdx.x < n_blocks)
-      reduce_block<T, NT>(blocks[blockIdx.x], nullptr, nullptr, V, H, W, 0, sm, lds_cap);
+      reduce_block<T, NT>(blocks[blockIdx.x], nullptr, nullptr, V, H, W, 0, sm, lds_cap, skip_large);
     return;
   }
   const int ci = xcd_chunk((int)blockIdx.x - g_red, n_chunks);
@@ -762,6 +764,61 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, 
       }
 }
 
+// G0: W_L = H_a V_L + V_L H_b^T for one 64 x 64 tile of a large block, stored
+// (it runs before the pn kernels add into W).  Two K phases: H_a (da x da) times
+// V_L (da x db), then V_L times H_b^T, where H_b is symmetric: H_b^T(k, j) = H_b(k, j).
+__global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, const double* __restrict__ V,
+                                              const double* __restrict__ H, double* __restrict__ W) {
+  __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+  const GTile3 t = tiles[blockIdx.x];
+  const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const double* v = V + t.v_off;
+  // phase 0: A = H_a (ld da), Y = V_L (ld da), K = da; phase 1: A = V_L (ld da), Y = H_b (ld db), K = db
+  for (int ph = 0; ph < 2; ++ph) {
+    const int64_t hoff = ph == 0 ? t.ha_off : t.hb_off;
+    if (hoff < 0) continue;
+    const double* Am = ph == 0 ? H + hoff : v;
+    const double* Ym = ph == 0 ? v : H + hoff;
+    const int K = ph == 0 ? da : db, ldy = ph == 0 ? da : db;
+    for (int k0 = 0; k0 < K; k0 += kBigK) {
+      double ra[4], ry[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+        ra[q] = (i < da && kk < K) ? Am[i + (int64_t)da * kk] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = k0 + tid % 16, j = j0 + tid / 16 + 16 * q;
+        ry[q] = (kk < K && j < db) ? Ym[kk + (int64_t)ldy * j] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sP[(tid % 16) * kBigLd + tid / 16 + 16 * q] = ry[q];
+      __syncthreads();
+      big_mma(sP, sQ, wj, wi, lane, acc);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + lane / 16 + 4 * q;
+        if (i < da && j < db) W[t.v_off + i + (int64_t)da * j] = acc[r][c][q];
+      }
+}
+
 const void* k_stk_ptr(int stage, int prec, int nt) {
 #define NSX_STK(T)                                                                                  \
   if (stage == 1) return nt == 64 ? (const void*)k_stk1<T, 64> : nt == 128 ? (const void*)k_stk1<T, 128> \
@@ -818,6 +875,10 @@ const void* k_pn_partial_ptr(int prec, bool atomic) {
 }
 
 // ---------------------------------------------------------------- launchers
+// FP64 plans in atomic mode run their large blocks on the GEMM tiles: the
+// like-particle part (k_big0, before the pn kernels) and the pn part (k_big1, k_big2)
+static bool big_like_active(const DevicePlan& d, int prec) { return prec == 0 && d.atomic && d.n_gtile3 > 0; }
+
 template <typename T>
 static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H, double* W,
                        hipStream_t s) {
@@ -832,9 +893,10 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     if (g_red + g1 > 0) {
       const T* Hs = H;
       const int cap = d.s_lds1;
+      const int skip = big_like_active(d, prec) ? 1 : 0;
       void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X,
                     (void*)&d.blocks,   (void*)&d.n_blocks, (void*)&g_red,    (void*)&Hs, (void*)&W,
-                    (void*)&cap};
+                    (void*)&cap,        (void*)&skip};
       hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3((unsigned)(g_red + g1)), dim3(d.stk_nt),
                                      a1, sizeof(T) * (size_t)d.s_lds1, s);
       if (e != hipSuccess) return e;
@@ -879,11 +941,15 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
 }
 
 template <typename T>
-static hipError_t reduce_T(const DevicePlan& d, const T* V, const T* H, double* W, hipStream_t s) {
-  if (d.stacked) return hipSuccess;  // the stacked stage-1 launch carries the like-particle blocks
-  if (d.n_blocks > 0)
+static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H, double* W, hipStream_t s) {
+  const bool big = big_like_active(d, prec);
+  // the stacked stage-1 launch carries the small like-particle blocks
+  if (!d.stacked && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
-                       V, H, W, d.atomic ? 0 : 1);
+                       V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
+  if (big)
+    hipLaunchKernelGGL(k_big0, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3,
+                       reinterpret_cast<const double*>(V), reinterpret_cast<const double*>(H), W);
   return hipGetLastError();
 }
 
@@ -903,9 +969,9 @@ hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const do
 hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                          hipStream_t s) {
   switch (prec) {
-    case 0: return reduce_T<double>(d, V, d.H, W, s);
-    case 1: return reduce_T<float>(d, (const float*)lp.vin, (const float*)lp.H, W, s);
-    default: return reduce_T<Bf16x2>(d, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s);
+    case 0: return reduce_T<double>(d, 0, V, d.H, W, s);
+    case 1: return reduce_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.H, W, s);
+    default: return reduce_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s);
   }
 }
 

@@ -40,9 +40,10 @@ struct DevicePlan {
   GTask1* g_task1 = nullptr;
   GTile1* g_tile1 = nullptr;
   GTile2* g_tile2 = nullptr;
+  GTile3* g_tile3 = nullptr;
   STerm2* g_term2 = nullptr;
   SXRef* g_xref = nullptr;
-  int n_gtile1 = 0, n_gtile2 = 0;
+  int n_gtile1 = 0, n_gtile2 = 0, n_gtile3 = 0;
   double* XG = nullptr;      // xg_size doubles
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
 };

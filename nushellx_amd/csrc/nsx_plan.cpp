@@ -355,8 +355,16 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
 }
 
 void build_big(HostPlan& p, const std::vector<const Model*>& models) {
-  p.g_task1.clear(); p.g_tile1.clear(); p.g_tile2.clear(); p.g_term2.clear(); p.g_xref.clear();
+  p.g_task1.clear(); p.g_tile1.clear(); p.g_tile2.clear(); p.g_tile3.clear(); p.g_term2.clear();
+  p.g_xref.clear();
   p.xg_size = 0;
+  // like-particle tiles of every large block (the reduce kernels skip these blocks)
+  for (const DBlock& b : p.blocks) {
+    if (b.j0 != 0 || std::max(b.da, b.db) <= kSmallMax) continue;
+    for (int j0 = 0; j0 < b.db; j0 += kBigTile)
+      for (int i0 = 0; i0 < b.da; i0 += kBigTile)
+        p.g_tile3.push_back(GTile3{b.v_off, b.ha_off, b.hb_off, b.da, b.db, i0, j0});
+  }
   std::map<std::pair<int64_t, int64_t>, int64_t> x_of;  // (b_off, v_off) -> x_off
   // terms of every left block (instance, L), in item order
   std::map<std::pair<size_t, int>, std::vector<STerm2>> terms_of;

@@ -134,6 +134,12 @@ struct GTile2 {
   int32_t da, db, i0, j0;
   int32_t t_begin, t_end;        // a-terms (STerm2 with lds unused) of this tile's K chunk
 };
+// G0: the like-particle part of a large block, W_L = H_a V_L + V_L H_b^T, one
+// 64 x 64 tile each (stored, so it initialises W_L before G2 adds into it)
+struct GTile3 {
+  int64_t v_off, ha_off, hb_off; // ha_off / hb_off < 0: no matrix on that side
+  int32_t da, db, i0, j0;
+};
 
 // Tunable plan shape (chosen by the autotuner, see nsx_tune.cpp).
 struct PlanConfig {
@@ -177,6 +183,7 @@ struct HostPlan {
   std::vector<GTask1> g_task1;
   std::vector<GTile1> g_tile1;
   std::vector<GTile2> g_tile2;
+  std::vector<GTile3> g_tile3;
   std::vector<STerm2> g_term2;
   std::vector<SXRef> g_xref;
   int64_t xg_size = 0;         // large-block X buffer (doubles)
