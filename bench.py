@@ -282,6 +282,56 @@ def separable(log, mf, cuts=((0.0, 0.0), (0.3, 0.05), (1.0, 0.1))):
     return out
 
 
+def large(log, steps: int = 5, cpu: bool = True):
+    """One large instance: pf shell (0f7/2 1p3/2 0f5/2 1p1/2), 6 protons + 6
+    neutrons, J = 0 (N = 1,777,116; blocks up to 615 x 615), seeded interaction
+    (TBMEs ~ N(0, 1.5 MeV), SPEs ~ U[-10, -4] MeV).  Its blocks run on the FP64
+    matrix-core tiles.  Reports the FP64 H*v time, the reference's sparse flops
+    per second, and one H*v of the CPU restatement on the host cores."""
+    from nushellx_amd.hv import DeviceBuffer, Hamiltonian, device_sync
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import PF, PNSpace
+    from nushellx_amd.synth.writer import PNModel, write_model
+    work = tempfile.mkdtemp(prefix="nsx_pf66_")
+    t0 = time.perf_counter()
+    log("large: writing pf 6p6n J=0 operator files")
+    mf = write_model(PNModel(PNSpace(PF), 6, 6, random_interaction(PF, seed=SEED, spe_range=(-10.0, -4.0))),
+                     work, "pf66", [0])
+    t_files = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h = Hamiltonian(*mf.args(0))
+    t_setup = time.perf_counter() - t0
+    flops, flops_like, _ = h.work()
+    v = np.random.default_rng(1).standard_normal(h.N)
+    din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+    h.multiply_dev(din, dout)
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.multiply_dev(din, dout)
+    device_sync()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    out = {"workload": "pf 6p6n J=0, N=%d, one FP64 H*v" % h.N, "files_s": t_files, "setup_s": t_setup,
+           "hv_ms": ms, "flops_ref": flops, "TFLOPs_ref": flops / (ms * 1e-3) / 1e12}
+    log(f"large: pf 6p6n H*v {ms:.1f} ms ({out['TFLOPs_ref']:.2f} TFLOP/s of the reference's sparse work)")
+    if cpu:
+        from oracle.refcpu import RefCPU
+        r = RefCPU(*mf.args(0), fp64=False, thresholds=True)
+        threads = min(os.cpu_count() or 1, 16)
+        r.set_threads(threads)
+        t0 = time.perf_counter()
+        r.apply(v)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_port_s"] = cpu_s
+        out["cpu_threads"] = threads
+        out["gpu_over_cpu"] = cpu_s / (ms * 1e-3)
+        log(f"large: CPU restatement {cpu_s:.2f} s per H*v on {threads} threads")
+    din.free()
+    dout.free()
+    h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +345,8 @@ def main():
     ap.add_argument("--ensemble-samples", type=int, default=256,
                     help="interaction samples of the batched ensemble (0 skips it)")
     ap.add_argument("--no-sep", action="store_true", help="skip the separable-operator (NuTRLZ) timings")
+    ap.add_argument("--large", action="store_true",
+                    help="also time one pf 6p6n J=0 H*v (N = 1.78 M; about 3 minutes of file writing)")
     ap.add_argument("--pf-warm-samples", type=int, default=0,
                     help="training samples of the pf 4p4n warm-start study (0 skips it; 512 takes ~8 min)")
     args = ap.parse_args()
@@ -381,6 +433,8 @@ def main():
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
     if not args.no_sep:
         out["separable"] = separable(log, mf)
+    if args.large:
+        out["large"] = large(log, cpu=not args.no_cpu)
     if args.ensemble_samples > 0:
         out["ensemble"] = ensemble(log, args.ensemble_samples)
     if args.pf_warm_samples > 0:
