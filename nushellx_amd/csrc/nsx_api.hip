@@ -91,6 +91,9 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
     d.n_gtile3 = (int)hp.g_tile3.size();
     if (hp.xg_size > 0 && hipMalloc(&d.XG, sizeof(double) * (size_t)hp.xg_size) != hipSuccess)
       return "hipMalloc failed (large-block X buffer)";
+    // X tiles that vanish identically are never computed: they stay zero
+    if (hp.xg_size > 0 && hipMemset(d.XG, 0, sizeof(double) * (size_t)hp.xg_size) != hipSuccess)
+      return "hipMemset failed (large-block X buffer)";
     d.atomic = 1;
   }
   if (!(e = up(d.items, hp.items)).empty()) return e;

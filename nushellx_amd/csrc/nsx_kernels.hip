@@ -624,8 +624,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
 // a time: both operand tiles go through LDS (k-major rows of 64, padded) and the
 // next tiles are loaded into registers while the current ones are multiplied.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-constexpr int kBigK = 16;
+constexpr int kBigK = kBigStep;       // K step (rows of each LDS tile)
+constexpr int kBigQ = kBigK / 4;      // staged elements per thread and operand (256 threads, 64 x kBigK)
+constexpr int kBigJS = 256 / kBigK;   // column stride of the k-major staging map
 constexpr int kBigLd = kBigTile + 1;  // padded LDS row
+
+// lowest set bit of the 128-bit step mask (m0, m1), cleared; -1 when empty
+// (selects on values, so that the two words stay in registers)
+__device__ __forceinline__ int pop_step(uint64_t& m0, uint64_t& m1) {
+  const bool lo = m0 != 0;
+  const uint64_t m = lo ? m0 : m1;
+  if (!m) return -1;
+  const int b = __builtin_ctzll(m);
+  const uint64_t rest = m & (m - 1);
+  m0 = lo ? rest : m0;
+  m1 = lo ? m1 : rest;
+  return lo ? b : 64 + b;
+}
 
 // one K step of the tile: sP holds the row operand of D (the columns j of C) as
 // [k][j], sQ the column operand (the rows i of C) as [k][i]
@@ -662,28 +677,32 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles, 
     for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
   // staging: V tile element (i = tid % 64, kk = tid / 64 + 4q), Bhat^T tile
   // element (kk = tid % 16, j = tid / 16 + 16q), zero outside the matrices
-  double rv[4], rb[4];
+  double rv[kBigQ], rb[kBigQ];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kBigQ; ++q) {
       const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
       rv[q] = (i < m && kk < K) ? V[g.v_off + i + (int64_t)m * kk] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int kk = k0 + tid % 16, j = j0 + tid / 16 + 16 * q;
+    for (int q = 0; q < kBigQ; ++q) {
+      const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
       rb[q] = (kk < K && j < n) ? Bt[g.b_off + kk + (int64_t)K * j] : 0.0;
     }
   };
-  load(0);
-  for (int k0 = 0; k0 < K; k0 += kBigK) {
+  // only the K steps where this tile's Bhat^T columns are not all zero
+  uint64_t m0 = t.kmask[0], m1 = t.kmask[1];
+  int step = pop_step(m0, m1);
+  if (step >= 0) load(step * kBigK);
+  while (step >= 0) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = rv[q];
+    for (int q = 0; q < kBigQ; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = rv[q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sP[(tid % 16) * kBigLd + tid / 16 + 16 * q] = rb[q];
+    for (int q = 0; q < kBigQ; ++q) sP[(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = rb[q];
     __syncthreads();
-    if (k0 + kBigK < K) load(k0 + kBigK);
+    step = pop_step(m0, m1);
+    if (step >= 0) load(step * kBigK);
     big_mma(sP, sQ, wj, wi, lane, acc);
   }
 #pragma unroll
@@ -699,7 +718,7 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles, 
 
 // G2: one 64 x 64 tile of W_L += sum_k A_k Y_k over a chunk of a-terms, with
 // Y_k = sum_u r_u X_u formed while staging; the tile is added into W atomically
-__global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, const STerm2* __restrict__ terms,
+__global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, const GTerm2* __restrict__ terms,
                                               const SXRef* __restrict__ xrefs, const double* __restrict__ Ad,
                                               const double* __restrict__ X, double* __restrict__ W) {
   __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
@@ -712,45 +731,56 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, 
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-  double ra[4], ry[4];
+  double ra[kBigQ], ry[kBigQ];
   // staging: A tile element (i = tid % 64, kk = tid / 64 + 4q), Y tile element
-  // (kk = tid % 16, j = tid / 16 + 16q)
-  auto load = [&](const STerm2& tm, int k0) {
+  // (kk = tid % kBigK, j = tid / kBigK + kBigJS q)
+  auto load = [&](const GTerm2& tm, int k0) {
     const int dar = tm.dar;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kBigQ; ++q) {
       const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
       ra[q] = (i < da && kk < dar) ? Ad[tm.a_off + i + (int64_t)da * kk] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ry[q] = 0.0;
+    for (int q = 0; q < kBigQ; ++q) ry[q] = 0.0;
     for (int x = tm.x_begin; x < tm.x_end; ++x) {
       const SXRef xr = xrefs[x];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = k0 + tid % 16, j = j0 + tid / 16 + 16 * q;
+      for (int q = 0; q < kBigQ; ++q) {
+        const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
         if (kk < dar && j < db) ry[q] = fma(xr.r, X[xr.x_off + kk + (int64_t)dar * j], ry[q]);
       }
     }
   };
-  int tc = t.t_begin, k0 = 0;
-  STerm2 tm = terms[tc];
-  load(tm, 0);
-  while (tc < t.t_end) {
+  // (term, K step) pairs in order, only the steps where the term's A rows of
+  // this tile are not all zero
+  int tc = t.t_begin;
+  GTerm2 tm = terms[tc];
+  uint64_t m0 = tm.kmask[0], m1 = tm.kmask[1];
+  int step = pop_step(m0, m1);
+  while (step < 0 && ++tc < t.t_end) {
+    tm = terms[tc];
+    m0 = tm.kmask[0];
+    m1 = tm.kmask[1];
+    step = pop_step(m0, m1);
+  }
+  if (step >= 0) load(tm, step * kBigK);
+  while (step >= 0) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
+    for (int q = 0; q < kBigQ; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sP[(tid % 16) * kBigLd + tid / 16 + 16 * q] = ry[q];
+    for (int q = 0; q < kBigQ; ++q) sP[(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = ry[q];
     __syncthreads();
-    // advance to the next K step (next term when this one is done) and prefetch it
-    k0 += kBigK;
-    if (k0 >= tm.dar) {
-      k0 = 0;
-      ++tc;
-      if (tc < t.t_end) tm = terms[tc];
+    // next (term, step) pair, prefetched before this step's products
+    step = pop_step(m0, m1);
+    while (step < 0 && ++tc < t.t_end) {
+      tm = terms[tc];
+      m0 = tm.kmask[0];
+      m1 = tm.kmask[1];
+      step = pop_step(m0, m1);
     }
-    if (tc < t.t_end) load(tm, k0);
+    if (step >= 0) load(tm, step * kBigK);
     big_mma(sP, sQ, wj, wi, lane, acc);
   }
 #pragma unroll
@@ -788,22 +818,22 @@ __global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, 
     const double* Ym = ph == 0 ? v : H + hoff;
     const int K = ph == 0 ? da : db, ldy = ph == 0 ? da : db;
     for (int k0 = 0; k0 < K; k0 += kBigK) {
-      double ra[4], ry[4];
+      double ra[kBigQ], ry[kBigQ];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kBigQ; ++q) {
         const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
         ra[q] = (i < da && kk < K) ? Am[i + (int64_t)da * kk] : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = k0 + tid % 16, j = j0 + tid / 16 + 16 * q;
+      for (int q = 0; q < kBigQ; ++q) {
+        const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
         ry[q] = (kk < K && j < db) ? Ym[kk + (int64_t)ldy * j] : 0.0;
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
+      for (int q = 0; q < kBigQ; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sP[(tid % 16) * kBigLd + tid / 16 + 16 * q] = ry[q];
+      for (int q = 0; q < kBigQ; ++q) sP[(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = ry[q];
       __syncthreads();
       big_mma(sP, sQ, wj, wi, lane, acc);
     }

@@ -41,7 +41,7 @@ struct DevicePlan {
   GTile1* g_tile1 = nullptr;
   GTile2* g_tile2 = nullptr;
   GTile3* g_tile3 = nullptr;
-  STerm2* g_term2 = nullptr;
+  GTerm2* g_term2 = nullptr;
   SXRef* g_xref = nullptr;
   int n_gtile1 = 0, n_gtile2 = 0, n_gtile3 = 0;
   double* XG = nullptr;      // xg_size doubles

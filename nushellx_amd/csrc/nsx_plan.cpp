@@ -354,10 +354,27 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
   }
 }
 
+// K-step mask of the columns [c0, c0 + w) x rows [r0, r0 + h) of a column-major
+// matrix M (ld rows), over steps of kBigStep along the columns (col_steps) or
+// along the rows (!col_steps)
+static void step_mask(const double* M, int ld, int r0, int h, int c0, int w, bool col_steps, uint64_t (&mask)[2]) {
+  mask[0] = mask[1] = 0;
+  for (int c = c0; c < c0 + w; ++c)
+    for (int r = r0; r < r0 + h; ++r)
+      if (M[r + (int64_t)ld * c] != 0.0) {
+        const int s = (col_steps ? c : r) / kBigStep;
+        mask[s >> 6] |= uint64_t(1) << (s & 63);
+      }
+}
+
 void build_big(HostPlan& p, const std::vector<const Model*>& models) {
   p.g_task1.clear(); p.g_tile1.clear(); p.g_tile2.clear(); p.g_tile3.clear(); p.g_term2.clear();
   p.g_xref.clear();
   p.xg_size = 0;
+  // the K-step masks cover inner dimensions up to kBigMaxK; larger blocks keep the direct kernel
+  for (size_t mi = 0; mi < models.size(); ++mi)
+    for (int L = 0; L < models[mi]->basd; ++L)
+      if (models[mi]->da[L] > kBigMaxK || models[mi]->db[L] > kBigMaxK) return;
   // like-particle tiles of every large block (the reduce kernels skip these blocks)
   for (const DBlock& b : p.blocks) {
     if (b.j0 != 0 || std::max(b.da, b.db) <= kSmallMax) continue;
@@ -365,9 +382,10 @@ void build_big(HostPlan& p, const std::vector<const Model*>& models) {
       for (int i0 = 0; i0 < b.da; i0 += kBigTile)
         p.g_tile3.push_back(GTile3{b.v_off, b.ha_off, b.hb_off, b.da, b.db, i0, j0});
   }
-  std::map<std::pair<int64_t, int64_t>, int64_t> x_of;  // (b_off, v_off) -> x_off
-  // terms of every left block (instance, L), in item order
-  std::map<std::pair<size_t, int>, std::vector<STerm2>> terms_of;
+  struct T { int64_t a_off; int dar; std::vector<std::pair<int, double>> u; };  // refs: (task, r)
+  std::map<std::pair<int64_t, int64_t>, int> task_of;      // (b_off, v_off) -> task
+  std::vector<std::vector<char>> rows_needed;               // per task: i-tiles of X some term reads
+  std::map<std::pair<size_t, int>, std::vector<T>> terms_of;
   std::map<std::pair<size_t, int>, std::array<int64_t, 3>> block_of;  // -> (w_off, da, db)
   for (size_t mi = 0; mi < models.size(); ++mi) {
     const Model& m = *models[mi];
@@ -383,49 +401,84 @@ void build_big(HostPlan& p, const std::vector<const Model*>& models) {
       block_of[{mi, it.L}] = {vb + m.posnv[it.L], da, db};
       auto& tl = terms_of[{mi, it.L}];
       for (const ATerm& at : it.at) {
-        STerm2 t{};
+        T t;
         t.a_off = a_off[at.adens];
         t.dar = dar;
-        t.x_begin = (int)p.g_xref.size();
+        // X rows this term reads: the columns of A that are not all zero
+        const double* A = p.Ad.data() + t.a_off;
+        std::vector<char> used((dar + kBigTile - 1) / kBigTile, 0);
+        for (int c = 0; c < dar; ++c)
+          for (int r = 0; r < da; ++r)
+            if (A[r + (int64_t)da * c] != 0.0) { used[c / kBigTile] = 1; break; }
         for (const RTerm& rt : at.rt) {
           const int64_t v = vb + m.posnv[rt.R], b = b_off[rt.bhat];
-          const int dbr = m.db[rt.R];
-          auto f = x_of.find({b, v});
-          int64_t x;
-          if (f == x_of.end()) {
-            x = p.xg_size;
+          auto f = task_of.find({b, v});
+          int task;
+          if (f == task_of.end()) {
+            task = (int)p.g_task1.size();
+            task_of[{b, v}] = task;
+            p.g_task1.push_back(GTask1{v, b, p.xg_size, dar, m.db[rt.R], db, 0});
             p.xg_size += int64_t(dar) * db;
-            x_of[{b, v}] = x;
-            const int task = (int)p.g_task1.size();
-            p.g_task1.push_back(GTask1{v, b, x, dar, dbr, db, 0});
-            for (int j0 = 0; j0 < db; j0 += kBigTile)
-              for (int i0 = 0; i0 < dar; i0 += kBigTile) p.g_tile1.push_back(GTile1{task, i0, j0, 0});
+            rows_needed.emplace_back(used.size(), 0);
           } else {
-            x = f->second;
+            task = f->second;
           }
-          p.g_xref.push_back({x, rt.r});
+          for (size_t q = 0; q < used.size(); ++q) rows_needed[task][q] |= used[q];
+          t.u.push_back({task, rt.r});
         }
-        t.x_end = (int)p.g_xref.size();
-        tl.push_back(t);
+        tl.push_back(std::move(t));
       }
     }
   }
+  // stage 1: the X tiles some term reads, with the K steps of their Bhat^T columns
+  for (size_t task = 0; task < p.g_task1.size(); ++task) {
+    const GTask1& g = p.g_task1[task];
+    const double* Bt = p.Bh.data() + g.b_off;
+    for (int j0 = 0; j0 < g.n; j0 += kBigTile) {
+      GTile1 t{};
+      step_mask(Bt, g.k, 0, g.k, j0, std::min(kBigTile, g.n - j0), false, t.kmask);
+      if (!(t.kmask[0] | t.kmask[1])) continue;  // X columns identically zero
+      for (int i0 = 0; i0 < g.m; i0 += kBigTile) {
+        if (!rows_needed[task][i0 / kBigTile]) continue;
+        t.task = (int)task;
+        t.i0 = i0;
+        t.j0 = j0;
+        p.g_tile1.push_back(t);
+      }
+    }
+  }
+  // stage 2: per left block and row tile, the terms whose A rows there are not
+  // all zero, with their K steps, in chunks of about kBigKChunk active steps
   for (auto& kv : terms_of) {
     const auto& blk = block_of[kv.first];
     const int da = (int)blk[1], db = (int)blk[2];
-    const auto& tl = kv.second;
-    size_t t = 0;
-    while (t < tl.size()) {  // split-K: chunks of terms covering about kBigKChunk
-      const int t_begin = (int)p.g_term2.size();
-      int kk = 0;
-      while (t < tl.size() && (kk == 0 || kk + tl[t].dar <= kBigKChunk)) {
-        kk += tl[t].dar;
-        p.g_term2.push_back(tl[t++]);
+    for (int i0 = 0; i0 < da; i0 += kBigTile) {
+      std::vector<GTerm2> row;
+      for (const T& t : kv.second) {
+        GTerm2 g{};
+        step_mask(p.Ad.data() + t.a_off, da, i0, std::min(kBigTile, da - i0), 0, t.dar, true, g.kmask);
+        if (!(g.kmask[0] | g.kmask[1])) continue;
+        g.a_off = t.a_off;
+        g.dar = t.dar;
+        g.x_begin = (int)p.g_xref.size();
+        for (auto& u : t.u) p.g_xref.push_back({p.g_task1[u.first].x_off, u.second});
+        g.x_end = (int)p.g_xref.size();
+        row.push_back(g);
       }
-      const int t_end = (int)p.g_term2.size();
-      for (int j0 = 0; j0 < db; j0 += kBigTile)
-        for (int i0 = 0; i0 < da; i0 += kBigTile)
+      size_t q = 0;
+      while (q < row.size()) {
+        const int t_begin = (int)p.g_term2.size();
+        int steps = 0;
+        while (q < row.size()) {
+          const int s = __builtin_popcountll(row[q].kmask[0]) + __builtin_popcountll(row[q].kmask[1]);
+          if (steps > 0 && steps + s > kBigKChunk) break;
+          steps += s;
+          p.g_term2.push_back(row[q++]);
+        }
+        const int t_end = (int)p.g_term2.size();
+        for (int j0 = 0; j0 < db; j0 += kBigTile)
           p.g_tile2.push_back(GTile2{blk[0], da, db, i0, j0, t_begin, t_end});
+      }
     }
   }
 }

@@ -120,19 +120,30 @@ struct SXRef {
 //   G1: X_u = V_R Bhat^T for every distinct (V block, Bhat block) pair;
 //   G2: W_L += sum_k A_k Y_k, Y_k = sum_u r_u X_u, tiles of W_L over chunks of
 //       its a-terms (split-K), added into W with FP64 atomics.
+// The densities are block-sparse (partition pairs), so every tile carries the
+// set of 16-deep K steps in which its operand is not identically zero, and
+// tiles whose operands vanish are not listed.
 constexpr int kBigTile = 64;     // output tile side
-constexpr int kBigKChunk = 2048; // inner dimension one stage-2 tile covers (split-K)
+constexpr int kBigStep = 16;     // K step of the tiles
+constexpr int kBigMaxK = 2048;   // largest inner dimension the K-step masks cover (128 steps)
+constexpr int kBigKChunk = 128;  // active K steps one stage-2 tile covers (split-K)
 struct GTask1 {
   int64_t v_off, b_off, x_off;   // V_R (m x k), Bhat^T (k x n), X (m x n), column-major
   int32_t m, k, n, pad;
 };
 struct GTile1 {
+  uint64_t kmask[2];             // K steps where this tile's Bhat^T columns are not all zero
   int32_t task, i0, j0, pad;
+};
+struct GTerm2 {
+  uint64_t kmask[2];             // K steps where A's rows of this tile are not all zero
+  int64_t a_off;                 // A (da x dar, column-major)
+  int32_t dar, x_begin, x_end, pad;
 };
 struct GTile2 {
   int64_t w_off;                 // W_L (da x db, column-major)
   int32_t da, db, i0, j0;
-  int32_t t_begin, t_end;        // a-terms (STerm2 with lds unused) of this tile's K chunk
+  int32_t t_begin, t_end;        // GTerm2 range (this row tile's terms of one K chunk)
 };
 // G0: the like-particle part of a large block, W_L = H_a V_L + V_L H_b^T, one
 // 64 x 64 tile each (stored, so it initialises W_L before G2 adds into it)
@@ -184,7 +195,7 @@ struct HostPlan {
   std::vector<GTile1> g_tile1;
   std::vector<GTile2> g_tile2;
   std::vector<GTile3> g_tile3;
-  std::vector<STerm2> g_term2;
+  std::vector<GTerm2> g_term2;
   std::vector<SXRef> g_xref;
   int64_t xg_size = 0;         // large-block X buffer (doubles)
   int64_t N = 0, part_size = 0;

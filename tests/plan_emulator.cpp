@@ -100,15 +100,18 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
     }
   }
   if (!big) return;
-  // large-block path: X_u = V_R Bhat^T per task, then W tiles += sum_k A_k sum_u r_u X_u
+  // large-block path: X_u = V_R Bhat^T on the listed tiles and their active K
+  // steps (X starts at zero), then W tiles += sum over the tile's terms and
+  // their active K steps of A_k sum_u r_u X_u
+  auto active = [](const uint64_t (&mk)[2], int k) { const int s = k / kBigStep; return (mk[s >> 6] >> (s & 63)) & 1u; };
   std::vector<double> XG(size_t(p.xg_size), 0.0);
-  std::vector<char> tile_done(p.g_tile1.size(), 0);
   for (const GTile1& t : p.g_tile1) {
     const GTask1& g = p.g_task1[t.task];
     for (int j = t.j0; j < std::min(t.j0 + kBigTile, g.n); ++j)
       for (int i = t.i0; i < std::min(t.i0 + kBigTile, g.m); ++i) {
         double s = 0.0;
-        for (int k = 0; k < g.k; ++k) s += vin[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
+        for (int k = 0; k < g.k; ++k)
+          if (active(t.kmask, k)) s += vin[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
         XG[g.x_off + i + int64_t(g.m) * j] = s;
       }
   }
@@ -117,8 +120,9 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
       for (int i = t.i0; i < std::min(t.i0 + kBigTile, t.da); ++i) {
         double s = 0.0;
         for (int q = t.t_begin; q < t.t_end; ++q) {
-          const STerm2& tm = p.g_term2[q];
+          const GTerm2& tm = p.g_term2[q];
           for (int k = 0; k < tm.dar; ++k) {
+            if (!active(tm.kmask, k)) continue;
             double y = 0.0;
             for (int x = tm.x_begin; x < tm.x_end; ++x)
               y += p.g_xref[x].r * XG[p.g_xref[x].x_off + k + int64_t(tm.dar) * j];
