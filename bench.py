@@ -345,8 +345,8 @@ def main():
     ap.add_argument("--ensemble-samples", type=int, default=256,
                     help="interaction samples of the batched ensemble (0 skips it)")
     ap.add_argument("--no-sep", action="store_true", help="skip the separable-operator (NuTRLZ) timings")
-    ap.add_argument("--large", action="store_true",
-                    help="also time one pf 6p6n J=0 H*v (N = 1.78 M; about 3 minutes of file writing)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the pf 6p6n J=0 H*v (N = 1.78 M; about a minute, mostly writing its files)")
     ap.add_argument("--pf-warm-samples", type=int, default=0,
                     help="training samples of the pf 4p4n warm-start study (0 skips it; 512 takes ~8 min)")
     args = ap.parse_args()
@@ -433,8 +433,12 @@ def main():
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
     if not args.no_sep:
         out["separable"] = separable(log, mf)
-    if args.large:
-        out["large"] = large(log, cpu=not args.no_cpu)
+    if not args.no_large:
+        try:
+            out["large"] = large(log, cpu=not args.no_cpu)
+        except Exception as e:  # the headline line must print whatever happens here
+            out["large"] = {"error": repr(e)}
+            log(f"large: failed: {e!r}")
     if args.ensemble_samples > 0:
         out["ensemble"] = ensemble(log, args.ensemble_samples)
     if args.pf_warm_samples > 0:
