@@ -501,6 +501,9 @@ PlanConfig plan_config(const nsx_tune_result& r) {
 // (same result as plan_add of each model into hp in turn).
 void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
   const int n = (int)models.size();
+  // test hook: a smaller block side above which the large-block GEMM path runs,
+  // so that small models exercise it
+  if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
   if (n == 1) {
     plan_add(hp, *models[0]);
     if (hp.cfg.stacked) build_stacked(hp, models);

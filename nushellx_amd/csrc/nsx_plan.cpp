@@ -125,7 +125,7 @@ void plan_add(HostPlan& p, const Model& m) {
     int maxdbr = 0;
     for (const ATerm& at : it.at)
       for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
-    if (std::max(std::max(da, dar), std::max(db, maxdbr)) <= kSmallMax) {
+    if (std::max(std::max(da, dar), std::max(db, maxdbr)) <= p.cfg.big_min) {
       // phases: one per a-term (split when its runs exceed kFatCap); items are
       // cut into chunks of <= kMaxPhases phases, each chunk owning a partial
       std::vector<DPhase> ph_item;
@@ -256,7 +256,7 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
       int maxdbr = 0;
       for (const ATerm& at : it.at)
         for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
-      if (std::max(std::max(da, dar), std::max(db, maxdbr)) > kSmallMax) continue;  // direct kernel
+      if (std::max(std::max(da, dar), std::max(db, maxdbr)) > p.cfg.big_min) continue;  // direct kernel
       for (const ATerm& at : it.at) {
         T2 t;
         t.a_off = a_off[at.adens];
@@ -397,7 +397,7 @@ void build_big(HostPlan& p, const std::vector<const Model*>& models) {
       int maxdbr = 0;
       for (const ATerm& at : it.at)
         for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
-      if (std::max(std::max(da, dar), std::max(db, maxdbr)) <= kSmallMax) continue;  // small-block paths
+      if (std::max(std::max(da, dar), std::max(db, maxdbr)) <= p.cfg.big_min) continue;  // small-block paths
       block_of[{mi, it.L}] = {vb + m.posnv[it.L], da, db};
       auto& tl = terms_of[{mi, it.L}];
       for (const ATerm& at : it.at) {

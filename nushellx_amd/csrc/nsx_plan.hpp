@@ -159,6 +159,7 @@ struct PlanConfig {
   int stacked = 0;           // 1: small blocks take the two-stage stacked path
   int stk_cap = 2048;        // stacked path: doubles one chunk may stage in LDS
   int stk_work = 512;        // stacked path: work units per chunk
+  int big_min = kSmallMax;   // items with a side above this take the direct / large-block paths
 };
 
 struct HostPlan {

@@ -24,6 +24,7 @@ def lib():
         L = C.CDLL(SO)
         L.emu_hv.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64]
         L.emu_hv_sep.argtypes = [C.c_char_p] * 5 + [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double)]
+        L.emu_hv_big.argtypes = [C.c_char_p] * 5 + [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.emu_plan_stats.argtypes = [C.c_char_p] * 4 + [C.c_void_p]
         L.emu_last_error.restype = C.c_char_p
         L.emu_sixj.argtypes = [C.c_int] * 6
@@ -55,6 +56,19 @@ def emu_hv_sep(args, dop, v):
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out, f.value
+
+
+def emu_hv_big(args, v, big_min, dop=None):
+    """H*v with the items above big_min on the large-block GEMM plan; returns
+    (w, (stage-1 tiles, stage-2 tiles, X references, stage-2 terms))."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    tiles = np.zeros(4, dtype=np.int64)
+    rc = lib().emu_hv_big(*[a.encode() for a in args], dop.encode() if dop else None, big_min, v.ctypes.data,
+                          out.ctypes.data, len(v), tiles.ctypes.data)
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out, tuple(int(x) for x in tiles)
 
 
 def plan_stats(args):

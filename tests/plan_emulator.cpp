@@ -146,6 +146,28 @@ extern "C" int emu_hv(const char* nus, const char* opa, const char* opb, const c
   return 0;
 }
 
+// The plan with items whose side exceeds big_min on the large-block GEMM path
+// (dop: optional .dop prefix of a separable model, or empty).
+extern "C" int emu_hv_big(const char* nus, const char* opa, const char* opb, const char* intf, const char* dop,
+                          int big_min, const double* vin, double* vout, int64_t n, int64_t* tiles) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m, dop ? dop : "");
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  p.cfg.big_min = big_min;
+  plan_add(p, m);
+  build_big(p, {&m});
+  emulate(p, vin, vout);
+  if (tiles) {
+    tiles[0] = (int64_t)p.g_tile1.size();
+    tiles[1] = (int64_t)p.g_tile2.size();
+    tiles[2] = (int64_t)p.g_xref.size();
+    tiles[3] = (int64_t)p.g_term2.size();
+  }
+  return 0;
+}
+
 // The separable (NuTRLZ) model of <dop>.dop through the same plan.
 extern "C" int emu_hv_sep(const char* nus, const char* opa, const char* opb, const char* intf, const char* dop,
                           const double* vin, double* vout, int64_t n, double* flops) {

@@ -55,3 +55,22 @@ def test_pf44_lanczos_matches_arpack(pf44):
     r = h.lanczos(nev=5, tol=1e-7)
     print("device", r["evals"], "arpack", ev_ref, r["matvecs"], flush=True)
     assert np.abs(r["evals"] - ev_ref).max() < 1e-6
+
+
+@pytest.mark.parametrize("name,J2,big_min,dop", [("sd44", 0, 8, None), ("sd44", 8, 8, None), ("sd23", 3, 4, None),
+                                                 ("sd44", 4, 8, "cut")])
+def test_large_block_path_on_small_models(models, dops, name, J2, big_min, dop, monkeypatch):
+    """The large-block GEMM kernels (MFMA tiles, K-step masks, split-K atomics,
+    several X references per a-term when J > 0) on sd models, reached by
+    lowering the block side above which they run."""
+    from nushellx_amd.hv import Hamiltonian
+    monkeypatch.setenv("NSX_BIG_MIN", str(big_min))
+    args = models[name].args(J2)
+    dopfile = dops[name][dop] if dop else None
+    print(f"case large-block {name} 2J={J2} big_min={big_min} dop={dop}", flush=True)
+    ref = RefCPU(*args, fp64=True, dopfile=dopfile)
+    h = Hamiltonian(*args, dopfile=dopfile)
+    v = np.random.default_rng(J2 + big_min).standard_normal(h.N)
+    w_ref = ref.apply(v)
+    assert np.abs(h.multiply(v) - w_ref).max() <= 1e-12 * np.abs(w_ref).max()
+    h.close()
