@@ -615,19 +615,56 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   }
 }
 
-// ---------------------------------------------------------------- large-block GEMM path (FP64)
-// 64 x 64 output tiles on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l
-// supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds D(l / 16 + 4 v, l % 16)).
+// ---------------------------------------------------------------- large-block GEMM path
+// 64 x 64 output tiles on the matrix cores: FP64 (v_mfma_f64_16x16x4f64) for
+// FP64 plans, FP32 (v_mfma_f32_16x16x4f32) for FP32 and BF16 hi/lo operands
+// (a hi/lo pair is expanded to the FP32 value hi + lo, which FP32 holds
+// exactly).  In both, lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16);
+// it holds D(l / 16 + 4 v, l % 16) in FP64 and D(4 (l / 16) + v, l % 16) in FP32.
 // Four wavefronts take 32 x 32 quarters as 2 x 2 MFMA blocks.  The product is
 // formed transposed, D = C^T, so that a lane's outputs run down a column of the
 // column-major result and stores (or atomic adds) coalesce.  K advances 16 at
 // a time: both operand tiles go through LDS (k-major rows of 64, padded) and the
 // next tiles are loaded into registers while the current ones are multiplied.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kBigK = kBigStep;       // K step (rows of each LDS tile)
 constexpr int kBigQ = kBigK / 4;      // staged elements per thread and operand (256 threads, 64 x kBigK)
 constexpr int kBigJS = 256 / kBigK;   // column stride of the k-major staging map
 constexpr int kBigLd = kBigTile + 1;  // padded LDS row
+
+template <typename C>
+struct BigMma;
+template <>
+struct BigMma<double> {
+  using vec = f64x4;
+  __device__ static vec mma(double a, double b, vec c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int v) { return lane / 16 + 4 * v; }
+};
+template <>
+struct BigMma<float> {
+  using vec = f32x4;
+  __device__ static vec mma(float a, float b, vec c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int v) { return 4 * (lane / 16) + v; }
+};
+// storage type -> compute type of the tiles
+template <typename T>
+struct BigIn;
+template <>
+struct BigIn<double> {
+  using C = double;
+  __device__ static double get(double x) { return x; }
+};
+template <>
+struct BigIn<float> {
+  using C = float;
+  __device__ static float get(float x) { return x; }
+};
+template <>
+struct BigIn<Bf16x2> {
+  using C = float;
+  __device__ static float get(Bf16x2 x) { return bf_hi(x) + bf_lo(x); }
+};
 
 // lowest set bit of the 128-bit step mask (m0, m1), cleared; -1 when empty
 // (selects on values, so that the two words stay in registers)
@@ -644,11 +681,12 @@ __device__ __forceinline__ int pop_step(uint64_t& m0, uint64_t& m1) {
 
 // one K step of the tile: sP holds the row operand of D (the columns j of C) as
 // [k][j], sQ the column operand (the rows i of C) as [k][i]
-__device__ __forceinline__ void big_mma(const double* sP, const double* sQ, int wj, int wi, int lane,
-                                        f64x4 (&acc)[2][2]) {
+template <typename C>
+__device__ __forceinline__ void big_mma(const C* sP, const C* sQ, int wj, int wi, int lane,
+                                        typename BigMma<C>::vec (&acc)[2][2]) {
 #pragma unroll
   for (int ks = 0; ks < kBigK; ks += 4) {
-    double a[2], b[2];
+    C a[2], b[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) a[r] = sP[(ks + lane / 16) * kBigLd + wj + r * 16 + lane % 16];
 #pragma unroll
@@ -656,38 +694,45 @@ __device__ __forceinline__ void big_mma(const double* sP, const double* sQ, int 
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-      for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+      for (int c = 0; c < 2; ++c) acc[r][c] = BigMma<C>::mma(a[r], b[c], acc[r][c]);
   }
 }
 
+template <typename C>
+__device__ __forceinline__ void big_zero(typename BigMma<C>::vec (&acc)[2][2]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = typename BigMma<C>::vec{C(0), C(0), C(0), C(0)};
+}
+
 // G1: X = V_R Bhat^T for one 64 x 64 tile of one pair (V_R m x k, Bhat^T k x n)
-__global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles, const GTask1* __restrict__ tasks,
-                                              const double* __restrict__ V, const double* __restrict__ Bt,
-                                              double* __restrict__ X) {
-  __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+template <typename T>
+__global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles,
+                                              const GTask1* __restrict__ tasks, const T* __restrict__ V,
+                                              const T* __restrict__ Bt, typename BigIn<T>::C* __restrict__ X) {
+  using C = typename BigIn<T>::C;
+  __shared__ C sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
   const GTile1 t = tiles[blockIdx.x];
   const GTask1 g = tasks[t.task];
   const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
-  f64x4 acc[2][2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+  typename BigMma<C>::vec acc[2][2];
+  big_zero<C>(acc);
   // staging: V tile element (i = tid % 64, kk = tid / 64 + 4q), Bhat^T tile
   // element (kk = tid % 16, j = tid / 16 + 16q), zero outside the matrices
-  double rv[kBigQ], rb[kBigQ];
+  C rv[kBigQ], rb[kBigQ];
   auto load = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < kBigQ; ++q) {
       const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
-      rv[q] = (i < m && kk < K) ? V[g.v_off + i + (int64_t)m * kk] : 0.0;
+      rv[q] = (i < m && kk < K) ? BigIn<T>::get(V[g.v_off + i + (int64_t)m * kk]) : C(0);
     }
 #pragma unroll
     for (int q = 0; q < kBigQ; ++q) {
       const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
-      rb[q] = (kk < K && j < n) ? Bt[g.b_off + kk + (int64_t)K * j] : 0.0;
+      rb[q] = (kk < K && j < n) ? BigIn<T>::get(Bt[g.b_off + kk + (int64_t)K * j]) : C(0);
     }
   };
   // only the K steps where this tile's Bhat^T columns are not all zero
@@ -703,7 +748,7 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles, 
     __syncthreads();
     step = pop_step(m0, m1);
     if (step >= 0) load(step * kBigK);
-    big_mma(sP, sQ, wj, wi, lane, acc);
+    big_mma<C>(sP, sQ, wj, wi, lane, acc);
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r)
@@ -711,27 +756,27 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles, 
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + lane / 16 + 4 * v;
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + BigMma<C>::row(lane, v);
         if (i < m && j < n) X[g.x_off + i + (int64_t)m * j] = acc[r][c][v];
       }
 }
 
 // G2: one 64 x 64 tile of W_L += sum_k A_k Y_k over a chunk of a-terms, with
 // Y_k = sum_u r_u X_u formed while staging; the tile is added into W atomically
-__global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, const GTerm2* __restrict__ terms,
-                                              const SXRef* __restrict__ xrefs, const double* __restrict__ Ad,
-                                              const double* __restrict__ X, double* __restrict__ W) {
-  __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+template <typename T>
+__global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
+                                              const GTerm2* __restrict__ terms, const SXRef* __restrict__ xrefs,
+                                              const T* __restrict__ Ad, const typename BigIn<T>::C* __restrict__ X,
+                                              double* __restrict__ W) {
+  using C = typename BigIn<T>::C;
+  __shared__ C sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
   const GTile2 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
-  f64x4 acc[2][2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-  double ra[kBigQ], ry[kBigQ];
+  typename BigMma<C>::vec acc[2][2];
+  big_zero<C>(acc);
+  C ra[kBigQ], ry[kBigQ];
   // staging: A tile element (i = tid % 64, kk = tid / 64 + 4q), Y tile element
   // (kk = tid % kBigK, j = tid / kBigK + kBigJS q)
   auto load = [&](const GTerm2& tm, int k0) {
@@ -739,16 +784,17 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, 
 #pragma unroll
     for (int q = 0; q < kBigQ; ++q) {
       const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
-      ra[q] = (i < da && kk < dar) ? Ad[tm.a_off + i + (int64_t)da * kk] : 0.0;
+      ra[q] = (i < da && kk < dar) ? BigIn<T>::get(Ad[tm.a_off + i + (int64_t)da * kk]) : C(0);
     }
 #pragma unroll
-    for (int q = 0; q < kBigQ; ++q) ry[q] = 0.0;
+    for (int q = 0; q < kBigQ; ++q) ry[q] = C(0);
     for (int x = tm.x_begin; x < tm.x_end; ++x) {
       const SXRef xr = xrefs[x];
+      const C r = C(xr.r);
 #pragma unroll
       for (int q = 0; q < kBigQ; ++q) {
         const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
-        if (kk < dar && j < db) ry[q] = fma(xr.r, X[xr.x_off + kk + (int64_t)dar * j], ry[q]);
+        if (kk < dar && j < db) ry[q] = fma(r, X[xr.x_off + kk + (int64_t)dar * j], ry[q]);
       }
     }
   };
@@ -781,7 +827,7 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, 
       step = pop_step(m0, m1);
     }
     if (step >= 0) load(tm, step * kBigK);
-    big_mma(sP, sQ, wj, wi, lane, acc);
+    big_mma<C>(sP, sQ, wj, wi, lane, acc);
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r)
@@ -789,45 +835,44 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles, 
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + lane / 16 + 4 * v;
-        if (i < da && j < db) unsafeAtomicAdd(W + t.w_off + i + (int64_t)da * j, acc[r][c][v]);
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + BigMma<C>::row(lane, v);
+        if (i < da && j < db) unsafeAtomicAdd(W + t.w_off + i + (int64_t)da * j, (double)acc[r][c][v]);
       }
 }
 
 // G0: W_L = H_a V_L + V_L H_b^T for one 64 x 64 tile of a large block, stored
 // (it runs before the pn kernels add into W).  Two K phases: H_a (da x da) times
 // V_L (da x db), then V_L times H_b^T, where H_b is symmetric: H_b^T(k, j) = H_b(k, j).
-__global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, const double* __restrict__ V,
-                                              const double* __restrict__ H, double* __restrict__ W) {
-  __shared__ double sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+template <typename T>
+__global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, const T* __restrict__ V,
+                                              const T* __restrict__ H, double* __restrict__ W) {
+  using C = typename BigIn<T>::C;
+  __shared__ C sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
   const GTile3 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
-  f64x4 acc[2][2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-  const double* v = V + t.v_off;
+  typename BigMma<C>::vec acc[2][2];
+  big_zero<C>(acc);
+  const T* v = V + t.v_off;
   // phase 0: A = H_a (ld da), Y = V_L (ld da), K = da; phase 1: A = V_L (ld da), Y = H_b (ld db), K = db
   for (int ph = 0; ph < 2; ++ph) {
     const int64_t hoff = ph == 0 ? t.ha_off : t.hb_off;
     if (hoff < 0) continue;
-    const double* Am = ph == 0 ? H + hoff : v;
-    const double* Ym = ph == 0 ? v : H + hoff;
+    const T* Am = ph == 0 ? H + hoff : v;
+    const T* Ym = ph == 0 ? v : H + hoff;
     const int K = ph == 0 ? da : db, ldy = ph == 0 ? da : db;
     for (int k0 = 0; k0 < K; k0 += kBigK) {
-      double ra[kBigQ], ry[kBigQ];
+      C ra[kBigQ], ry[kBigQ];
 #pragma unroll
       for (int q = 0; q < kBigQ; ++q) {
         const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
-        ra[q] = (i < da && kk < K) ? Am[i + (int64_t)da * kk] : 0.0;
+        ra[q] = (i < da && kk < K) ? BigIn<T>::get(Am[i + (int64_t)da * kk]) : C(0);
       }
 #pragma unroll
       for (int q = 0; q < kBigQ; ++q) {
         const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
-        ry[q] = (kk < K && j < db) ? Ym[kk + (int64_t)ldy * j] : 0.0;
+        ry[q] = (kk < K && j < db) ? BigIn<T>::get(Ym[kk + (int64_t)ldy * j]) : C(0);
       }
       __syncthreads();
 #pragma unroll
@@ -835,7 +880,7 @@ __global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, 
 #pragma unroll
       for (int q = 0; q < kBigQ; ++q) sP[(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = ry[q];
       __syncthreads();
-      big_mma(sP, sQ, wj, wi, lane, acc);
+      big_mma<C>(sP, sQ, wj, wi, lane, acc);
     }
   }
 #pragma unroll
@@ -844,8 +889,8 @@ __global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, 
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + lane / 16 + 4 * q;
-        if (i < da && j < db) W[t.v_off + i + (int64_t)da * j] = acc[r][c][q];
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + BigMma<C>::row(lane, q);
+        if (i < da && j < db) W[t.v_off + i + (int64_t)da * j] = (double)acc[r][c][q];
       }
 }
 
@@ -907,7 +952,7 @@ const void* k_pn_partial_ptr(int prec, bool atomic) {
 // ---------------------------------------------------------------- launchers
 // FP64 plans in atomic mode run their large blocks on the GEMM tiles: the
 // like-particle part (k_big0, before the pn kernels) and the pn part (k_big1, k_big2)
-static bool big_like_active(const DevicePlan& d, int prec) { return prec == 0 && d.atomic && d.n_gtile3 > 0; }
+static bool big_like_active(const DevicePlan& d) { return d.atomic && d.n_gtile3 > 0; }
 
 template <typename T>
 static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H, double* W,
@@ -923,7 +968,7 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     if (g_red + g1 > 0) {
       const T* Hs = H;
       const int cap = d.s_lds1;
-      const int skip = big_like_active(d, prec) ? 1 : 0;
+      const int skip = big_like_active(d) ? 1 : 0;
       void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X,
                     (void*)&d.blocks,   (void*)&d.n_blocks, (void*)&g_red,    (void*)&Hs, (void*)&W,
                     (void*)&cap,        (void*)&skip};
@@ -950,14 +995,15 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     hipError_t e = hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt), args, lds, s);
     if (e != hipSuccess) return e;
   }
-  if (prec == 0 && at && d.n_gtile2 > 0) {  // large blocks on the matrix cores (FP64)
-    const double* Vd = reinterpret_cast<const double*>(V);
-    const double* Bd = reinterpret_cast<const double*>(Bh);
-    const double* Adp = reinterpret_cast<const double*>(Ad);
+  if (at && d.n_gtile2 > 0) {  // large blocks on the matrix cores
+    // the plan's precision is fixed, so XG only ever holds X in this compute type
+    using C = typename BigIn<T>::C;
+    C* XG = reinterpret_cast<C*>(d.XG);
     if (d.n_gtile1 > 0)
-      hipLaunchKernelGGL(k_big1, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1, Vd, Bd, d.XG);
-    hipLaunchKernelGGL(k_big2, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2, d.g_xref, Adp,
-                       (const double*)d.XG, W);
+      hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1,
+                         V, Bh, XG);
+    hipLaunchKernelGGL(k_big2<T>, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2,
+                       d.g_xref, Ad, (const C*)XG, W);
     return hipGetLastError();
   }
   if (d.n_items > 0) {
@@ -972,14 +1018,13 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
 
 template <typename T>
 static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H, double* W, hipStream_t s) {
-  const bool big = big_like_active(d, prec);
+  const bool big = big_like_active(d);
   // the stacked stage-1 launch carries the small like-particle blocks
   if (!d.stacked && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
                        V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
   if (big)
-    hipLaunchKernelGGL(k_big0, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3,
-                       reinterpret_cast<const double*>(V), reinterpret_cast<const double*>(H), W);
+    hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
   return hipGetLastError();
 }
 
