@@ -712,7 +712,7 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles,
                                               const GTask1* __restrict__ tasks, const T* __restrict__ V,
                                               const T* __restrict__ Bt, typename BigIn<T>::C* __restrict__ X) {
   using C = typename BigIn<T>::C;
-  __shared__ C sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+  __shared__ C sP[2][kBigK * kBigLd], sQ[2][kBigK * kBigLd];  // double-buffered: one barrier per K step
   const GTile1 t = tiles[blockIdx.x];
   const GTask1 g = tasks[t.task];
   const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
@@ -735,20 +735,29 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles,
       rb[q] = (kk < K && j < n) ? BigIn<T>::get(Bt[g.b_off + kk + (int64_t)K * j]) : C(0);
     }
   };
-  // only the K steps where this tile's Bhat^T columns are not all zero
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) sQ[buf][(tid / 64 + 4 * q) * kBigLd + tid % 64] = rv[q];
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) sP[buf][(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = rb[q];
+  };
+  // only the K steps where this tile's Bhat^T columns are not all zero.  Step
+  // s is multiplied from buffer cur while step s+1 is loaded and then staged
+  // into the other buffer, whose last readers finished before the previous barrier
   uint64_t m0 = t.kmask[0], m1 = t.kmask[1];
-  int step = pop_step(m0, m1);
-  if (step >= 0) load(step * kBigK);
+  int step = pop_step(m0, m1), cur = 0;
+  if (step >= 0) {
+    load(step * kBigK);
+    stage(0);
+    __syncthreads();
+  }
   while (step >= 0) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kBigQ; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = rv[q];
-#pragma unroll
-    for (int q = 0; q < kBigQ; ++q) sP[(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = rb[q];
-    __syncthreads();
     step = pop_step(m0, m1);
     if (step >= 0) load(step * kBigK);
-    big_mma<C>(sP, sQ, wj, wi, lane, acc);
+    big_mma<C>(sP[cur], sQ[cur], wj, wi, lane, acc);
+    if (step >= 0) stage(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r)
@@ -769,7 +778,7 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
                                               const T* __restrict__ Ad, const typename BigIn<T>::C* __restrict__ X,
                                               double* __restrict__ W) {
   using C = typename BigIn<T>::C;
-  __shared__ C sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
+  __shared__ C sP[2][kBigK * kBigLd], sQ[2][kBigK * kBigLd];  // double-buffered: one barrier per K step
   const GTile2 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -810,15 +819,21 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
     m1 = tm.kmask[1];
     step = pop_step(m0, m1);
   }
-  if (step >= 0) load(tm, step * kBigK);
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) sQ[buf][(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) sP[buf][(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = ry[q];
+  };
+  int cur = 0;
+  if (step >= 0) {
+    load(tm, step * kBigK);
+    stage(0);
+    __syncthreads();
+  }
   while (step >= 0) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kBigQ; ++q) sQ[(tid / 64 + 4 * q) * kBigLd + tid % 64] = ra[q];
-#pragma unroll
-    for (int q = 0; q < kBigQ; ++q) sP[(tid % kBigK) * kBigLd + tid / kBigK + kBigJS * q] = ry[q];
-    __syncthreads();
-    // next (term, step) pair, prefetched before this step's products
+    // next (term, step) pair, loaded while this step's products run and then
+    // staged into the other buffer
     step = pop_step(m0, m1);
     while (step < 0 && ++tc < t.t_end) {
       tm = terms[tc];
@@ -827,7 +842,10 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
       step = pop_step(m0, m1);
     }
     if (step >= 0) load(tm, step * kBigK);
-    big_mma<C>(sP, sQ, wj, wi, lane, acc);
+    big_mma<C>(sP[cur], sQ[cur], wj, wi, lane, acc);
+    if (step >= 0) stage(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r)
