@@ -245,3 +245,42 @@ def test_standard_lanczos_matches_dense(models):
         print(name, r["matvecs"], r["evals"] - ev, flush=True)
         assert r["converged"] == 5
         assert np.abs(r["evals"] - ev).max() < 1e-7
+
+
+def test_empty_and_ragged_instances(tmp_path):
+    """A J with no basis states (sd 2p2n, 2J = 30): the handle has N = 0, its
+    H*v is empty and a solve fails with an error; in a batch it sits between
+    non-empty instances without moving them, and a batch solve, which would ask
+    it for more states than it has, fails with an error."""
+    from nushellx_amd.hv import Batch, DeviceBuffer, Hamiltonian, NsxError, device_sync
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import SD, PNSpace
+    from nushellx_amd.synth.writer import PNModel, write_model
+    mf = write_model(PNModel(PNSpace(SD), 2, 2, random_interaction(SD, seed=3)), str(tmp_path), "sd22", [0, 4, 30])
+    print("case empty instance sd22 2J=30", flush=True)
+    h0 = Hamiltonian(*mf.args(30))
+    assert h0.N == 0
+    assert h0.multiply(np.zeros(0)).shape == (0,)
+    with pytest.raises(NsxError, match="nothing to solve"):
+        h0.lanczos(nev=1)
+    hams = [Hamiltonian(*mf.args(0)), h0, Hamiltonian(*mf.args(4))]
+    refs = [RefCPU(*mf.args(J2), fp64=True) for J2 in (0, 30, 4)]
+    b = Batch(hams)
+    assert b.offsets == [0, hams[0].N, hams[0].N]
+    rng = np.random.default_rng(8)
+    v = rng.standard_normal(b.N)
+    din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+    b.multiply_dev(din, dout)
+    device_sync()
+    w = np.empty_like(v)
+    dout.download(w)
+    for k, r in enumerate(refs):
+        o = b.offsets[k]
+        if r.N:
+            ref = r.apply(v[o:o + r.N])
+            assert np.abs(w[o:o + r.N] - ref).max() <= 1e-12 * np.abs(ref).max()
+    with pytest.raises(NsxError, match="more eigenvalues"):
+        b.lanczos(nev=1)
+    din.free()
+    dout.free()
+    b.close()
