@@ -62,6 +62,20 @@ __global__ __launch_bounds__(kVT) void k_seg_dot(const double* __restrict__ Q, i
   }
 }
 
+// Long instances: k_seg_dot runs over row chunks of at most kDotChunk rows
+// (a segment table of chunks, partial dots per chunk), and this kernel sums the
+// chunks of instance i in order: c[i, col] = sum_{ch in [cfirst[i], cfirst[i+1])} part[ch, col]
+constexpr int64_t kDotChunk = 16384;
+__global__ __launch_bounds__(kVT) void k_dot_fin(const double* __restrict__ part, const int* __restrict__ cfirst,
+                                                 int ncols, double* __restrict__ c, int ldc) {
+  const int i = blockIdx.x;
+  for (int col = threadIdx.x; col < ncols; col += kVT) {
+    double t = 0.0;
+    for (int ch = cfirst[i]; ch < cfirst[i + 1]; ++ch) t += part[(int64_t)ch * ldc + col];
+    c[(int64_t)i * ldc + col] = t;
+  }
+}
+
 // Subtraction passes: a workgroup takes 64 rows and its four wavefronts split
 // the columns (wave p sums columns p, p+4, ...), so a thread's chain of loads
 // is a quarter as long and four times as many workgroups share the GPU; the
@@ -195,7 +209,7 @@ struct TrlWork {
   uint64_t plan_gen = 0;
   void *Q = nullptr, *W = nullptr, *Y = nullptr, *inst_of = nullptr, *seg = nullptr, *c1 = nullptr,
        *c2 = nullptr, *ah = nullptr, *bh = nullptr, *scale = nullptr, *dim = nullptr, *S = nullptr,
-       *wmask = nullptr;
+       *wmask = nullptr, *cseg = nullptr, *cfirst = nullptr, *cpart = nullptr;  // chunked dots
   double *hha = nullptr, *hhb = nullptr;  // pinned alpha / beta histories
   int* hdim[2] = {nullptr, nullptr};      // pinned dimension flags, one per block parity
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -206,8 +220,8 @@ struct TrlWork {
   }
   void release() {
     clear_graphs();
-    for (void* p : {Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S, wmask}) (void)hipFree(p);
-    Q = W = Y = inst_of = seg = c1 = c2 = ah = bh = scale = dim = S = wmask = nullptr;
+    for (void* p : {Q, W, Y, inst_of, seg, c1, c2, ah, bh, scale, dim, S, wmask, cseg, cfirst, cpart}) (void)hipFree(p);
+    Q = W = Y = inst_of = seg = c1 = c2 = ah = bh = scale = dim = S = wmask = cseg = cfirst = cpart = nullptr;
     (void)hipHostFree(hha);
     (void)hipHostFree(hhb);
     (void)hipHostFree(hdim[0]);
@@ -261,6 +275,9 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   const int64_t ldq = N;
   const int ldc = m + 1;
 
+  // row chunks of the dot products: every instance split into pieces of at most
+  // kDotChunk rows (at most B + N / kDotChunk of them)
+  const int64_t max_chunks = B + N / kDotChunk + 1;
   if (!work) work = new TrlWork();
   TrlWork& ws = *work;
   const bool fresh = !(ws.N == N && ws.B == B && ws.m == m && ws.keep == keep && ws.nev == nev);
@@ -273,7 +290,8 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
               dm(ws.c2, sizeof(double) * B * ldc) && dm(ws.ah, sizeof(double) * (m + 1) * B) &&
               dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * 2 * B) &&
               dm(ws.dim, sizeof(int) * B) && dm(ws.S, sizeof(double) * (size_t)B * m * std::max(keep, nev)) &&
-              dm(ws.wmask, (size_t)N) &&
+              dm(ws.wmask, (size_t)N) && dm(ws.cseg, sizeof(int64_t) * (max_chunks + 1)) &&
+              dm(ws.cfirst, sizeof(int) * (B + 1)) && dm(ws.cpart, sizeof(double) * max_chunks * ldc) &&
               hipHostMalloc((void**)&ws.hha, sizeof(double) * (m + 1) * B) == hipSuccess &&
               hipHostMalloc((void**)&ws.hhb, sizeof(double) * (m + 1) * B) == hipSuccess &&
               hipHostMalloc((void**)&ws.hdim[0], sizeof(int) * B) == hipSuccess &&
@@ -293,7 +311,8 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     ws.prec = prec;
   }
   const DPtr Q{ws.Q}, W{ws.W}, Y{ws.Y}, inst_of{ws.inst_of}, seg{ws.seg}, c1{ws.c1}, c2{ws.c2}, ah{ws.ah},
-      bh{ws.bh}, scale{ws.scale}, dim{ws.dim}, S{ws.S}, kmask{ws.wmask};
+      bh{ws.bh}, scale{ws.scale}, dim{ws.dim}, S{ws.S}, kmask{ws.wmask}, cseg{ws.cseg}, cfirst{ws.cfirst},
+      cpart{ws.cpart};
 #define TRL_HIP(call)                                                          \
   do {                                                                         \
     hipError_t _e = (call);                                                    \
@@ -315,6 +334,25 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     if (inst[i].off + inst[i].n != inst[i + 1].off) { set_error("instances must be contiguous"); return NSX_EARG; }
   TRL_HIP(hipMemcpy(seg.p, hseg.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice));
   TRL_HIP(hipMemcpy(inst_of.p, hinst.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  // an instance longer than kDotChunk rows would leave one workgroup per column
+  // streaming it alone: its dots then run over row chunks
+  bool chunked = false;
+  for (const auto& x : inst) chunked = chunked || x.n > kDotChunk;
+  int nchunks = 0;
+  if (chunked) {
+    std::vector<int64_t> hcs;
+    std::vector<int> hcf(B + 1, 0);
+    for (int i = 0; i < B; ++i) {
+      hcf[i] = (int)hcs.size();
+      for (int64_t r = inst[i].off; r < inst[i].off + inst[i].n; r += kDotChunk) hcs.push_back(r);
+    }
+    hcf[B] = (int)hcs.size();
+    nchunks = (int)hcs.size();
+    hcs.push_back(hseg[B]);
+    if (nchunks > max_chunks) { set_error("internal: dot chunk table overflow"); return NSX_EGPU; }
+    TRL_HIP(hipMemcpy(cseg.p, hcs.data(), sizeof(int64_t) * hcs.size(), hipMemcpyHostToDevice));
+    TRL_HIP(hipMemcpy(cfirst.p, hcf.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice));
+  }
   // window masks of truncated instances (others keep every row: H conserves parity)
   bool any_masked = false;
   for (const auto& x : inst) any_masked = any_masked || x.masked;
@@ -356,6 +394,17 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
     TRL_HIP(hipMemcpy(Q.p, v0.data(), sizeof(double) * N, hipMemcpyHostToDevice));
   }
 
+  // c[i, col] = Q[:, col] . W over instance i's rows, col < ncols
+  auto dots = [&](int ncols, double* c) {
+    if (!chunked) {
+      hipLaunchKernelGGL(k_seg_dot, dim3(B, ncols), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
+                         seg.as<int64_t>(), c, ldc);
+      return;
+    }
+    hipLaunchKernelGGL(k_seg_dot, dim3(nchunks, ncols), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
+                       cseg.as<int64_t>(), cpart.as<double>(), ldc);
+    hipLaunchKernelGGL(k_dot_fin, dim3(B), dim3(kVT), 0, s, cpart.as<double>(), cfirst.as<int>(), ncols, c, ldc);
+  };
   // Lanczos steps [j0, j1) on the stream: H*v, two Gram-Schmidt passes, normalise
   auto launch_steps = [&](int j0, int j1) -> hipError_t {
     // norm accumulator of the block's first step (later steps are cleared by k_seg_finish)
@@ -364,12 +413,10 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s);
       if (e != hipSuccess) return e;
       if (any_masked) hipLaunchKernelGGL(k_mask, gv, dim3(kVT), 0, s, W.as<double>(), kmask.as<uint8_t>(), N);
-      hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
-                         seg.as<int64_t>(), c1.as<double>(), ldc);
+      dots(j + 1, c1.as<double>());
       hipLaunchKernelGGL(k_seg_sub, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
                          c1.as<double>(), ldc, inst_of.as<int>(), N);
-      hipLaunchKernelGGL(k_seg_dot, dim3(B, j + 1), dim3(kVT), 0, s, Q.as<double>(), ldq, 0, W.as<double>(),
-                         seg.as<int64_t>(), c2.as<double>(), ldc);
+      dots(j + 1, c2.as<double>());
       double* nrm_j = scale.as<double>() + (size_t)(j & 1) * B;
       double* nrm_n = scale.as<double>() + (size_t)((j + 1) & 1) * B;
       hipLaunchKernelGGL(k_seg_sub_norm, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
