@@ -36,7 +36,10 @@ static std::string up(T*& dst, const std::vector<T>& src) {
   return "";
 }
 
-std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
+std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
+  // the product buffers hold the plan's accumulation type: FP64, or FP32 for
+  // the FP32 and BF16 hi/lo modes
+  const size_t xbytes = precision == NSX_FP64 ? sizeof(double) : sizeof(float);
   static std::atomic<uint64_t> generation{0};
   d.gen = ++generation;
   std::string e;
@@ -67,7 +70,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
     d.n_chunk2 = (int)hp.s_chunk2.size();
     d.s_lds1 = hp.s_lds1;
     d.s_lds2 = hp.s_lds2;
-    if (hp.x_size > 0 && hipMalloc(&d.X, sizeof(double) * (size_t)hp.x_size) != hipSuccess)
+    if (hp.x_size > 0 && hipMalloc(&d.X, xbytes * (size_t)hp.x_size) != hipSuccess)
       return "hipMalloc failed (stacked X buffer)";
     const size_t lds = sizeof(double) * (size_t)std::max(hp.s_lds1, hp.s_lds2);
     if (lds > 64 * 1024 - 4096)
@@ -89,10 +92,10 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d) {
     d.n_gtile1 = (int)hp.g_tile1.size();
     d.n_gtile2 = (int)hp.g_tile2.size();
     d.n_gtile3 = (int)hp.g_tile3.size();
-    if (hp.xg_size > 0 && hipMalloc(&d.XG, sizeof(double) * (size_t)hp.xg_size) != hipSuccess)
+    if (hp.xg_size > 0 && hipMalloc(&d.XG, xbytes * (size_t)hp.xg_size) != hipSuccess)
       return "hipMalloc failed (large-block X buffer)";
     // X tiles that vanish identically are never computed: they stay zero
-    if (hp.xg_size > 0 && hipMemset(d.XG, 0, sizeof(double) * (size_t)hp.xg_size) != hipSuccess)
+    if (hp.xg_size > 0 && hipMemset(d.XG, 0, xbytes * (size_t)hp.xg_size) != hipSuccess)
       return "hipMemset failed (large-block X buffer)";
     d.atomic = 1;
   }
@@ -181,7 +184,7 @@ static int create_impl(const char* nus_prefix, const char* opfile_a, const char*
     e = build_lowp(h->hplan, precision, h->lowp);
     if (!e.empty()) { set_error(e); return NSX_EGPU; }
   }
-  e = upload_plan(h->hplan, h->dplan);
+  e = upload_plan(h->hplan, h->dplan, precision);
   if (!e.empty()) { set_error(e); return NSX_EGPU; }
   NSX_HIP(hipStreamCreate(&h->stream));
   const int64_t N = std::max<int64_t>(h->model.N, 1);
@@ -317,7 +320,7 @@ int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const n
     set_error(e);
     return NSX_EGPU;
   }
-  if (!(e = upload_plan(b->hplan, b->dplan)).empty()) { set_error(e); return NSX_EGPU; }
+  if (!(e = upload_plan(b->hplan, b->dplan, precision)).empty()) { set_error(e); return NSX_EGPU; }
   if (cfg) {
     b->dplan.staged_nt = cfg->threads;
     b->dplan.staged_mode = cfg->mode;
@@ -533,7 +536,7 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
   plan_models(models, hp);
   std::string e;
   if (precision != NSX_FP64 && !(e = build_lowp(hp, precision, lp)).empty()) return e;
-  if (!(e = upload_plan(hp, d)).empty()) return e;
+  if (!(e = upload_plan(hp, d, precision)).empty()) return e;
   d.staged_nt = nt;
   d.staged_mode = mode;
   d.atomic = cfg.stacked ? 1 : at;

@@ -1016,7 +1016,7 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
   if (at && d.n_gtile2 > 0) {  // large blocks on the matrix cores
     // the plan's precision is fixed, so XG only ever holds X in this compute type
     using C = typename BigIn<T>::C;
-    C* XG = reinterpret_cast<C*>(d.XG);
+    C* XG = static_cast<C*>(d.XG);
     if (d.n_gtile1 > 0)
       hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1,
                          V, Bh, XG);

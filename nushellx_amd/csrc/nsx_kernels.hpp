@@ -35,7 +35,7 @@ struct DevicePlan {
   STerm2* s_term2 = nullptr;
   SXRef* s_xref = nullptr;
   int n_chunk1 = 0, n_chunk2 = 0, s_lds1 = 1, s_lds2 = 1;
-  void* X = nullptr;         // x_size elements of the accumulation type (8 bytes reserved each)
+  void* X = nullptr;         // x_size elements of the accumulation type (FP64, or FP32 in reduced precision)
   // large-block GEMM path (FP64, atomic mode)
   GTask1* g_task1 = nullptr;
   GTile1* g_tile1 = nullptr;
@@ -44,12 +44,12 @@ struct DevicePlan {
   GTerm2* g_term2 = nullptr;
   SXRef* g_xref = nullptr;
   int n_gtile1 = 0, n_gtile2 = 0, n_gtile3 = 0;
-  double* XG = nullptr;      // xg_size doubles
+  void* XG = nullptr;        // xg_size elements of the accumulation type (FP64, or FP32 in reduced precision)
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
 };
 
 // Uploads a host plan; returns "" or an error message.
-std::string upload_plan(const HostPlan& hp, DevicePlan& d);
+std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision);
 void free_plan(DevicePlan& d);
 
 // Operand storage of the BF16 hi/lo split: x ~ hi + lo with hi = bf16(x) and
