@@ -40,16 +40,22 @@ __global__ __launch_bounds__(kVT) void k_seg_dot(const double* __restrict__ Q, i
   const int i = blockIdx.x, col = col0 + blockIdx.y;
   const int64_t r0 = seg[i], r1 = seg[i + 1];
   const double* q = Q + ldq * col;
-  // four independent chains so that four loads of each operand are in flight
+  // rounds of four rows per thread (four independent chains, every load of a
+  // round in flight together); the last round is guarded instead of serialised
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int64_t r = r0 + threadIdx.x;
-  for (; r + 3 * kVT < r1; r += 4 * kVT) {
-    s0 = fma(q[r], w[r], s0);
-    s1 = fma(q[r + kVT], w[r + kVT], s1);
-    s2 = fma(q[r + 2 * kVT], w[r + 2 * kVT], s2);
-    s3 = fma(q[r + 3 * kVT], w[r + 3 * kVT], s3);
+  for (int64_t base = r0 + threadIdx.x; base < r1; base += 4 * kVT) {
+    double qa[4], wa[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t r = base + (int64_t)u * kVT;
+      qa[u] = r < r1 ? q[r] : 0.0;
+      wa[u] = r < r1 ? w[r] : 0.0;
+    }
+    s0 = fma(qa[0], wa[0], s0);
+    s1 = fma(qa[1], wa[1], s1);
+    s2 = fma(qa[2], wa[2], s2);
+    s3 = fma(qa[3], wa[3], s3);
   }
-  for (; r < r1; r += kVT) s0 = fma(q[r], w[r], s0);
   double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   __shared__ double red[kVT / 64];
@@ -87,12 +93,28 @@ constexpr int kSubRows = kVT / 4;
 __device__ __forceinline__ double sub_partial(const double* __restrict__ Q, int64_t ldq, int ncols,
                                               const double* __restrict__ c, int ldc, const int* __restrict__ inst,
                                               int64_t n, int64_t r, int p) {
-  double s = 0.0;
+  // eight columns of the class per round, every load issued before the first
+  // product, so a round costs one memory latency
+  double s0 = 0.0, s1 = 0.0;
   if (r < n) {
     const double* ci = c + (int64_t)inst[r] * ldc;
-    for (int k = p; k < ncols; k += 4) s = fma(Q[r + ldq * k], ci[k], s);
+    const double* qr = Q + r;
+    for (int k = p; k < ncols; k += 32) {
+      double qv[8], cv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int kk = k + 4 * u;
+        qv[u] = kk < ncols ? qr[ldq * kk] : 0.0;
+        cv[u] = kk < ncols ? ci[kk] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        s0 = fma(qv[u], cv[u], s0);
+        s1 = fma(qv[u + 1], cv[u + 1], s1);
+      }
+    }
   }
-  return s;
+  return s0 + s1;
 }
 
 // w[r] -= sum_{col < ncols} Q[r, col] c[inst(r), col]; grid ceil(n / 64)
