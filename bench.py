@@ -204,6 +204,43 @@ def ensemble(log, samples: int = 256):
     return out
 
 
+def precision_study(log):
+    """sd shell 6p6n, J = 0, 2, 4 (N = 3372, 13562, 15089; same seeded
+    interaction as the headline workload): the lowest five states of every J
+    with the block contractions in FP64, FP32 and BF16 hi/lo (the recurrence,
+    reorthogonalisation and Ritz solve stay FP64), to residual 1e-6 MeV, in one
+    batch per precision.  Eigenvalue errors are against the FP64 solve at
+    residual 1e-9 MeV (the solver that the tests pin to dense eigvalsh)."""
+    from nushellx_amd.hv import Batch, Hamiltonian
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import SD, PNSpace
+    from nushellx_amd.synth.writer import PNModel, write_model
+    work = tempfile.mkdtemp(prefix="nsx_prec_")
+    J2s = [0, 4, 8]
+    mf = write_model(PNModel(PNSpace(SD), 6, 6, random_interaction(SD, seed=SEED)), work, "sd66", J2s)
+    hams = [Hamiltonian(*mf.args(J2)) for J2 in J2s]
+    ref_b = Batch(hams)
+    ref = ref_b.lanczos(nev=5, tol=1e-9)["evals"]
+    ref_b.close()
+    out = {"workload": "sd 6p6n J=0,2,4 (N = %s), lowest 5 each, residual 1e-6 MeV" % "/".join(str(h.N) for h in hams),
+           "reference": "FP64 solve at residual 1e-9 MeV", "evals_MeV": ref.tolist()}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        b = Batch(hams, precision=prec)
+        b.autotune(iters=5)
+        b.lanczos(nev=5, tol=1e-6)  # first solve captures the graphs
+        t0 = time.perf_counter()
+        r = b.lanczos(nev=5, tol=1e-6)
+        el = time.perf_counter() - t0
+        err = np.abs(r["evals"] - ref)
+        out[prec] = {"s": el, "matvecs": int(r["matvecs"]), "max_err_MeV": float(err.max()),
+                     "err_MeV": err.tolist()}
+        log(f"precision study {prec}: {el * 1e3:.1f} ms, {r['matvecs']} H*v, max eigenvalue error {err.max():.2e} MeV")
+        b.close()
+    for h in hams:
+        h.close()
+    return out
+
+
 def pf_warm(log, train_samples: int):
     """Learned warm start on pf 4p4n J=0 (N = 41,355; 195 TBMEs ~ N(0, 1.5 MeV),
     SPEs ~ U[-10, -4] MeV): network trained on the lowest-8 Lanczos eigenvectors
@@ -354,6 +391,7 @@ def main():
     ap.add_argument("--ensemble-samples", type=int, default=256,
                     help="interaction samples of the batched ensemble (0 skips it)")
     ap.add_argument("--no-sep", action="store_true", help="skip the separable-operator (NuTRLZ) timings")
+    ap.add_argument("--no-prec", action="store_true", help="skip the sd 6p6n precision study")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the pf 6p6n J=0 H*v (N = 1.78 M; about a minute, mostly writing its files)")
     ap.add_argument("--pf-warm-samples", type=int, default=0,
@@ -442,6 +480,8 @@ def main():
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
     if not args.no_sep:
         out["separable"] = separable(log, mf)
+    if not args.no_prec:
+        out["precision"] = precision_study(log)
     if not args.no_large:
         try:
             out["large"] = large(log, cpu=not args.no_cpu)
