@@ -766,7 +766,7 @@ __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + BigMma<C>::row(lane, v);
-        if (i < m && j < n) X[g.x_off + i + (int64_t)m * j] = acc[r][c][v];
+        if (i < m && j < n) X[g.x_off + i + (int64_t)big_ldx(m) * j] = acc[r][c][v];
       }
 }
 
@@ -789,7 +789,7 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
   // staging: A tile element (i = tid % 64, kk = tid / 64 + 4q), Y tile element
   // (kk = tid % kBigK, j = tid / kBigK + kBigJS q)
   auto load = [&](const GTerm2& tm, int k0) {
-    const int dar = tm.dar;
+    const int dar = tm.dar, ldx = big_ldx(dar);
 #pragma unroll
     for (int q = 0; q < kBigQ; ++q) {
       const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
@@ -803,7 +803,7 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
 #pragma unroll
       for (int q = 0; q < kBigQ; ++q) {
         const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
-        if (kk < dar && j < db) ry[q] = fma(r, X[xr.x_off + kk + (int64_t)dar * j], ry[q]);
+        if (kk < dar && j < db) ry[q] = fma(r, X[xr.x_off + kk + (int64_t)ldx * j], ry[q]);
       }
     }
   };

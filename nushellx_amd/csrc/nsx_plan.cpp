@@ -418,7 +418,7 @@ void build_big(HostPlan& p, const std::vector<const Model*>& models) {
             task = (int)p.g_task1.size();
             task_of[{b, v}] = task;
             p.g_task1.push_back(GTask1{v, b, p.xg_size, dar, m.db[rt.R], db, 0});
-            p.xg_size += int64_t(dar) * db;
+            p.xg_size += int64_t(big_ldx(dar)) * db;  // stays a multiple of kBigStep
             rows_needed.emplace_back(used.size(), 0);
           } else {
             task = f->second;

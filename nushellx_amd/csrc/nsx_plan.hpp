@@ -127,8 +127,11 @@ constexpr int kBigTile = 64;     // output tile side
 constexpr int kBigStep = 16;     // K step of the tiles
 constexpr int kBigMaxK = 2048;   // largest inner dimension the K-step masks cover (128 steps)
 constexpr int kBigKChunk = 128;  // active K steps one stage-2 tile covers (split-K)
+// X of the large-block path: columns padded to a multiple of kBigStep rows and
+// 128-byte aligned, so that a K step of a column is one cache line
+constexpr int big_ldx(int m) { return (m + kBigStep - 1) / kBigStep * kBigStep; }
 struct GTask1 {
-  int64_t v_off, b_off, x_off;   // V_R (m x k), Bhat^T (k x n), X (m x n), column-major
+  int64_t v_off, b_off, x_off;   // V_R (m x k), Bhat^T (k x n), X (m x n, leading dimension big_ldx(m))
   int32_t m, k, n, pad;
 };
 struct GTile1 {

@@ -112,7 +112,7 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
         double s = 0.0;
         for (int k = 0; k < g.k; ++k)
           if (active(t.kmask, k)) s += vin[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
-        XG[g.x_off + i + int64_t(g.m) * j] = s;
+        XG[g.x_off + i + int64_t(big_ldx(g.m)) * j] = s;
       }
   }
   for (const GTile2& t : p.g_tile2)
@@ -125,7 +125,7 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
             if (!active(tm.kmask, k)) continue;
             double y = 0.0;
             for (int x = tm.x_begin; x < tm.x_end; ++x)
-              y += p.g_xref[x].r * XG[p.g_xref[x].x_off + k + int64_t(tm.dar) * j];
+              y += p.g_xref[x].r * XG[p.g_xref[x].x_off + k + int64_t(big_ldx(tm.dar)) * j];
             s += p.Ad[tm.a_off + i + int64_t(t.da) * k] * y;
           }
         }
