@@ -70,15 +70,20 @@ __global__ __launch_bounds__(kVT) void k_seg_dot(const double* __restrict__ Q, i
 
 // Long instances: k_seg_dot runs over row chunks of at most kDotChunk rows
 // (a segment table of chunks, partial dots per chunk), and this kernel sums the
-// chunks of instance i in order: c[i, col] = sum_{ch in [cfirst[i], cfirst[i+1])} part[ch, col]
+// chunks of instance i in a fixed order: c[i, col] = sum_{ch in [cfirst[i], cfirst[i+1])} part[ch, col]
 constexpr int64_t kDotChunk = 16384;
 __global__ __launch_bounds__(kVT) void k_dot_fin(const double* __restrict__ part, const int* __restrict__ cfirst,
                                                  int ncols, double* __restrict__ c, int ldc) {
   const int i = blockIdx.x;
+  const int ch0 = cfirst[i], ch1 = cfirst[i + 1];
   for (int col = threadIdx.x; col < ncols; col += kVT) {
-    double t = 0.0;
-    for (int ch = cfirst[i]; ch < cfirst[i + 1]; ++ch) t += part[(int64_t)ch * ldc + col];
-    c[(int64_t)i * ldc + col] = t;
+    // eight chunks per round, their loads in flight together
+    double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int ch = ch0; ch < ch1; ch += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] += ch + u < ch1 ? part[(int64_t)(ch + u) * ldc + col] : 0.0;
+    }
+    c[(int64_t)i * ldc + col] = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
   }
 }
 
