@@ -59,9 +59,16 @@ def cpu_baseline(mf, seconds: float = 10.0):
         el = time.perf_counter() - t0
         if el >= seconds and n >= 20:
             break
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n / el, "unit": "matvecs/s", "cores": threads, "kind": "port",
             "sample": f"{n} H*v over the nine sd 4p4n J=0..8 instances in {el:.1f} s "
-                      f"(REAL*4 build with the reference's 1e-7 product thresholds)"}
+                      f"(REAL*4 build with the reference's 1e-7 product thresholds)",
+            "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
 def traffic_from_profile(stacked: bool):
