@@ -495,7 +495,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   const int nt = c.t_end - c.t_begin;
   const int tid = threadIdx.x;
   for (int t = tid; t < nt; t += NT) st[t] = terms[c.t_begin + t];
-  const int x0 = terms[c.t_begin].x_begin, nx = terms[c.t_end - 1].x_end - x0;
+  const int x0 = c.x_begin, nx = c.x_end - x0;  // issued with the term loads
   for (int x = tid; x < nx; x += NT) sx[x] = xrefs[x0 + x];
   __syncthreads();
   // staging: A elements are plain copies; a Y element sums r X over its term's

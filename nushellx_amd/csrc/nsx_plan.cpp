@@ -330,14 +330,20 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
       c.w_off = g.first[3]; c.da = da; c.db = db;
       c.t_begin = c.t_end = (int)p.s_term2.size();
     };
+    auto close = [&]() {
+      if (c.t_end == c.t_begin) return;
+      c.x_begin = p.s_term2[c.t_begin].x_begin;
+      c.x_end = p.s_term2[c.t_end - 1].x_end;
+      p.s_chunk2.push_back(c);
+      p.s_lds2 = std::max(p.s_lds2, c.lds);
+    };
     open();
     for (const T2& t : g.second) {
       const int need = da * t.dar + t.dar * db;
       const int nx = (int)p.s_xref.size() - (c.t_end > c.t_begin ? p.s_term2[c.t_begin].x_begin : 0);
       if (c.t_end > c.t_begin && (c.lds + need > cap || c.t_end - c.t_begin >= kStkTerms ||
                                   nx + (int)t.x.size() > kStkXref)) {
-        p.s_chunk2.push_back(c);
-        p.s_lds2 = std::max(p.s_lds2, c.lds);
+        close();
         open();
       }
       STerm2 s{};
@@ -349,8 +355,7 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
       c.t_end++;
       c.lds += need;
     }
-    p.s_chunk2.push_back(c);
-    p.s_lds2 = std::max(p.s_lds2, c.lds);
+    close();
   }
 }
 

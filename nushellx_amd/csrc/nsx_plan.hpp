@@ -103,6 +103,7 @@ struct SChunk2 {
   int32_t da, db;
   int32_t t_begin, t_end;   // STerm2 range (a-terms of this left block)
   int32_t lds;              // doubles staged: sum over terms of da*dar + dar*db
+  int32_t x_begin, x_end;   // SXRef range of those terms (read with the terms, not after them)
   int32_t pad;
 };
 struct STerm2 {
