@@ -248,12 +248,14 @@ def precision_study(log):
     return out
 
 
-def pf_warm(log, train_samples: int):
+def pf_warm(log, train_samples: int, nprot: int = 4, nneut: int = 4, chunk: int = 64, held_n: int = 8,
+            threads: int = 16, train_tol: float = 1e-6):
     """Learned warm start on pf 4p4n J=0 (N = 41,355; 195 TBMEs ~ N(0, 1.5 MeV),
-    SPEs ~ U[-10, -4] MeV): network trained on the lowest-8 Lanczos eigenvectors
-    of `train_samples` perturbed interactions, then 8 held-out interactions
-    solved in one lockstep batch from the random and from the predicted start
-    (lowest 5, residual 1e-4 MeV)."""
+    SPEs ~ U[-10, -4] MeV), or pf 6p6n J=0 (N = 1,777,116) with nprot = nneut
+    = 6: network trained on the lowest-8 Lanczos eigenvectors of
+    `train_samples` perturbed interactions (solved `chunk` at a time), then
+    `held_n` held-out interactions solved in one lockstep batch from the random
+    and from the predicted start (lowest 5, residual 1e-4 MeV)."""
     from nushellx_amd.ensemble import build_instances, solve_ensemble, write_ensemble
     from nushellx_amd.synth.interaction import random_interaction
     from nushellx_amd.synth.mspace import PF, PNSpace
@@ -262,15 +264,17 @@ def pf_warm(log, train_samples: int):
     space, J2s = PNSpace(PF), [0]
     work = tempfile.mkdtemp(prefix="nsx_pfw_")
     t0 = time.perf_counter()
-    F, T, _, _ = build_training_set_batched(space, 4, 4, J2s, base, range(1000, 1000 + train_samples), work,
-                                            log=log)
+    F, T, _, _ = build_training_set_batched(space, nprot, nneut, J2s, base, range(1000, 1000 + train_samples), work,
+                                            tol=train_tol, chunk=chunk, threads=threads, log=log)
     t_data = time.perf_counter() - t0
     ws = WarmStart(F.shape[1], T.shape[1], seed=0)
     t0 = time.perf_counter()
     losses = ws.fit(F, T, epochs=200, batch=16, lr=1e-3)
     t_fit = time.perf_counter() - t0
-    held = list(range(9001, 9009))
-    hams = build_instances(write_ensemble(space, [(4, 4)], J2s, base, held, work + "/held"), J2s)
+    log(f"warm start fitted ({t_fit:.1f} s)")
+    held = list(range(9001, 9001 + held_n))
+    hams = build_instances(write_ensemble(space, [(nprot, nneut)], J2s, base, held, work + "/held"), J2s,
+                           threads=threads)
     feats = np.array([base.perturbed(s).parameter_vector() for s in held])
     r0 = solve_ensemble(hams, nev=5, tol=1e-4)
     b = r0["batch"]
@@ -281,7 +285,8 @@ def pf_warm(log, train_samples: int):
     t0 = time.perf_counter()
     rr = b.lanczos(nev=5, tol=1e-4)
     t_rand = time.perf_counter() - t0
-    out = {"workload": f"pf 4p4n J=0, {len(F)} training samples, 8 held-out interactions in one batch",
+    out = {"workload": f"pf {nprot}p{nneut}n J=0, {len(F)} training samples, {held_n} held-out interactions "
+                       f"in one batch",
            "train_solve_s": t_data, "fit_s": t_fit, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
            "random": {"s": t_rand, "matvecs": rr["matvecs"], "converged_at": rr["converged_at"].tolist()},
            "warm": {"s": t_warm, "matvecs": rw["matvecs"], "converged_at": rw["converged_at"].tolist(),
@@ -401,6 +406,8 @@ def main():
     ap.add_argument("--no-prec", action="store_true", help="skip the sd 6p6n precision study")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the pf 6p6n J=0 H*v (N = 1.78 M; about a minute, mostly writing its files)")
+    ap.add_argument("--pf66-warm-samples", type=int, default=0,
+                    help="training samples of the pf 6p6n J=0 warm-start study (0 skips it; 16 take ~6 min)")
     ap.add_argument("--pf-warm-samples", type=int, default=0,
                     help="training samples of the pf 4p4n warm-start study (0 skips it; 512 takes ~8 min)")
     args = ap.parse_args()
@@ -499,6 +506,9 @@ def main():
         out["ensemble"] = ensemble(log, args.ensemble_samples)
     if args.pf_warm_samples > 0:
         out["pf_warm"] = pf_warm(log, args.pf_warm_samples)
+    if args.pf66_warm_samples > 0:
+        out["pf66_warm"] = pf_warm(log, args.pf66_warm_samples, nprot=6, nneut=6, chunk=4, held_n=4, threads=4,
+                                   train_tol=1e-5)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mf, args.cpu_seconds)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
