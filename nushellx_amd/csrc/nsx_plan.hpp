@@ -1,12 +1,14 @@
 // Device work plan of a batch of pn J-scheme H*v instances.
 //
 // The vectors of all instances are concatenated (instance i occupies
-// [vbase_i, vbase_i + N_i)).  Two kernels make one H*v:
-//   1. pn_partial: one workgroup per (item, column tile); an item is one
-//      (block L, a-side J') pair.  It forms Y = sum_R r V_R Bhat^T for each
-//      a-density and accumulates A Y into a private partial tile (no atomics).
-//   2. reduce: one workgroup per (block L, column tile); sums the item
-//      partials of L in a fixed order, adds H_a V_L + V_L H_b^T, writes W_L.
+// [vbase_i, vbase_i + N_i)).  An item is one (block L, a-side J') pair of the
+// pn part, W_L += sum_k A_k sum_R r V_R Bhat^T.  Items whose blocks all have
+// sides up to kSmallMax take the stacked two-stage path (or the staged kernel);
+// larger items take the large-block matrix-core tiles, or, when a block is too
+// large for their K-step masks, the direct kernel (pn_partial: per (item,
+// column tile), partials or atomics).  The like-particle part H_a V_L +
+// V_L H_b^T is a reduce pass per block (inside the stacked stage-1 launch on
+// that path), or matrix-core tiles for large blocks.
 #pragma once
 #include <cstdint>
 #include <unordered_map>
@@ -116,8 +118,9 @@ struct SXRef {
   double r;
 };
 
-// Large-block path (blocks with a side above kSmallMax, FP64): the same two
-// stages as tiled GEMMs on the matrix cores, 64 x 64 output tiles.
+// Large-block path (blocks with a side above kSmallMax, in every precision of
+// an atomic-mode plan): the same two stages as tiled GEMMs on the matrix
+// cores, 64 x 64 output tiles.
 //   G1: X_u = V_R Bhat^T for every distinct (V block, Bhat block) pair;
 //   G2: W_L += sum_k A_k Y_k, Y_k = sum_u r_u X_u, tiles of W_L over chunks of
 //       its a-terms (split-K), added into W with FP64 atomics.
@@ -221,8 +224,9 @@ void plan_merge(HostPlan& p, const HostPlan& q);
 // (the instances planned into p, in order).  Clears p.cfg.stacked (the staged
 // kernel then runs) if an a-term has more references than a chunk can hold.
 void build_stacked(HostPlan& p, const std::vector<const Model*>& models);
-// Fills the large-block GEMM arrays of p for the items the direct kernel would
-// take (a side above kSmallMax).
+// Fills the large-block GEMM arrays of p for the items with a side above
+// p.cfg.big_min (kSmallMax unless a test lowers it); leaves them empty, so the
+// direct kernel takes those items, when a block exceeds kBigMaxK.
 void build_big(HostPlan& p, const std::vector<const Model*>& models);
 
 }  // namespace nsx
