@@ -9,7 +9,9 @@
 //   Bf16x2   BF16 hi/lo pairs x = hi + lo, a*b = ah*bh + ah*bl + al*bh, FP32 accumulation
 // W is always written in FP64.  FP64 vector FMA runs at the FP64 matrix-core
 // rate on MI355X (78.6 TFLOP/s both), so the ragged, tiny (<= 33 x 33 in the sd
-// shell) contractions use VALU FMAs with 4 output columns per thread.
+// shell) contractions use VALU FMAs with 4 output columns per thread; blocks
+// with a side above 40 (pf shell, up to 615) run as 64 x 64 matrix-core tiles
+// (k_big0/1/2, FP64 MFMA in FP64 and FP32 MFMA in the two other modes).
 #include <hip/hip_runtime.h>
 
 #include "nsx_kernels.hpp"
