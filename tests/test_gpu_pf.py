@@ -58,11 +58,12 @@ def test_pf44_lanczos_matches_arpack(pf44):
 
 
 @pytest.mark.parametrize("name,J2,big_min,dop", [("sd44", 0, 8, None), ("sd44", 8, 8, None), ("sd23", 3, 4, None),
-                                                 ("sd44", 4, 8, "cut")])
+                                                 ("sd44", 4, 8, "cut"), ("psd33t", 2, 4, None)])
 def test_large_block_path_on_small_models(models, dops, name, J2, big_min, dop, monkeypatch):
     """The large-block GEMM kernels (MFMA tiles, K-step masks, split-K atomics,
-    several X references per a-term when J > 0) on sd models, reached by
-    lowering the block side above which they run."""
+    several X references per a-term when J > 0) on sd models, a separable
+    operator and a truncated (N-hbar-omega window) basis, reached by lowering
+    the block side above which they run."""
     from nushellx_amd.hv import Hamiltonian
     monkeypatch.setenv("NSX_BIG_MIN", str(big_min))
     args = models[name].args(J2)
