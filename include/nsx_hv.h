@@ -24,8 +24,8 @@ enum {
 
 /* Arithmetic of the block contractions.  Everything outside them (the Lanczos
  * recurrence, reorthogonalisation, Ritz solve, residuals) is FP64 in all modes.
- * Blocks with a side above 40 run on matrix-core tiles: FP64 ones in NSX_FP64,
- * FP32 ones in the two other modes (a BF16 pair enters as its FP32 value hi + lo). */
+ * Blocks with a side above 40 run on matrix-core tiles: FP64 MFMA in NSX_FP64,
+ * FP32 MFMA in NSX_FP32, BF16 MFMA (the three products) in NSX_BF16X3. */
 enum {
   NSX_FP64 = 0,   /* FP64 GEMMs: the plain drop-in */
   NSX_FP32 = 1,   /* FP32 operands, FP32 accumulation */

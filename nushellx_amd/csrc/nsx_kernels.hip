@@ -11,8 +11,11 @@
 // rate on MI355X (78.6 TFLOP/s both), so the ragged, tiny (<= 33 x 33 in the sd
 // shell) contractions use VALU FMAs with 4 output columns per thread; blocks
 // with a side above 40 (pf shell, up to 615) run as 64 x 64 matrix-core tiles
-// (k_big0/1/2, FP64 MFMA in FP64 and FP32 MFMA in the two other modes).
+// (k_big0/1/2: FP64 MFMA in FP64, FP32 MFMA in FP32; k_big*_bf: BF16 MFMA for
+// BF16 hi/lo pairs).
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "nsx_kernels.hpp"
 
@@ -619,9 +622,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
 
 // ---------------------------------------------------------------- large-block GEMM path
 // 64 x 64 output tiles on the matrix cores: FP64 (v_mfma_f64_16x16x4f64) for
-// FP64 plans, FP32 (v_mfma_f32_16x16x4f32) for FP32 and BF16 hi/lo operands
-// (a hi/lo pair is expanded to the FP32 value hi + lo, which FP32 holds
-// exactly).  In both, lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16);
+// FP64 plans, FP32 (v_mfma_f32_16x16x4f32) for FP32 plans.  (The BigIn<Bf16x2>
+// expansion of a hi/lo pair to its FP32 value hi + lo, exact in FP32, keeps
+// these templates complete for BF16 operands; BF16 plans run the k_big*_bf
+// kernels below on the BF16 matrix cores instead.)  In both, lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16);
 // it holds D(l / 16 + 4 v, l % 16) in FP64 and D(4 (l / 16) + v, l % 16) in FP32.
 // Four wavefronts take 32 x 32 quarters as 2 x 2 MFMA blocks.  The product is
 // formed transposed, D = C^T, so that a lane's outputs run down a column of the
@@ -914,6 +918,243 @@ __global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, 
       }
 }
 
+// ---------------------------------------------------------------- large blocks, BF16 hi/lo
+// The same tiles for BF16 hi/lo operands on the BF16 matrix cores
+// (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation): per 16-deep K step and
+// 16 x 16 block, three products ah*bh + ah*bl + al*bh (the lo*lo term is below
+// the pair's resolution), one instruction each.  Lane l supplies A(l % 16,
+// 4 (l / 16) + t) and B(4 (l / 16) + t, l % 16), t = 0..3, as four packed
+// BF16, and holds D(4 (l / 16) + v, l % 16).  LDS keeps the hi and lo planes
+// of both operands as [row][k], 16 K values per row padded to 20 (40 bytes),
+// so that a lane's four K values are one conflict-free 8-byte read.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+constexpr int kBfLd = kBigK + 4;
+constexpr int kBfPlane = kBigTile * kBfLd;  // one operand plane of one buffer (uint16)
+
+struct alignas(16) BfTile {  // one buffer: hi / lo planes of the row operand (P, [j][k]) and column operand (Q, [i][k])
+  uint16_t ph[kBfPlane], pl[kBfPlane], qh[kBfPlane], ql[kBfPlane];
+};
+
+__device__ __forceinline__ void bf_put(uint16_t* hi, uint16_t* lo, int idx, Bf16x2 x) {
+  hi[idx] = (uint16_t)(x.bits >> 16);
+  lo[idx] = (uint16_t)(x.bits & 0xffffu);
+}
+
+__device__ __forceinline__ void big_mma_bf(const BfTile& b, int wj, int wi, int lane, f32x4 (&acc)[2][2]) {
+  const int kq = 4 * (lane / 16);
+  bf16x4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int o = (wj + r * 16 + lane % 16) * kBfLd + kq;
+    ah[r] = *reinterpret_cast<const bf16x4*>(b.ph + o);
+    al[r] = *reinterpret_cast<const bf16x4*>(b.pl + o);
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int o = (wi + c * 16 + lane % 16) * kBfLd + kq;
+    bh[c] = *reinterpret_cast<const bf16x4*>(b.qh + o);
+    bl[c] = *reinterpret_cast<const bf16x4*>(b.ql + o);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[r], bh[c], acc[r][c], 0, 0, 0);
+      acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[r], bl[c], acc[r][c], 0, 0, 0);
+      acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[r], bh[c], acc[r][c], 0, 0, 0);
+    }
+}
+
+// G1 in BF16 hi/lo: X = V_R Bhat^T for one 64 x 64 tile, X in FP32
+__global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tiles, const GTask1* __restrict__ tasks,
+                                                 const Bf16x2* __restrict__ V, const Bf16x2* __restrict__ Bt,
+                                                 float* __restrict__ X) {
+  __shared__ BfTile sb[2];  // double-buffered: one barrier per K step
+  const GTile1 t = tiles[blockIdx.x];
+  const GTask1 g = tasks[t.task];
+  const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  f32x4 acc[2][2];
+  big_zero<float>(acc);
+  // staging: V tile element (i = tid % 64, kk = tid / 64 + 4q) into Q[i][kk],
+  // Bhat^T tile element (kk = tid % 16, j = tid / 16 + 16q) into P[j][kk]
+  Bf16x2 rv[kBigQ], rb[kBigQ];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) {
+      const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+      rv[q] = (i < m && kk < K) ? V[g.v_off + i + (int64_t)m * kk] : Bf16x2{0u};
+    }
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) {
+      const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
+      rb[q] = (kk < K && j < n) ? Bt[g.b_off + kk + (int64_t)K * j] : Bf16x2{0u};
+    }
+  };
+  auto stage = [&](BfTile& b) {
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) bf_put(b.qh, b.ql, (tid % 64) * kBfLd + tid / 64 + 4 * q, rv[q]);
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) bf_put(b.ph, b.pl, (tid / kBigK + kBigJS * q) * kBfLd + tid % kBigK, rb[q]);
+  };
+  uint64_t m0 = t.kmask[0], m1 = t.kmask[1];
+  int step = pop_step(m0, m1), cur = 0;
+  if (step >= 0) {
+    load(step * kBigK);
+    stage(sb[0]);
+    __syncthreads();
+  }
+  while (step >= 0) {
+    step = pop_step(m0, m1);
+    if (step >= 0) load(step * kBigK);
+    big_mma_bf(sb[cur], wj, wi, lane, acc);
+    if (step >= 0) stage(sb[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + 4 * (lane / 16) + v;
+        if (i < m && j < n) X[g.x_off + i + (int64_t)big_ldx(m) * j] = acc[r][c][v];
+      }
+}
+
+// G2 in BF16 hi/lo: W_L += sum_k A_k Y_k, Y_k = sum_u r_u X_u formed in FP32
+// and split into a hi/lo pair while staging
+__global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tiles, const GTerm2* __restrict__ terms,
+                                                 const SXRef* __restrict__ xrefs, const Bf16x2* __restrict__ Ad,
+                                                 const float* __restrict__ X, double* __restrict__ W) {
+  __shared__ BfTile sb[2];
+  const GTile2 t = tiles[blockIdx.x];
+  const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  f32x4 acc[2][2];
+  big_zero<float>(acc);
+  Bf16x2 ra[kBigQ];
+  float ry[kBigQ];
+  auto load = [&](const GTerm2& tm, int k0) {
+    const int dar = tm.dar, ldx = big_ldx(dar);
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) {
+      const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+      ra[q] = (i < da && kk < dar) ? Ad[tm.a_off + i + (int64_t)da * kk] : Bf16x2{0u};
+    }
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) ry[q] = 0.0f;
+    for (int x = tm.x_begin; x < tm.x_end; ++x) {
+      const SXRef xr = xrefs[x];
+      const float r = (float)xr.r;
+#pragma unroll
+      for (int q = 0; q < kBigQ; ++q) {
+        const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
+        if (kk < dar && j < db) ry[q] = fmaf(r, X[xr.x_off + kk + (int64_t)ldx * j], ry[q]);
+      }
+    }
+  };
+  auto stage = [&](BfTile& b) {
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) bf_put(b.qh, b.ql, (tid % 64) * kBfLd + tid / 64 + 4 * q, ra[q]);
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q)
+      bf_put(b.ph, b.pl, (tid / kBigK + kBigJS * q) * kBfLd + tid % kBigK, bf16x2_split(ry[q]));
+  };
+  int tc = t.t_begin;
+  GTerm2 tm = terms[tc];
+  uint64_t m0 = tm.kmask[0], m1 = tm.kmask[1];
+  int step = pop_step(m0, m1);
+  while (step < 0 && ++tc < t.t_end) {
+    tm = terms[tc];
+    m0 = tm.kmask[0];
+    m1 = tm.kmask[1];
+    step = pop_step(m0, m1);
+  }
+  int cur = 0;
+  if (step >= 0) {
+    load(tm, step * kBigK);
+    stage(sb[0]);
+    __syncthreads();
+  }
+  while (step >= 0) {
+    step = pop_step(m0, m1);
+    while (step < 0 && ++tc < t.t_end) {
+      tm = terms[tc];
+      m0 = tm.kmask[0];
+      m1 = tm.kmask[1];
+      step = pop_step(m0, m1);
+    }
+    if (step >= 0) load(tm, step * kBigK);
+    big_mma_bf(sb[cur], wj, wi, lane, acc);
+    if (step >= 0) stage(sb[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + 4 * (lane / 16) + v;
+        if (i < da && j < db) unsafeAtomicAdd(W + t.w_off + i + (int64_t)da * j, (double)acc[r][c][v]);
+      }
+}
+
+// G0 in BF16 hi/lo: W_L = H_a V_L + V_L H_b^T for one 64 x 64 tile, stored
+__global__ __launch_bounds__(256) void k_big0_bf(const GTile3* __restrict__ tiles, const Bf16x2* __restrict__ V,
+                                                 const Bf16x2* __restrict__ H, double* __restrict__ W) {
+  __shared__ BfTile sb[1];
+  const GTile3 t = tiles[blockIdx.x];
+  const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  f32x4 acc[2][2];
+  big_zero<float>(acc);
+  const Bf16x2* v = V + t.v_off;
+  for (int ph = 0; ph < 2; ++ph) {
+    const int64_t hoff = ph == 0 ? t.ha_off : t.hb_off;
+    if (hoff < 0) continue;
+    const Bf16x2* Am = ph == 0 ? H + hoff : v;
+    const Bf16x2* Ym = ph == 0 ? v : H + hoff;
+    const int K = ph == 0 ? da : db, ldy = ph == 0 ? da : db;
+    for (int k0 = 0; k0 < K; k0 += kBigK) {
+      Bf16x2 ra[kBigQ], ry[kBigQ];
+#pragma unroll
+      for (int q = 0; q < kBigQ; ++q) {
+        const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+        ra[q] = (i < da && kk < K) ? Am[i + (int64_t)da * kk] : Bf16x2{0u};
+      }
+#pragma unroll
+      for (int q = 0; q < kBigQ; ++q) {
+        const int kk = k0 + tid % kBigK, j = j0 + tid / kBigK + kBigJS * q;
+        ry[q] = (kk < K && j < db) ? Ym[kk + (int64_t)ldy * j] : Bf16x2{0u};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kBigQ; ++q) bf_put(sb[0].qh, sb[0].ql, (tid % 64) * kBfLd + tid / 64 + 4 * q, ra[q]);
+#pragma unroll
+      for (int q = 0; q < kBigQ; ++q)
+        bf_put(sb[0].ph, sb[0].pl, (tid / kBigK + kBigJS * q) * kBfLd + tid % kBigK, ry[q]);
+      __syncthreads();
+      big_mma_bf(sb[0], wj, wi, lane, acc);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + 4 * (lane / 16) + q;
+        if (i < da && j < db) W[t.v_off + i + (int64_t)da * j] = (double)acc[r][c][q];
+      }
+}
+
 const void* k_stk_ptr(int stage, int prec, int nt) {
 #define NSX_STK(T)                                                                                  \
   if (stage == 1) return nt == 64 ? (const void*)k_stk1<T, 64> : nt == 128 ? (const void*)k_stk1<T, 128> \
@@ -1019,11 +1260,18 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     // the plan's precision is fixed, so XG only ever holds X in this compute type
     using C = typename BigIn<T>::C;
     C* XG = static_cast<C*>(d.XG);
-    if (d.n_gtile1 > 0)
-      hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1,
-                         V, Bh, XG);
-    hipLaunchKernelGGL(k_big2<T>, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2,
-                       d.g_xref, Ad, (const C*)XG, W);
+    if constexpr (std::is_same<T, Bf16x2>::value) {  // BF16 pairs on the BF16 matrix cores
+      if (d.n_gtile1 > 0)
+        hipLaunchKernelGGL(k_big1_bf, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1, V, Bh, XG);
+      hipLaunchKernelGGL(k_big2_bf, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2, d.g_xref, Ad,
+                         (const C*)XG, W);
+    } else {
+      if (d.n_gtile1 > 0)
+        hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1,
+                           V, Bh, XG);
+      hipLaunchKernelGGL(k_big2<T>, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2,
+                         d.g_xref, Ad, (const C*)XG, W);
+    }
     return hipGetLastError();
   }
   if (d.n_items > 0) {
@@ -1043,8 +1291,12 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
   if (!d.stacked && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
                        V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
-  if (big)
-    hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+  if (big) {
+    if constexpr (std::is_same<T, Bf16x2>::value)
+      hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+    else
+      hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+  }
   return hipGetLastError();
 }
 
