@@ -336,8 +336,8 @@ def large(log, steps: int = 5, cpu: bool = True):
     neutrons, J = 0 (N = 1,777,116; blocks up to 615 x 615), seeded interaction
     (TBMEs ~ N(0, 1.5 MeV), SPEs ~ U[-10, -4] MeV).  Its blocks run on the FP64
     matrix-core tiles.  Reports the FP64 H*v time, the reference's sparse flops
-    per second, the time to the lowest five states, and one H*v of the CPU
-    restatement on the host cores."""
+    per second, the time to the lowest five states (FP64, and with the BF16
+    hi/lo contractions), and one H*v of the CPU restatement on the host cores."""
     from nushellx_amd.hv import DeviceBuffer, Hamiltonian, device_sync
     from nushellx_amd.synth.interaction import random_interaction
     from nushellx_amd.synth.mspace import PF, PNSpace
@@ -373,6 +373,27 @@ def large(log, steps: int = 5, cpu: bool = True):
                       "restarts": int(r["restarts"]), "converged": int(r["converged"]),
                       "evals": [float(e) for e in r["evals"]], "max_residual": float(np.max(r["residuals"]))}
     log(f"large: lowest 5 to 1e-4 MeV in {out['lanczos']['seconds']:.2f} s, {r['matvecs']} H*v")
+    # the same solve with the BF16 hi/lo contractions (BF16 matrix cores), Lanczos in FP64
+    hb = Hamiltonian(*mf.args(0), precision="bf16x3")
+    din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+    hb.multiply_dev(din, dout)
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hb.multiply_dev(din, dout)
+    device_sync()
+    ms_b = 1e3 * (time.perf_counter() - t0) / steps
+    din.free()
+    dout.free()
+    t0 = time.perf_counter()
+    rb = hb.lanczos(nev=5, tol=1e-4)
+    out["bf16x3"] = {"hv_ms": ms_b, "lanczos_s": time.perf_counter() - t0, "matvecs": int(rb["matvecs"]),
+                     "max_eval_diff_vs_fp64_MeV": float(np.abs(rb["evals"] - r["evals"]).max())}
+    out["bf16x3"]["speedup_vs_fp64"] = out["lanczos"]["seconds"] / out["bf16x3"]["lanczos_s"]
+    log(f"large: BF16 hi/lo H*v {ms_b:.2f} ms, lowest 5 in {out['bf16x3']['lanczos_s']:.2f} s "
+        f"({out['bf16x3']['speedup_vs_fp64']:.2f}x FP64), eigenvalues within "
+        f"{out['bf16x3']['max_eval_diff_vs_fp64_MeV']:.1e} MeV of the FP64 solve")
+    hb.close()
     if cpu:
         from oracle.refcpu import RefCPU
         r = RefCPU(*mf.args(0), fp64=False, thresholds=True)
