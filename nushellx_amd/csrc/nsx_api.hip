@@ -92,6 +92,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
     d.n_gtile1 = (int)hp.g_tile1.size();
     d.n_gtile2 = (int)hp.g_tile2.size();
     d.n_gtile3 = (int)hp.g_tile3.size();
+    d.big_rows1 = hp.cfg.big_rows1;
     if (hp.xg_size > 0 && hipMalloc(&d.XG, xbytes * (size_t)hp.xg_size) != hipSuccess)
       return "hipMalloc failed (large-block X buffer)";
     // X tiles that vanish identically are never computed: they stay zero
@@ -179,7 +180,7 @@ static int create_impl(const char* nus_prefix, const char* opfile_a, const char*
     *out = h.release();
     return NSX_OK;
   }
-  plan_models({&h->model}, h->hplan);
+  plan_models({&h->model}, h->hplan, precision);
   if (precision != NSX_FP64) {
     e = build_lowp(h->hplan, precision, h->lowp);
     if (!e.empty()) { set_error(e); return NSX_EGPU; }
@@ -314,7 +315,7 @@ int nsx_batch_create_config(nsx_hv* const* hs, int32_t n, int precision, const n
     if (!hs[i]) { set_error("null handle in batch"); return NSX_EARG; }
     b->models.push_back(&hs[i]->model);
   }
-  plan_models(b->models, b->hplan);
+  plan_models(b->models, b->hplan, precision);
   std::string e;
   if (precision != NSX_FP64 && !(e = build_lowp(b->hplan, precision, b->lowp)).empty()) {
     set_error(e);
@@ -502,8 +503,11 @@ PlanConfig plan_config(const nsx_tune_result& r) {
 
 // Plans every model on its own in parallel, then merges the plans in order
 // (same result as plan_add of each model into hp in turn).
-void plan_models(const std::vector<const Model*>& models, HostPlan& hp) {
+void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int precision) {
   const int n = (int)models.size();
+  // BF16 stage-1 tiles are bound by operand traffic rather than the matrix
+  // cores: 128-row tiles halve the re-reads of each Bhat^T panel
+  hp.cfg.big_rows1 = precision == NSX_BF16X3 ? 2 * kBigTile : kBigTile;
   // test hook: a smaller block side above which the large-block GEMM path runs,
   // so that small models exercise it
   if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
@@ -533,7 +537,7 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
   free_lowp(lp);
   hp = HostPlan();
   hp.cfg = cfg;
-  plan_models(models, hp);
+  plan_models(models, hp, precision);
   std::string e;
   if (precision != NSX_FP64 && !(e = build_lowp(hp, precision, lp)).empty()) return e;
   if (!(e = upload_plan(hp, d, precision)).empty()) return e;

@@ -40,7 +40,7 @@ struct nsx_batch {
 
 namespace nsx {
 void set_error(const std::string& e);
-void plan_models(const std::vector<const Model*>& models, HostPlan& hp);
+void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int precision);
 PlanConfig plan_config(const nsx_tune_result& r);
 // Rebuilds and uploads the plan of `models` with `cfg`; keeps kernel choices of `d`.
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,

@@ -704,12 +704,12 @@ __device__ __forceinline__ void big_mma(const C* sP, const C* sQ, int wj, int wi
   }
 }
 
-template <typename C>
-__device__ __forceinline__ void big_zero(typename BigMma<C>::vec (&acc)[2][2]) {
+template <typename C, int CB = 2>
+__device__ __forceinline__ void big_zero(typename BigMma<C>::vec (&acc)[2][CB]) {
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = typename BigMma<C>::vec{C(0), C(0), C(0), C(0)};
+    for (int c = 0; c < CB; ++c) acc[r][c] = typename BigMma<C>::vec{C(0), C(0), C(0), C(0)};
 }
 
 // G1: X = V_R Bhat^T for one 64 x 64 tile of one pair (V_R m x k, Bhat^T k x n)
@@ -931,8 +931,9 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 constexpr int kBfLd = kBigK + 4;
 constexpr int kBfPlane = kBigTile * kBfLd;  // one operand plane of one buffer (uint16)
 
+template <int TM = kBigTile>  // TM: rows of C (the Q operand) per tile
 struct alignas(16) BfTile {  // one buffer: hi / lo planes of the row operand (P, [j][k]) and column operand (Q, [i][k])
-  uint16_t ph[kBfPlane], pl[kBfPlane], qh[kBfPlane], ql[kBfPlane];
+  uint16_t ph[kBfPlane], pl[kBfPlane], qh[TM * kBfLd], ql[TM * kBfLd];
 };
 
 __device__ __forceinline__ void bf_put(uint16_t* hi, uint16_t* lo, int idx, Bf16x2 x) {
@@ -940,9 +941,11 @@ __device__ __forceinline__ void bf_put(uint16_t* hi, uint16_t* lo, int idx, Bf16
   lo[idx] = (uint16_t)(x.bits & 0xffffu);
 }
 
-__device__ __forceinline__ void big_mma_bf(const BfTile& b, int wj, int wi, int lane, f32x4 (&acc)[2][2]) {
+// a wavefront's 32 x (16 CB) block of D
+template <int TM, int CB = 2>
+__device__ __forceinline__ void big_mma_bf(const BfTile<TM>& b, int wj, int wi, int lane, f32x4 (&acc)[2][CB]) {
   const int kq = 4 * (lane / 16);
-  bf16x4 ah[2], al[2], bh[2], bl[2];
+  bf16x4 ah[2], al[2], bh[CB], bl[CB];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int o = (wj + r * 16 + lane % 16) * kBfLd + kq;
@@ -950,7 +953,7 @@ __device__ __forceinline__ void big_mma_bf(const BfTile& b, int wj, int wi, int 
     al[r] = *reinterpret_cast<const bf16x4*>(b.pl + o);
   }
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < CB; ++c) {
     const int o = (wi + c * 16 + lane % 16) * kBfLd + kq;
     bh[c] = *reinterpret_cast<const bf16x4*>(b.qh + o);
     bl[c] = *reinterpret_cast<const bf16x4*>(b.ql + o);
@@ -958,32 +961,34 @@ __device__ __forceinline__ void big_mma_bf(const BfTile& b, int wj, int wi, int 
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CB; ++c) {
       acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[r], bh[c], acc[r][c], 0, 0, 0);
       acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[r], bl[c], acc[r][c], 0, 0, 0);
       acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[r], bh[c], acc[r][c], 0, 0, 0);
     }
 }
 
-// G1 in BF16 hi/lo: X = V_R Bhat^T for one 64 x 64 tile, X in FP32
+// G1 in BF16 hi/lo: X = V_R Bhat^T for one TM x 64 tile (TM = 64 or 128), X in FP32
+template <int TM>
 __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tiles, const GTask1* __restrict__ tasks,
                                                  const Bf16x2* __restrict__ V, const Bf16x2* __restrict__ Bt,
                                                  float* __restrict__ X) {
-  __shared__ BfTile sb[2];  // double-buffered: one barrier per K step
+  constexpr int CB = TM / 32, QV = TM * kBigK / 256, VS = 256 / TM;
+  __shared__ BfTile<TM> sb[2];  // double-buffered: one barrier per K step
   const GTile1 t = tiles[blockIdx.x];
   const GTask1 g = tasks[t.task];
   const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
-  f32x4 acc[2][2];
-  big_zero<float>(acc);
-  // staging: V tile element (i = tid % 64, kk = tid / 64 + 4q) into Q[i][kk],
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * (TM / 2);
+  f32x4 acc[2][CB];
+  big_zero<float, CB>(acc);
+  // staging: V tile element (i = tid % TM, kk = tid / TM + VS q) into Q[i][kk],
   // Bhat^T tile element (kk = tid % 16, j = tid / 16 + 16q) into P[j][kk]
-  Bf16x2 rv[kBigQ], rb[kBigQ];
+  Bf16x2 rv[QV], rb[kBigQ];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < kBigQ; ++q) {
-      const int i = i0 + tid % 64, kk = k0 + tid / 64 + 4 * q;
+    for (int q = 0; q < QV; ++q) {
+      const int i = i0 + tid % TM, kk = k0 + tid / TM + VS * q;
       rv[q] = (i < m && kk < K) ? V[g.v_off + i + (int64_t)m * kk] : Bf16x2{0u};
     }
 #pragma unroll
@@ -992,9 +997,9 @@ __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tile
       rb[q] = (kk < K && j < n) ? Bt[g.b_off + kk + (int64_t)K * j] : Bf16x2{0u};
     }
   };
-  auto stage = [&](BfTile& b) {
+  auto stage = [&](BfTile<TM>& b) {
 #pragma unroll
-    for (int q = 0; q < kBigQ; ++q) bf_put(b.qh, b.ql, (tid % 64) * kBfLd + tid / 64 + 4 * q, rv[q]);
+    for (int q = 0; q < QV; ++q) bf_put(b.qh, b.ql, (tid % TM) * kBfLd + tid / TM + VS * q, rv[q]);
 #pragma unroll
     for (int q = 0; q < kBigQ; ++q) bf_put(b.ph, b.pl, (tid / kBigK + kBigJS * q) * kBfLd + tid % kBigK, rb[q]);
   };
@@ -1008,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tile
   while (step >= 0) {
     step = pop_step(m0, m1);
     if (step >= 0) load(step * kBigK);
-    big_mma_bf(sb[cur], wj, wi, lane, acc);
+    big_mma_bf<TM, CB>(sb[cur], wj, wi, lane, acc);
     if (step >= 0) stage(sb[cur ^ 1]);
     __syncthreads();
     cur ^= 1;
@@ -1016,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tile
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < CB; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + 4 * (lane / 16) + v;
@@ -1029,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tile
 __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tiles, const GTerm2* __restrict__ terms,
                                                  const SXRef* __restrict__ xrefs, const Bf16x2* __restrict__ Ad,
                                                  const float* __restrict__ X, double* __restrict__ W) {
-  __shared__ BfTile sb[2];
+  __shared__ BfTile<> sb[2];
   const GTile2 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1057,7 +1062,7 @@ __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tile
       }
     }
   };
-  auto stage = [&](BfTile& b) {
+  auto stage = [&](BfTile<>& b) {
 #pragma unroll
     for (int q = 0; q < kBigQ; ++q) bf_put(b.qh, b.ql, (tid % 64) * kBfLd + tid / 64 + 4 * q, ra[q]);
 #pragma unroll
@@ -1089,7 +1094,7 @@ __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tile
       step = pop_step(m0, m1);
     }
     if (step >= 0) load(tm, step * kBigK);
-    big_mma_bf(sb[cur], wj, wi, lane, acc);
+    big_mma_bf<kBigTile>(sb[cur], wj, wi, lane, acc);
     if (step >= 0) stage(sb[cur ^ 1]);
     __syncthreads();
     cur ^= 1;
@@ -1108,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tile
 // G0 in BF16 hi/lo: W_L = H_a V_L + V_L H_b^T for one 64 x 64 tile, stored
 __global__ __launch_bounds__(256) void k_big0_bf(const GTile3* __restrict__ tiles, const Bf16x2* __restrict__ V,
                                                  const Bf16x2* __restrict__ H, double* __restrict__ W) {
-  __shared__ BfTile sb[1];
+  __shared__ BfTile<> sb[1];
   const GTile3 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1141,7 +1146,7 @@ __global__ __launch_bounds__(256) void k_big0_bf(const GTile3* __restrict__ tile
       for (int q = 0; q < kBigQ; ++q)
         bf_put(sb[0].ph, sb[0].pl, (tid / kBigK + kBigJS * q) * kBfLd + tid % kBigK, ry[q]);
       __syncthreads();
-      big_mma_bf(sb[0], wj, wi, lane, acc);
+      big_mma_bf<kBigTile>(sb[0], wj, wi, lane, acc);
     }
   }
 #pragma unroll
@@ -1261,8 +1266,14 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     using C = typename BigIn<T>::C;
     C* XG = static_cast<C*>(d.XG);
     if constexpr (std::is_same<T, Bf16x2>::value) {  // BF16 pairs on the BF16 matrix cores
-      if (d.n_gtile1 > 0)
-        hipLaunchKernelGGL(k_big1_bf, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1, V, Bh, XG);
+      if (d.n_gtile1 > 0) {
+        if (d.big_rows1 == 2 * kBigTile)
+          hipLaunchKernelGGL(k_big1_bf<2 * kBigTile>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1,
+                             d.g_task1, V, Bh, XG);
+        else
+          hipLaunchKernelGGL(k_big1_bf<kBigTile>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1,
+                             d.g_task1, V, Bh, XG);
+      }
       hipLaunchKernelGGL(k_big2_bf, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2, d.g_xref, Ad,
                          (const C*)XG, W);
     } else {

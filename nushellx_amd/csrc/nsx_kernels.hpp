@@ -44,6 +44,7 @@ struct DevicePlan {
   GTerm2* g_term2 = nullptr;
   SXRef* g_xref = nullptr;
   int n_gtile1 = 0, n_gtile2 = 0, n_gtile3 = 0;
+  int big_rows1 = 64;        // rows of the stage-1 tiles (64, or 128 in BF16 plans)
   void* XG = nullptr;        // xg_size elements of the accumulation type (FP64, or FP32 in reduced precision)
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
 };

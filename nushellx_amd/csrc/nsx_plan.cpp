@@ -443,8 +443,11 @@ void build_big(HostPlan& p, const std::vector<const Model*>& models) {
       GTile1 t{};
       step_mask(Bt, g.k, 0, g.k, j0, std::min(kBigTile, g.n - j0), false, t.kmask);
       if (!(t.kmask[0] | t.kmask[1])) continue;  // X columns identically zero
-      for (int i0 = 0; i0 < g.m; i0 += kBigTile) {
-        if (!rows_needed[task][i0 / kBigTile]) continue;
+      for (int i0 = 0; i0 < g.m; i0 += p.cfg.big_rows1) {
+        bool need = false;  // rows_needed is kept per kBigTile rows
+        for (int q = i0 / kBigTile; q < (int)rows_needed[task].size() && q * kBigTile < i0 + p.cfg.big_rows1; ++q)
+          need = need || rows_needed[task][q];
+        if (!need) continue;
         t.task = (int)task;
         t.i0 = i0;
         t.j0 = j0;

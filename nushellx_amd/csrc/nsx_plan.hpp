@@ -167,6 +167,7 @@ struct PlanConfig {
   int stk_cap = 2048;        // stacked path: doubles one chunk may stage in LDS
   int stk_work = 512;        // stacked path: work units per chunk
   int big_min = kSmallMax;   // items with a side above this take the direct / large-block paths
+  int big_rows1 = kBigTile;  // rows of the stage-1 large-block tiles: 64, or 128 (BF16 plans)
 };
 
 struct HostPlan {

@@ -108,7 +108,7 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
   for (const GTile1& t : p.g_tile1) {
     const GTask1& g = p.g_task1[t.task];
     for (int j = t.j0; j < std::min(t.j0 + kBigTile, g.n); ++j)
-      for (int i = t.i0; i < std::min(t.i0 + kBigTile, g.m); ++i) {
+      for (int i = t.i0; i < std::min(t.i0 + p.cfg.big_rows1, g.m); ++i) {
         double s = 0.0;
         for (int k = 0; k < g.k; ++k)
           if (active(t.kmask, k)) s += vin[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
