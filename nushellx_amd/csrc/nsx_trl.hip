@@ -134,8 +134,13 @@ __global__ __launch_bounds__(kVT) void k_seg_sub(double* __restrict__ w, const d
   if (p == 0 && r < n) w[r] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
+// Norm accumulators: kNrmSlots per instance, a workgroup adding into slot
+// (block % kNrmSlots), so that the workgroups of one long instance do not all
+// contend for one address; k_seg_finish sums the slots.
+constexpr int kNrmSlots = 16;
+
 // Second Gram-Schmidt pass fused with the norm: w[r] -= sum_k Q[r, k] c[inst(r), k],
-// and per instance sum w^2 accumulated into nrm[inst] (a wavefront reduction per
+// and per instance sum w^2 accumulated into the instance's nrm slots (a wavefront reduction per
 // instance present in the block's rows, one FP64 atomic add each).
 __global__ __launch_bounds__(kVT) void k_seg_sub_norm(double* __restrict__ w, const double* __restrict__ Q,
                                                       int64_t ldq, int ncols, const double* __restrict__ c, int ldc,
@@ -161,11 +166,11 @@ __global__ __launch_bounds__(kVT) void k_seg_sub_norm(double* __restrict__ w, co
   for (int i = i_lo; i <= i_hi; ++i) {  // instances are contiguous: usually one per block
     double v = (my == i) ? sq : 0.0;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) unsafeAtomicAdd(nrm + i, v);
+    if (lane == 0) unsafeAtomicAdd(nrm + (int64_t)i * kNrmSlots + blockIdx.x % kNrmSlots, v);
   }
 }
 
-// Finishes step j: beta = sqrt(nrm[inst]), alpha = c1[j] + c2[j]; the row that
+// Finishes step j: beta = sqrt(sum of the instance's nrm slots), alpha = c1[j] + c2[j]; the row that
 // starts an instance records alpha/beta, marks a breakdown (dim[i] = j + 1: an
 // invariant subspace was reached) and clears the other norm slot for step
 // j + 1; every row writes q_{j+1} = w / beta (0 once the instance stopped).
@@ -182,7 +187,10 @@ __global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, cons
   if (r >= n) return;
   const int i = inst[r];
   const double a = c1[(int64_t)i * ldc + j] + c2[(int64_t)i * ldc + j];
-  const double b = sqrt(nrm[i]);
+  double ss = 0.0;
+#pragma unroll
+  for (int q = 0; q < kNrmSlots; ++q) ss += nrm[(int64_t)i * kNrmSlots + q];
+  const double b = sqrt(ss);
   const bool live = dim[i] < 0;
   const bool breakdown = !(b > 1e-10 * fmax(1.0, fabs(a)));
   const bool ok = live && !breakdown;
@@ -190,7 +198,8 @@ __global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, cons
     alpha_h[(int64_t)j * B + i] = a;
     beta_h[(int64_t)j * B + i] = ok ? b : 0.0;
     if (live && breakdown) dim[i] = j + 1;
-    nrm_next[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < kNrmSlots; ++q) nrm_next[(int64_t)i * kNrmSlots + q] = 0.0;
   }
   q[r] = ok ? w[r] / b : 0.0;
 }
@@ -315,7 +324,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
               dm(ws.Y, sizeof(double) * N * std::max(keep, nev)) && dm(ws.inst_of, sizeof(int) * N) &&
               dm(ws.seg, sizeof(int64_t) * (B + 1)) && dm(ws.c1, sizeof(double) * B * ldc) &&
               dm(ws.c2, sizeof(double) * B * ldc) && dm(ws.ah, sizeof(double) * (m + 1) * B) &&
-              dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * 2 * B) &&
+              dm(ws.bh, sizeof(double) * (m + 1) * B) && dm(ws.scale, sizeof(double) * 2 * B * kNrmSlots) &&
               dm(ws.dim, sizeof(int) * B) && dm(ws.S, sizeof(double) * (size_t)B * m * std::max(keep, nev)) &&
               dm(ws.wmask, (size_t)N) && dm(ws.cseg, sizeof(int64_t) * (max_chunks + 1)) &&
               dm(ws.cfirst, sizeof(int) * (B + 1)) && dm(ws.cpart, sizeof(double) * max_chunks * ldc) &&
@@ -435,7 +444,8 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   // Lanczos steps [j0, j1) on the stream: H*v, two Gram-Schmidt passes, normalise
   auto launch_steps = [&](int j0, int j1) -> hipError_t {
     // norm accumulator of the block's first step (later steps are cleared by k_seg_finish)
-    hipLaunchKernelGGL(k_fill0, dim3((B + kVT - 1) / kVT), dim3(kVT), 0, s, scale.as<double>() + (size_t)(j0 & 1) * B, B);
+    hipLaunchKernelGGL(k_fill0, dim3((B * kNrmSlots + kVT - 1) / kVT), dim3(kVT), 0, s,
+                       scale.as<double>() + (size_t)(j0 & 1) * B * kNrmSlots, B * kNrmSlots);
     for (int j = j0; j < j1; ++j) {
       hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s);
       if (e != hipSuccess) return e;
@@ -444,8 +454,8 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       hipLaunchKernelGGL(k_seg_sub, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
                          c1.as<double>(), ldc, inst_of.as<int>(), N);
       dots(j + 1, c2.as<double>());
-      double* nrm_j = scale.as<double>() + (size_t)(j & 1) * B;
-      double* nrm_n = scale.as<double>() + (size_t)((j + 1) & 1) * B;
+      double* nrm_j = scale.as<double>() + (size_t)(j & 1) * B * kNrmSlots;
+      double* nrm_n = scale.as<double>() + (size_t)((j + 1) & 1) * B * kNrmSlots;
       hipLaunchKernelGGL(k_seg_sub_norm, gsub, dim3(kVT), 0, s, W.as<double>(), Q.as<double>(), ldq, j + 1,
                          c2.as<double>(), ldc, inst_of.as<int>(), N, nrm_j);
       hipLaunchKernelGGL(k_seg_finish, gv, dim3(kVT), 0, s, Q.as<double>() + ldq * (j + 1), W.as<double>(),
