@@ -30,6 +30,16 @@ def cases():
     out.append(G + G.T)
     Q, _ = np.linalg.qr(rng.normal(size=(20, 20)))
     out.append(Q @ np.diag(np.repeat([1.0, 2.0, 3.0, 4.0], 5)) @ Q.T)  # 5-fold degenerate
+    # Wilkinson's W21+: tridiagonal, diagonal |10 - i|, unit couplings; its top
+    # eigenvalues come in pairs that agree to ~1e-14
+    W = np.diag(np.abs(np.arange(21) - 10.0)) + np.diag(np.ones(20), 1) + np.diag(np.ones(20), -1)
+    out.append(W)
+    # graded tridiagonal (diagonal spans 12 decades) and one with exact zero couplings (split)
+    g = 10.0 ** -np.arange(13)
+    out.append(np.diag(g) + np.diag(g[1:] * 0.3, 1) + np.diag(g[1:] * 0.3, -1))
+    S = arrow_tridiag(rng, 0, 40)
+    S[13, 14] = S[14, 13] = S[27, 28] = S[28, 27] = 0.0
+    out.append(S)
     return out
 
 
