@@ -579,8 +579,10 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   auto record = [&](int i, double beta_last) {
     int conv = 0;
     for (int q = 0; q < nev; ++q) {
-      theta_all[size_t(i) * nev + q] = Th[i][q];
-      const double r = std::fabs(beta_last * LC[i][q]);
+      // a Krylov space that closed after mm_of[i] < nev steps holds only mm_of[i] pairs
+      const bool have = q < mm_of[i];
+      theta_all[size_t(i) * nev + q] = have ? Th[i][q] : NAN;
+      const double r = have ? std::fabs(beta_last * LC[i][q]) : NAN;
       res_all[size_t(i) * nev + q] = r;
       conv += r <= tol;
     }
@@ -646,13 +648,14 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
         record(i, beta_last);
         if (broke) {  // invariant subspace: eigenpairs exact, Ritz vectors from Sv
           done[i] = 1;
-          if (converged_at[i] < 0) converged_at[i] = matvecs;
+          // fewer pairs than wanted: the missing ones stay NaN and unconverged
+          if (converged_at[i] < 0 && last >= nev) converged_at[i] = matvecs;
         }
       }, 2);
       host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
       if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
-      all_conv = true;
-      for (int i = 0; i < B; ++i) all_conv = all_conv && converged_at[i] >= 0;
+      all_conv = true;  // every instance converged, or exhausted (done) with nothing left to find
+      for (int i = 0; i < B; ++i) all_conv = all_conv && (converged_at[i] >= 0 || done[i]);
       if (all_conv || matvecs >= max_iter || !spec) break;
       j0 = k0;
       j1 = k1;
@@ -725,7 +728,7 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
       }
       const int mm = mm_of[i];
       if (!done[i] && !solve(i, mm, true)) { failed = 1; return; }  // the last test formed only the last row
-      for (int c = 0; c < nev; ++c)
+      for (int c = 0; c < std::min(nev, mm); ++c)
         for (int j = 0; j < mm; ++j) Si[(size_t)c * mmax + j] = Sv[i][size_t(c) * mm + j];
     }, 1);
     if (failed) { set_error("projected eigenproblem did not converge"); return NSX_EGPU; }
@@ -764,6 +767,11 @@ extern "C" int nsx_trl_solve(nsx_hv* h, const nsx_trl_opts* opt, const double* s
   int rc = trl_run(h->dplan, h->lowp, h->precision, h->stream, h->trl, inst, o, start, evals, evecs, residuals,
                    &conv_at, stats);
   if (rc == NSX_OK && stats) stats->converged = conv_at >= 0 ? opt->nev : 0;
+  if (rc == NSX_OK && opt->nev > 0 && std::isnan(evals[opt->nev - 1])) {
+    set_error("Krylov space closed with fewer than nev eigenpairs (the start vector lies in a small invariant "
+              "subspace, e.g. an eigenvector, or H has few distinct eigenvalues); the missing ones are NaN");
+    return NSX_ECAP;
+  }
   return rc;
 }
 

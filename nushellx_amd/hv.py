@@ -217,14 +217,6 @@ class Batch:
             check(lib().nsx_batch_offset(b, i, C.byref(o)), "nsx_batch_offset")
             self.offsets.append(int(o.value))
 
-    def write_vectors(self, prefix: str, evals, vecs, no_level: int = 5, tt: float = 0.0) -> None:
-        """<prefix>.lnz / <prefix>.abz in the layout NuVec reads (vecs: nvec x N)."""
-        vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
-        evals = np.ascontiguousarray(evals, dtype=np.float64)
-        assert vecs.shape == (len(evals), self.N)
-        check(lib().nsx_hv_write_vectors(self._h, prefix.encode(), len(evals), evals.ctypes.data,
-                                         vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_vectors")
-
     def work(self) -> Tuple[float, float, int]:
         f, fl, by = C.c_double(), C.c_double(), C.c_int64()
         check(lib().nsx_batch_work(self._b, C.byref(f), C.byref(fl), C.byref(by)), "nsx_batch_work")

@@ -10,9 +10,29 @@ and form each state as sum_ix eigv(ix, i) * record ix of <prefix>.lnz
 """
 from __future__ import annotations
 
+import struct
+
 import numpy as np
 
-from nushellx_amd.synth.fortio import read_records
+
+def read_records(path: str):
+    """The payloads of a Fortran unformatted sequential file: each record is a
+    4-byte little-endian byte count, the bytes, and the count again.  Written
+    here, apart from the product's readers, so that a reader bug cannot hide
+    in both."""
+    with open(path, "rb") as f:
+        data = f.read()
+    out, pos = [], 0
+    while pos < len(data):
+        (n,) = struct.unpack_from("<i", data, pos)
+        if n < 0 or pos + 8 + n > len(data):
+            raise ValueError(f"{path}: bad record length {n} at byte {pos}")
+        (n2,) = struct.unpack_from("<i", data, pos + 4 + n)
+        if n2 != n:
+            raise ValueError(f"{path}: record markers differ at byte {pos}")
+        out.append(data[pos + 4:pos + 4 + n])
+        pos += 8 + n
+    return out
 
 
 def read_abz_lnz(prefix: str):

@@ -9,8 +9,9 @@ pytestmark = pytest.mark.gpu
 
 # relative H*v error bounds (operand rounding: FP32 2^-24, BF16 hi/lo pair ~2^-16)
 HV_TOL = {"fp32": 2e-6, "bf16x3": 2e-4}
-# eigenvalue error bounds (MeV) of a converged Lanczos with the reduced-precision H*v
-EV_TOL = {"fp32": 1e-5, "bf16x3": 5e-3}
+# eigenvalue error bounds (MeV) of a converged Lanczos with the reduced-precision H*v:
+# the 1e-4 MeV target of the mixed-precision solver for BF16 hi/lo
+EV_TOL = {"fp32": 1e-5, "bf16x3": 1e-4}
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
