@@ -188,6 +188,14 @@ int nsx_hv_expect(nsx_hv *h, int32_t nvec, const double *vecs, double *M);
  * one REAL*4 record per column j).  nlev_out (may be NULL) receives the number of
  * states used: those present, at most the count in the file's header. */
 int nsx_hv_xpct_files(nsx_hv *h, const char *prefix, const char *operat, int32_t *nlev_out);
+/* nsx_hv_expect without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  The
+ * states, their images H v_i and M live in a static 8 MiB device scratch array
+ * of the library (one per GPU, held until the process ends), so
+ * 2*N*nvec + nvec*nvec must not exceed 1048576 doubles (NSX_ECAP otherwise).
+ * Calls that overlap on one GPU share that array and race: serialise them. */
+int nsx_hv_expect_noalloc(nsx_hv *h, int32_t nvec, const double *vecs, double *M);
 
 /* ---------------------------------------------------------------- device helpers
  * For hosts without their own HIP binding (Fortran, ctypes): device buffers,
