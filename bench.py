@@ -296,6 +296,37 @@ def pf_warm(log, train_samples: int, nprot: int = 4, nneut: int = 4, chunk: int 
     return out
 
 
+def deterministic(log, hams, ms_default):
+    """The headline batch planned for a bitwise reproducible H*v (the fused
+    column path: X and Y in LDS, column partials summed in plan order): its
+    H*v time beside the default path's, and three H*v compared bit for bit."""
+    from nushellx_amd.hv import Batch, DeviceBuffer, device_sync
+    os.environ["NSX_DETERMINISTIC"] = "1"
+    try:
+        b = Batch(hams)
+    finally:
+        os.environ.pop("NSX_DETERMINISTIC", None)
+    v = np.random.default_rng(3).standard_normal(b.N)
+    din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+    ms_hv, ms_pn, ms_red = b.time(din, dout, warmup=10, iters=200)
+    outs = []
+    for _ in range(3):
+        b.multiply_dev(din, dout)
+        device_sync()
+        w = np.empty_like(v)
+        dout.download(w)
+        outs.append(w)
+    out = {"path": b.path(), "hv_ms": ms_hv, "pn_ms": ms_pn, "reduce_ms": ms_red,
+           "cost_vs_default": ms_hv / ms_default,
+           "bitwise_identical": bool(all(np.array_equal(o, outs[0]) for o in outs[1:]))}
+    log(f"deterministic: {out['path']} H*v {ms_hv * 1e3:.1f} us ({out['cost_vs_default']:.2f}x the default), "
+        f"bitwise identical {out['bitwise_identical']}")
+    din.free()
+    dout.free()
+    b.close()
+    return out
+
+
 def separable(log, mf, cuts=((0.0, 0.0), (0.3, 0.05), (1.0, 0.1))):
     """NuTRLZ's separable pn operator on the same workload: the .dop files NuDiop
     would write at several cuts (eigenvalue / sqrt(2lam+1) in MeV, amplitude), the
@@ -394,6 +425,30 @@ def large(log, steps: int = 5, cpu: bool = True):
         f"({out['bf16x3']['speedup_vs_fp64']:.2f}x FP64), eigenvalues within "
         f"{out['bf16x3']['max_eval_diff_vs_fp64_MeV']:.1e} MeV of the FP64 solve")
     hb.close()
+    # deterministic FP64 plan: one stage-2 workgroup per large W tile
+    os.environ["NSX_DETERMINISTIC"] = "1"
+    try:
+        hd = Hamiltonian(*mf.args(0))
+    finally:
+        os.environ.pop("NSX_DETERMINISTIC", None)
+    din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+    hd.multiply_dev(din, dout)
+    device_sync()
+    w1 = np.empty_like(v)
+    dout.download(w1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hd.multiply_dev(din, dout)
+    device_sync()
+    ms_d = 1e3 * (time.perf_counter() - t0) / steps
+    w2 = np.empty_like(v)
+    dout.download(w2)
+    out["deterministic"] = {"hv_ms": ms_d, "cost_vs_default": ms_d / ms, "bitwise_identical": bool(np.array_equal(w1, w2))}
+    log(f"large: deterministic FP64 H*v {ms_d:.2f} ms ({ms_d / ms:.2f}x), bitwise identical "
+        f"{out['deterministic']['bitwise_identical']}")
+    din.free()
+    dout.free()
+    hd.close()
     if cpu:
         from oracle.refcpu import RefCPU
         r = RefCPU(*mf.args(0), fp64=False, thresholds=True)
@@ -429,8 +484,10 @@ def main():
                     help="skip the pf 6p6n J=0 H*v (N = 1.78 M; about a minute, mostly writing its files)")
     ap.add_argument("--pf66-warm-samples", type=int, default=0,
                     help="training samples of the pf 6p6n J=0 warm-start study (0 skips it; 16 take ~6 min)")
-    ap.add_argument("--pf-warm-samples", type=int, default=0,
-                    help="training samples of the pf 4p4n warm-start study (0 skips it; 512 takes ~8 min)")
+    ap.add_argument("--pf-warm-samples", type=int, default=64,
+                    help="training samples of the pf 4p4n warm-start study (0 skips it; 64 take ~1 min, "
+                         "512 ~8 min)")
+    ap.add_argument("--no-det", action="store_true", help="skip the deterministic-mode timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -467,18 +524,26 @@ def main():
     value = len(hams) * args.steps / el
     log(f"timed {args.steps} batched H*v: {value:.0f} matvecs/s")
 
-    # HIP events on the batch stream around every launch of the timed loop
+    # HIP events on the batch stream around loops of back-to-back launches: the
+    # whole H*v, and its pn kernels alone (no per-launch event overhead)
     ms_hv, ms_pn, ms_red = batch.time(din, dout, warmup=10, iters=min(args.steps, 1000))
-    stacked = bool(tuned.get("stacked"))
-    # the stacked stage-1 launch also carries the like-particle blocks
+    path = batch.path()
+    stacked = path.startswith("stacked")
+    # the stacked stage-1 launch also carries the like-particle blocks, so its
+    # pn launches are the whole H*v; the other paths do that part in the reduce
     flops_pn = flops if stacked else flops - flops_like
     achieved = flops_pn / (ms_pn * 1e-3) / 1e12
+    traffic = traffic_from_profile(stacked)
+    kernel = {"stacked": "k_stk1 + k_stk2 (stacked two-stage pn path with the like-particle blocks, FP64)",
+              "staged": "k_pn_staged (pn two-body part, FP64)",
+              "fused": "k_fuse (fused column path, pn two-body part, FP64)"}[path.split()[0]]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(stacked),
-                "kernel": ("k_stk1 + k_stk2 (stacked two-stage pn path with the like-particle blocks, FP64)"
-                           if stacked else "k_pn_staged (pn two-body part, FP64)"), "kernel_ms": ms_pn,
-                "flops_per_launch": flops_pn,
-                "hbm_GBs_alg": (bytes_alg / (ms_hv * 1e-3)) / 1e9}
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                "engine": "FP64 vector FMA (VALU), whose peak equals the FP64 matrix-core peak on gfx950; "
+                          "the kernels are bound by load latency and instruction issue, not by either peak",
+                "kernel": kernel, "kernel_ms": ms_pn, "flops_per_launch": flops_pn,
+                "hbm_GBs_alg": (bytes_alg / (ms_hv * 1e-3)) / 1e9,
+                "hbm_GBs_counters": (traffic / (ms_pn * 1e-3)) / 1e9 if traffic else None}
 
     out = {
         "metric": "time-to-5-eigenvalues @1e-4 MeV & matvecs/s: mixed+warm vs plain FP64",
@@ -499,6 +564,7 @@ def main():
         "roofline": roofline,
         "kernel_ms": {"hv": ms_hv, "pn": ms_pn, "reduce": ms_red},
         "tuned": tuned,
+        "path": path,
     }
     try:
         out["device"] = list(device_info())
@@ -511,6 +577,8 @@ def main():
     din.free()
     dout.free()
     batch.close()
+    if not args.no_det:
+        out["deterministic"] = deterministic(log, hams, ms_hv)
     if not args.no_t5:
         out["time_to_5"] = time_to_5(log, train_samples=args.train_samples)
     if not args.no_sep:

@@ -117,13 +117,26 @@ typedef struct {
                           (summation order varies run to run at the 1e-16 relative level) */
   double ms;           /* measured time of one H*v with this shape */
   int32_t stacked;     /* 1: two-stage stacked path (X = V Bhat^T shared across left
-                          blocks, then W += A sum r X; atomic accumulation) */
-  int32_t stk_cap;     /* stacked path: doubles a chunk stages in LDS (0 = default) */
-  int32_t stk_work;    /* stacked path: work units per chunk (0 = default) */
+                          blocks, then W += A sum r X; atomic accumulation);
+                          2: fused column path (X and Y in LDS, column partials
+                          summed in a fixed order: deterministic) */
+  int32_t stk_cap;     /* stacked path: doubles a chunk stages in LDS (0 = default);
+                          fused path: LDS elements of one round */
+  int32_t stk_work;    /* stacked path: work units per chunk (0 = default);
+                          fused path: flops one workgroup takes, about */
+  int32_t f_rounds;    /* fused path: rounds one workgroup takes at most (0 = default) */
+  int32_t deterministic; /* 1: bitwise reproducible H*v: the fused path for the small
+                            blocks and one stage-2 workgroup per large-block W tile
+                            (also set for any plan by NSX_DETERMINISTIC=1) */
 } nsx_tune_result;
 int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);
 int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);
 int nsx_batch_set_config(nsx_batch *b, const nsx_tune_result *cfg);
+/* The small-block pn path the batch's plan runs: 0 staged kernel, 1 stacked
+ * two-stage path, 2 fused column path (a requested path falls back to the
+ * staged kernel where its limits are exceeded), plus 4 when the plan is
+ * deterministic.  No counterpart in the reference. */
+int nsx_batch_path(const nsx_batch *b, int32_t *path);
 /* nsx_batch_create with a plan shape and kernel choice known in advance (for
  * instance tuned on a small representative batch): plans the batch once. */
 int nsx_batch_create_config(nsx_hv *const *hs, int32_t n, int precision, const nsx_tune_result *cfg,
@@ -206,8 +219,10 @@ int nsx_dev_free(void *ptr);
 int nsx_dev_upload(void *dst, const void *src, int64_t bytes);
 int nsx_dev_download(void *dst, const void *src, int64_t bytes);
 int nsx_dev_sync(void);
-/* warmup + iters batched H*v on the batch stream; HIP events bracket every
- * launch.  Averages in ms: whole H*v, pn kernel, reduce kernel. */
+/* warmup launches, then three loops of iters back-to-back launches on the
+ * batch stream, each bracketed by one pair of HIP events: the whole H*v, its
+ * pn kernels alone, its reduce / like-particle kernels alone.  Average ms per
+ * launch of each. */
 int nsx_batch_time(nsx_batch *b, const double *d_in, double *d_out, int32_t warmup, int32_t iters,
                    double *ms_hv, double *ms_pn, double *ms_reduce);
 /* iters launches of a plain 16-byte-per-lane copy kernel (bytes % 16 == 0) on

@@ -22,7 +22,7 @@ EXPORTS = [
     "nsx_batch_create", "nsx_batch_destroy", "nsx_batch_dims", "nsx_batch_offset",
     "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
-    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_trl_solve", "nsx_dev_copy", "nsx_batch_create_config", "nsx_hv_write_vectors",
+    "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_path", "nsx_batch_trl_solve", "nsx_dev_copy", "nsx_batch_create_config", "nsx_hv_write_vectors",
     "nsx_hv_create_sep", "nsx_hv_write_xvc", "nsx_hv_expect", "nsx_hv_expect_noalloc", "nsx_hv_xpct_files", "nsx_hv_trunc",
     "nsx_mlp_create", "nsx_mlp_destroy", "nsx_mlp_num_params", "nsx_mlp_get_params",
     "nsx_mlp_set_params", "nsx_mlp_grad", "nsx_mlp_train", "nsx_mlp_predict",
@@ -40,7 +40,8 @@ class TrlOpts(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("fat_cap", C.c_int32), ("max_phases", C.c_int32), ("threads", C.c_int32),
                 ("mode", C.c_int32), ("atomic", C.c_int32), ("ms", C.c_double),
-                ("stacked", C.c_int32), ("stk_cap", C.c_int32), ("stk_work", C.c_int32)]
+                ("stacked", C.c_int32), ("stk_cap", C.c_int32), ("stk_work", C.c_int32),
+                ("f_rounds", C.c_int32), ("deterministic", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -92,6 +93,7 @@ def lib() -> C.CDLL:
     L.nsx_hv_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_autotune.argtypes = [vp, i32, C.POINTER(TuneResult)]
     L.nsx_batch_set_config.argtypes = [vp, C.POINTER(TuneResult)]
+    L.nsx_batch_path.argtypes = [vp, C.POINTER(i32)]
     L.nsx_batch_create_config.argtypes = [C.POINTER(vp), i32, C.c_int, C.POINTER(TuneResult), C.POINTER(vp)]
     L.nsx_hv_write_vectors.argtypes = [vp, C.c_char_p, i32, vp, vp, i32, C.c_float]
     L.nsx_hv_trunc.argtypes = [vp, vp]

@@ -59,7 +59,30 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
   if (const char* v = std::getenv("NSX_STAGED_NT")) d.staged_nt = std::atoi(v);
   if (const char* v = std::getenv("NSX_STAGED_MODE")) d.staged_mode = std::atoi(v);
   if (const char* v = std::getenv("NSX_ATOMIC")) d.atomic = std::atoi(v);
-  d.stacked = hp.cfg.stacked;
+  d.stacked = hp.cfg.stacked == 1;
+  if (hp.cfg.stacked == 2 && !hp.f_task.empty()) {  // fused column path
+    if (!(e = up(d.f_task, hp.f_task)).empty()) return e;
+    if (!(e = up(d.f_blk, hp.f_blk)).empty()) return e;
+    if (!(e = up(d.f_round, hp.f_round)).empty()) return e;
+    if (!(e = up(d.f_seg, hp.f_seg)).empty()) return e;
+    if (!(e = up(d.f_x, hp.f_x)).empty()) return e;
+    if (!(e = up(d.f_cons, hp.f_cons)).empty()) return e;
+    if (!(e = up(d.f_ref, hp.f_ref)).empty()) return e;
+    if (!(e = up(d.f_blocks, hp.f_blocks)).empty()) return e;
+    if (!(e = up(d.f_part_list, hp.f_part_list)).empty()) return e;
+    if (hp.f_part_size > 0 && hipMalloc(&d.f_part, sizeof(double) * (size_t)hp.f_part_size) != hipSuccess)
+      return "hipMalloc failed (column partials)";
+    d.n_ftask = (int)hp.f_task.size();
+    d.n_fblocks = (int)hp.f_blocks.size();
+    d.f_lds = hp.f_lds;
+    d.f_u = hp.f_units <= 2 * kFuseThreads ? 2 : hp.f_units <= 4 * kFuseThreads ? 4 : 8;
+    const size_t lds = (precision == NSX_FP64 ? sizeof(double) : sizeof(float)) * (size_t)hp.f_lds;
+    if (lds > 48 * 1024)
+      if (hipFuncSetAttribute(k_fuse_ptr(precision, d.f_u), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds) != hipSuccess)
+        return "LDS request too large (fused kernel)";
+    d.fused = 1;
+  }
   if (d.stacked) {
     if (!(e = up(d.s_chunk1, hp.s_chunk1)).empty()) return e;
     if (!(e = up(d.s_unit1, hp.s_unit1)).empty()) return e;
@@ -132,6 +155,9 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
   (void)hipFree(d.g_tile3);
   (void)hipFree(d.g_xref); (void)hipFree(d.XG);
+  (void)hipFree(d.f_task); (void)hipFree(d.f_blk); (void)hipFree(d.f_round); (void)hipFree(d.f_seg);
+  (void)hipFree(d.f_x); (void)hipFree(d.f_cons); (void)hipFree(d.f_ref); (void)hipFree(d.f_blocks);
+  (void)hipFree(d.f_part_list); (void)hipFree(d.f_part);
   d = DevicePlan();
 }
 
@@ -398,6 +424,14 @@ int nsx_dev_sync(void) {
   return NSX_OK;
 }
 
+int nsx_batch_path(const nsx_batch* b, int32_t* path) {
+  if (!b || !path) { set_error("null argument"); return NSX_EARG; }
+  *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : 0;
+  // deterministic: the fused path for the small blocks and unsplit large tiles
+  if (b->dplan.fused && b->hplan.cfg.deterministic) *path |= 4;
+  return NSX_OK;
+}
+
 int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warmup, int32_t iters,
                    double* ms_hv, double* ms_pn, double* ms_reduce) {
   if (!b || iters <= 0) { set_error("bad timing arguments"); return NSX_EARG; }
@@ -406,40 +440,33 @@ int nsx_batch_time(nsx_batch* b, const double* d_in, double* d_out, int32_t warm
     int rc = nsx_batch_apply_dev(b, d_in, d_out, s);
     if (rc) return rc;
   }
-  std::vector<hipEvent_t> ev(3 * (size_t)iters);
-  for (auto& e : ev) NSX_HIP(hipEventCreate(&e));
-  for (int i = 0; i < iters; ++i) {
-    // events: [pn kernels] [reduce / like-particle kernel], in launch order
-    const bool at = b->dplan.atomic != 0;
-    auto pn = [&]() { return launch_pn(b->dplan, b->lowp, b->precision, d_in, d_out, s); };
-    auto red = [&]() { return launch_reduce(b->dplan, b->lowp, b->precision, d_in, d_out, s); };
-    {
-      hipError_t ce = launch_convert(b->dplan, b->lowp, b->precision, d_in, s);
-      if (ce != hipSuccess) { set_error(hipGetErrorString(ce)); return NSX_EGPU; }
+  // one event pair around `iters` back-to-back launches of each part: the
+  // average launch time without per-launch event overhead
+  hipEvent_t e0, e1;
+  NSX_HIP(hipEventCreate(&e0));
+  NSX_HIP(hipEventCreate(&e1));
+  auto loop = [&](int part, double* out) -> int {
+    NSX_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; ++i) {
+      hipError_t e = part == 0   ? launch_hv(b->dplan, b->lowp, b->precision, d_in, d_out, s)
+                     : part == 1 ? launch_pn(b->dplan, b->lowp, b->precision, d_in, d_out, s)
+                                 : launch_reduce(b->dplan, b->lowp, b->precision, d_in, d_out, s);
+      if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
     }
-    NSX_HIP(hipEventRecord(ev[3 * i], s));
-    hipError_t e = at ? red() : pn();
-    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
-    NSX_HIP(hipEventRecord(ev[3 * i + 1], s));
-    e = at ? pn() : red();
-    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return NSX_EGPU; }
-    NSX_HIP(hipEventRecord(ev[3 * i + 2], s));
-  }
-  NSX_HIP(hipStreamSynchronize(s));
-  double pn = 0.0, red = 0.0;
-  for (int i = 0; i < iters; ++i) {
-    float a = 0.f, c = 0.f;
-    NSX_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
-    NSX_HIP(hipEventElapsedTime(&c, ev[3 * i + 1], ev[3 * i + 2]));
-    if (b->dplan.atomic) { red += a; pn += c; } else { pn += a; red += c; }
-  }
-  float tot = 0.f;
-  NSX_HIP(hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)iters - 1]));
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (ms_hv) *ms_hv = tot / iters;
-  if (ms_pn) *ms_pn = pn / iters;
-  if (ms_reduce) *ms_reduce = red / iters;
-  return NSX_OK;
+    NSX_HIP(hipEventRecord(e1, s));
+    NSX_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    NSX_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (out) *out = t / iters;
+    return NSX_OK;
+  };
+  int rc = loop(0, ms_hv);
+  // the pn and reduce parts read the converted input the last H*v left
+  if (!rc) rc = loop(1, ms_pn);
+  if (!rc) rc = loop(2, ms_reduce);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 }  // extern "C"
@@ -495,9 +522,11 @@ PlanConfig plan_config(const nsx_tune_result& r) {
   PlanConfig pc;
   pc.fat_cap = std::min(std::max(r.fat_cap, 256), kFatCap);
   pc.max_phases = std::max(r.max_phases, 1);
-  pc.stacked = r.stacked ? 1 : 0;
+  pc.stacked = r.stacked == 2 ? 2 : r.stacked ? 1 : 0;
   if (r.stk_cap > 0) pc.stk_cap = r.stk_cap;
   if (r.stk_work > 0) pc.stk_work = r.stk_work;
+  if (r.f_rounds > 0) pc.f_rounds = r.f_rounds;
+  pc.deterministic = r.deterministic ? 1 : 0;
   return pc;
 }
 
@@ -511,10 +540,25 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int prec
   // test hook: a smaller block side above which the large-block GEMM path runs,
   // so that small models exercise it
   if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
+  if (const char* v = std::getenv("NSX_DETERMINISTIC")) hp.cfg.deterministic = std::atoi(v) != 0;
+  if (hp.cfg.deterministic) {
+    if (hp.cfg.stacked != 2) {  // the fused shape measured fastest on sd 4p4n J=0..8
+      hp.cfg.stk_cap = 6144;
+      hp.cfg.stk_work = 250000;
+      hp.cfg.f_rounds = 1;
+    }
+    hp.cfg.stacked = 2;
+    hp.cfg.big_kchunk = 1 << 30;
+  }
+  auto finish = [&]() {
+    if (hp.cfg.stacked == 1) build_stacked(hp, models);
+    build_big(hp, models);
+    // fused column path, or the staged kernel where a column exceeds its limits
+    if (hp.cfg.stacked == 2 && !build_fused(hp, models, kFuseThreads)) hp.cfg.stacked = 0;
+  };
   if (n == 1) {
     plan_add(hp, *models[0]);
-    if (hp.cfg.stacked) build_stacked(hp, models);
-    build_big(hp, models);
+    finish();
     return;
   }
   std::vector<HostPlan> local(n);
@@ -526,8 +570,7 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int prec
     plan_merge(hp, local[i]);
     local[i] = HostPlan();  // release as we go
   }
-  if (hp.cfg.stacked) build_stacked(hp, models);
-  build_big(hp, models);
+  finish();
 }
 
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
@@ -543,7 +586,7 @@ std::string replan(const std::vector<const Model*>& models, const PlanConfig& cf
   if (!(e = upload_plan(hp, d, precision)).empty()) return e;
   d.staged_nt = nt;
   d.staged_mode = mode;
-  d.atomic = cfg.stacked ? 1 : at;
+  d.atomic = hp.cfg.stacked ? 1 : at;
   d.stk_nt = snt;
   return "";
 }
@@ -582,7 +625,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
   std::vector<double> h(N);
   for (int64_t i = 0; i < N; ++i) h[i] = std::sin(0.37 * double(i + 1));
   (void)hipMemcpy(in, h.data(), sizeof(double) * N, hipMemcpyHostToDevice);
-  nsx_tune_result best{0, 0, 0, 0, 0, 1e300, 0, 0, 0};
+  nsx_tune_result best{0, 0, 0, 0, 0, 1e300, 0, 0, 0, 0, 0};
   const int caps[] = {768, 1536, 2560, kFatCap};
   const int chunks[] = {1, 2, 4, 8};
   const int nts[] = {64, 128, 256};
@@ -600,7 +643,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
             d.staged_mode = mode;
             d.atomic = at;
             const double ms = time_hv(d, lp, precision, in, out, s, iters);
-            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms, 0, 0, 0};
+            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms, 0, 0, 0, 0, 0};
           }
     }
   // stacked two-stage path: chunk capacity (LDS doubles) x work units x threads
@@ -616,8 +659,22 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
         d.stk_nt = nt;
         d.atomic = 1;
         const double ms = time_hv(d, lp, precision, in, out, s, iters);
-        if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, nt, 0, 1, ms, 1, scap, swork};
+        if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, nt, 0, 1, ms, 1, scap, swork, 0, 0};
       }
+    }
+  // fused column path: LDS of a round x work of a task
+  for (int fcap : {3072, 4096, 6144})
+    for (int frounds : {1, 2, 4}) {
+      PlanConfig cfg;
+      cfg.stacked = 2;
+      cfg.stk_cap = fcap;
+      cfg.stk_work = 250000;
+      cfg.f_rounds = frounds;
+      std::string e = replan(models, cfg, precision, hp, d, lp);
+      if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+      if (!d.fused) break;  // a column exceeds the fused kernel's limits
+      const double ms = time_hv(d, lp, precision, in, out, s, iters);
+      if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, kFuseThreads, 0, 1, ms, 2, fcap, 250000, frounds, 0};
     }
   (void)hipFree(in);
   (void)hipFree(out);

@@ -620,6 +620,300 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   }
 }
 
+// ---------------------------------------------------------------- fused column path
+// One workgroup per task: the output column Jb of one instance for a range of
+// its slabs, in rounds.  Per round r:
+//   X   X_x = V_R Bhat_R^T, units (row, 4 columns) of every product; meanwhile
+//       the loads of round r+1's tables (products, consumers, references,
+//       segments) are in flight, and they are stored to the other table buffer;
+//   Y   Y_c = sum_f r_f X_f per consumer, into the space of V_R / Bhat^T;
+//       meanwhile round r+1's operands (its V_R, Bhat^T and A blocks) are
+//       loaded into registers, kFPre per thread;
+//   W   every thread adds A_c Y_c of the round's consumers of its own output
+//       units' block, in plan order, to register accumulators;
+// then round r+1's operands go from the registers to LDS.  No round waits on a
+// global load that was not issued a phase earlier.  The column's outputs are
+// owned by the threads (at most U units each) for the whole task and written
+// once as its partial: no atomics, and the sum order is the plan's, so the
+// H*v is bitwise reproducible.
+constexpr int kFPre = 16;  // operand elements of the next round a thread holds in registers
+
+template <typename T>
+__device__ __forceinline__ const T* fuse_src(const FSeg& g, const T* V, const T* Ad, const T* Bh) {
+  return (g.kind == 0 ? V : g.kind == 1 ? Bh : Ad) + g.src;
+}
+
+template <typename T, int NT, int U>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(U <= 2 ? 3 : 2))) void k_fuse(const FTask* __restrict__ tasks, int n_tasks,
+                                             const FBlk* __restrict__ blks, const FRound* __restrict__ rounds,
+                                             const FSeg* __restrict__ segs, const FX* __restrict__ fxs,
+                                             const FCons* __restrict__ fcons, const FRef* __restrict__ frefs,
+                                             const T* __restrict__ V, const T* __restrict__ Ad,
+                                             const T* __restrict__ Bh, double* __restrict__ part) {
+  using Acc = typename Prec<T>::acc;
+  static_assert(sizeof(T) == sizeof(Acc), "LDS regions are shared between operand and product types");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  Acc* smx = reinterpret_cast<Acc*>(smem_raw);
+  __shared__ FRound srs[kFMaxRounds];
+  __shared__ __align__(16) FX sx[2][kFMaxX];
+  __shared__ __align__(16) FCons sc[2][kFMaxC];
+  __shared__ double sfr[2][kFMaxRef];
+  __shared__ int16_t sfx[2][kFMaxRef];
+  __shared__ __align__(16) FSeg sg[2][kFMaxSeg];
+  const int ti = xcd_chunk((int)blockIdx.x, n_tasks);
+  if (ti >= n_tasks) return;
+  __shared__ FBlk sb[kFMaxBlk];
+  const FTask tk = tasks[ti];
+  const int tid = threadIdx.x;
+  const int db = tk.db, nr = tk.r_end - tk.r_begin, nblk = tk.b_end - tk.b_begin;
+  for (int k = tid; k < nr; k += NT) srs[k] = rounds[tk.r_begin + k];
+  if (tid < nblk) sb[tid] = blks[tk.b_begin + tid];
+  __syncthreads();
+  // this thread's output units of the column: block (task-relative) and unit
+  // within the block (row lu % da, columns 4 (lu / da) ..)
+  int ub[U], ul[U];
+  Acc acc[U][kColBlk];
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int u = tid + NT * q;
+    ub[q] = -1;
+    ul[q] = 0;
+    if (u < tk.units) {
+      const int b = seg_of(u, nblk, [&](int x) { return sb[x].unit0; });
+      ub[q] = b;
+      ul[q] = u - sb[b].unit0;
+    }
+#pragma unroll
+    for (int c = 0; c < kColBlk; ++c) acc[q][c] = Acc(0);
+  }
+  // round tables: at most one entry of each table per thread (NT >= the table
+  // limits), moved as raw words so that they stay in registers
+  static_assert(sizeof(FX) == 12 && sizeof(FCons) == 32 && sizeof(FRef) == 16 && sizeof(FSeg) == 16,
+                "round table layouts");
+  static_assert(NT >= kFMaxX && NT >= kFMaxC && NT >= kFMaxRef && NT >= kFMaxSeg, "one table entry per thread");
+  uint32_t tx0 = 0, tx1 = 0, tx2 = 0;
+  uint4 tc = make_uint4(0, 0, 0, 0), tf = make_uint4(0, 0, 0, 0), tg = make_uint4(0, 0, 0, 0);
+  uint32_t tc4 = 0;  // FCons.w0
+  auto tables_load = [&](const FRound& Q) {
+    if (tid < Q.x_end - Q.x_begin) {
+      const uint32_t* w = reinterpret_cast<const uint32_t*>(fxs + Q.x_begin + tid);
+      tx0 = w[0];
+      tx1 = w[1];
+      tx2 = w[2];
+    }
+    if (tid < Q.c_end - Q.c_begin) {
+      tc = *reinterpret_cast<const uint4*>(fcons + Q.c_begin + tid);
+      tc4 = reinterpret_cast<const uint32_t*>(fcons + Q.c_begin + tid)[4];
+    }
+    if (tid < Q.f_end - Q.f_begin) tf = *reinterpret_cast<const uint4*>(frefs + Q.f_begin + tid);
+    if (tid < Q.s_end - Q.s_begin) tg = *reinterpret_cast<const uint4*>(segs + Q.s_begin + tid);
+  };
+  auto tables_store = [&](const FRound& Q, int b) {
+    if (tid < Q.x_end - Q.x_begin) {
+      uint32_t* w = reinterpret_cast<uint32_t*>(&sx[b][tid]);
+      w[0] = tx0;
+      w[1] = tx1;
+      w[2] = tx2;
+    }
+    if (tid < Q.c_end - Q.c_begin) {
+      *reinterpret_cast<uint4*>(&sc[b][tid]) = tc;
+      reinterpret_cast<uint32_t*>(&sc[b][tid])[4] = tc4;
+    }
+    if (tid < Q.f_end - Q.f_begin) {  // FRef: r (8 bytes), then x_lds
+      sfr[b][tid] = __hiloint2double((int)tf.y, (int)tf.x);
+      sfx[b][tid] = (int16_t)tf.z;
+    }
+    if (tid < Q.s_end - Q.s_begin) *reinterpret_cast<uint4*>(&sg[b][tid]) = tg;
+  };
+  // element e of a round's staging order lives in segment k: a binary search
+  // for a thread's first element (k < 0), then upward steps (its elements are
+  // NT apart, a few segments at most)
+  auto seg_at = [&](int b, int ns, int e, int& k) {
+    if (k < 0) { k = seg_of(e, ns, [&](int x) { return (int)sg[b][x].pre; }); return; }
+    while (k + 1 < ns && sg[b][k + 1].pre <= e) ++k;
+  };
+  tables_load(srs[0]);
+  tables_store(srs[0], 0);
+  __syncthreads();
+  {  // round 0's operands, loaded in batches of kFPre with every load of a batch in flight
+    const FRound& Q = srs[0];
+    const int ns = Q.s_end - Q.s_begin;
+    int k = -1;
+    for (int e0 = tid; e0 < Q.stage; e0 += NT * kFPre) {
+      T v[kFPre];
+      int kk = k;  // the stores below repeat the loads' segment walk
+#pragma unroll
+      for (int q = 0; q < kFPre; ++q) {
+        const int e = e0 + NT * q;
+        if (e < Q.stage) {
+          seg_at(0, ns, e, kk);
+          v[q] = fuse_src(sg[0][kk], V, Ad, Bh)[e - sg[0][kk].pre];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kFPre; ++q) {
+        const int e = e0 + NT * q;
+        if (e < Q.stage) {
+          seg_at(0, ns, e, k);
+          sm[sg[0][k].lds + (e - sg[0][k].pre)] = v[q];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < nr; ++r) {
+    const FRound& R = srs[r];
+    const int b = r & 1, nb = b ^ 1;
+    const bool more = r + 1 < nr;
+    const int nx = R.x_end - R.x_begin, nc = R.c_end - R.c_begin;
+    if (more) tables_load(srs[r + 1]);
+    // X = V_R Bhat^T: unit (row i, columns j0 .. j0+3) of product x
+    {
+      int x = tid < R.xwork ? seg_of(tid, nx, [&](int i) { return (int)sx[b][i].work0; }) : 0;
+      for (int w = tid; w < R.xwork; w += NT) {
+        while (x + 1 < nx && sx[b][x + 1].work0 <= w) ++x;
+        const FX X = sx[b][x];
+        const int dar = X.dar, dbr = X.dbr;
+        const int lw = w - X.work0, i = lw % dar, j0 = (lw / dar) * kColBlk;
+        const T* sV = sm + X.v_lds + i;
+        const T* bc[kColBlk];
+#pragma unroll
+        for (int q = 0; q < kColBlk; ++q) bc[q] = sm + X.b_lds + dbr * min(j0 + q, db - 1);
+        Acc s0[kColBlk], s1[kColBlk];
+#pragma unroll
+        for (int q = 0; q < kColBlk; ++q) { s0[q] = Acc(0); s1[q] = Acc(0); }
+        int k = 0;
+        for (; k + 1 < dbr; k += 2) {
+          const T v0 = sV[dar * k], v1 = sV[dar * (k + 1)];
+#pragma unroll
+          for (int q = 0; q < kColBlk; ++q) {
+            s0[q] = Prec<T>::fma3(v0, bc[q][k], s0[q]);
+            s1[q] = Prec<T>::fma3(v1, bc[q][k + 1], s1[q]);
+          }
+        }
+        if (k < dbr) {
+          const T v0 = sV[dar * k];
+#pragma unroll
+          for (int q = 0; q < kColBlk; ++q) s0[q] = Prec<T>::fma3(v0, bc[q][k], s0[q]);
+        }
+        Acc* xo = smx + X.x_lds + i;
+#pragma unroll
+        for (int q = 0; q < kColBlk; ++q)
+          if (j0 + q < db) xo[dar * (j0 + q)] = s0[q] + s1[q];
+      }
+    }
+    if (more) tables_store(srs[r + 1], nb);
+    __syncthreads();
+    // round r+1's operands into registers while this round forms Y and W
+    T pre[kFPre];
+    const int nstage = more ? srs[r + 1].stage : 0;
+    const int ns1 = more ? srs[r + 1].s_end - srs[r + 1].s_begin : 0;
+    {
+      int k = -1;
+#pragma unroll
+      for (int q = 0; q < kFPre; ++q) {
+        const int e = tid + NT * q;
+        if (e < nstage) {
+          seg_at(nb, ns1, e, k);
+          pre[q] = fuse_src(sg[nb][k], V, Ad, Bh)[e - sg[nb][k].pre];
+        }
+      }
+    }
+    // Y_c = sum_f r_f X_f, element by element (the X of one consumer share a layout)
+    {
+      int c = tid < R.ywork ? seg_of(tid, nc, [&](int i) { return sc[b][i].y0; }) : 0;
+      for (int e = tid; e < R.ywork; e += NT) {
+        while (c + 1 < nc && sc[b][c + 1].y0 <= e) ++c;
+        const FCons C = sc[b][c];
+        const int le = e - C.y0;
+        Acc y = Acc(0);
+        for (int f = C.f_begin; f < C.f_end; ++f) y += Acc(sfr[b][f]) * smx[sfx[b][f] + le];
+        sm[C.y_lds + le] = Prec<T>::from(y);
+      }
+    }
+    __syncthreads();
+    // W items (consumer c, row, 4 columns) of A_c Y_c, dealt over all threads:
+    // each writes its four sums over the X space (dead once Y is formed)
+    Acc* wp = smx + R.wp_lds;
+    {
+      int c = tid < R.wwork ? seg_of(tid, nc, [&](int i) { return sc[b][i].w0; }) : 0;
+      for (int it = tid; it < R.wwork; it += NT) {
+        while (c + 1 < nc && sc[b][c + 1].w0 <= it) ++c;
+        const int da = sc[b][c].da, dar = sc[b][c].dar, lu = it - sc[b][c].w0;
+        const int i = lu % da, j0 = (lu / da) * kColBlk;
+        const T* sA = sm + sc[b][c].a_lds + i;
+        const T* yc[kColBlk];
+#pragma unroll
+        for (int j = 0; j < kColBlk; ++j) yc[j] = sm + sc[b][c].y_lds + dar * min(j0 + j, db - 1);
+        Acc s0[kColBlk], s1[kColBlk];
+#pragma unroll
+        for (int j = 0; j < kColBlk; ++j) { s0[j] = Acc(0); s1[j] = Acc(0); }
+        int k = 0;
+        for (; k + 1 < dar; k += 2) {
+          const T a0 = sA[da * k], a1 = sA[da * (k + 1)];
+          T y0[kColBlk], y1[kColBlk];
+#pragma unroll
+          for (int j = 0; j < kColBlk; ++j) { y0[j] = yc[j][k]; y1[j] = yc[j][k + 1]; }
+#pragma unroll
+          for (int j = 0; j < kColBlk; ++j) {
+            s0[j] = Prec<T>::fma3(a0, y0[j], s0[j]);
+            s1[j] = Prec<T>::fma3(a1, y1[j], s1[j]);
+          }
+        }
+        if (k < dar) {
+          const T a0 = sA[da * k];
+#pragma unroll
+          for (int j = 0; j < kColBlk; ++j) s0[j] = Prec<T>::fma3(a0, yc[j][k], s0[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kColBlk; ++j) wp[kColBlk * it + j] = s0[j] + s1[j];
+      }
+    }
+    __syncthreads();
+    // every owner adds its units' item sums, consumer by consumer in plan order
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      if (ub[q] < 0) continue;
+      const int c0 = R.cb[ub[q]], c1 = R.cb[ub[q] + 1];
+      for (int c = c0; c < c1; ++c) {
+        const Acc* w = wp + kColBlk * (sc[b][c].w0 + ul[q]);
+#pragma unroll
+        for (int j = 0; j < kColBlk; ++j) acc[q][j] += w[j];
+      }
+    }
+    if (!more) break;
+    __syncthreads();
+    {  // round r+1's operands to LDS: the prefetched ones, then any beyond kFPre per thread
+      int k = -1;
+#pragma unroll
+      for (int q = 0; q < kFPre; ++q) {
+        const int e = tid + NT * q;
+        if (e < nstage) {
+          seg_at(nb, ns1, e, k);
+          sm[sg[nb][k].lds + (e - sg[nb][k].pre)] = pre[q];
+        }
+      }
+      for (int e = tid + NT * kFPre; e < nstage; e += NT) {
+        seg_at(nb, ns1, e, k);
+        sm[sg[nb][k].lds + (e - sg[nb][k].pre)] = fuse_src(sg[nb][k], V, Ad, Bh)[e - sg[nb][k].pre];
+      }
+    }
+    __syncthreads();
+  }
+  // the column partial: every owned unit, touched or not
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    if (ub[q] < 0) continue;
+    const FBlk B = sb[ub[q]];
+    const int i = ul[q] % B.da, j0 = (ul[q] / B.da) * kColBlk;
+    double* po = part + tk.part_off + B.wc_off + i;
+#pragma unroll
+    for (int j = 0; j < kColBlk; ++j)
+      if (j0 + j < db) po[(int64_t)B.da * (j0 + j)] = (double)acc[q][j];
+  }
+}
+
 // ---------------------------------------------------------------- large-block GEMM path
 // 64 x 64 output tiles on the matrix cores: FP64 (v_mfma_f64_16x16x4f64) for
 // FP64 plans, FP32 (v_mfma_f32_16x16x4f32) for FP32 plans.  (The BigIn<Bf16x2>
@@ -1173,6 +1467,18 @@ const void* k_stk_ptr(int stage, int prec, int nt) {
 #undef NSX_STK
 }
 
+const void* k_fuse_ptr(int prec, int u) {
+#define NSX_FUSE(T) \
+  return u <= 2 ? (const void*)k_fuse<T, kFuseThreads, 2> : u <= 4 ? (const void*)k_fuse<T, kFuseThreads, 4> \
+                                                               : (const void*)k_fuse<T, kFuseThreads, 8>;
+  switch (prec) {
+    case 0: { NSX_FUSE(double) }
+    case 1: { NSX_FUSE(float) }
+    default: { NSX_FUSE(Bf16x2) }
+  }
+#undef NSX_FUSE
+}
+
 // ---------------------------------------------------------------- instance tables
 struct StagedInst {
   int prec, nt, mu, mode, atomic;
@@ -1224,7 +1530,17 @@ template <typename T>
 static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H, double* W,
                        hipStream_t s) {
   const bool at = d.atomic != 0;
-  if (d.stacked) {  // two-stage path for the small blocks (always accumulates into W)
+  if (d.fused) {  // column partials of the small blocks (summed by the reduce)
+    if (d.n_ftask > 0) {
+      void* a[] = {(void*)&d.f_task, (void*)&d.n_ftask, (void*)&d.f_blk, (void*)&d.f_round, (void*)&d.f_seg,
+                   (void*)&d.f_x,    (void*)&d.f_cons,  (void*)&d.f_ref, (void*)&V,         (void*)&Ad,
+                   (void*)&Bh,       (void*)&d.f_part};
+      const unsigned g = (unsigned)(((d.n_ftask + kXcds - 1) / kXcds) * kXcds);
+      hipError_t e = hipLaunchKernel(k_fuse_ptr(prec, d.f_u), dim3(g), dim3(kFuseThreads), a,
+                                     sizeof(T) * (size_t)d.f_lds, s);
+      if (e != hipSuccess) return e;
+    }
+  } else if (d.stacked) {  // two-stage path for the small blocks (always accumulates into W)
     if (!at) return hipErrorInvalidConfiguration;
     using Acc = typename Prec<T>::acc;
     Acc* X = reinterpret_cast<Acc*>(d.X);
@@ -1298,8 +1614,13 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
 template <typename T>
 static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H, double* W, hipStream_t s) {
   const bool big = big_like_active(d);
+  if (d.fused) {  // W = column partials + like-particle part, stored (large blocks: k_big0)
+    if (d.n_fblocks > 0)
+      hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_fblocks), dim3(kThreads), 0, s, d.f_blocks, d.f_part_list,
+                         d.f_part, V, H, W, 1, big ? 1 : 0);
+  }
   // the stacked stage-1 launch carries the small like-particle blocks
-  if (!d.stacked && d.n_blocks > 0)
+  if (!d.stacked && !d.fused && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
                        V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
   if (big) {
@@ -1345,10 +1666,50 @@ hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, con
 
 // One H*v: deterministic mode runs pn partials then the reduce; atomic mode
 // writes the like-particle part first and lets the pn kernels add into W.
+// fused plans: k_big0 (large blocks' like-particle part, stored), k_fuse (column
+// partials), k_reduce (small blocks stored), then k_big1 / k_big2 add the
+// large-block terms
+template <typename T>
+static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H,
+                             double* W, hipStream_t s) {
+  const bool big = big_like_active(d);
+  if (big) {
+    if constexpr (std::is_same<T, Bf16x2>::value)
+      hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+    else
+      hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+  }
+  DevicePlan small = d;   // k_fuse alone (the large-block items run on the tiles below)
+  small.n_gtile2 = 0;
+  small.n_items = 0;
+  hipError_t e = pn_T<T>(small, prec, V, Ad, Bh, H, W, s);
+  if (e != hipSuccess) return e;
+  if (d.n_fblocks > 0)
+    hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_fblocks), dim3(kThreads), 0, s, d.f_blocks, d.f_part_list, d.f_part,
+                       V, H, W, 1, big ? 1 : 0);
+  if (d.n_gtile2 > 0) {
+    DevicePlan lg = d;    // the large-block tiles alone
+    lg.n_ftask = 0;
+    return pn_T<T>(lg, prec, V, Ad, Bh, H, W, s);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s) {
   hipError_t e = launch_convert(d, lp, prec, V, s);
   if (e != hipSuccess) return e;
+  if (d.fused) {
+    switch (prec) {
+      case 0: return fused_hv_T<double>(d, 0, V, d.Ad, d.Bh, d.H, W, s);
+      case 1:
+        return fused_hv_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.Ad, (const float*)lp.Bh,
+                                 (const float*)lp.H, W, s);
+      default:
+        return fused_hv_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.Ad, (const Bf16x2*)lp.Bh,
+                                  (const Bf16x2*)lp.H, W, s);
+    }
+  }
   if (d.atomic) {
     e = launch_reduce(d, lp, prec, V, W, s);
     if (e != hipSuccess) return e;

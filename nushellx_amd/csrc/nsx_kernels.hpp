@@ -46,6 +46,20 @@ struct DevicePlan {
   int n_gtile1 = 0, n_gtile2 = 0, n_gtile3 = 0;
   int big_rows1 = 64;        // rows of the stage-1 tiles (64, or 128 in BF16 plans)
   void* XG = nullptr;        // xg_size elements of the accumulation type (FP64, or FP32 in reduced precision)
+  // fused column path (small blocks; deterministic): k_fuse writes column
+  // partials, k_reduce sums them with the like-particle part
+  int fused = 0, f_u = 2;    // f_u: output units per thread of the k_fuse instance
+  FTask* f_task = nullptr;
+  FBlk* f_blk = nullptr;
+  FRound* f_round = nullptr;
+  FSeg* f_seg = nullptr;
+  FX* f_x = nullptr;
+  FCons* f_cons = nullptr;
+  FRef* f_ref = nullptr;
+  DBlock* f_blocks = nullptr;
+  int64_t* f_part_list = nullptr;
+  double* f_part = nullptr;
+  int n_ftask = 0, n_fblocks = 0, f_lds = 1;
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
 };
 
@@ -103,5 +117,8 @@ int num_staged();
 const void* k_pn_staged_ptr(int i);
 const void* k_pn_partial_ptr(int prec, bool atomic);
 const void* k_stk_ptr(int stage, int prec, int nt);
+// k_fuse instance of a precision with u output units per thread (2, 4 or 8; 256 threads)
+const void* k_fuse_ptr(int prec, int u);
+constexpr int kFuseThreads = 256;
 
 }  // namespace nsx

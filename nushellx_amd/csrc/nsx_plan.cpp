@@ -1,9 +1,11 @@
 #include "nsx_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <array>
 #include <map>
+#include <set>
 
 namespace nsx {
 
@@ -359,6 +361,341 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
   }
 }
 
+bool build_fused(HostPlan& p, const std::vector<const Model*>& models, int threads) {
+  p.f_task.clear(); p.f_blk.clear(); p.f_round.clear(); p.f_seg.clear(); p.f_x.clear();
+  p.f_cons.clear(); p.f_ref.clear(); p.f_blocks.clear(); p.f_part_list.clear();
+  p.f_part_size = 0;
+  p.f_lds = p.f_units = 1;
+  auto fail = [&]() {
+    p.f_task.clear(); p.f_blk.clear(); p.f_round.clear(); p.f_seg.clear(); p.f_x.clear();
+    p.f_cons.clear(); p.f_ref.clear(); p.f_blocks.clear(); p.f_part_list.clear();
+    p.f_part_size = 0;
+    return false;
+  };
+  // items that only the direct kernel takes would add partials of their own
+  if (!p.items.empty() && p.g_tile2.empty()) return fail();
+  const int cap = std::max(p.cfg.stk_cap, 256);
+  const double target = std::max(p.cfg.stk_work, 1000);
+  struct Cons { int bl, da; int64_t a_off; std::vector<std::pair<int, double>> refs; };  // refs: (slab X, r)
+  struct XP { int64_t v_off, b_off; int dbr; };
+  struct Slab { int dar = 0; std::vector<XP> x; std::vector<Cons> cons; double work = 0.0; };
+  std::map<int64_t, std::vector<int64_t>> bases_of;  // block (by its v_off) -> partial bases
+  for (size_t mi = 0; mi < models.size(); ++mi) {
+    const Model& m = *models[mi];
+    const int64_t vb = p.vbase[mi];
+    const auto& a_off = p.inst_a_off[mi];
+    const auto& b_off = p.inst_b_off[mi];
+    std::map<int, std::vector<int>> col_blocks;  // Jb -> blocks, Ja ascending (basis order)
+    for (int L = 0; L < m.basd; ++L) col_blocks[m.bas_b[L]].push_back(L);
+    std::map<int, std::map<std::array<int, 4>, Slab>> cols;
+    for (const Item& it : m.items) {
+      const int da = m.da[it.L], db = m.db[it.L], dar = m.A.d(it.ajr);
+      int maxdbr = 0;
+      for (const ATerm& at : it.at)
+        for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
+      if (std::max(std::max(da, dar), std::max(db, maxdbr)) > p.cfg.big_min) continue;  // large-block tiles
+      const int Jb = m.bas_b[it.L];
+      const std::vector<int>& cb = col_blocks[Jb];
+      const int bl = (int)(std::find(cb.begin(), cb.end(), it.L) - cb.begin());
+      for (const ATerm& at : it.at) {
+        const auto& bk = m.bhat_key[at.rt[0].bhat];
+        Slab& sl = cols[Jb][{it.ajr, bk[2], bk[3], bk[4]}];
+        sl.dar = dar;
+        Cons c;
+        c.bl = bl;
+        c.da = da;
+        c.a_off = a_off[at.adens];
+        for (const RTerm& rt : at.rt) {
+          const int64_t v = vb + m.posnv[rt.R], b = b_off[rt.bhat];
+          int xi = 0;
+          while (xi < (int)sl.x.size() && !(sl.x[xi].v_off == v && sl.x[xi].b_off == b)) ++xi;
+          if (xi == (int)sl.x.size()) {
+            sl.x.push_back({v, b, m.db[rt.R]});
+            sl.work += 2.0 * dar * m.db[rt.R] * db;
+          }
+          c.refs.push_back({xi, rt.r});
+        }
+        sl.work += 2.0 * da * dar * db + double(c.refs.size()) * dar * db;
+        sl.cons.push_back(std::move(c));
+      }
+    }
+    for (auto& col : cols) {
+      const std::vector<int>& blocks = col_blocks[col.first];
+      const int db = m.B.d(col.first);
+      const int ncb = (db + kColBlk - 1) / kColBlk;
+      if ((int)blocks.size() > kFMaxBlk) return fail();
+      const int b_begin = (int)p.f_blk.size();
+      int wcol = 0, units = 0;
+      for (int L : blocks) {
+        p.f_blk.push_back(FBlk{wcol, m.da[L], units, 0});
+        wcol += m.da[L] * db;
+        units += m.da[L] * ncb;
+      }
+      const int b_end = (int)p.f_blk.size();
+      if (units > kFMaxU * threads) return fail();
+      p.f_units = std::max(p.f_units, units);
+      // a slab whose round alone would exceed the LDS cap is split into groups of
+      // its consumers, each with the products it reads (computed once per group)
+      std::vector<Slab> split;
+      auto lds_of = [&](const Slab& sl) {
+        std::set<int64_t> v, b, a;
+        int na = 0, np = 0;
+        for (const XP& x : sl.x) {
+          if (v.insert(x.v_off).second) np += sl.dar * x.dbr;
+          if (b.insert(x.b_off).second) np += x.dbr * db;
+        }
+        for (const Cons& c : sl.cons)
+          if (a.insert(c.a_off).second) na += c.da * sl.dar;
+        const int nx = (int)sl.x.size() * sl.dar * db, ny = (int)sl.cons.size() * sl.dar * db;
+        return na + nx + std::max(np, ny);
+      };
+      for (auto& sp : col.second) {
+        const Slab& sl = sp.second;
+        if (lds_of(sl) <= cap || sl.cons.size() < 2) {
+          split.push_back(sl);
+          continue;
+        }
+        size_t c0 = 0;
+        while (c0 < sl.cons.size()) {
+          Slab g;
+          g.dar = sl.dar;
+          std::map<int, int> xmap;  // slab X -> group X
+          size_t c1 = c0;
+          while (c1 < sl.cons.size()) {
+            Slab t = g;
+            std::map<int, int> tm = xmap;
+            Cons c = sl.cons[c1];
+            for (auto& f : c.refs) {
+              auto it = tm.find(f.first);
+              if (it == tm.end()) {
+                it = tm.emplace(f.first, (int)t.x.size()).first;
+                t.x.push_back(sl.x[f.first]);
+              }
+              f.first = it->second;
+            }
+            t.cons.push_back(std::move(c));
+            if (c1 > c0 && lds_of(t) > cap) break;
+            g = std::move(t);
+            xmap = std::move(tm);
+            ++c1;
+          }
+          const int db_ = db;
+          for (const XP& x : g.x) g.work += 2.0 * g.dar * x.dbr * db_;
+          for (const Cons& c : g.cons) g.work += 2.0 * c.da * g.dar * db_ + double(c.refs.size()) * g.dar * db_;
+          split.push_back(std::move(g));
+          c0 = c1;
+        }
+      }
+      std::vector<const Slab*> slabs;
+      double total = 0.0;
+      for (const Slab& sl : split) {
+        slabs.push_back(&sl);
+        total += sl.work;
+      }
+      const int ntask = std::max(1, (int)std::ceil(total / target));
+      const double per = total / ntask;
+      size_t si = 0;
+      FTask tk{};
+      auto open_task = [&]() {
+        tk = FTask{};
+        tk.part_off = p.f_part_size;
+        p.f_part_size += wcol;
+        tk.b_begin = b_begin;
+        tk.b_end = b_end;
+        tk.db = db;
+        tk.units = units;
+        tk.r_begin = (int)p.f_round.size();
+        for (size_t k = 0; k < blocks.size(); ++k)
+          bases_of[vb + m.posnv[blocks[k]]].push_back(tk.part_off + p.f_blk[b_begin + k].wc_off);
+      };
+      for (int t = 0; t < ntask && si < slabs.size(); ++t) {
+        open_task();
+        // this task's slabs: up to its share of the column's work (the last takes the rest)
+        double acc = 0.0;
+        const size_t s0 = si;
+        while (si < slabs.size() && (si == s0 || t == ntask - 1 || acc + 0.5 * slabs[si]->work <= per)) {
+          acc += slabs[si]->work;
+          ++si;
+        }
+        // rounds of slabs that fit the LDS buffer and the round tables.  A round's
+        // LDS holds [A blocks][X products][V_R and Bhat^T blocks, later Y]: the
+        // staged V_R and Bhat^T are dead once X is formed, so Y reuses them.
+        size_t ri = s0;
+        while (ri < si) {
+          if ((int)p.f_round.size() - tk.r_begin == std::min(std::max(p.cfg.f_rounds, 1), kFMaxRounds)) {
+            // continue in a task of its own
+            tk.r_end = (int)p.f_round.size();
+            p.f_task.push_back(tk);
+            open_task();
+          }
+          FRound R{};
+          R.s_begin = (int)p.f_seg.size();
+          R.x_begin = (int)p.f_x.size();
+          std::map<int64_t, int> v_lds, b_lds, a_lds;  // region-relative offsets
+          int na_tot = 0, nx_tot = 0, np_tot = 0, ny_tot = 0, nw_tot = 0, nref = 0;
+          struct RC { int bl, da, dar, a_lds, y_lds; std::vector<std::pair<int, double>> refs; };  // refs: (x_lds, r)
+          std::vector<RC> rc;
+          int last_seg[3] = {-1, -1, -1};
+          // lengths and region-relative offsets of this round's segments (FSeg
+          // stores the LDS offset and the prefix in 16 bits once the round is laid out)
+          std::vector<int> seg_len, seg_off;
+          auto seg = [&](int64_t src, int len, int kind, int off) {
+            const int ls = last_seg[kind == 2 ? 2 : 0];
+            if (ls >= 0) {
+              const int k = ls - R.s_begin;
+              if (p.f_seg[ls].kind == kind && p.f_seg[ls].src + seg_len[k] == src && seg_off[k] + seg_len[k] == off) {
+                seg_len[k] += len;
+                return;
+              }
+            }
+            last_seg[kind == 2 ? 2 : 0] = (int)p.f_seg.size();
+            FSeg g{};
+            g.src = src;
+            g.kind = (int16_t)kind;
+            p.f_seg.push_back(g);
+            seg_len.push_back(len);
+            seg_off.push_back(off);
+          };
+          while (ri < si) {
+            const Slab& sl = *slabs[ri];
+            const int dar = sl.dar;
+            int da_ = 0, dx = 0, dp = 0, dy = 0;
+            std::set<int64_t> nv, nb, na;
+            for (const XP& x : sl.x) {
+              if (!v_lds.count(x.v_off) && nv.insert(x.v_off).second) dp += dar * x.dbr;
+              if (!b_lds.count(x.b_off) && nb.insert(x.b_off).second) dp += x.dbr * db;
+              dx += dar * db;
+            }
+            int nr = 0;
+            for (const Cons& c : sl.cons) {
+              if (!a_lds.count(c.a_off) && na.insert(c.a_off).second) da_ += c.da * dar;
+              dy += dar * db;
+              nr += (int)c.refs.size();
+            }
+            int dw = 0;  // W-item partial sums (4 per item), in the X space once Y is formed
+            for (const Cons& c : sl.cons) dw += kColBlk * c.da * ncb;
+            const int after = na_tot + da_ + std::max(nx_tot + dx, nw_tot + dw) + std::max(np_tot + dp, ny_tot + dy);
+            const int nx = (int)p.f_x.size() - R.x_begin;
+            const int ns = (int)p.f_seg.size() - R.s_begin;
+            const int new_segs = (int)(nv.size() + nb.size() + na.size());
+            if (!rc.empty() && (after > cap || nx + (int)sl.x.size() > kFMaxX ||
+                                (int)(rc.size() + sl.cons.size()) > kFMaxC || nref + nr > kFMaxRef ||
+                                ns + new_segs > kFMaxSeg))
+              break;
+            if ((int)sl.x.size() > kFMaxX || (int)sl.cons.size() > kFMaxC || nr > kFMaxRef || new_segs > kFMaxSeg)
+              return fail();
+            std::vector<int> x_lds(sl.x.size());
+            for (size_t k = 0; k < sl.x.size(); ++k) {
+              const XP& x = sl.x[k];
+              auto fv = v_lds.find(x.v_off);
+              if (fv == v_lds.end()) {
+                fv = v_lds.emplace(x.v_off, np_tot).first;
+                seg(x.v_off, dar * x.dbr, 0, np_tot);
+                np_tot += dar * x.dbr;
+              }
+              auto fb = b_lds.find(x.b_off);
+              if (fb == b_lds.end()) {
+                fb = b_lds.emplace(x.b_off, np_tot).first;
+                seg(x.b_off, x.dbr * db, 1, np_tot);
+                np_tot += x.dbr * db;
+              }
+              FX fx{};
+              fx.v_lds = fv->second;
+              fx.b_lds = fb->second;
+              fx.x_lds = nx_tot;
+              fx.dar = dar;
+              fx.dbr = x.dbr;
+              x_lds[k] = nx_tot;
+              nx_tot += dar * db;
+              p.f_x.push_back(fx);
+            }
+            for (const Cons& c : sl.cons) {
+              auto fa = a_lds.find(c.a_off);
+              if (fa == a_lds.end()) {
+                fa = a_lds.emplace(c.a_off, na_tot).first;
+                seg(c.a_off, c.da * dar, 2, na_tot);
+                na_tot += c.da * dar;
+              }
+              RC r{c.bl, c.da, dar, fa->second, ny_tot, {}};
+              ny_tot += dar * db;
+              nw_tot += kColBlk * c.da * ncb;
+              for (auto& f : c.refs) r.refs.push_back({x_lds[f.first], f.second});
+              rc.push_back(std::move(r));
+            }
+            nref += nr;
+            ++ri;
+          }
+          // region bases: A at 0, X (later the W partials) after A, V_R / Bhat^T
+          // (later Y) after that
+          const int xbase = na_tot, pbase = na_tot + std::max(nx_tot, nw_tot);
+          const int lds = pbase + std::max(np_tot, ny_tot);
+          R.wp_lds = xbase;
+          if (lds > kFMaxLds) return fail();
+          R.s_end = (int)p.f_seg.size();
+          int pre = 0;
+          for (int k = R.s_begin; k < R.s_end; ++k) {
+            p.f_seg[k].lds = (int16_t)(seg_off[k - R.s_begin] + (p.f_seg[k].kind == 2 ? 0 : pbase));
+            p.f_seg[k].pre = (int16_t)pre;
+            pre += seg_len[k - R.s_begin];
+          }
+          R.stage = pre;
+          R.x_end = (int)p.f_x.size();
+          int w = 0;
+          for (int k = R.x_begin; k < R.x_end; ++k) {
+            FX& fx = p.f_x[k];
+            fx.v_lds += pbase;
+            fx.b_lds += pbase;
+            fx.x_lds += xbase;
+            fx.work0 = w;
+            w += fx.dar * ncb;
+          }
+          R.xwork = w;
+          std::stable_sort(rc.begin(), rc.end(), [](const RC& a, const RC& b) { return a.bl < b.bl; });
+          R.c_begin = (int)p.f_cons.size();
+          R.f_begin = (int)p.f_ref.size();
+          int y0 = 0, w0 = 0;
+          for (const RC& r : rc) {
+            FCons c{};
+            c.w0 = w0;
+            w0 += r.da * ncb;
+            c.a_lds = r.a_lds;
+            c.y_lds = pbase + r.y_lds;
+            c.da = r.da;
+            c.dar = r.dar;
+            c.f_begin = (int)p.f_ref.size() - R.f_begin;
+            for (auto& f : r.refs) p.f_ref.push_back(FRef{f.second, xbase + f.first, 0});
+            c.f_end = (int)p.f_ref.size() - R.f_begin;
+            c.y0 = y0;
+            y0 += r.dar * db;
+            p.f_cons.push_back(c);
+          }
+          R.c_end = (int)p.f_cons.size();
+          R.f_end = (int)p.f_ref.size();
+          R.ywork = y0;
+          R.wwork = w0;
+          for (int b = 0, c = 0; b <= kFMaxBlk; ++b) {
+            while (c < (int)rc.size() && rc[c].bl < b) ++c;
+            R.cb[b] = c;
+          }
+          p.f_lds = std::max(p.f_lds, lds);
+          p.f_round.push_back(R);
+        }
+        tk.r_end = (int)p.f_round.size();
+        p.f_task.push_back(tk);
+      }
+    }
+  }
+  for (const DBlock& b : p.blocks) {
+    DBlock f = b;
+    f.p_begin = (int)p.f_part_list.size();
+    auto it = bases_of.find(b.v_off);
+    if (it != bases_of.end()) p.f_part_list.insert(p.f_part_list.end(), it->second.begin(), it->second.end());
+    f.p_end = (int)p.f_part_list.size();
+    p.f_blocks.push_back(f);
+  }
+  return true;
+}
+
 // K-step mask of the columns [c0, c0 + w) x rows [r0, r0 + h) of a column-major
 // matrix M (ld rows), over steps of kBigStep along the columns (col_steps) or
 // along the rows (!col_steps)
@@ -479,7 +816,7 @@ void build_big(HostPlan& p, const std::vector<const Model*>& models) {
         int steps = 0;
         while (q < row.size()) {
           const int s = __builtin_popcountll(row[q].kmask[0]) + __builtin_popcountll(row[q].kmask[1]);
-          if (steps > 0 && steps + s > kBigKChunk) break;
+          if (steps > 0 && steps + s > p.cfg.big_kchunk) break;
           steps += s;
           p.g_term2.push_back(row[q++]);
         }

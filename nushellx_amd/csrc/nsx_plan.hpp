@@ -118,6 +118,66 @@ struct SXRef {
   double r;
 };
 
+// Fused small-block path (owner-computes, no atomics, deterministic):
+// one workgroup owns the output column Jb of one instance (the blocks
+// L = (Ja, Jb), concatenated into a column partial) for a range of the column's
+// slabs.  A slab is one (Ja', lam, sa, sa') (separable: (Ja', lam, n, parity)):
+// its products X_R = V_R Bhat_R^T over the right blocks R = (Ja', Jb'), and
+// its consumers, the a-terms of the column's blocks that read them.  Slabs
+// are taken in rounds that fit the LDS buffer,
+//   stage V_R, Bhat_R^T, A_k  ->  X_R = V_R Bhat_R^T  ->  Y_k = sum r X_R  ->  W_L += A_k Y_k,
+// so X and Y never leave LDS; the column's outputs stay in registers across
+// rounds and the task writes its column partial once.  k_reduce then sums a
+// block's partials in task order and adds H_a V_L + V_L H_b^T.
+constexpr int kFMaxBlk = 16;     // blocks of one column
+constexpr int kFMaxX = 64;       // products of one round
+constexpr int kFMaxC = 64;       // consumers (a-terms) of one round
+constexpr int kFMaxRef = 256;    // X references of one round
+constexpr int kFMaxSeg = 128;    // staging segments of one round
+constexpr int kFMaxRounds = 24;  // rounds of one task
+constexpr int kFMaxU = 8;        // output units (row, 4 columns) per thread
+constexpr int kFMaxLds = 32767;  // LDS element offsets fit 16 bits
+struct FTask {
+  int64_t part_off;         // column partial: the column's blocks, each da x db column-major
+  int32_t r_begin, r_end;   // FRound range
+  int32_t b_begin, b_end;   // FBlk range (the column's blocks, Ja ascending)
+  int32_t db, units;        // column width; output units = sum over blocks of da * ceil(db / 4)
+};
+struct FBlk {
+  int32_t wc_off, da, unit0, pad;  // offset in the column partial, rows, first output unit
+};
+struct FRound {
+  int32_t s_begin, s_end;   // FSeg range (staging)
+  int32_t x_begin, x_end;   // FX range
+  int32_t c_begin, c_end;   // FCons range, sorted by column block
+  int32_t f_begin, f_end;   // FRef range
+  int32_t xwork, ywork;     // X units (row, 4 columns); Y elements
+  int32_t wwork, wp_lds;    // W items (consumer, row, 4 columns); their partial sums (4 each) in LDS
+  int32_t stage;            // elements staged (sum of the segment lengths)
+  int32_t cb[kFMaxBlk + 1]; // consumers of the task's block b: [cb[b], cb[b + 1]) (round-relative)
+};
+// The round tables hold LDS offsets and sizes in 16 bits (the plan checks they fit).
+struct FSeg {
+  int64_t src;              // element offset in V (kind 0), Bh (1) or Ad (2)
+  int16_t lds, pre, kind, pad;  // LDS offset; first element in the round's staging order
+};
+struct FX {
+  int16_t v_lds, b_lds, x_lds;  // V_R (dar x dbr), Bhat^T (dbr x db), X (dar x db) in LDS
+  int16_t dar, dbr, work0;      // first X unit
+};
+struct FCons {
+  int16_t a_lds, y_lds;     // A (da x dar), Y (dar x db) in LDS
+  int16_t da, dar;
+  int16_t f_begin, f_end;   // round-relative FRef range
+  int32_t y0;               // first Y element of the round
+  int32_t w0;               // first W item of the round: items are (row, 4 columns) of A Y
+  int32_t pad[3];
+};
+struct FRef {
+  double r;
+  int32_t x_lds, pad;
+};
+
 // Large-block path (blocks with a side above kSmallMax, in every precision of
 // an atomic-mode plan): the same two stages as tiled GEMMs on the matrix
 // cores, 64 x 64 output tiles.
@@ -168,6 +228,17 @@ struct PlanConfig {
   int stk_work = 512;        // stacked path: work units per chunk
   int big_min = kSmallMax;   // items with a side above this take the direct / large-block paths
   int big_rows1 = kBigTile;  // rows of the stage-1 large-block tiles: 64, or 128 (BF16 plans)
+  // stacked == 2: the fused path, stk_cap = LDS elements of a round,
+  // stk_work = flops one task (workgroup) takes at most, about, and
+  // f_rounds = rounds one task takes at most
+  int f_rounds = 4;
+  // stage-2 large-block tiles take about this many active K steps each (split-K
+  // over several workgroups adding into one W tile)
+  int big_kchunk = kBigKChunk;
+  // bitwise reproducible H*v: the fused path for the small blocks and one
+  // stage-2 workgroup per large W tile (its single atomic add lands on the
+  // stored like-particle value, so the order of additions cannot vary)
+  int deterministic = 0;
 };
 
 struct HostPlan {
@@ -207,6 +278,18 @@ struct HostPlan {
   std::vector<GTerm2> g_term2;
   std::vector<SXRef> g_xref;
   int64_t xg_size = 0;         // large-block X buffer (doubles)
+  // fused path (filled by build_fused)
+  std::vector<FTask> f_task;
+  std::vector<FBlk> f_blk;
+  std::vector<FRound> f_round;
+  std::vector<FSeg> f_seg;
+  std::vector<FX> f_x;
+  std::vector<FCons> f_cons;
+  std::vector<FRef> f_ref;
+  std::vector<DBlock> f_blocks;      // p.blocks with the fused partials
+  std::vector<int64_t> f_part_list;
+  int64_t f_part_size = 0;           // doubles of the column partials
+  int f_lds = 1, f_units = 1;        // largest round (LDS elements); largest task's output units
   int64_t N = 0, part_size = 0;
   int max_lds = 0;                  // doubles of LDS the pn kernel needs
   double flops_ref = 0.0;           // algorithmic work of one batched H*v
@@ -225,6 +308,11 @@ void plan_merge(HostPlan& p, const HostPlan& q);
 // (the instances planned into p, in order).  Clears p.cfg.stacked (the staged
 // kernel then runs) if an a-term has more references than a chunk can hold.
 void build_stacked(HostPlan& p, const std::vector<const Model*>& models);
+// Fills the fused-path arrays of p for the small-block items of `models`.
+// Returns false (and leaves them empty) if a column exceeds the kernel's
+// limits (blocks per column, output units) or the plan has items that only
+// the direct kernel takes.
+bool build_fused(HostPlan& p, const std::vector<const Model*>& models, int threads);
 // Fills the large-block GEMM arrays of p for the items with a side above
 // p.cfg.big_min (kSmallMax unless a test lowers it); leaves them empty, so the
 // direct kernel takes those items, when a block exceeds kBigMaxK.

@@ -234,10 +234,16 @@ class Batch:
         check(lib().nsx_batch_autotune(self._b, iters, C.byref(r)), "nsx_batch_autotune")
         return r.as_dict()
 
+    def path(self) -> str:
+        """The small-block pn path of the plan: "staged", "stacked" or "fused"."""
+        v = C.c_int32()
+        check(lib().nsx_batch_path(self._b, C.byref(v)), "nsx_batch_path")
+        return ("staged", "stacked", "fused")[v.value & 3] + (" deterministic" if v.value & 4 else "")
+
     def set_config(self, cfg: dict) -> None:
         r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"],
                        cfg.get("atomic", 0), 0.0, cfg.get("stacked", 0), cfg.get("stk_cap", 0),
-                       cfg.get("stk_work", 0))
+                       cfg.get("stk_work", 0), cfg.get("f_rounds", 0), cfg.get("deterministic", 0))
         check(lib().nsx_batch_set_config(self._b, C.byref(r)), "nsx_batch_set_config")
 
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,

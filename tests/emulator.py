@@ -33,6 +33,8 @@ def lib():
                                    C.c_void_p]
         L.emu_hv_stacked.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int]
         L.emu_sym_eig.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.emu_hv_fused.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                                      C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -107,6 +109,19 @@ def emu_hv_batch(arg_lists, v, merge):
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out, sizes
+
+
+def emu_hv_fused(args, v, cap=3072, work=250000, threads=256):
+    """Emulated H*v through the fused column plan; returns (W, stats) with stats =
+    (tasks, rounds, largest round LDS, partial doubles, largest task units)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    st = np.zeros(5, dtype=np.int64)
+    rc = lib().emu_hv_fused(*[a.encode() for a in args], v.ctypes.data, out.ctypes.data, len(v), cap, work, threads,
+                            st.ctypes.data)
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out, st
 
 
 def emu_hv_stacked(args, v, cap=2048, work=512):

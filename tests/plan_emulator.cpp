@@ -2,6 +2,9 @@
 // plain loops, so the plan construction (block lists, phase runs, Bhat^T
 // layout, partial bookkeeping) is checked against the oracle without a GPU.
 // It mirrors the arithmetic of the HIP kernels, not their scheduling.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -320,5 +323,166 @@ extern "C" int emu_hv_stacked(const char* nus, const char* opa, const char* opb,
       }
       for (int i = 0; i < it.da; ++i) vout[it.w_off + i + int64_t(it.da) * j] += w[i];
     }
+  return 0;
+}
+
+// Fused path: every task of the plan replayed in the kernel's order (rounds of
+// staging, X = V Bhat^T, Y = sum r X, W += A Y into the task's column outputs),
+// through an LDS image of the round, then the reduce over the column partials
+// with the like-particle part, and the large-block items added on top.
+// stats (optional, 5 values): tasks, rounds, largest round LDS, partial doubles, largest task units.
+extern "C" int emu_hv_fused(const char* nus, const char* opa, const char* opb, const char* intf,
+                            const double* vin, double* vout, int64_t n, int cap, int work, int threads,
+                            int64_t* stats) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  p.cfg.stacked = 2;
+  p.cfg.stk_cap = cap;
+  p.cfg.stk_work = work;
+  if (const char* v = std::getenv("NSX_BIG_MIN")) p.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
+  plan_add(p, m);
+  std::vector<const Model*> ms{&m};
+  build_big(p, ms);
+  if (!build_fused(p, ms, threads)) { g_err = "fused plan not possible"; return -4; }
+  std::vector<double> part(p.f_part_size, 0.0);
+  std::vector<double> lds(p.f_lds + 1);
+  for (const FTask& t : p.f_task) {
+    const int ncb = (t.db + kColBlk - 1) / kColBlk;
+    if (t.units > kFMaxU * threads) { g_err = "task units above the kernel's limit"; return -3; }
+    std::vector<double> acc(size_t(t.units) * kColBlk, 0.0);
+    for (int r = t.r_begin; r < t.r_end; ++r) {
+      const FRound& R = p.f_round[r];
+      std::fill(lds.begin(), lds.end(), std::nan(""));
+      if (R.x_end - R.x_begin > kFMaxX || R.c_end - R.c_begin > kFMaxC || R.f_end - R.f_begin > kFMaxRef ||
+          R.s_end - R.s_begin > kFMaxSeg || t.r_end - t.r_begin > kFMaxRounds) {
+        g_err = "round tables above the kernel's limits";
+        return -3;
+      }
+      for (int s = R.s_begin; s < R.s_end; ++s) {
+        const FSeg& g = p.f_seg[s];
+        const int len = (s + 1 < R.s_end ? p.f_seg[s + 1].pre : R.stage) - g.pre;
+        if (g.lds < 0 || len <= 0 || g.lds + len > p.f_lds) { g_err = "segment outside the round's LDS"; return -3; }
+        const double* src = g.kind == 0 ? vin : g.kind == 1 ? p.Bh.data() : p.Ad.data();
+        for (int k = 0; k < len; ++k) lds[g.lds + k] = src[g.src + k];
+      }
+      for (int x = R.x_begin; x < R.x_end; ++x) {
+        const FX& X = p.f_x[x];
+        for (int w = 0; w < X.dar * ncb; ++w) {
+          const int i = w % X.dar, j0 = (w / X.dar) * kColBlk;
+          for (int q = 0; q < kColBlk && j0 + q < t.db; ++q) {
+            double sum = 0.0;
+            for (int k = 0; k < X.dbr; ++k) sum += lds[X.v_lds + i + X.dar * k] * lds[X.b_lds + k + X.dbr * (j0 + q)];
+            lds[X.x_lds + i + X.dar * (j0 + q)] = sum;
+          }
+        }
+      }
+      // Y overwrites the staged V / Bhat^T: form every Y from X first, as the kernel's barrier does
+      std::vector<std::vector<double>> Y;
+      for (int c = R.c_begin; c < R.c_end; ++c) {
+        const FCons& C = p.f_cons[c];
+        std::vector<double> y(size_t(C.dar) * t.db, 0.0);
+        for (int f = C.f_begin; f < C.f_end; ++f) {
+          const FRef& F = p.f_ref[R.f_begin + f];
+          for (size_t q = 0; q < y.size(); ++q) y[q] += F.r * lds[F.x_lds + q];
+        }
+        Y.push_back(std::move(y));
+      }
+      for (int c = R.c_begin; c < R.c_end; ++c) {
+        const FCons& C = p.f_cons[c];
+        for (size_t q = 0; q < Y[c - R.c_begin].size(); ++q) lds[C.y_lds + q] = Y[c - R.c_begin][q];
+      }
+      // W items (consumer, row, 4 columns) write their partial sums over the dead X space
+      for (int c = R.c_begin; c < R.c_end; ++c) {
+        const FCons& C = p.f_cons[c];
+        for (int lu = 0; lu < C.da * ncb; ++lu) {
+          const int it = C.w0 + lu, i = lu % C.da, j0 = (lu / C.da) * kColBlk;
+          if (R.wp_lds + kColBlk * (it + 1) > p.f_lds) { g_err = "W partials outside the round's LDS"; return -3; }
+          for (int q = 0; q < kColBlk; ++q) {
+            double sum = 0.0;
+            if (j0 + q < t.db)
+              for (int k = 0; k < C.dar; ++k) sum += lds[C.a_lds + i + C.da * k] * lds[C.y_lds + k + C.dar * (j0 + q)];
+            lds[R.wp_lds + kColBlk * it + q] = sum;
+          }
+        }
+      }
+      for (int u = 0; u < t.units; ++u) {
+        int b = t.b_begin;
+        while (b + 1 < t.b_end && p.f_blk[b + 1].unit0 <= u) ++b;
+        const FBlk& B = p.f_blk[b];
+        const int lu = u - B.unit0, j0 = (lu / B.da) * kColBlk;
+        for (int c = R.cb[b - t.b_begin]; c < R.cb[b - t.b_begin + 1]; ++c) {
+          const FCons& C = p.f_cons[R.c_begin + c];
+          if (C.da != B.da) { g_err = "consumer rows differ from its block's"; return -3; }
+          for (int q = 0; q < kColBlk && j0 + q < t.db; ++q) acc[size_t(u) * kColBlk + q] += lds[R.wp_lds + kColBlk * (C.w0 + lu) + q];
+        }
+      }
+    }
+    for (int u = 0; u < t.units; ++u) {
+      int b = t.b_begin;
+      while (b + 1 < t.b_end && p.f_blk[b + 1].unit0 <= u) ++b;
+      const FBlk& B = p.f_blk[b];
+      const int lu = u - B.unit0, i = lu % B.da, j0 = (lu / B.da) * kColBlk;
+      for (int q = 0; q < kColBlk && j0 + q < t.db; ++q)
+        part[t.part_off + B.wc_off + i + B.da * (j0 + q)] = acc[size_t(u) * kColBlk + q];
+    }
+  }
+  std::fill(vout, vout + n, 0.0);
+  for (const DBlock& b : p.f_blocks)
+    for (int jj = 0; jj < b.nc; ++jj) {
+      const int j = b.j0 + jj;
+      for (int i = 0; i < b.da; ++i) {
+        const int64_t ij = i + int64_t(b.da) * j;
+        double w = 0.0;
+        for (int q = b.p_begin; q < b.p_end; ++q) w += part[p.f_part_list[q] + ij];
+        if (b.ha_off >= 0)
+          for (int k = 0; k < b.da; ++k) w += p.H[b.ha_off + i + int64_t(b.da) * k] * vin[b.v_off + k + int64_t(b.da) * j];
+        if (b.hb_off >= 0)
+          for (int k = 0; k < b.db; ++k) w += vin[b.v_off + i + int64_t(b.da) * k] * p.H[b.hb_off + j + int64_t(b.db) * k];
+        vout[b.v_off + ij] = w;
+      }
+    }
+  // large-block items (the GEMM tiles on the GPU) add into W
+  for (const DItem& it : p.items)
+    for (int jj = 0; jj < it.nc; ++jj) {
+      const int j = it.j0 + jj;
+      std::vector<double> w(it.da, 0.0);
+      for (int a = it.a_begin; a < it.a_end; ++a) {
+        const DATerm& T = p.aterms[a];
+        std::vector<double> y(it.dar, 0.0);
+        for (int u = T.r_begin; u < T.r_end; ++u) {
+          const DRTerm& R = p.rterms[u];
+          for (int i = 0; i < it.dar; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < R.dbr; ++k) s += vin[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
+            y[i] += R.r * s;
+          }
+        }
+        for (int i = 0; i < it.da; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < it.dar; ++k) s += p.Ad[T.a_off + i + int64_t(it.da) * k] * y[k];
+          w[i] += s;
+        }
+      }
+      for (int i = 0; i < it.da; ++i) vout[it.w_off + i + int64_t(it.da) * j] += w[i];
+    }
+  if (const char* v = std::getenv("NSX_EMU_ROUND_STATS")) {
+    int mx = 0, mc = 0, mf = 0, ms = 0, mr = 0;
+    for (const FRound& R : p.f_round) {
+      mx = std::max(mx, R.x_end - R.x_begin); mc = std::max(mc, R.c_end - R.c_begin);
+      mf = std::max(mf, R.f_end - R.f_begin); ms = std::max(ms, R.s_end - R.s_begin);
+    }
+    for (const FTask& t : p.f_task) mr = std::max(mr, t.r_end - t.r_begin);
+    std::fprintf(stderr, "round max: X %d cons %d refs %d segs %d; rounds per task %d\n", mx, mc, mf, ms, mr);
+  }
+  if (stats) {
+    stats[0] = (int64_t)p.f_task.size();
+    stats[1] = (int64_t)p.f_round.size();
+    stats[2] = p.f_lds;
+    stats[3] = p.f_part_size;
+    stats[4] = p.f_units;
+  }
   return 0;
 }

@@ -2,24 +2,12 @@
 so the direct (non-staged) pn kernel runs beside the staged one.  H*v against
 the FP64 oracle, and the device Lanczos against ARPACK (scipy eigsh) driven by
 the oracle's H*v."""
-import tempfile
-
 import numpy as np
 import pytest
 
 from oracle.refcpu import RefCPU
 
 pytestmark = pytest.mark.gpu
-
-
-@pytest.fixture(scope="module")
-def pf44():
-    from nushellx_amd.synth.interaction import random_interaction
-    from nushellx_amd.synth.mspace import PF, PNSpace
-    from nushellx_amd.synth.writer import PNModel, write_model
-    print("generating pf 4p4n J=0 operator files", flush=True)
-    inter = random_interaction(PF, seed=2024, spe_range=(-10.0, -4.0))
-    return write_model(PNModel(PNSpace(PF), 4, 4, inter), tempfile.mkdtemp(prefix="nsx_pf44_"), "pf44", [0])
 
 
 def test_pf44_hv_matches_oracle(pf44):

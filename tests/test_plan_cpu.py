@@ -58,3 +58,35 @@ def test_stacked_plan_matches_refcpu(models):
             for cap, work in ((512, 64), (2048, 512), (8192, 4096)):
                 w = emu_hv_stacked(args, v, cap, work)
                 assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, cap)
+
+
+def test_fused_plan_matches_refcpu(models):
+    """The fused column plan (per task: rounds of V_R / Bhat^T / A staging,
+    X = V Bhat^T, Y = sum r X, W += A Y into the column's outputs; then the
+    reduce over the column partials) gives the oracle's H*v, with LDS caps and
+    task sizes that force many rounds and many tasks per column, on a
+    truncated basis, and beside large-block items."""
+    from tests.emulator import emu_hv_fused
+    rng = np.random.default_rng(13)
+    cases = [("sd22", 0), ("sd22", 4), ("sd44", 0), ("sd44", 8), ("sd66", 4), ("sd23", 3), ("psd33t", 2)]
+    for name, J2 in cases:
+        args = models[name].args(J2)
+        r = RefCPU(*args, fp64=True)
+        v = rng.standard_normal(r.N)
+        ref = r.apply(v)
+        for cap, work in ((256, 2000), (1024, 20000), (3072, 250000), (16384, 10 ** 8)):
+            w, st = emu_hv_fused(args, v, cap, work)
+            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, cap, work)
+            assert st[0] > 0 and st[1] >= st[0]
+
+
+def test_fused_plan_beside_large_blocks(models, monkeypatch):
+    from tests.emulator import emu_hv_fused
+    monkeypatch.setenv("NSX_BIG_MIN", "20")
+    args = models["sd44"].args(4)
+    r = RefCPU(*args, fp64=True)
+    v = np.random.default_rng(2).standard_normal(r.N)
+    ref = r.apply(v)
+    w, st = emu_hv_fused(args, v, 2048, 50000)
+    assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert st[0] > 0
