@@ -296,6 +296,24 @@ def pf_warm(log, train_samples: int, nprot: int = 4, nneut: int = 4, chunk: int 
     return out
 
 
+def host_f32(log, hams, iters: int = 50):
+    """The drop-in as NuTRL calls it: nsx_hv_apply_f32 on host REAL*4 vectors
+    (coefR in, coefL out; PCIe transfers, device widening and narrowing
+    included), per instance, one instance after another."""
+    per = []
+    for h in hams:
+        v = np.random.default_rng(h.N).standard_normal(h.N).astype(np.float32)
+        h.multiply(v)
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            h.multiply(v)
+        per.append(1e6 * (time.perf_counter() - t0) / iters)
+    out = {"workload": "sd 4p4n J=0..8, one REAL*4 host-vector H*v per instance in turn",
+           "us_per_instance": per, "matvecs_per_s": len(hams) / (1e-6 * sum(per))}
+    log(f"host REAL*4 path: {out['matvecs_per_s']:.0f} H*v/s ({min(per):.0f}-{max(per):.0f} us per instance)")
+    return out
+
+
 def deterministic(log, hams, ms_default):
     """The headline batch planned for a bitwise reproducible H*v (the fused
     column path: X and Y in LDS, column partials summed in plan order): its
@@ -394,6 +412,12 @@ def large(log, steps: int = 5, cpu: bool = True):
     ms = 1e3 * (time.perf_counter() - t0) / steps
     out = {"workload": "pf 6p6n J=0, N=%d, one FP64 H*v" % h.N, "files_s": t_files, "setup_s": t_setup,
            "hv_ms": ms, "flops_ref": flops, "TFLOPs_ref": flops / (ms * 1e-3) / 1e12}
+    v32 = v.astype(np.float32)
+    h.multiply(v32)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.multiply(v32)
+    out["host_f32_ms"] = 1e3 * (time.perf_counter() - t0) / steps  # REAL*4 host vectors, PCIe included
     log(f"large: pf 6p6n H*v {ms:.1f} ms ({out['TFLOPs_ref']:.2f} TFLOP/s of the reference's sparse work)")
     din.free()
     dout.free()
@@ -577,6 +601,7 @@ def main():
     din.free()
     dout.free()
     batch.close()
+    out["host_f32"] = host_f32(log, hams)
     if not args.no_det:
         out["deterministic"] = deterministic(log, hams, ms_hv)
     if not args.no_t5:

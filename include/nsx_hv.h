@@ -186,6 +186,24 @@ int nsx_trl_solve(nsx_hv *h, const nsx_trl_opts *opt, const double *start, doubl
  * Works on host-only handles too. */
 int nsx_hv_write_vectors(const nsx_hv *h, const char *prefix, int32_t nvec, const double *evals,
                          const double *vecs, int32_t no_level, float tt);
+/* The thick-restart layout of the same states for NuVec: <prefix>.trl (direct
+ * access, one REAL*4 record of N values per vector) and the four-record
+ * <prefix>.abz that TRL.Lnz_OP writes (src/NuTRL/NuCoreX.f90:526-534):
+ * isave, alpha, betat, matrix(isave, isave) / imax, no_level, tt, no_state,
+ * basd, count, no_keep, no_conv, first_iter, second_iter, noiter / bas /
+ * alpha_copy(noiter), betat(isave+1), indx(noiter).  The Ritz vectors are the
+ * records and matrix = diag(Ritz values), count = 2 (a converged restart
+ * state), which NuVec's tred2 + tqli path (src/NuVec/NuVec.f90:181-190) returns
+ * exactly.  nvec <= 300. */
+int nsx_hv_write_trl(const nsx_hv *h, const char *prefix, int32_t nvec, const double *evals,
+                     const double *vecs, int32_t no_level, float tt);
+/* NuVec's program step (Vec_OP, src/NuVec/NuVec.f90:99-296): reads <prefix>.abz
+ * and <prefix>.lnz or, failing that, <prefix>.trl; diagonalises the projected
+ * matrix; forms the lowest min(no_level, no_state, isave) states on the device;
+ * writes <prefix>.xvc, .xva (energy, average shell occupations of both sides,
+ * percentage per (a partition, b partition, 2Ja, 2Jb)), .lev and .lp.  nlev_out
+ * (may be NULL) receives the number of states. */
+int nsx_nuvec(nsx_hv *h, const char *prefix, int32_t *nlev_out);
 /* <prefix>.xvc, the state file NuVec writes (src/NuVec/NuVec.f90:166,200,291) and
  * NuXpct reads: record (no_state, nvec), then per state (energy, vector), REAL*4. */
 int nsx_hv_write_xvc(const nsx_hv *h, const char *prefix, int32_t nvec, const double *evals,

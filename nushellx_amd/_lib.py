@@ -23,7 +23,7 @@ EXPORTS = [
     "nsx_batch_work", "nsx_batch_apply_dev", "nsx_trl_solve", "nsx_dev_alloc", "nsx_dev_free",
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
     "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_path", "nsx_batch_trl_solve", "nsx_dev_copy", "nsx_batch_create_config", "nsx_hv_write_vectors",
-    "nsx_hv_create_sep", "nsx_hv_write_xvc", "nsx_hv_expect", "nsx_hv_expect_noalloc", "nsx_hv_xpct_files", "nsx_hv_trunc",
+    "nsx_hv_create_sep", "nsx_hv_write_xvc", "nsx_hv_write_trl", "nsx_nuvec", "nsx_hv_expect", "nsx_hv_expect_noalloc", "nsx_hv_xpct_files", "nsx_hv_trunc",
     "nsx_mlp_create", "nsx_mlp_destroy", "nsx_mlp_num_params", "nsx_mlp_get_params",
     "nsx_mlp_set_params", "nsx_mlp_grad", "nsx_mlp_train", "nsx_mlp_predict",
     "nsx_last_error",
@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
     L.nsx_hv_trunc.argtypes = [vp, vp]
     L.nsx_hv_create_sep.argtypes = [C.c_char_p] * 5 + [C.c_int, C.POINTER(vp)]
     L.nsx_hv_write_xvc.argtypes = [vp, C.c_char_p, i32, vp, vp]
+    L.nsx_hv_write_trl.argtypes = [vp, C.c_char_p, i32, vp, vp, i32, C.c_float]
+    L.nsx_nuvec.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
     L.nsx_hv_expect.argtypes = [vp, i32, vp, vp]
     L.nsx_hv_expect_noalloc.argtypes = [vp, i32, vp, vp]
     L.nsx_hv_xpct_files.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(i32)]

@@ -182,6 +182,16 @@ static std::string trimmed(const char* s) {
     }                                                                        \
   } while (0)
 
+__global__ __launch_bounds__(256) void k_widen(const float* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+
+__global__ __launch_bounds__(256) void k_narrow(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
 extern "C" {
 
 const char* nsx_last_error(void) { return nsx::g_err.c_str(); }
@@ -217,6 +227,8 @@ static int create_impl(const char* nus_prefix, const char* opfile_a, const char*
   const int64_t N = std::max<int64_t>(h->model.N, 1);
   NSX_HIP(hipMalloc(&h->d_in, sizeof(double) * N));
   NSX_HIP(hipMalloc(&h->d_out, sizeof(double) * N));
+  NSX_HIP(hipMalloc(&h->d_in32, sizeof(float) * N));
+  NSX_HIP(hipMalloc(&h->d_out32, sizeof(float) * N));
   *out = h.release();
   return NSX_OK;
 }
@@ -246,6 +258,8 @@ void nsx_hv_destroy(nsx_hv* h) {
   free_lowp(h->lowp);
   (void)hipFree(h->d_in);
   (void)hipFree(h->d_out);
+  (void)hipFree(h->d_in32);
+  (void)hipFree(h->d_out32);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -316,12 +330,23 @@ int nsx_hv_apply_f64(nsx_hv* h, const double* v_in, double* v_out) {
   return NSX_OK;
 }
 
+// The REAL*4 drop-in: the 4-byte vectors cross PCIe as they are and are
+// widened to FP64 (and the result narrowed) on the device.
 int nsx_hv_apply_f32(nsx_hv* h, const float* v_in, float* v_out) {
   if (!h || (h->model.N && (!v_in || !v_out))) { set_error("null argument"); return NSX_EARG; }
-  std::vector<double> a(v_in, v_in + h->model.N), b(h->model.N);
-  int rc = nsx_hv_apply_f64(h, a.data(), b.data());
+  if (h->host_only) { set_error("host-only handle: apply it through a batch"); return NSX_EARG; }
+  const int64_t N = h->model.N;
+  if (!N) return NSX_OK;
+  const dim3 g((unsigned)((N + 255) / 256));
+  NSX_HIP(hipMemcpyAsync(h->d_in32, v_in, sizeof(float) * N, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_widen, g, dim3(256), 0, h->stream, h->d_in32, h->d_in, N);
+  NSX_HIP(hipGetLastError());
+  int rc = nsx_hv_apply_dev(h, h->d_in, h->d_out, h->stream);
   if (rc) return rc;
-  for (int64_t i = 0; i < h->model.N; ++i) v_out[i] = (float)b[i];
+  hipLaunchKernelGGL(k_narrow, g, dim3(256), 0, h->stream, h->d_out, h->d_out32, N);
+  NSX_HIP(hipGetLastError());
+  NSX_HIP(hipMemcpyAsync(v_out, h->d_out32, sizeof(float) * N, hipMemcpyDeviceToHost, h->stream));
+  NSX_HIP(hipStreamSynchronize(h->stream));
   return NSX_OK;
 }
 

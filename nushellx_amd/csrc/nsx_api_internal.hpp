@@ -1,5 +1,6 @@
 // Handle types behind the C ABI.
 #pragma once
+#include <cstdio>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -26,6 +27,8 @@ struct nsx_hv {
   hipStream_t stream = nullptr;
   double* d_in = nullptr;
   double* d_out = nullptr;
+  float* d_in32 = nullptr;   // REAL*4 vectors of the drop-in call (nsx_hv_apply_f32)
+  float* d_out32 = nullptr;
 };
 
 struct nsx_batch {
@@ -40,6 +43,9 @@ struct nsx_batch {
 
 namespace nsx {
 void set_error(const std::string& e);
+// Fortran sequential unformatted records (4-byte length markers)
+bool write_record(FILE* f, const std::vector<unsigned char>& payload);
+bool read_records(const std::string& path, std::vector<std::vector<unsigned char>>& out);
 void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int precision);
 PlanConfig plan_config(const nsx_tune_result& r);
 // Rebuilds and uploads the plan of `models` with `cfg`; keeps kernel choices of `d`.

@@ -24,28 +24,14 @@
 #include "../../include/nsx_hv.h"
 #include "nsx_api_internal.hpp"
 
-namespace {
-
-struct File {
-  FILE* f = nullptr;
-  explicit File(const std::string& p) : f(std::fopen(p.c_str(), "wb")) {}
-  ~File() {
-    if (f) std::fclose(f);
-  }
-};
+namespace nsx {
 
 // one Fortran sequential record: length marker, payload, length marker
-bool record(FILE* f, const std::vector<unsigned char>& payload) {
+bool write_record(FILE* f, const std::vector<unsigned char>& payload) {
   const int32_t n = (int32_t)payload.size();
   return std::fwrite(&n, 4, 1, f) == 1 &&
          (payload.empty() || std::fwrite(payload.data(), 1, payload.size(), f) == payload.size()) &&
          std::fwrite(&n, 4, 1, f) == 1;
-}
-
-template <typename T>
-void put(std::vector<unsigned char>& b, T v) {
-  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
-  b.insert(b.end(), p, p + sizeof(T));
 }
 
 // Fortran sequential unformatted records of a whole file
@@ -64,6 +50,28 @@ bool read_records(const std::string& path, std::vector<std::vector<unsigned char
     pos += 8 + size_t(n);
   }
   return pos == d.size();
+}
+
+}  // namespace nsx
+
+namespace {
+
+using nsx::read_records;
+
+struct File {
+  FILE* f = nullptr;
+  explicit File(const std::string& p) : f(std::fopen(p.c_str(), "wb")) {}
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+
+bool record(FILE* f, const std::vector<unsigned char>& payload) { return nsx::write_record(f, payload); }
+
+template <typename T>
+void put(std::vector<unsigned char>& b, T v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  b.insert(b.end(), p, p + sizeof(T));
 }
 
 }  // namespace
@@ -103,6 +111,64 @@ extern "C" int nsx_hv_write_vectors(const nsx_hv* h, const char* prefix, int32_t
     r3.insert(r3.end(), key, key + 8);
   }
   if (!record(f.f, r1) || !record(f.f, r2) || !record(f.f, r3)) {
+    set_error("write failed: " + base + ".abz");
+    return NSX_EIO;
+  }
+  return NSX_OK;
+}
+
+// The thick-restart layout NuTRL leaves behind (src/NuTRL/NuCoreX.f90:526-534,
+// records of .trl as :553-555), read by NuVec (src/NuVec/NuVec.f90:124-130,
+// 156-157, 174-190):
+//   .abz  rec 1: isave, alpha(1:isave), betat(1:isave), matrix(1:isave,1:isave)
+//         rec 2: imax, no_level, tt, no_state, basd, count, no_keep, no_conv,
+//                first_iter, second_iter, noiter
+//         rec 3: bas(1:basd)
+//         rec 4: alpha_copy(1:noiter), betat(isave+1), indx(1:noiter)
+//   .trl  direct access, record r = vector r (no_state REAL*4, no markers)
+// The records are the converged Ritz vectors and the projected matrix is
+// diag(Ritz values) with count = 2, the state after a thick restart whose
+// arrow couplings have converged: NuVec's tred2/tqli path then returns exactly
+// these values and vectors.
+extern "C" int nsx_hv_write_trl(const nsx_hv* h, const char* prefix, int32_t nvec, const double* evals,
+                                const double* vecs, int32_t no_level, float tt) {
+  using nsx::set_error;
+  constexpr int32_t kNoiter = 300, kFirstIterT = 80, kIncIter = 3;  // src/NuCommon/NuCommon.f90:111-124
+  if (!h || !prefix || !evals || !vecs || nvec < 1 || nvec > kNoiter) { set_error("bad arguments"); return NSX_EARG; }
+  const nsx::Model& m = h->model;
+  const std::string base(prefix);
+  {
+    File f(base + ".trl");
+    if (!f.f) { set_error("cannot open " + base + ".trl for writing"); return NSX_EIO; }
+    std::vector<float> rec(size_t(m.N));
+    for (int q = 0; q < nvec; ++q) {
+      const double* v = vecs + size_t(q) * m.N;
+      for (int64_t i = 0; i < m.N; ++i) rec[i] = (float)v[i];
+      if (std::fwrite(rec.data(), sizeof(float), rec.size(), f.f) != rec.size()) {
+        set_error("write failed: " + base + ".trl");
+        return NSX_EIO;
+      }
+    }
+  }
+  File f(base + ".abz");
+  if (!f.f) { set_error("cannot open " + base + ".abz for writing"); return NSX_EIO; }
+  std::vector<unsigned char> r1, r2, r3, r4;
+  put<int32_t>(r1, nvec);
+  for (int q = 0; q < nvec; ++q) put<float>(r1, (float)evals[q]);
+  for (int q = 0; q < nvec; ++q) put<float>(r1, 0.0f);
+  for (int c = 0; c < nvec; ++c)
+    for (int r = 0; r < nvec; ++r) put<float>(r1, r == c ? (float)evals[c] : 0.0f);
+  for (int32_t x : {nvec, no_level}) put<int32_t>(r2, x);
+  put<float>(r2, tt);
+  for (int32_t x : {(int32_t)m.N, (int32_t)m.basd, 2, nvec, nvec, kFirstIterT, kIncIter, kNoiter}) put<int32_t>(r2, x);
+  for (int b = 0; b < m.basd; ++b) {
+    unsigned char key[8] = {(unsigned char)m.bas_a[b], (unsigned char)m.bas_b[b], 0, 0, 0, 0, 0, 0};
+    r3.insert(r3.end(), key, key + 8);
+  }
+  for (int q = 0; q < kNoiter; ++q) put<float>(r4, q < nvec ? (float)evals[q] : 0.0f);
+  put<float>(r4, 0.0f);
+  for (int q = 0; q < kNoiter; ++q) put<int32_t>(r4, q + 1);
+  if (!record(f.f, r1) || !record(f.f, r2) || !record(f.f, r3) || !record(f.f, r4)) {
     set_error("write failed: " + base + ".abz");
     return NSX_EIO;
   }

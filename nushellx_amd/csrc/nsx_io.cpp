@@ -81,6 +81,8 @@ std::string read_nus(const std::string& prefix, NusInput& n) {
   size_t i = 4;
   auto need = [&](size_t k) { return lines.size() > k; };
   if (!need(10)) return prefix + ".nus: too short";
+  n.nucleus = lines[0];
+  for (int k = 0; k < 3; ++k) n.desc[k] = lines[1 + k];
   std::istringstream(lines[i++]) >> n.no_cI >> n.no_cN;
   std::istringstream(lines[i++]) >> n.tparity;
   std::istringstream(lines[i++]) >> n.no_shells;
@@ -141,11 +143,13 @@ std::string read_side(const std::string& opfile, const NusInput& nus, SideFiles&
   const int ptdim = len / (2 * s.nsp);
   s.parity.assign(s.nsp, 1);
   s.nq.assign(s.nsp, 0);
+  s.occ.assign(size_t(s.nsp) * nus.no_shells, 0);
   for (int q = 0; q < s.nsp; ++q) {  // cParity, src/NuCommon/NuCommon.f90:2918-2937
     int par = 1, nq = 0;
     for (int k = 0; k < nus.no_shells && k < ptdim; ++k) {
       int16_t occ;
       std::memcpy(&occ, p + 2 * (size_t(q) * ptdim + k), 2);
+      s.occ[size_t(q) * nus.no_shells + k] = occ;
       for (int t = 0; t < occ; ++t) par *= nus.sp[k];
       nq += (2 * nus.sn[k] + nus.sl[k]) * occ;  // cNN, src/NuTRL/NuCoreX.f90:814-817
     }

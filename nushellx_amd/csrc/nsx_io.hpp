@@ -12,6 +12,7 @@
 namespace nsx {
 
 struct NusInput {                     // src/NuCommon/NuCommon.f90:2546-2736
+  std::string nucleus, desc[3];       // the name and the three description lines
   int no_cI = 0, no_cN = 0, tparity = 0, no_shells = 0, np_shells = 0;
   std::vector<int> sj2, sp;           // per shell: 2j, parity
   std::vector<int> sn, sl;            // per shell: radial n and l (oscillator quanta 2n + l)
@@ -33,6 +34,7 @@ struct SideFiles {
   std::vector<int> nJT;               // [(J2/2) * nsp + p]
   std::vector<int> parity;            // per partition
   std::vector<int> nq;                // per partition: sum over shells of (2n + l) * occupation
+  std::vector<int16_t> occ;           // [p * no_shells + shell]: occupation of each shell (all shells)
   std::vector<Density> dens;          // sorted by key
   std::vector<double> vals;           // density values (file REAL*4 or REAL*8, widened)
   // like-particle Hamiltonian per J: dense d(J) x d(J), symmetric, empty if absent

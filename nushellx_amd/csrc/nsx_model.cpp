@@ -257,6 +257,7 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
                         const std::string& dopfile) {
   std::string err;
   if (!(err = read_nus(nus_prefix, m.nus)).empty()) return err;
+  m.intfile = intfile;
   const NusInput& nus = m.nus;
   if (!(err = read_side(opfile_a, nus, m.A)).empty()) return err;
   if (!(err = read_side(opfile_b, nus, m.B)).empty()) return err;

@@ -43,6 +43,7 @@ struct Item {   // one (L, Ja') work item
 
 struct Model {
   NusInput nus;
+  std::string intfile;                     // interaction name as given (NuVec prints it)
   SideFiles A, B;
   int J2 = 0, basd = 0, no_level = 5;
   double renorm = 1.0;

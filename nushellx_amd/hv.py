@@ -90,6 +90,20 @@ class Hamiltonian:
         check(lib().nsx_hv_write_vectors(self._h, prefix.encode(), len(evals), evals.ctypes.data,
                                          vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_vectors")
 
+    def write_trl(self, prefix: str, evals, vecs, no_level: int = 5, tt: float = 0.0) -> None:
+        """<prefix>.trl / <prefix>.abz in the thick-restart layout NuVec reads (vecs: nvec x N)."""
+        vecs = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+        evals = np.ascontiguousarray(evals, dtype=np.float64)
+        assert vecs.shape == (len(evals), self.N)
+        check(lib().nsx_hv_write_trl(self._h, prefix.encode(), len(evals), evals.ctypes.data,
+                                     vecs.ctypes.data, no_level, C.c_float(tt)), "nsx_hv_write_trl")
+
+    def nuvec(self, prefix: str) -> int:
+        """NuVec's step: <prefix>.abz + .lnz/.trl in, .xvc/.xva/.lev/.lp out; returns the state count."""
+        n = C.c_int32()
+        check(lib().nsx_nuvec(self._h, prefix.encode(), C.byref(n)), "nsx_nuvec")
+        return int(n.value)
+
     def trunc(self) -> Tuple[bool, np.ndarray]:
         """(truncated, mask): the states a solve spans (the N-hbar-omega window
         mask on a truncated basis, else the parity mask)."""

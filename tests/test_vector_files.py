@@ -32,3 +32,30 @@ def test_write_vectors_round_trip_through_nuvec(models):
     for q in range(3):
         assert np.abs(np.abs(X[q] @ vecs[q]) - 1.0) < 1e-6
         assert np.abs(X[q] - vecs[q]).max() < 1e-6  # diagonal tridiagonal: no sign flip, REAL*4 rounding
+
+
+def test_write_trl_round_trip_through_nuvec(models):
+    """The thick-restart layout (.trl + four-record .abz) read by the oracle's
+    restatement of NuVec's TRL branch: count = 2, the projected matrix, the
+    fourth record, and the states it rebuilds."""
+    from oracle.nuvec import nuvec_states_any, read_abz
+    from nushellx_amd.hv import Hamiltonian
+    mf = models["sd44"]
+    h = Hamiltonian(*mf.args(2), device=False)
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((h.N, 6)))
+    vecs = Q.T
+    evals = np.array([-30.5, -29.0, -28.25, -27.0, -26.5, -25.0])
+    prefix = os.path.join(tempfile.mkdtemp(), "sd44" + "1")
+    h.write_trl(prefix, evals, vecs, no_level=5, tt=2.0)
+    assert not os.path.exists(prefix + ".lnz")
+    abz = read_abz(prefix)
+    assert abz["trl"] and abz["count"] == 2 and abz["isave"] == 6 and abz["no_level"] == 5
+    assert abz["no_state"] == h.N and abz["basd"] == h.basd
+    assert np.allclose(abz["matrix"], np.diag(evals), atol=1e-5)
+    assert len(abz["rec4"]) == 4 * 300 + 4 + 4 * 300
+    assert os.path.getsize(prefix + ".trl") == 6 * 4 * h.N
+    E, X = nuvec_states_any(prefix)
+    assert np.allclose(E, evals[:5], atol=1e-5)
+    for q in range(5):
+        assert np.abs(np.abs(X[q] @ vecs[q]) - 1.0) < 1e-6
