@@ -174,25 +174,34 @@ def time_to_5(log, train_samples: int = 256, epochs: int = 200, repeats: int = 3
 def ensemble(log, samples: int = 256):
     """Batched ensemble: `samples` perturbed interactions (V_0 + N(0, (0.1 |V_0|)^2))
     x J = 0, 2, 4 x sd 4p4n, 5p5n, 6p6n, all instances in one lockstep batch,
-    lowest 5 states each to residual 1e-4 MeV; aggregate eigenpairs/s."""
-    from nushellx_amd.ensemble import build_instances, solve_ensemble, write_ensemble
-    from nushellx_amd.hv import DeviceBuffer
+    lowest 5 states each to residual 1e-4 MeV; aggregate eigenpairs/s.  A linear
+    ensemble: one template per (nucleus, J) from the base interaction's files,
+    every sample's operators made on the device from its parameters."""
+    from nushellx_amd.ensemble import ensemble_templates, linear_ensemble
+    from nushellx_amd.hv import Batch, DeviceBuffer
     from nushellx_amd.synth.interaction import random_interaction
     from nushellx_amd.synth.mspace import SD, PNSpace
     J2s, nuclei = [0, 4, 8], [(4, 4), (5, 5), (6, 6)]
-    t0 = time.perf_counter()
-    entries = write_ensemble(PNSpace(SD), nuclei, J2s, random_interaction(SD, seed=SEED),
-                             range(1000, 1000 + samples), tempfile.mkdtemp(prefix="nsx_ens_"), log=log)
-    t_files = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    hams = build_instances(entries, J2s, threads=16)
-    t_inst = time.perf_counter() - t0
-    tune_on = build_instances(entries[:len(nuclei)], J2s)
+    space, base = PNSpace(SD), random_interaction(SD, seed=SEED)
+    seeds = range(1000, 1000 + samples)
+    templates = ensemble_templates(space, nuclei, J2s, base, tempfile.mkdtemp(prefix="nsx_ens_"))
+    tmpl = templates[0]
     out = {"workload": f"{samples} samples x J=0,2,4 x sd 4p4n/5p5n/6p6n, lowest 5 each, residual 1e-4 MeV",
-           "instances": len(hams), "files_s": t_files, "instances_s": t_inst}
+           "instances": samples * len(tmpl), "mode": "linear ensemble (operators made on the device)",
+           "template_files_s": templates[2]["template_files_s"]}
     for prec in ("fp64", "fp32"):
-        r = solve_ensemble(hams, nev=5, tol=1e-4, precision=prec, tune_on=tune_on, log=log)
-        b = r["batch"]
+        t0 = time.perf_counter()  # tuning: the templates' own batch; its stacked shape is used
+        small = Batch(tmpl, precision=prec)
+        tuned = small.autotune(iters=5)
+        small.close()
+        t_tune = time.perf_counter() - t0
+        cfg = tuned if tuned.get("stacked") == 1 else None
+        b, _, info = linear_ensemble(space, nuclei, J2s, base, seeds, "", precision=prec, config=cfg,
+                                     templates=templates, log=log)
+        setup = info["templates_s"] + info["theta_s"] + info["batch_s"]
+        t0 = time.perf_counter()
+        r = b.lanczos(nev=5, tol=1e-4)
+        t_solve = time.perf_counter() - t0
         flops = b.work()[0]
         din, dout = DeviceBuffer(8 * b.N), DeviceBuffer(8 * b.N)
         din.upload(np.random.default_rng(0).standard_normal(b.N))
@@ -200,13 +209,15 @@ def ensemble(log, samples: int = 256):
         din.free()
         dout.free()
         out["N_total"] = int(b.N)
-        out[prec] = {"eigenpairs_per_s": r["eigenpairs_per_s"], "solve_s": r["t_solve"],
-                     "matvecs": r["matvecs"], "converged": int((r["converged_at"] >= 0).sum()),
-                     "plan_s": r["t_plan"], "tune_s": r["t_tune"], "host_s": r["host_seconds"],
-                     "hv_ms": ms_hv, "hv_TFLOPs": flops / (ms_hv * 1e-3) / 1e12, "tuned": r["tuned"]}
-        log(f"ensemble {prec}: {r['eigenpairs_per_s']:.0f} eigenpairs/s")
+        n_inst = len(tmpl) * samples
+        out[prec] = {"eigenpairs_per_s": n_inst * 5 / t_solve, "solve_s": t_solve, "matvecs": r["matvecs"],
+                     "converged": int((r["converged_at"] >= 0).sum()), "setup_s": setup,
+                     "templates_s": info["templates_s"], "theta_s": info["theta_s"], "batch_s": info["batch_s"],
+                     "tune_s": t_tune, "host_s": r["host_seconds"], "hv_ms": ms_hv,
+                     "hv_TFLOPs": flops / (ms_hv * 1e-3) / 1e12, "tuned": cfg}
+        log(f"ensemble {prec}: {n_inst * 5 / t_solve:.0f} eigenpairs/s, setup {setup:.2f} s, solve {t_solve:.2f} s")
         b.close()
-    for h in hams + tune_on:
+    for h in tmpl:
         h.close()
     return out
 

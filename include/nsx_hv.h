@@ -37,6 +37,12 @@ enum {
  * used through nsx_batch_create; its own apply/solve calls fail with NSX_EARG.
  * For ensembles of many instances, whose operators are planned once per batch. */
 #define NSX_HOST_ONLY 0x100
+/* Flag or-ed into the precision of nsx_hv_create: a template of a linear
+ * ensemble (nsx_ens_create).  Its pn term structure keeps every key of the
+ * .oph file, zero-valued ones too, so that any interaction with the same keys
+ * fits it, and it records how each folded b-operator is made from the .oph
+ * values.  Usually combined with NSX_HOST_ONLY. */
+#define NSX_TEMPLATE 0x200
 
 typedef struct nsx_hv nsx_hv;
 
@@ -258,6 +264,42 @@ int nsx_device_info(char *name, int32_t len, int32_t *cus);
  * instance i's rows = its q-th Ritz vector (same layout as the start vector). */
 int nsx_batch_trl_solve(nsx_batch *b, const nsx_trl_opts *opt, const double *start, double *evals,
                         double *evecs, double *residuals, int32_t *converged_at, nsx_trl_stats *stats);
+
+/* ---------------------------------------------------------------- linear ensembles
+ * Many interaction samples theta_s (s < S) of the same nuclei and J values as
+ * one batch whose operators are made on the device.  Replaces writing and
+ * reading one operator file set per sample: the .mat like-particle matrices
+ * (NuOrth, src/NuOrth/NuOrth.f90:206-218) and the .oph pn multipoles (NuOp,
+ * src/NuOp/NuOp.f90:352-363) are linear in the P interaction parameters, the
+ * densities (.tri/.trd) not at all.  So each template (a handle created with
+ * NSX_TEMPLATE from the files of one nucleus and J) takes a basis, the values
+ * of its files for each unit parameter vector, and sample s gets
+ *   values_s = sum_i theta[s*P + i] basis_t[i*K_t + k]
+ * in FP64 on the device, from which its folded b-operators and H(J) blocks are
+ * built (.oph values below 1e-10 count as zero, as at load).  The batch then
+ * holds n_t*S instances, sample-major (instance s*n_t + t is template t with
+ * sample s) and works with nsx_batch_apply_dev, nsx_batch_trl_solve,
+ * nsx_batch_time and the other batch calls; autotune and set_config are
+ * refused (pass a tuning result at creation).  It runs on the stacked path
+ * (block sides up to 40).  No counterpart in the reference, which reads one
+ * file set per interaction.
+ *
+ * Value layout of a template (K_t = n_oph + sum d^2): the .oph record values in
+ * file order (the raw me, before renorm), then per H block (side a, then side b,
+ * 2J ascending) its d x d matrix column-major.  nsx_ens_layout gives n_oph and,
+ * per H block, (side 0/1, 2J, d) in h_desc (3 ints each, up to cap blocks).
+ * nsx_ens_values writes the template's own file values in that layout. */
+int nsx_ens_layout(const nsx_hv *h, int64_t *n_oph, int32_t *n_h, int32_t *h_desc, int32_t cap);
+int nsx_ens_values(const nsx_hv *h, double *values);
+/* theta: S x P row-major; basis[t]: P x K_t row-major; cfg: optional tuning
+ * result (its stacked-path shape is used). */
+int nsx_ens_create(nsx_hv *const *templates, int32_t n_t, int32_t S, int32_t P, const double *theta,
+                   const double *const *basis, int precision, const nsx_tune_result *cfg, nsx_batch **b);
+/* New parameters for the same S samples (S x P); the operators are rebuilt on
+ * the device, the plan is kept. */
+int nsx_ens_update(nsx_batch *b, const double *theta);
+/* S samples of n_t templates (S = 0 for a batch of separate instances). */
+int nsx_ens_samples(const nsx_batch *b, int32_t *S, int32_t *n_t);
 
 /* ---------------------------------------------------------------- learned warm start
  * No counterpart in the reference, whose Lanczos always starts from a random

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libnsx.so")
 
 NSX_FP64, NSX_FP32, NSX_BF16X3 = 0, 1, 2
 NSX_HOST_ONLY = 0x100
+NSX_TEMPLATE = 0x200
 PRECISIONS = {"fp64": NSX_FP64, "fp32": NSX_FP32, "bf16x3": NSX_BF16X3}
 
 # every symbol include/nsx_hv.h declares
@@ -24,6 +25,7 @@ EXPORTS = [
     "nsx_dev_upload", "nsx_dev_download", "nsx_dev_sync", "nsx_batch_time", "nsx_device_info",
     "nsx_hv_autotune", "nsx_batch_autotune", "nsx_batch_set_config", "nsx_batch_path", "nsx_batch_trl_solve", "nsx_dev_copy", "nsx_batch_create_config", "nsx_hv_write_vectors",
     "nsx_hv_create_sep", "nsx_hv_write_xvc", "nsx_hv_write_trl", "nsx_nuvec", "nsx_hv_expect", "nsx_hv_expect_noalloc", "nsx_hv_xpct_files", "nsx_hv_trunc",
+    "nsx_ens_layout", "nsx_ens_values", "nsx_ens_create", "nsx_ens_update", "nsx_ens_samples",
     "nsx_mlp_create", "nsx_mlp_destroy", "nsx_mlp_num_params", "nsx_mlp_get_params",
     "nsx_mlp_set_params", "nsx_mlp_grad", "nsx_mlp_train", "nsx_mlp_predict",
     "nsx_last_error",
@@ -106,6 +108,12 @@ def lib() -> C.CDLL:
     L.nsx_hv_xpct_files.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(i32)]
     L.nsx_dev_copy.argtypes = [vp, vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.nsx_batch_trl_solve.argtypes = [vp, C.POINTER(TrlOpts), vp, vp, vp, vp, vp, C.POINTER(TrlStats)]
+    L.nsx_ens_layout.argtypes = [vp, C.POINTER(i64), C.POINTER(i32), vp, i32]
+    L.nsx_ens_values.argtypes = [vp, vp]
+    L.nsx_ens_create.argtypes = [C.POINTER(vp), i32, i32, i32, vp, C.POINTER(vp), C.c_int, C.POINTER(TuneResult),
+                                 C.POINTER(vp)]
+    L.nsx_ens_update.argtypes = [vp, vp]
+    L.nsx_ens_samples.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     L.nsx_mlp_create.argtypes = [i32, i32, i32, i32, C.c_uint64, C.POINTER(vp)]
     L.nsx_mlp_destroy.argtypes = [vp]
     L.nsx_mlp_destroy.restype = None

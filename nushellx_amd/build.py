@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnsx.so")
 SOURCES = ["nsx_io.cpp", "nsx_model.cpp", "nsx_plan.cpp", "nsx_kernels.hip", "nsx_lowp.hip",
-           "nsx_trl.hip", "nsx_mlp.hip", "nsx_api.hip", "nsx_files.cpp", "nsx_xpct.hip", "nsx_nuvec.hip"]
+           "nsx_trl.hip", "nsx_mlp.hip", "nsx_api.hip", "nsx_files.cpp", "nsx_xpct.hip", "nsx_nuvec.hip",
+           "nsx_ens.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]

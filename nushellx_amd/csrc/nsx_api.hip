@@ -204,13 +204,15 @@ static int create_impl(const char* nus_prefix, const char* opfile_a, const char*
   if (!nus_prefix || !opfile_a || !opfile_b || !intfile) { set_error("null file name"); return NSX_EARG; }
   *out = nullptr;
   const bool host_only = (precision & NSX_HOST_ONLY) != 0;
-  precision &= ~NSX_HOST_ONLY;
+  const bool templ = (precision & NSX_TEMPLATE) != 0;
+  precision &= ~(NSX_HOST_ONLY | NSX_TEMPLATE);
+  if (templ && dopfile) { set_error("templates of linear ensembles take .oph interactions"); return NSX_EARG; }
   if (precision < NSX_FP64 || precision > NSX_BF16X3) { set_error("unknown precision"); return NSX_EARG; }
   std::unique_ptr<nsx_hv> h(new nsx_hv());
   h->precision = precision;
   h->host_only = host_only;
   std::string e = build_model(trimmed(nus_prefix), trimmed(opfile_a), trimmed(opfile_b),
-                              trimmed(intfile), h->model, dopfile ? trimmed(dopfile) : std::string());
+                              trimmed(intfile), h->model, dopfile ? trimmed(dopfile) : std::string(), templ);
   if (!e.empty()) { set_error(e); return code_of(e); }
   if (host_only) {  // operators stay on the host until a batch plans them
     *out = h.release();
@@ -390,6 +392,7 @@ void nsx_batch_destroy(nsx_batch* b) {
   free_trl_work(b->trl);
   free_plan(b->dplan);
   free_lowp(b->lowp);
+  free_ens(b->ens);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -727,11 +730,13 @@ int nsx_hv_autotune(nsx_hv* h, int32_t iters, nsx_tune_result* best) {
 
 int nsx_batch_autotune(nsx_batch* b, int32_t iters, nsx_tune_result* best) {
   if (!b) { set_error("null batch"); return NSX_EARG; }
+  if (b->ens.S > 0) { set_error("linear ensembles take their configuration at creation"); return NSX_EARG; }
   return autotune(b->models, b->precision, b->hplan, b->dplan, b->lowp, b->stream, iters > 0 ? iters : 10, best);
 }
 
 int nsx_batch_set_config(nsx_batch* b, const nsx_tune_result* cfg) {
   if (!b || !cfg) { set_error("null argument"); return NSX_EARG; }
+  if (b->ens.S > 0) { set_error("linear ensembles take their configuration at creation"); return NSX_EARG; }
   const PlanConfig pc = plan_config(*cfg);
   b->dplan.staged_nt = cfg->threads;
   b->dplan.staged_mode = cfg->mode;

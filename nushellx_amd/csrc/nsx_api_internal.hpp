@@ -31,6 +31,19 @@ struct nsx_hv {
   float* d_out32 = nullptr;
 };
 
+// Linear ensemble of a batch (nsx_ens_create): per sample s the operator
+// values are values[s] = theta[s] . basis (P parameters, K columns: the
+// templates' .oph sections, then the H store), and the Bhat^T and H stores of
+// every sample are produced from them on the device.
+struct EnsData {
+  int S = 0, P = 0;
+  int64_t K = 0, n_o = 0, b_t = 0, h_t = 0;
+  int n_fold = 0;
+  double *theta = nullptr, *basis = nullptr, *values = nullptr, *bsrc = nullptr;
+  nsx::EnsFold* fold = nullptr;
+  nsx::EnsPair* pair = nullptr;
+};
+
 struct nsx_batch {
   std::vector<const nsx::Model*> models;  // owned by the instance handles
   nsx::HostPlan hplan;
@@ -39,6 +52,7 @@ struct nsx_batch {
   int precision = 0;
   nsx::TrlWork* trl = nullptr;
   hipStream_t stream = nullptr;
+  EnsData ens;                            // S = 0: an ordinary batch
 };
 
 namespace nsx {
@@ -48,6 +62,7 @@ bool write_record(FILE* f, const std::vector<unsigned char>& payload);
 bool read_records(const std::string& path, std::vector<std::vector<unsigned char>>& out);
 void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int precision);
 PlanConfig plan_config(const nsx_tune_result& r);
+void free_ens(EnsData& e);
 // Rebuilds and uploads the plan of `models` with `cfg`; keeps kernel choices of `d`.
 std::string replan(const std::vector<const Model*>& models, const PlanConfig& cfg, int precision,
                    HostPlan& hp, DevicePlan& d, LowpPlan& lp);

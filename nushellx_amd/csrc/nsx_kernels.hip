@@ -396,6 +396,20 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
   return (b % kXcds) * per + b / kXcds;
 }
 
+// Linear ensembles launch (workgroups per sample) x (samples rounded up to 8):
+// the dispatch index L = blockIdx.y * gridDim.x + blockIdx.x runs on XCD L % 8,
+// and is mapped so that every workgroup of a sample runs on one XCD, in order
+// (sample s on XCD s % 8), which keeps the sample's vectors and operators in
+// that XCD's L2.  Returns the workgroup index within the sample, or -1 for
+// the padding samples.
+__device__ __forceinline__ int ens_block(const EnsStride& es, int64_t& smp) {
+  const int gx = gridDim.x;
+  const int64_t L = (int64_t)blockIdx.y * gx + blockIdx.x;
+  const int64_t r = L / kXcds;
+  smp = (r / gx) * kXcds + L % kXcds;
+  return smp < es.n ? (int)(r % gx) : -1;
+}
+
 // Occupancy targets: both stages fit 8 workgroups of LDS per CU (2048-double
 // staging buffers) and, told so, the compiler keeps them within 64 VGPRs
 // without spills (stage 1: 60 instead of 114).
@@ -403,21 +417,36 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
 // stage 2's atomics) ride in the same launch: workgroups [0, g_red) take one
 // block each, the rest take stage-1 chunks.  g_red is a multiple of 8, so the
 // XCD mapping of the chunks is unchanged.
+// Linear ensembles run every sample through the same tables: blockIdx.y is the
+// sample, whose vectors, X, Bhat^T and H stores sit at fixed strides (es; all
+// zero for one sample).  Grids along x are multiples of 8, so the XCD of a
+// workgroup still follows blockIdx.x.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_stk1(
     const SChunk1* __restrict__ chunks, int n_chunks, const SUnit1* __restrict__ units, const T* __restrict__ V,
     const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X, const DBlock* __restrict__ blocks,
-    int n_blocks, int g_red, const T* __restrict__ H, double* __restrict__ W, int lds_cap, int skip_large) {
+    int n_blocks, int g_red, const T* __restrict__ H, double* __restrict__ W, int lds_cap, int skip_large,
+    EnsStride es) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   __shared__ SUnit1 su[kStkMeta];
-  if ((int)blockIdx.x < g_red) {
-    if ((int)blockIdx.x < n_blocks)
-      reduce_block<T, NT>(blocks[blockIdx.x], nullptr, nullptr, V, H, W, 0, sm, lds_cap, skip_large);
+  int bx = blockIdx.x;
+  if (es.n > 1) {
+    int64_t smp;
+    bx = ens_block(es, smp);
+    if (bx < 0) return;
+    V += smp * es.v;
+    W += smp * es.v;
+    Bt += smp * es.b;
+    H += smp * es.h;
+    X += smp * es.x;
+  }
+  if (bx < g_red) {
+    if (bx < n_blocks) reduce_block<T, NT>(blocks[bx], nullptr, nullptr, V, H, W, 0, sm, lds_cap, skip_large);
     return;
   }
-  const int ci = xcd_chunk((int)blockIdx.x - g_red, n_chunks);
+  const int ci = es.n > 1 ? bx - g_red : xcd_chunk(bx - g_red, n_chunks);
   if (ci >= n_chunks) return;
   const SChunk1 c = chunks[ci];
   const int nu = c.u_end - c.u_begin;
@@ -488,13 +517,22 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_stk2(const SChunk2* __restrict__ chunks, int n_chunks,
                                              const STerm2* __restrict__ terms, const SXRef* __restrict__ xrefs,
                                              const T* __restrict__ Ad, const typename Prec<T>::acc* __restrict__ X,
-                                             double* __restrict__ W) {
+                                             double* __restrict__ W, EnsStride es) {
   using Acc = typename Prec<T>::acc;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   __shared__ STerm2 st[kStkTerms];
   __shared__ SXRef sx[kStkXref];
-  const int ci = xcd_chunk(blockIdx.x, n_chunks);
+  int ci;
+  if (es.n > 1) {
+    int64_t smp;
+    ci = ens_block(es, smp);
+    if (ci < 0) return;
+    X += smp * es.x;
+    W += smp * es.v;
+  } else {
+    ci = xcd_chunk(blockIdx.x, n_chunks);
+  }
   if (ci >= n_chunks) return;
   const SChunk2 c = chunks[ci];
   const int nt = c.t_end - c.t_begin;
@@ -1553,18 +1591,20 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
       const int skip = big_like_active(d) ? 1 : 0;
       void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X,
                     (void*)&d.blocks,   (void*)&d.n_blocks, (void*)&g_red,    (void*)&Hs, (void*)&W,
-                    (void*)&cap,        (void*)&skip};
-      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3((unsigned)(g_red + g1)), dim3(d.stk_nt),
-                                     a1, sizeof(T) * (size_t)d.s_lds1, s);
+                    (void*)&cap,        (void*)&skip,       (void*)&d.es};
+      const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
+      hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3((unsigned)(g_red + g1), gy),
+                                     dim3(d.stk_nt), a1, sizeof(T) * (size_t)d.s_lds1, s);
       if (e != hipSuccess) return e;
     }
     if (d.n_chunk2 > 0) {
       const Acc* Xc = X;
       void* a2[] = {(void*)&d.s_chunk2, (void*)&d.n_chunk2, (void*)&d.s_term2, (void*)&d.s_xref,
-                    (void*)&Ad,         (void*)&Xc,         (void*)&W};
+                    (void*)&Ad,         (void*)&Xc,         (void*)&W,          (void*)&d.es};
       const unsigned g2 = (unsigned)(((d.n_chunk2 + kXcds - 1) / kXcds) * kXcds);
       const size_t lds2 = std::max(sizeof(T) * (size_t)d.s_lds2, sizeof(Acc) * (size_t)d.stk_nt * kColBlk);
-      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2), dim3(d.stk_nt), a2, lds2, s);
+      const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
+      hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2, gy), dim3(d.stk_nt), a2, lds2, s);
       if (e != hipSuccess) return e;
     }
   } else if (d.n_items2 > 0) {
@@ -1652,6 +1692,16 @@ hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, cons
     case 1: return reduce_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.H, W, s);
     default: return reduce_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s);
   }
+}
+
+hipError_t convert_to(int prec, const double* x, void* y, int64_t n, hipStream_t s) {
+  if (prec == 0 || n == 0) return hipSuccess;
+  const dim3 g((unsigned)((n + 255) / 256));
+  if (prec == 1)
+    hipLaunchKernelGGL(k_convert<float>, g, dim3(256), 0, s, x, (float*)y, n);
+  else
+    hipLaunchKernelGGL(k_convert<Bf16x2>, g, dim3(256), 0, s, x, (Bf16x2*)y, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s) {

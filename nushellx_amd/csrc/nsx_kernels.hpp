@@ -8,6 +8,13 @@
 
 namespace nsx {
 
+// Per-sample element strides of a linear ensemble (vectors, X, Bhat^T store,
+// H store) and its sample count n; all zero for an ordinary plan.
+struct EnsStride {
+  int64_t v = 0, x = 0, b = 0, h = 0;
+  int32_t n = 0, pad = 0;
+};
+
 struct DevicePlan {
   DItem2* items2 = nullptr;
   DPhase* phases = nullptr;
@@ -61,6 +68,9 @@ struct DevicePlan {
   double* f_part = nullptr;
   int n_ftask = 0, n_fblocks = 0, f_lds = 1;
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
+  // linear ensemble: the stacked kernels run ens_s samples (grid y) at strides es
+  int ens_s = 1;
+  EnsStride es;
 };
 
 // Uploads a host plan; returns "" or an error message.
@@ -109,6 +119,8 @@ void free_lowp(LowpPlan& lp);
 hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s);
 hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s);
+// n FP64 values -> FP32 or BF16 hi/lo (prec 1, 2), as the plan's operands are stored
+hipError_t convert_to(int prec, const double* x, void* y, int64_t n, hipStream_t s);
 hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s);
 hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,

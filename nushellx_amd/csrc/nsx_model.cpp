@@ -72,7 +72,7 @@ static inline bool bit(const std::vector<uint32_t>& mask, int words, int col, in
 
 // Dense d(Jl) x d(Jr) matrix of one density: partition blocks in next_ij order
 // (column partition outer, row partition inner), each block column-major.
-static std::vector<double> densify(const SideFiles& s, const Density& dd, int J2l, int J2r) {
+std::vector<double> densify(const SideFiles& s, const Density& dd, int J2l, int J2r) {
   const int dl = s.d(J2l), dr = s.d(J2r);
   std::vector<double> D(size_t(dl) * dr, 0.0);
   int64_t pt = dd.pos;
@@ -254,10 +254,13 @@ static std::string build_separable(Model& m, const std::string& dopfile, char ty
 
 std::string build_model(const std::string& nus_prefix, const std::string& opfile_a,
                         const std::string& opfile_b, const std::string& intfile, Model& m,
-                        const std::string& dopfile) {
+                        const std::string& dopfile, bool keep_zero_ops) {
   std::string err;
+  m.keep_zero_ops = keep_zero_ops;
   if (!(err = read_nus(nus_prefix, m.nus)).empty()) return err;
   m.intfile = intfile;
+  m.opfile_a = opfile_a;
+  m.opfile_b = opfile_b;
   const NusInput& nus = m.nus;
   if (!(err = read_side(opfile_a, nus, m.A)).empty()) return err;
   if (!(err = read_side(opfile_b, nus, m.B)).empty()) return err;
@@ -341,12 +344,14 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
   if (m.separable) return build_separable(m, dopfile, typea, bidx);
   for (const auto& d : A.dens) m.nnz_a += d.count;
   for (const auto& d : B.dens) m.nnz_b += d.count;
-  std::map<std::array<uint8_t, 8>, double> opn;
+  std::map<std::array<uint8_t, 8>, std::pair<double, int>> opn;  // key -> (renorm * me, record index)
+  m.oph_me.clear();
   for (auto& e : oph) {
     std::array<uint8_t, 8> k;
     std::memcpy(k.data(), e.key, 8);
     double v = std::fabs(e.me) < 1e-10 ? 0.0 : e.me;
-    opn.emplace(k, m.renorm * v);
+    opn.emplace(k, std::make_pair(m.renorm * v, (int)m.oph_me.size()));
+    m.oph_me.push_back(e.me);
   }
   int max_sj2u = 0;
   for (int s : nus.sj2) max_sj2u = std::max(max_sj2u, s);
@@ -396,6 +401,7 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
               const int R = bi->second;
               // b-densities of this (Jb, Jb', lam) coupled to (sa, sa')
               std::vector<std::pair<int, double>> bl;
+              std::vector<std::pair<int, int>> src;  // (b-density, .oph record)
               for (int bshl = sb0; bshl <= sb1; ++bshl)
                 for (int bshr = sb0; bshr <= sb1; ++bshr) {
                   const int k1 = nus.sj2[bshl - 1], k2 = nus.sj2[bshr - 1];
@@ -406,8 +412,9 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
                   if (btsc < 0) continue;
                   std::array<uint8_t, 8> op = {(uint8_t)ashl, (uint8_t)ashr, (uint8_t)bshl, (uint8_t)bshr, (uint8_t)lm, 0, 0, 0};
                   auto oi = opn.find(op);
-                  if (oi == opn.end() || oi->second == 0.0) continue;
-                  bl.push_back({btsc, oi->second});
+                  if (oi == opn.end() || (oi->second.first == 0.0 && !keep_zero_ops)) continue;
+                  bl.push_back({btsc, oi->second.first});
+                  src.push_back({btsc, oi->second.second});
                 }
               if (bl.empty()) continue;
               const std::array<int, 5> bk = {bjl, bjr, lm, ashl, ashr};
@@ -423,6 +430,7 @@ std::string build_model(const std::string& nus_prefix, const std::string& opfile
                 m.bhat.push_back(std::move(acc));
                 m.bhat_J.push_back({bjl, bjr});
                 m.bhat_key.push_back(bk);
+                m.bhat_src.push_back(std::move(src));
                 bhat_of[bk] = slot;
               } else {
                 slot = f->second;

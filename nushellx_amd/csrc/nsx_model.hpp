@@ -44,6 +44,7 @@ struct Item {   // one (L, Ja') work item
 struct Model {
   NusInput nus;
   std::string intfile;                     // interaction name as given (NuVec prints it)
+  std::string opfile_a, opfile_b;          // side file names as given
   SideFiles A, B;
   int J2 = 0, basd = 0, no_level = 5;
   double renorm = 1.0;
@@ -67,6 +68,13 @@ struct Model {
   std::vector<std::array<int, 2>> bhat_J;    // (2Jb, 2Jb')
   std::vector<std::array<int, 5>> bhat_key;  // (2Jb, 2Jb', 2lam, sa, sa') or (2Jb, 2Jb', 2lam, n, parity)
   int64_t nnz_a = 0, nnz_b = 0;              // operator values the reference stores per side
+  // linear-ensemble templates: the .oph values as read (file order), and per
+  // bhat slot its fold recipe (b-density index, .oph record index); the pn
+  // structure then keeps every .oph key present in the file, zero or not, so
+  // that any interaction with the same keys fits it
+  bool keep_zero_ops = false;
+  std::vector<double> oph_me;
+  std::vector<std::vector<std::pair<int, int>>> bhat_src;
   // algorithmic (reference sparse) work of one H*v, and term counts
   double flops_ref = 0.0;               // all terms
   double flops_like = 0.0;              // H_a V + V H_b^T part of flops_ref
@@ -77,11 +85,14 @@ struct Model {
 // With a non-empty `dopfile` the pn part comes from <dopfile>.dop as in NuTRLZ.
 std::string build_model(const std::string& nus_prefix, const std::string& opfile_a,
                         const std::string& opfile_b, const std::string& intfile, Model& m,
-                        const std::string& dopfile = "");
+                        const std::string& dopfile = "", bool keep_zero_ops = false);
 
 // Restatement of the reference's float-bisection key search
 // (src/NuCommonX/NuCommonX.f90:147-183); 1-based index or 0.
 int tfind_ref(const std::vector<std::array<uint8_t, 8>>& keys, const uint8_t* key);
+
+// Dense d(Jl) x d(Jr) column-major matrix of one stored density of a side.
+std::vector<double> densify(const SideFiles& s, const Density& dd, int J2l, int J2r);
 
 double sixj(int a, int b, int c, int d, int e, int f);
 double racah_w(int j1, int j2, int j3, int j4, int J, int K);

@@ -6,6 +6,7 @@
 #include <array>
 #include <map>
 #include <set>
+#include <string>
 
 namespace nsx {
 
@@ -34,6 +35,78 @@ static int64_t intern(std::vector<double>& store, std::unordered_map<uint64_t, s
   store.insert(store.end(), v, v + n);
   cands.push_back(off);
   blocks.push_back({off, (int64_t)n});
+  return off;
+}
+
+// Linear-ensemble interning: a Bhat^T run is identified by the files it is
+// folded from (b side, interaction) and its group key, Jb' range and slots; a
+// new run records, per slot, its fold recipe over the b densities.
+static int64_t ens_intern_run(HostPlan& p, const Model& m, const std::array<int, 4>& grp,
+                              const std::map<int, int>& slot_of, std::map<int, int64_t>& within,
+                              const std::vector<double>& run) {
+  std::string key = "B|" + m.opfile_b + "|" + m.intfile;
+  for (int x : grp) key += "," + std::to_string(x);
+  for (auto& sl : slot_of) key += ";" + std::to_string(sl.first);
+  key += "|" + std::to_string(run.size());
+  auto f = p.ens_key.find(key);
+  if (f != p.ens_key.end()) return f->second;
+  const int64_t off = (int64_t)p.Bh.size();
+  p.Bh.insert(p.Bh.end(), run.begin(), run.end());
+  p.b_blocks.push_back({off, (int64_t)run.size()});
+  p.ens_key[key] = off;
+  const int bjl = grp[0], db = m.B.d(bjl);
+  for (auto& sl : slot_of) {
+    const int bjr = sl.first, dbr = m.B.d(bjr);
+    EnsFold fd;
+    fd.dst = off + within[bjr];
+    fd.n = dbr * db;
+    fd.p_begin = (int32_t)p.ens_pair.size();
+    for (const auto& pr : m.bhat_src[sl.second]) {
+      // the density in the Bhat^T layout, stored once per (b side, density, Jb, Jb')
+      const std::string dk = "D|" + m.opfile_b + "|" + std::to_string(pr.first) + "," + std::to_string(bjl) +
+                             "," + std::to_string(bjr);
+      auto g = p.ens_key.find(dk);
+      int64_t src;
+      if (g == p.ens_key.end()) {
+        const std::vector<double> D = densify(m.B, m.B.dens[pr.first], bjl, bjr);  // db x dbr
+        src = (int64_t)p.ens_bsrc.size();
+        for (int c = 0; c < db; ++c)
+          for (int i = 0; i < dbr; ++i) p.ens_bsrc.push_back(D[size_t(i) * db + c]);
+        p.ens_key[dk] = src;
+      } else {
+        src = g->second;
+      }
+      EnsPair ep;
+      ep.src = src;
+      ep.c = m.renorm;
+      ep.val = p.ens_opcol + pr.second;
+      ep.pad = 0;
+      p.ens_pair.push_back(ep);
+    }
+    fd.p_end = (int32_t)p.ens_pair.size();
+    fd.pad = 0;
+    p.ens_fold.push_back(fd);
+  }
+  return off;
+}
+
+// Linear-ensemble interning of a like-particle H(J) block: identified by the
+// side's file name, the interaction suffix and J.
+static int64_t ens_intern_h(HostPlan& p, const Model& m, int side, int j) {
+  const SideFiles& sf = side ? m.B : m.A;
+  const std::string key = "H|" + (side ? m.opfile_b : m.opfile_a) + "|" +
+                          (m.intfile.empty() ? std::string("x") : m.intfile.substr(m.intfile.size() - 1)) + "|" +
+                          std::to_string(j);
+  auto f = p.ens_key.find(key);
+  if (f != p.ens_key.end()) {
+    p.ens_h_alias.push_back({f->second, (int64_t)sf.H[j].size(), p.ens_t, side, j, 0});
+    return f->second;
+  }
+  const int64_t off = (int64_t)p.H.size();
+  p.H.insert(p.H.end(), sf.H[j].begin(), sf.H[j].end());
+  p.h_blocks.push_back({off, (int64_t)sf.H[j].size()});
+  p.ens_key[key] = off;
+  p.ens_h.push_back({off, (int64_t)sf.H[j].size(), p.ens_t, side, j, 0});
   return off;
 }
 
@@ -81,7 +154,8 @@ void plan_add(HostPlan& p, const Model& m) {
           for (int i = 0; i < dbr; ++i)
             run.push_back(it == slot_of.end() ? 0.0 : m.bhat[it->second][size_t(i) * db + c]);
       }
-      const int64_t off = intern(p.Bh, p.dedup_b, run.data(), run.size(), p.b_blocks);
+      const int64_t off = p.ens ? ens_intern_run(p, m, g.first, slot_of, within, run)
+                                : intern(p.Bh, p.dedup_b, run.data(), run.size(), p.b_blocks);
       for (int k : g.second) b_off[k] = off + within[m.bhat_key[k][1]];
     }
   }
@@ -91,12 +165,14 @@ void plan_add(HostPlan& p, const Model& m) {
   int64_t nnz_like = 0;
   for (int j = 0; j < m.A.nJ(); ++j)
     if (!m.A.H.empty() && !m.A.H[j].empty()) {
-      ha_off[j] = intern(p.H, p.dedup_h, m.A.H[j].data(), m.A.H[j].size(), p.h_blocks);
+      ha_off[j] = p.ens ? ens_intern_h(p, m, 0, j)
+                        : intern(p.H, p.dedup_h, m.A.H[j].data(), m.A.H[j].size(), p.h_blocks);
       for (double v : m.A.H[j]) nnz_like += v != 0.0;
     }
   for (int j = 0; j < m.B.nJ(); ++j)
     if (!m.B.H.empty() && !m.B.H[j].empty()) {
-      hb_off[j] = intern(p.H, p.dedup_h, m.B.H[j].data(), m.B.H[j].size(), p.h_blocks);
+      hb_off[j] = p.ens ? ens_intern_h(p, m, 1, j)
+                        : intern(p.H, p.dedup_h, m.B.H[j].data(), m.B.H[j].size(), p.h_blocks);
       for (double v : m.B.H[j]) nnz_like += v != 0.0;
     }
   // items: partial tiles

@@ -12,6 +12,7 @@
 #pragma once
 #include <cstdint>
 #include <unordered_map>
+#include <string>
 #include <vector>
 
 #include "nsx_model.hpp"
@@ -241,6 +242,28 @@ struct PlanConfig {
   int deterministic = 0;
 };
 
+// Linear ensembles: every sample's operators are linear in its parameters, so
+// one template plan serves all samples.  A folded Bhat^T sub-block of the store
+// is sum_p c_p val[p] Bsrc_p over the recipe's pairs (val: the sample's
+// .oph value, zeroed below 1e-10 as at load); a like-particle H block is a copy
+// of the sample's values.
+struct EnsFold {   // one Bhat^T sub-block (dbr x db, column-major) of the Bh store
+  int64_t dst;     // store offset
+  int32_t n;       // elements
+  int32_t p_begin, p_end;
+  int32_t pad;
+};
+struct EnsPair {   // one b-density of a fold
+  int64_t src;     // offset into the transposed dense densities
+  double c;        // renorm of the template
+  int32_t val;     // column of the sample's values
+  int32_t pad;
+};
+struct EnsHBlk {   // one H block of the store and where its values come from
+  int64_t dst, n;
+  int32_t t, side, j, pad;  // template, side (0 = a, 1 = b), 2J / 2
+};
+
 struct HostPlan {
   PlanConfig cfg;
   int max_units = 1;                 // staged kernel: largest (row, column block) count of an item
@@ -290,6 +313,17 @@ struct HostPlan {
   std::vector<int64_t> f_part_list;
   int64_t f_part_size = 0;           // doubles of the column partials
   int f_lds = 1, f_units = 1;        // largest round (LDS elements); largest task's output units
+  // linear ensembles (filled by plan_add when ens is set): Bh and H blocks are
+  // interned by their source (files and keys) instead of their values
+  bool ens = false;
+  int32_t ens_t = 0;                   // template being added
+  int32_t ens_opcol = 0;               // first value column of its .oph section
+  std::unordered_map<std::string, int64_t> ens_key;
+  std::vector<EnsFold> ens_fold;
+  std::vector<EnsPair> ens_pair;
+  std::vector<double> ens_bsrc;        // dense b densities in the Bhat^T layout
+  std::vector<EnsHBlk> ens_h;
+  std::vector<EnsHBlk> ens_h_alias;    // later templates' blocks that share a stored one
   int64_t N = 0, part_size = 0;
   int max_lds = 0;                  // doubles of LDS the pn kernel needs
   double flops_ref = 0.0;           // algorithmic work of one batched H*v

@@ -14,22 +14,25 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import (NSX_FP64, NSX_HOST_ONLY, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, TuneResult, check,
+from ._lib import (NSX_FP64, NSX_HOST_ONLY, NSX_TEMPLATE, PRECISIONS, DeviceBuffer, NsxError, TrlOpts, TrlStats, TuneResult, check,
                    device_sync, lib)
 
 
 class Hamiltonian:
     def __init__(self, nus_prefix: str, opfile_a: str, opfile_b: str, intfile: str,
-                 precision: str = "fp64", device: bool = True, dopfile: Optional[str] = None):
+                 precision: str = "fp64", device: bool = True, dopfile: Optional[str] = None,
+                 template: bool = False):
         """``device=False`` builds a host-only instance (no device memory), usable
         only as a member of a Batch.  ``dopfile`` takes the pn part from the
-        separable operator ``<dopfile>.dop`` (program NuTRLZ / NuXpct)."""
+        separable operator ``<dopfile>.dop`` (program NuTRLZ / NuXpct).
+        ``template``: a template of a linear ensemble (every .oph key kept in the
+        term structure; see ``Batch.linear_ensemble``)."""
         L = lib()
         self.precision = precision
         self.device = device
         self.dopfile = dopfile
         h = C.c_void_p()
-        flags = PRECISIONS[precision] | (0 if device else NSX_HOST_ONLY)
+        flags = PRECISIONS[precision] | (0 if device else NSX_HOST_ONLY) | (NSX_TEMPLATE if template else 0)
         if dopfile is None:
             check(L.nsx_hv_create(nus_prefix.encode(), opfile_a.encode(), opfile_b.encode(),
                                   intfile.encode(), flags, C.byref(h)), "nsx_hv_create")
@@ -41,6 +44,22 @@ class Hamiltonian:
         n, b = C.c_int64(), C.c_int32()
         check(L.nsx_hv_dims(h, C.byref(n), C.byref(b)), "nsx_hv_dims")
         self.N, self.basd = int(n.value), int(b.value)
+
+    def ens_layout(self):
+        """Value layout of a template: (number of .oph records, [(side, 2J, d), ...])
+        of its H blocks (side 0 = a, 1 = b)."""
+        no, nh = C.c_int64(), C.c_int32()
+        check(lib().nsx_ens_layout(self._h, C.byref(no), C.byref(nh), None, 0), "nsx_ens_layout")
+        desc = np.zeros(3 * max(nh.value, 1), dtype=np.int32)
+        check(lib().nsx_ens_layout(self._h, C.byref(no), C.byref(nh), desc.ctypes.data, nh.value), "nsx_ens_layout")
+        return int(no.value), [tuple(int(x) for x in desc[3 * i:3 * i + 3]) for i in range(nh.value)]
+
+    def ens_values(self) -> np.ndarray:
+        """The template's own file values in its ensemble layout."""
+        no, hl = self.ens_layout()
+        out = np.zeros(no + sum(d * d for _, _, d in hl))
+        check(lib().nsx_ens_values(self._h, out.ctypes.data), "nsx_ens_values")
+        return out
 
     @classmethod
     def from_args(cls, args: Sequence[str], precision: str = "fp64") -> "Hamiltonian":
@@ -222,11 +241,50 @@ class Batch:
                   "nsx_batch_create_config")
         self._b = b
         self._hams = list(hams)
+        self.samples = 0
+        self._init_dims()
+
+    @classmethod
+    def linear_ensemble(cls, templates: Sequence[Hamiltonian], theta: np.ndarray, bases: Sequence[np.ndarray],
+                        precision: str = "fp64", config: Optional[dict] = None) -> "Batch":
+        """S interaction samples (theta: S x P) of every template in one batch, the
+        operators of each sample made on the device from the templates' bases
+        (basis t: P x K_t, the template's values for each unit parameter vector,
+        in the layout of ``Hamiltonian.ens_layout``).  Instances are sample-major:
+        instance s * len(templates) + t is template t with sample s."""
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        S, P = theta.shape
+        bases = [np.ascontiguousarray(B, dtype=np.float64) for B in bases]
+        assert len(bases) == len(templates) and all(B.shape[0] == P for B in bases)
+        arr = (C.c_void_p * len(templates))(*[h.handle for h in templates])
+        bp = (C.c_void_p * len(bases))(*[B.ctypes.data for B in bases])
+        cfg = None
+        if config is not None:
+            cfg = TuneResult(**{k: config.get(k, 0) for k, _ in TuneResult._fields_})
+        b = C.c_void_p()
+        check(lib().nsx_ens_create(arr, len(templates), S, P, theta.ctypes.data, bp, PRECISIONS[precision],
+                                   C.byref(cfg) if cfg is not None else None, C.byref(b)), "nsx_ens_create")
+        self = cls.__new__(cls)
+        self._b = b
+        self._hams = [h for _ in range(S) for h in templates]
+        self._bases = bases
+        self.samples = S
+        self._init_dims()
+        return self
+
+    def ens_update(self, theta: np.ndarray) -> None:
+        """New parameters for the same samples; operators rebuilt on the device."""
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        assert theta.shape[0] == self.samples
+        check(lib().nsx_ens_update(self._b, theta.ctypes.data), "nsx_ens_update")
+
+    def _init_dims(self) -> None:
+        b = self._b
         n = C.c_int64()
         check(lib().nsx_batch_dims(b, C.byref(n)), "nsx_batch_dims")
         self.N = int(n.value)
         self.offsets = []
-        for i in range(len(hams)):
+        for i in range(len(self._hams)):
             o = C.c_int64()
             check(lib().nsx_batch_offset(b, i, C.byref(o)), "nsx_batch_offset")
             self.offsets.append(int(o.value))

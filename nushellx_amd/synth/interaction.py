@@ -193,3 +193,54 @@ def multipole_pn(space: PNSpace, inter: Interaction):
                                 recs[key] = s
     items = sorted(recs.items())
     return items, max_lmo
+
+
+def multipole_pn_basis(space: PNSpace, inter: Interaction):
+    """The .oph records as linear functions of the interaction parameters: keys
+    (ascending, as ``multipole_pn`` writes them), max_lmo, and M (P x records)
+    with ``multipole_pn(space, V)`` values = theta(V) @ M for every interaction V
+    with the TBME keys of ``inter`` (parameters of ``Interaction.parameter_keys``;
+    the SPE rows are zero)."""
+    o = inter.orbits
+    pkeys = sorted(inter.tbme)
+    pidx = {k: len(inter.spe) + i for i, k in enumerate(pkeys)}
+    P = len(inter.spe) + len(pkeys)
+    recs: Dict[bytes, Dict[int, float]] = {}
+    max_lmo = 0
+    for a_side, b_side in (("p", "n"), ("n", "p")):
+        for sh1 in space.shells(a_side):
+            j1, o1 = space.j2(sh1), space.orbit_index(sh1)
+            for sh2 in space.shells(b_side):
+                j2, o2 = space.j2(sh2), space.orbit_index(sh2)
+                d12 = np.sqrt(2.0) if o1 == o2 else 1.0
+                for sh3 in space.shells(a_side):
+                    j3, o3 = space.j2(sh3), space.orbit_index(sh3)
+                    for sh4 in space.shells(b_side):
+                        j4, o4 = space.j2(sh4), space.orbit_index(sh4)
+                        d34 = np.sqrt(2.0) if o3 == o4 else 1.0
+                        lmn = max(abs(j1 - j3), abs(j2 - j4))
+                        lmx = min(j1 + j3, j2 + j4)
+                        max_lmo = max(max_lmo, lmx)
+                        for lam in range(lmx, lmn - 1, -2):
+                            key = bytes([sh1, sh3, sh2, sh4, lam, 0, 0, 0])
+                            if key in recs:
+                                continue
+                            col: Dict[int, float] = {}
+                            for J2 in range(max(abs(j1 - j2), abs(j3 - j4)), min(j1 + j2, j3 + j4) + 1, 2):
+                                for T2 in (0, 2):
+                                    if (o1 == o2 or o3 == o4) and ((J2 // 2 + T2 // 2) % 2) == 0:
+                                        continue
+                                    k, phv = inter.v_key(o1, o2, o3, o4, J2, T2)
+                                    if k not in pidx:
+                                        continue
+                                    ph = -1.0 if ((j1 + j4 - J2 - lam) // 2) % 2 else 1.0
+                                    c = (phv * (J2 + 1) * np.sqrt(lam + 1.0) * ph
+                                         * racah(j1, j2, j3, j4, J2, lam) * d12 * d34 / 2.0)
+                                    col[pidx[k]] = col.get(pidx[k], 0.0) + c
+                            recs[key] = col
+    keys = sorted(recs)
+    M = np.zeros((P, len(keys)))
+    for r, k in enumerate(keys):
+        for i, c in recs[k].items():
+            M[i, r] += c
+    return keys, max_lmo, M

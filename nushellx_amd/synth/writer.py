@@ -259,6 +259,29 @@ class SideCache:
             out[J2] = [(np_, npr, np.tensordot(theta, M, axes=(0, 0))) for (np_, npr), M in zip(pairs, mats)]
         return out
 
+    def h_dense_basis(self, space: PNSpace, kind: str, n: int, inter: Interaction) -> Dict[int, np.ndarray]:
+        """Per 2J the like-particle H(J) of each unit parameter as dense d x d
+        matrices flattened column-major (P x d^2), in the partition order the
+        .mat reader assembles them (block (npr, np) and its transpose)."""
+        key = (space.orbits, kind, n)
+        if key not in self._basis:
+            self._basis[key] = self.side(space, kind, n).hamiltonian_basis(inter)
+        side = self.side(space, kind, n)
+        out = {}
+        for J2, (pairs, mats) in self._basis[key].items():
+            dims = [side.phi(J2)[p].shape[1] for p in range(side.no_spart)]
+            off = np.concatenate([[0], np.cumsum(dims)]).astype(int)
+            d = int(off[-1])
+            P = mats[0].shape[0] if mats else len(inter.parameter_vector())
+            D = np.zeros((P, d, d))
+            for (np_, npr), M in zip(pairs, mats):
+                p, q = np_ - 1, npr - 1
+                D[:, off[q]:off[q] + dims[q], off[p]:off[p] + dims[p]] = M
+                if p != q:
+                    D[:, off[p]:off[p] + dims[p], off[q]:off[q] + dims[q]] = M.transpose(0, 2, 1)
+            out[J2] = D.transpose(0, 2, 1).reshape(P, d * d)
+        return out
+
     def dens(self, space: PNSpace, kind: str, n: int) -> Dict:
         key = (space.orbits, kind, n)
         if key not in self._dens:
