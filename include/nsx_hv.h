@@ -80,8 +80,11 @@ int nsx_hv_lcoef(const nsx_hv *h, uint8_t *mask);
  * of the requested parity whose partitions lie inside the window (:990-1037),
  * and the call returns 1; otherwise it is lcoef and the call returns 0.
  * On a truncated basis H*v leaves the blocks with no partition pair inside the
- * window at zero (bas_log, :86-88, :155-157) and computes the others in full;
- * the solvers zero W outside trunc after every H*v, as Lnz_OP does (:307-344). */
+ * window at zero (bas_log, :86-88, :155-157); in the others the pn part reads
+ * v masked to trunc, as single_productm gathers the lcoef states and adds only
+ * the window ones (src/NuCommonX/NuCommonX.f90:760-812, 885-889), and the
+ * like-particle part reads v in full (mulmtx, :1009-1104).  W is not masked:
+ * the solvers zero it outside trunc after every H*v, as Lnz_OP does (:307-344). */
 int nsx_hv_trunc(const nsx_hv *h, uint8_t *mask);
 /* Algorithmic work of one H*v: the reference loop nest's sparse flops (all terms,
  * and the like-particle H_a V + V H_b^T part of them) and FP64 bytes

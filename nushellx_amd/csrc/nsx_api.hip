@@ -137,6 +137,11 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
   d.n_blocks = (int)hp.blocks.size();
   d.max_lds = std::max(hp.max_lds, 1);
   d.N = hp.N;
+  if (!hp.vmask.empty()) {  // truncated instances: the pn part reads a masked copy of V
+    if (!(e = up(d.vmask, hp.vmask)).empty()) return e;
+    if (hipMalloc(&d.Vm, sizeof(double) * (size_t)std::max<int64_t>(hp.N, 1)) != hipSuccess)
+      return "hipMalloc failed (masked vector)";
+  }
   if (sizeof(double) * (size_t)d.max_lds > 64 * 1024)
     for (int prec = 0; prec < 3; ++prec)
       for (int at = 0; at < 2; ++at)
@@ -158,6 +163,7 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.f_task); (void)hipFree(d.f_blk); (void)hipFree(d.f_round); (void)hipFree(d.f_seg);
   (void)hipFree(d.f_x); (void)hipFree(d.f_cons); (void)hipFree(d.f_ref); (void)hipFree(d.f_blocks);
   (void)hipFree(d.f_part_list); (void)hipFree(d.f_part);
+  (void)hipFree(d.vmask); (void)hipFree(d.Vm);
   d = DevicePlan();
 }
 

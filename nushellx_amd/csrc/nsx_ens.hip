@@ -336,6 +336,11 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
       b->models.push_back(tms[t]);
       hp.vbase.push_back((int64_t)s * N_t + vb_t[t]);
     }
+  if (!hp.vmask.empty()) {  // window masks of the templates, per sample
+    const std::vector<uint8_t> vm_t = hp.vmask;
+    hp.vmask.clear();
+    for (int s = 0; s < S; ++s) hp.vmask.insert(hp.vmask.end(), vm_t.begin(), vm_t.end());
+  }
   hp.N = (int64_t)S * N_t;
   hp.x_size = (int64_t)S * x_t;
   hp.flops_ref *= S;

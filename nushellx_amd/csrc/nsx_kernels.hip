@@ -58,6 +58,15 @@ __global__ __launch_bounds__(256) void k_convert(const double* __restrict__ x, T
   if (i < n) y[i] = Prec<T>::from((typename Prec<T>::acc)x[i]);
 }
 
+// The pn part's input on truncated plans: V masked to the window states
+// (mask 0 elsewhere), in the working precision
+template <typename T>
+__global__ __launch_bounds__(256) void k_mask_in(const double* __restrict__ x, const uint8_t* __restrict__ mask,
+                                                 T* __restrict__ y, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < n) y[i] = Prec<T>::from((typename Prec<T>::acc)(mask[i] ? x[i] : 0.0));
+}
+
 // ---------------------------------------------------------------- direct kernel (large blocks)
 template <typename T, bool ATOMIC>
 __global__ __launch_bounds__(kThreads) void k_pn_partial(
@@ -417,6 +426,8 @@ __device__ __forceinline__ int ens_block(const EnsStride& es, int64_t& smp) {
 // stage 2's atomics) ride in the same launch: workgroups [0, g_red) take one
 // block each, the rest take stage-1 chunks.  g_red is a multiple of 8, so the
 // XCD mapping of the chunks is unchanged.
+// V feeds the like-particle blocks, Vp (V masked to the N-hbar-omega window on
+// truncated instances, else V) the stage-1 chunks.
 // Linear ensembles run every sample through the same tables: blockIdx.y is the
 // sample, whose vectors, X, Bhat^T and H stores sit at fixed strides (es; all
 // zero for one sample).  Grids along x are multiples of 8, so the XCD of a
@@ -424,7 +435,7 @@ __device__ __forceinline__ int ens_block(const EnsStride& es, int64_t& smp) {
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_stk1(
     const SChunk1* __restrict__ chunks, int n_chunks, const SUnit1* __restrict__ units, const T* __restrict__ V,
-    const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X, const DBlock* __restrict__ blocks,
+    const T* __restrict__ Vp, const T* __restrict__ Bt, typename Prec<T>::acc* __restrict__ X, const DBlock* __restrict__ blocks,
     int n_blocks, int g_red, const T* __restrict__ H, double* __restrict__ W, int lds_cap, int skip_large,
     EnsStride es) {
   using Acc = typename Prec<T>::acc;
@@ -437,6 +448,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     bx = ens_block(es, smp);
     if (bx < 0) return;
     V += smp * es.v;
+    Vp += smp * es.v;
     W += smp * es.v;
     Bt += smp * es.b;
     H += smp * es.h;
@@ -466,7 +478,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
           r[q] = Bt[c.b_off + e];
         } else {
           const int u = seg_of(e, nu, [&](int x) { return su[x].lds; });
-          r[q] = V[su[u].v_off + (e - su[u].lds)];
+          r[q] = Vp[su[u].v_off + (e - su[u].lds)];
         }
       }
     }
@@ -1564,14 +1576,16 @@ const void* k_pn_partial_ptr(int prec, bool atomic) {
 // like-particle part (k_big0, before the pn kernels) and the pn part (k_big1, k_big2)
 static bool big_like_active(const DevicePlan& d) { return d.atomic && d.n_gtile3 > 0; }
 
+// V: the like-particle input (read by the stacked stage-1 launch, which carries
+// those blocks); Vp: the pn input (V masked on truncated plans)
 template <typename T>
-static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H, double* W,
-                       hipStream_t s) {
+static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, const T* Ad, const T* Bh, const T* H,
+                       double* W, hipStream_t s) {
   const bool at = d.atomic != 0;
   if (d.fused) {  // column partials of the small blocks (summed by the reduce)
     if (d.n_ftask > 0) {
       void* a[] = {(void*)&d.f_task, (void*)&d.n_ftask, (void*)&d.f_blk, (void*)&d.f_round, (void*)&d.f_seg,
-                   (void*)&d.f_x,    (void*)&d.f_cons,  (void*)&d.f_ref, (void*)&V,         (void*)&Ad,
+                   (void*)&d.f_x,    (void*)&d.f_cons,  (void*)&d.f_ref, (void*)&Vp,        (void*)&Ad,
                    (void*)&Bh,       (void*)&d.f_part};
       const unsigned g = (unsigned)(((d.n_ftask + kXcds - 1) / kXcds) * kXcds);
       hipError_t e = hipLaunchKernel(k_fuse_ptr(prec, d.f_u), dim3(g), dim3(kFuseThreads), a,
@@ -1589,9 +1603,10 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
       const T* Hs = H;
       const int cap = d.s_lds1;
       const int skip = big_like_active(d) ? 1 : 0;
-      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V, (void*)&Bh, (void*)&X,
-                    (void*)&d.blocks,   (void*)&d.n_blocks, (void*)&g_red,    (void*)&Hs, (void*)&W,
-                    (void*)&cap,        (void*)&skip,       (void*)&d.es};
+      void* a1[] = {(void*)&d.s_chunk1, (void*)&d.n_chunk1, (void*)&d.s_unit1, (void*)&V,   (void*)&Vp,
+                    (void*)&Bh,         (void*)&X,          (void*)&d.blocks,  (void*)&d.n_blocks,
+                    (void*)&g_red,      (void*)&Hs,         (void*)&W,         (void*)&cap, (void*)&skip,
+                    (void*)&d.es};
       const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
       hipError_t e = hipLaunchKernel(k_stk_ptr(1, prec, d.stk_nt), dim3((unsigned)(g_red + g1), gy),
                                      dim3(d.stk_nt), a1, sizeof(T) * (size_t)d.s_lds1, s);
@@ -1611,7 +1626,7 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
     const int k = pick_staged(prec, d.staged_nt, d.staged_mode, at, d.max_units);
     if (k < 0) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(T) * (size_t)((d.staged_mode == 1 ? 2 : 1) * d.lds_buf + d.lds_y);
-    void* args[] = {(void*)&d.items2, (void*)&d.phases, (void*)&d.pairs, (void*)&V,
+    void* args[] = {(void*)&d.items2, (void*)&d.phases, (void*)&d.pairs, (void*)&Vp,
                     (void*)&Ad,       (void*)&Bh,       (void*)&d.part,  (void*)&W,
                     (void*)&d.lds_buf};
     hipError_t e = hipLaunchKernel(kStaged[k].fn, dim3(d.n_items2), dim3(d.staged_nt), args, lds, s);
@@ -1625,17 +1640,17 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
       if (d.n_gtile1 > 0) {
         if (d.big_rows1 == 2 * kBigTile)
           hipLaunchKernelGGL(k_big1_bf<2 * kBigTile>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1,
-                             d.g_task1, V, Bh, XG);
+                             d.g_task1, Vp, Bh, XG);
         else
           hipLaunchKernelGGL(k_big1_bf<kBigTile>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1,
-                             d.g_task1, V, Bh, XG);
+                             d.g_task1, Vp, Bh, XG);
       }
       hipLaunchKernelGGL(k_big2_bf, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2, d.g_xref, Ad,
                          (const C*)XG, W);
     } else {
       if (d.n_gtile1 > 0)
         hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1,
-                           V, Bh, XG);
+                           Vp, Bh, XG);
       hipLaunchKernelGGL(k_big2<T>, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2,
                          d.g_xref, Ad, (const C*)XG, W);
     }
@@ -1643,7 +1658,7 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Ad, c
   }
   if (d.n_items > 0) {
     const size_t lds = sizeof(T) * (size_t)std::max(d.max_lds, 1);
-    void* args[] = {(void*)&d.items, (void*)&d.aterms, (void*)&d.rterms, (void*)&V,
+    void* args[] = {(void*)&d.items, (void*)&d.aterms, (void*)&d.rterms, (void*)&Vp,
                     (void*)&Ad,      (void*)&Bh,       (void*)&d.part,   (void*)&W};
     hipError_t e = hipLaunchKernel(k_pn_partial_ptr(prec, at), dim3(d.n_items), dim3(kThreads), args, lds, s);
     if (e != hipSuccess) return e;
@@ -1672,16 +1687,23 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
   return hipGetLastError();
 }
 
+// the pn input of a plan: the masked copy made by launch_convert on truncated plans
+static const void* pn_input(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V) {
+  if (prec == 0) return d.vmask ? (const void*)d.Vm : (const void*)V;
+  return d.vmask ? lp.vin_m : lp.vin;
+}
+
 hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s) {
+  const void* Vp = pn_input(d, lp, prec, V);
   switch (prec) {
-    case 0: return pn_T<double>(d, 0, V, d.Ad, d.Bh, d.H, W, s);
+    case 0: return pn_T<double>(d, 0, V, (const double*)Vp, d.Ad, d.Bh, d.H, W, s);
     case 1:
-      return pn_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.Ad, (const float*)lp.Bh, (const float*)lp.H,
-                         W, s);
+      return pn_T<float>(d, 1, (const float*)lp.vin, (const float*)Vp, (const float*)lp.Ad, (const float*)lp.Bh,
+                         (const float*)lp.H, W, s);
     default:
-      return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.Ad, (const Bf16x2*)lp.Bh,
-                          (const Bf16x2*)lp.H, W, s);
+      return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)Vp, (const Bf16x2*)lp.Ad,
+                          (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s);
   }
 }
 
@@ -1705,8 +1727,17 @@ hipError_t convert_to(int prec, const double* x, void* y, int64_t n, hipStream_t
 }
 
 hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s) {
-  if (prec == 0 || d.N == 0) return hipSuccess;
+  if (d.N == 0) return hipSuccess;
   const dim3 g((unsigned)((d.N + 255) / 256));
+  if (d.vmask) {  // the pn part's masked input
+    if (prec == 0)
+      hipLaunchKernelGGL(k_mask_in<double>, g, dim3(256), 0, s, V, d.vmask, d.Vm, d.N);
+    else if (prec == 1)
+      hipLaunchKernelGGL(k_mask_in<float>, g, dim3(256), 0, s, V, d.vmask, (float*)lp.vin_m, d.N);
+    else
+      hipLaunchKernelGGL(k_mask_in<Bf16x2>, g, dim3(256), 0, s, V, d.vmask, (Bf16x2*)lp.vin_m, d.N);
+  }
+  if (prec == 0) return hipGetLastError();
   if (prec == 1)
     hipLaunchKernelGGL(k_convert<float>, g, dim3(256), 0, s, V, (float*)lp.vin, d.N);
   else
@@ -1720,8 +1751,8 @@ hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, con
 // partials), k_reduce (small blocks stored), then k_big1 / k_big2 add the
 // large-block terms
 template <typename T>
-static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T* Ad, const T* Bh, const T* H,
-                             double* W, hipStream_t s) {
+static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T* Vp, const T* Ad, const T* Bh,
+                             const T* H, double* W, hipStream_t s) {
   const bool big = big_like_active(d);
   if (big) {
     if constexpr (std::is_same<T, Bf16x2>::value)
@@ -1732,7 +1763,7 @@ static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T*
   DevicePlan small = d;   // k_fuse alone (the large-block items run on the tiles below)
   small.n_gtile2 = 0;
   small.n_items = 0;
-  hipError_t e = pn_T<T>(small, prec, V, Ad, Bh, H, W, s);
+  hipError_t e = pn_T<T>(small, prec, V, Vp, Ad, Bh, H, W, s);
   if (e != hipSuccess) return e;
   if (d.n_fblocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_fblocks), dim3(kThreads), 0, s, d.f_blocks, d.f_part_list, d.f_part,
@@ -1740,7 +1771,7 @@ static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T*
   if (d.n_gtile2 > 0) {
     DevicePlan lg = d;    // the large-block tiles alone
     lg.n_ftask = 0;
-    return pn_T<T>(lg, prec, V, Ad, Bh, H, W, s);
+    return pn_T<T>(lg, prec, V, Vp, Ad, Bh, H, W, s);
   }
   return hipGetLastError();
 }
@@ -1750,14 +1781,15 @@ hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const do
   hipError_t e = launch_convert(d, lp, prec, V, s);
   if (e != hipSuccess) return e;
   if (d.fused) {
+    const void* Vp = pn_input(d, lp, prec, V);
     switch (prec) {
-      case 0: return fused_hv_T<double>(d, 0, V, d.Ad, d.Bh, d.H, W, s);
+      case 0: return fused_hv_T<double>(d, 0, V, (const double*)Vp, d.Ad, d.Bh, d.H, W, s);
       case 1:
-        return fused_hv_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.Ad, (const float*)lp.Bh,
-                                 (const float*)lp.H, W, s);
+        return fused_hv_T<float>(d, 1, (const float*)lp.vin, (const float*)Vp, (const float*)lp.Ad,
+                                 (const float*)lp.Bh, (const float*)lp.H, W, s);
       default:
-        return fused_hv_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.Ad, (const Bf16x2*)lp.Bh,
-                                  (const Bf16x2*)lp.H, W, s);
+        return fused_hv_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)Vp, (const Bf16x2*)lp.Ad,
+                                  (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s);
     }
   }
   if (d.atomic) {

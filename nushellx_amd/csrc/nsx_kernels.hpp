@@ -68,6 +68,9 @@ struct DevicePlan {
   double* f_part = nullptr;
   int n_ftask = 0, n_fblocks = 0, f_lds = 1;
   uint64_t gen = 0;          // changes at every upload (captured graphs hold plan pointers)
+  // truncated plans: the window mask of the pn input and the masked FP64 copy
+  uint8_t* vmask = nullptr;
+  double* Vm = nullptr;
   // linear ensemble: the stacked kernels run ens_s samples (grid y) at strides es
   int ens_s = 1;
   EnsStride es;
@@ -111,6 +114,7 @@ struct LowpPlan {
   void* Bh = nullptr;
   void* H = nullptr;
   void* vin = nullptr;
+  void* vin_m = nullptr;  // the masked input of the pn part (truncated plans)
 };
 std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp);
 void free_lowp(LowpPlan& lp);

@@ -40,6 +40,8 @@ std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp) {
     return "";
   }
   if (hipMalloc(&lp.vin, esz * (size_t)(hp.N > 0 ? hp.N : 1)) != hipSuccess) return "hipMalloc failed (vector)";
+  if (!hp.vmask.empty() && hipMalloc(&lp.vin_m, esz * (size_t)(hp.N > 0 ? hp.N : 1)) != hipSuccess)
+    return "hipMalloc failed (masked vector)";
   return "";
 }
 
@@ -48,6 +50,7 @@ void free_lowp(LowpPlan& lp) {
   (void)hipFree(lp.Bh);
   (void)hipFree(lp.H);
   (void)hipFree(lp.vin);
+  (void)hipFree(lp.vin_m);
   lp = LowpPlan();
 }
 

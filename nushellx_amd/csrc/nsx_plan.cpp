@@ -119,6 +119,11 @@ static int col_tile(int rows) {
 void plan_add(HostPlan& p, const Model& m) {
   const int64_t vb = p.N;
   p.vbase.push_back(vb);
+  if (m.truncate && p.vmask.empty()) p.vmask.assign(size_t(vb), 1);
+  if (m.truncate)
+    p.vmask.insert(p.vmask.end(), m.trunc.begin(), m.trunc.end());
+  else if (!p.vmask.empty())
+    p.vmask.insert(p.vmask.end(), size_t(m.N), 1);
   // operator values of this instance
   std::vector<int64_t> a_off(m.adense.size()), b_off(m.bhat.size());
   for (size_t k = 0; k < m.adense.size(); ++k)
@@ -936,6 +941,13 @@ void plan_merge(HostPlan& p, const HostPlan& q) {
   const int i_aterms = (int)p.aterms.size(), i_rterms = (int)p.rterms.size();
   const int i_plist = (int)p.part_list.size();
   for (int64_t v : q.vbase) p.vbase.push_back(v + sv);
+  if (!q.vmask.empty() || !p.vmask.empty()) {
+    p.vmask.resize(size_t(sv), 1);
+    if (q.vmask.empty())
+      p.vmask.insert(p.vmask.end(), size_t(q.N), 1);
+    else
+      p.vmask.insert(p.vmask.end(), q.vmask.begin(), q.vmask.end());
+  }
   for (auto a : q.inst_a_off) {
     for (auto& x : a) x = ma(x);
     p.inst_a_off.push_back(std::move(a));

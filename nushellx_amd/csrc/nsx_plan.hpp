@@ -279,6 +279,11 @@ struct HostPlan {
   std::vector<int64_t> part_list;   // partial base of every item of each block, in item order
   std::vector<double> Ad, Bh, H;
   std::vector<int64_t> vbase;       // per instance
+  // N-hbar-omega windows: the pn part reads its input masked to the window
+  // states (NuTRL's single_productm gathers lcoef states and adds only window
+  // ones); 1 for every state of an untruncated instance.  Empty when no
+  // instance is truncated.  The like-particle part reads the input unmasked.
+  std::vector<uint8_t> vmask;
   // identical operator matrices are stored once (content hash -> offsets)
   std::unordered_map<uint64_t, std::vector<int64_t>> dedup_a, dedup_b, dedup_h;
   // (offset, length) of every block stored in Ad, Bh, H, in insertion order

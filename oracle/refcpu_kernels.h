@@ -6,13 +6,19 @@
 /* single_productm: gather V_R (masked a rows laco, masked b columns lbco) into
    B1 (dar x cols, col-major), then for each b-density of the group and each set
    bit (i=row b-partition of Jb, j=column b-partition of Jb'):
-   B3[:, (i,l)] += (trdb*meV) * B1[:, (j,k)]. */
+   B3[:, (i,l)] += (trdb*meV) * B1[:, (j,k)].
+   On a truncated basis (:760-767, :807-812) the gather keeps the lcoef states
+   only, F1 flags the gathered states inside the window (lbuffer1), an element
+   adds only where its source is flagged, and F2 (lbuffer2) flags the elements
+   of B3 that received a flagged source. */
 static void SFX(single_productm)(const refcpu_t *h, const R *bveci, int64_t posi, const R *meV,
                                  int ajr, int bjl, int bjr, const uint32_t *laco,
                                  const uint32_t *lbco, const int *blocb, int dar, int dbl,
-                                 int zero, R *B1, R *B3, int ntrb, const int *locb) {
+                                 int zero, R *B1, R *B3, int ntrb, const int *locb, uint8_t *F1,
+                                 uint8_t *F2) {
   const side_t *A = &h->A, *B = &h->B;
   int nsa = A->nsp, nsb = B->nsp;
+  const int trunc = h->trunc != NULL;
   int64_t p = posi;
   int64_t pp = 0;
   for (int j = 0; j < nsb; j++)
@@ -20,7 +26,14 @@ static void SFX(single_productm)(const refcpu_t *h, const R *bveci, int64_t posi
       for (int i = 0; i < nsa; i++) {
         int nr = nJTs(A, ajr, i);
         if (bit_of(lbco, j) && bit_of(laco, i) && nr != 0) {
-          for (int q = 0; q < nr; q++) B1[pp + q] = bveci[p + q];
+          if (trunc) {
+            for (int q = 0; q < nr; q++) {
+              B1[pp + q] = h->lcoef[p + q] ? bveci[p + q] : (R)0;
+              F1[pp + q] = (uint8_t)(h->lcoef[p + q] && h->trunc[p + q]);
+            }
+          } else {
+            for (int q = 0; q < nr; q++) B1[pp + q] = bveci[p + q];
+          }
           pp += nr;
         }
         p += nr;
@@ -28,6 +41,7 @@ static void SFX(single_productm)(const refcpu_t *h, const R *bveci, int64_t posi
   int64_t dp = (int64_t)dbl * dar;
   if (zero)
     for (int64_t q = 0; q < dp; q++) B3[q] = (R)0;
+  if (zero && trunc) memset(F2, 0, (size_t)dp);
   int *snJTb = (int *)alloca(sizeof(int) * (nsb + 1));
   snJTb[0] = 0;
   for (int j = 1; j < nsb; j++) snJTb[j] = snJTb[j - 1] + (bit_of(lbco, j - 1) ? dar * nJTs(B, bjr, j - 1) : 0);
@@ -45,7 +59,15 @@ static void SFX(single_productm)(const refcpu_t *h, const R *bveci, int64_t posi
           for (int l = 0; l < nJTs(B, bjl, i); l++) {
             R me = trdb[pt++] * meV[ntb];
             if (!h->thr_on || fabs((double)me) >= 0.0000001) {
-              for (int q = 0; q < dar; q++) B3[pm + q] += me * B1[pu + q];
+              if (trunc) {
+                for (int q = 0; q < dar; q++)
+                  if (F1[pu + q]) {
+                    B3[pm + q] += me * B1[pu + q];
+                    F2[pm + q] = 1;
+                  }
+              } else {
+                for (int q = 0; q < dar; q++) B3[pm + q] += me * B1[pu + q];
+              }
             }
             pm += dar;
           }
@@ -58,17 +80,22 @@ static void SFX(single_productm)(const refcpu_t *h, const R *bveci, int64_t posi
 
 /* single_product1: transpose B3 (dar x dbl) to dbl-fastest, apply the a-density
    (rows i = a-partitions of Ja, columns j = a-partitions of Ja'), transpose back
-   and scatter-add into the output block (masked a rows laro, b columns lbro). */
+   and scatter-add into the output block (masked a rows laro, b columns lbro).
+   On a truncated basis the flags of B3 (F2) travel with it (F1) and an element
+   adds only where flagged (:846, :885-889). */
 static void SFX(single_product1)(const refcpu_t *h, R *bveco, int64_t poso, int ajl, int ajr,
                                  int bjl, int atsc, int dar, int dal, int dbl,
                                  const uint32_t *laco, const uint32_t *laro, const uint32_t *lbro,
-                                 R *B2, const R *B3, R *B4) {
+                                 R *B2, const R *B3, R *B4, uint8_t *F1, const uint8_t *F2) {
   const side_t *A = &h->A, *B = &h->B;
   int nsa = A->nsp, nsb = B->nsp;
+  const int trunc = h->trunc != NULL;
   for (int64_t q = 0; q < (int64_t)dal * dbl; q++) B4[q] = (R)0;
   int64_t p = 0;
   for (int j = 0; j < dbl; j++) {
     for (int r = 0; r < dar; r++) B2[j + (int64_t)dbl * r] = B3[p + r];
+    if (trunc)
+      for (int r = 0; r < dar; r++) F1[j + (int64_t)dbl * r] = F2[p + r];
     p += dar;
   }
   int *snJTal = (int *)alloca(sizeof(int) * (nsa + 1));
@@ -91,7 +118,12 @@ static void SFX(single_product1)(const refcpu_t *h, R *bveco, int64_t poso, int 
         for (int l = 0; l < nJTs(A, ajl, i); l++) {
           R me = trda[pt++];
           if (!h->thr_on || fabs((double)me) >= 0.0000001) {
-            for (int q = 0; q < dbl; q++) B4[pm + q] += me * B2[pu + q];
+            if (trunc) {
+              for (int q = 0; q < dbl; q++)
+                if (F1[pu + q]) B4[pm + q] += me * B2[pu + q];
+            } else {
+              for (int q = 0; q < dbl; q++) B4[pm + q] += me * B2[pu + q];
+            }
           }
           pm += dbl;
         }
@@ -208,6 +240,7 @@ static int SFX(hv)(refcpu_t *h, const double *v_in, double *v_out) {
       R *B1 = (R *)malloc(sizeof(R) * maxdimv);
       R *B2 = (R *)malloc(sizeof(R) * maxdimv);
       R *B3 = (R *)calloc(maxdimv, sizeof(R));
+      uint8_t *F1 = (uint8_t *)calloc(maxdimv, 1), *F2 = (uint8_t *)calloc(maxdimv, 1);
       int wa = A->words, wb = B->words;
       uint32_t *lvac = (uint32_t *)calloc(wa, 4), *lvar = (uint32_t *)calloc(wa, 4);
       uint32_t *lvbc = (uint32_t *)calloc(wb, 4), *lvbr = (uint32_t *)calloc(wb, 4);
@@ -258,7 +291,7 @@ static int SFX(hv)(refcpu_t *h, const double *v_in, double *v_out) {
             if (lscs != lsc) {
               crproj(nplrbop, B->nsp, B->nsp, wb, lvbc, lvbr);
               SFX(single_productm)(h, coefR, h->posnv[lscs], mev, ajr, bjl, bjrs, lvac, lvbc, blocb,
-                                   dar, dbl, zero, B1, B2, ntr, locb);
+                                   dar, dbl, zero, B1, B2, ntr, locb, F1, F2);
               ntr = 0;
               memset(nplrbop, 0, 4 * (size_t)B->nsp * wb);
               bjrs = bjr;
@@ -275,14 +308,14 @@ static int SFX(hv)(refcpu_t *h, const double *v_in, double *v_out) {
           if (ntr != 0) {
             crproj(nplrbop, B->nsp, B->nsp, wb, lvbc, lvbr);
             SFX(single_productm)(h, coefR, h->posnv[lscs], mev, ajr, bjl, bjrs, lvac, lvbc, blocb,
-                                 dar, dbl, zero, B1, B2, ntr, locb);
+                                 dar, dbl, zero, B1, B2, ntr, locb, F1, F2);
           }
           crproj(nplrbo, B->nsp, B->nsp, wb, lvbc, lvbr);
           SFX(single_product1)(h, X, h->posnv[bsc], ajl, ajr, bjl, atsc, dar, dal, dbl, lvac, lvar,
-                               lvbr, B1, B2, B3);
+                               lvbr, B1, B2, B3, F1, F2);
         }
       }
-      free(B1); free(B2); free(B3); free(lvac); free(lvar); free(lvbc); free(lvbr);
+      free(B1); free(B2); free(B3); free(F1); free(F2); free(lvac); free(lvar); free(lvbc); free(lvbr);
       free(nplrbo); free(nplrbop); free(blocb); free(locb); free(mev);
     }
   }

@@ -20,7 +20,19 @@ static thread_local std::string g_err;
 extern "C" const char* emu_last_error() { return g_err.c_str(); }
 
 // H*v of every instance of a plan, computed on the CPU from the plan arrays
-static void emulate(const HostPlan& p, const double* vin, double* vout) {
+// The pn part's input: v masked to the window states of truncated instances.
+static const double* pn_input(const HostPlan& p, const double* vin, std::vector<double>& store) {
+  if (p.vmask.empty()) return vin;
+  store.assign(vin, vin + p.N);
+  for (int64_t i = 0; i < p.N; ++i)
+    if (!p.vmask[i]) store[i] = 0.0;
+  return store.data();
+}
+
+static void emulate(const HostPlan& p, const double* vin_all, double* vout) {
+  std::vector<double> vm;
+  const double* vin = vin_all;            // like-particle input
+  const double* vp = pn_input(p, vin_all, vm);
   std::vector<double> part(p.part_size, 0.0), buf;
   // staged items
   for (const DItem2& it : p.items2) {
@@ -28,7 +40,7 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
     for (int ph = it.ph_begin; ph < it.ph_end; ++ph) {
       const DPhase& P = p.phases[ph];
       buf.assign(P.nv + P.nb + P.na, 0.0);
-      for (int i = 0; i < P.nv; ++i) buf[i] = vin[P.v_off + i];
+      for (int i = 0; i < P.nv; ++i) buf[i] = vp[P.v_off + i];
       for (int i = 0; i < P.nb; ++i) buf[P.nv + i] = p.Bh[P.b_off + i];
       for (int i = 0; i < P.na; ++i) buf[P.nv + P.nb + i] = p.Ad[P.a_off + i];
       for (int q = P.pair_begin; q < P.pair_end; ++q) {
@@ -67,7 +79,7 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
           const DRTerm& R = p.rterms[u];
           for (int i = 0; i < it.dar; ++i) {
             double s = 0.0;
-            for (int k = 0; k < R.dbr; ++k) s += vin[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
+            for (int k = 0; k < R.dbr; ++k) s += vp[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
             y[i] += R.r * s;
           }
         }
@@ -114,7 +126,7 @@ static void emulate(const HostPlan& p, const double* vin, double* vout) {
       for (int i = t.i0; i < std::min(t.i0 + p.cfg.big_rows1, g.m); ++i) {
         double s = 0.0;
         for (int k = 0; k < g.k; ++k)
-          if (active(t.kmask, k)) s += vin[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
+          if (active(t.kmask, k)) s += vp[g.v_off + i + int64_t(g.m) * k] * p.Bh[g.b_off + k + int64_t(g.k) * j];
         XG[g.x_off + i + int64_t(big_ldx(g.m)) * j] = s;
       }
   }
@@ -255,6 +267,8 @@ extern "C" int emu_hv_stacked(const char* nus, const char* opa, const char* opb,
   plan_add(p, m);
   std::vector<const Model*> ms{&m};
   build_stacked(p, ms);
+  std::vector<double> vm;
+  const double* vp = pn_input(p, vin, vm);
   std::vector<double> X(p.x_size, 0.0);
   for (const SChunk1& c : p.s_chunk1) {
     if (c.lds > p.s_lds1) { g_err = "chunk LDS above the plan maximum"; return -3; }
@@ -263,7 +277,7 @@ extern "C" int emu_hv_stacked(const char* nus, const char* opa, const char* opb,
       for (int j = 0; j < c.db; ++j)
         for (int i = 0; i < U.dar; ++i) {
           double s = 0.0;
-          for (int k = 0; k < c.dbr; ++k) s += vin[U.v_off + i + int64_t(U.dar) * k] * p.Bh[c.b_off + k + int64_t(c.dbr) * j];
+          for (int k = 0; k < c.dbr; ++k) s += vp[U.v_off + i + int64_t(U.dar) * k] * p.Bh[c.b_off + k + int64_t(c.dbr) * j];
           X[U.x_off + i + int64_t(U.dar) * j] = s;
         }
     }
@@ -311,7 +325,7 @@ extern "C" int emu_hv_stacked(const char* nus, const char* opa, const char* opb,
           const DRTerm& R = p.rterms[u];
           for (int i = 0; i < it.dar; ++i) {
             double s = 0.0;
-            for (int k = 0; k < R.dbr; ++k) s += vin[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
+            for (int k = 0; k < R.dbr; ++k) s += vp[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
             y[i] += R.r * s;
           }
         }
@@ -347,6 +361,8 @@ extern "C" int emu_hv_fused(const char* nus, const char* opa, const char* opb, c
   std::vector<const Model*> ms{&m};
   build_big(p, ms);
   if (!build_fused(p, ms, threads)) { g_err = "fused plan not possible"; return -4; }
+  std::vector<double> vm;
+  const double* vp = pn_input(p, vin, vm);
   std::vector<double> part(p.f_part_size, 0.0);
   std::vector<double> lds(p.f_lds + 1);
   for (const FTask& t : p.f_task) {
@@ -365,7 +381,7 @@ extern "C" int emu_hv_fused(const char* nus, const char* opa, const char* opb, c
         const FSeg& g = p.f_seg[s];
         const int len = (s + 1 < R.s_end ? p.f_seg[s + 1].pre : R.stage) - g.pre;
         if (g.lds < 0 || len <= 0 || g.lds + len > p.f_lds) { g_err = "segment outside the round's LDS"; return -3; }
-        const double* src = g.kind == 0 ? vin : g.kind == 1 ? p.Bh.data() : p.Ad.data();
+        const double* src = g.kind == 0 ? vp : g.kind == 1 ? p.Bh.data() : p.Ad.data();
         for (int k = 0; k < len; ++k) lds[g.lds + k] = src[g.src + k];
       }
       for (int x = R.x_begin; x < R.x_end; ++x) {
@@ -456,7 +472,7 @@ extern "C" int emu_hv_fused(const char* nus, const char* opa, const char* opb, c
           const DRTerm& R = p.rterms[u];
           for (int i = 0; i < it.dar; ++i) {
             double s = 0.0;
-            for (int k = 0; k < R.dbr; ++k) s += vin[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
+            for (int k = 0; k < R.dbr; ++k) s += vp[R.v_off + i + int64_t(it.dar) * k] * p.Bh[R.b_off + int64_t(R.dbr) * j + k];
             y[i] += R.r * s;
           }
         }

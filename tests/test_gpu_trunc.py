@@ -50,3 +50,58 @@ def test_truncated_lanczos_matches_window_spectrum(models):
     rb = Batch(hams + [full0]).lanczos(nev=4, tol=1e-8, max_basis=24, keep=8)
     for k, ev in enumerate(evs + [ev_full]):
         assert np.abs(rb["evals"][k] - ev).max() < 1e-7, (k, rb["evals"][k], ev)
+
+
+def _batch_hv(b, v):
+    from nushellx_amd.hv import DeviceBuffer, device_sync
+    din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
+    b.multiply_dev(din, dout)
+    device_sync()
+    w = np.empty_like(v)
+    dout.download(w)
+    din.free()
+    dout.free()
+    return w
+
+
+@pytest.mark.parametrize("config", [None, {"stacked": 1, "threads": 256, "fat_cap": 5120, "max_phases": 6,
+                                           "mode": 0, "atomic": 1},
+                                    {"stacked": 2, "threads": 256, "fat_cap": 5120, "max_phases": 6, "mode": 0,
+                                     "atomic": 1, "stk_cap": 4096, "stk_work": 150000}])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
+def test_truncated_pn_input_is_masked(models, config, prec, tol):
+    """A vector nonzero outside the window: the pn part sees only its window
+    states, the like-particle part all of them (the oracle restates NuTRL's
+    element masks); truncated and full instances in one batch, every path."""
+    from nushellx_amd.hv import Batch, Hamiltonian
+    cases = [("psd33t", 0), ("psd33", 0), ("psd33t", 2), ("psd33w8", 0), ("psd33t", 4)]
+    print(f"case trunc mask {prec} {config and config['stacked']}", flush=True)
+    hams = [Hamiltonian(*models[n].args(J2), precision=prec, device=False) for n, J2 in cases]
+    b = Batch(hams, precision=prec, config=config)
+    v = np.random.default_rng(43).standard_normal(b.N)
+    w = _batch_hv(b, v)
+    for k, (n, J2) in enumerate(cases):
+        ref = RefCPU(*models[n].args(J2), fp64=True)
+        o = b.offsets[k]
+        w_ref = ref.apply(v[o:o + ref.N])
+        assert np.abs(w[o:o + ref.N] - w_ref).max() <= tol * np.abs(w_ref).max(), (n, J2)
+    b.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("bf16x3", 2e-4)])
+def test_truncated_pn_input_is_masked_on_large_blocks(models, monkeypatch, prec, tol):
+    """The same with the blocks above side 12 on the matrix-core tiles."""
+    from nushellx_amd.hv import Batch, Hamiltonian
+    monkeypatch.setenv("NSX_BIG_MIN", "12")
+    cases = [("psd33t", 2), ("psd33", 2), ("psd33t", 4)]
+    print(f"case trunc mask large blocks {prec}", flush=True)
+    hams = [Hamiltonian(*models[n].args(J2), precision=prec, device=False) for n, J2 in cases]
+    b = Batch(hams, precision=prec)
+    v = np.random.default_rng(47).standard_normal(b.N)
+    w = _batch_hv(b, v)
+    for k, (n, J2) in enumerate(cases):
+        ref = RefCPU(*models[n].args(J2), fp64=True)
+        o = b.offsets[k]
+        w_ref = ref.apply(v[o:o + ref.N])
+        assert np.abs(w[o:o + ref.N] - w_ref).max() <= tol * np.abs(w_ref).max(), (n, J2)
+    b.close()
