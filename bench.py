@@ -523,6 +523,9 @@ def main():
                     help="training samples of the pf 4p4n warm-start study (0 skips it; 64 take ~1 min, "
                          "512 ~8 min)")
     ap.add_argument("--no-det", action="store_true", help="skip the deterministic-mode timings")
+    ap.add_argument("--shape", default="",
+                    help='fixed plan shape for the headline batch instead of autotuning, as JSON, e.g. '
+                         '\'{"stacked": 1, "stk_cap": 2048, "stk_work": 256}\'')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -539,9 +542,15 @@ def main():
     tmp = tempfile.mkdtemp(prefix="nsx_bench_")
     mf = make_workload(tmp)
     hams = [Hamiltonian(*mf.args(J2)) for J2 in mf.J2s]
-    batch = Batch(hams)
-    tuned = batch.autotune(iters=10)   # staged-kernel shape, chosen before timing
-    log(f"autotuned: {tuned}")
+    if args.shape:  # a fixed plan shape instead of the autotuner (reproducible profiles)
+        tuned = dict({"fat_cap": 5120, "max_phases": 6, "threads": 256, "mode": 0, "atomic": 1, "stacked": 1},
+                     **json.loads(args.shape))
+        batch = Batch(hams, config=tuned)
+        log(f"fixed shape: {tuned}")
+    else:
+        batch = Batch(hams)
+        tuned = batch.autotune(iters=10)   # plan shape and kernel choice, chosen before timing
+        log(f"autotuned: {tuned}")
     flops, flops_like, bytes_alg = batch.work()
     v = np.random.default_rng(1).standard_normal(batch.N)
     din = DeviceBuffer.from_array(v)

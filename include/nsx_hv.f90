@@ -21,6 +21,7 @@ module nsx_hv_api
                                        NSX_ECAP = -4, NSX_EGPU = -5, NSX_EARG = -6
   integer(c_int), parameter, public :: NSX_FP64 = 0, NSX_FP32 = 1, NSX_BF16X3 = 2
   integer(c_int), parameter, public :: NSX_HOST_ONLY = 256   ! or-ed into the precision
+  integer(c_int), parameter, public :: NSX_TEMPLATE = 512    ! or-ed: a linear-ensemble template
 
   ! C: nsx_trl_opts (seed is uint64_t in C; any 64-bit pattern is accepted)
   type, bind(C) :: nsx_trl_opts
@@ -143,6 +144,88 @@ module nsx_hv_api
       character(kind=c_char), dimension(*), intent(in) :: prefix, operat
       integer(c_int32_t), intent(out) :: nlev
     end function nsx_hv_xpct_files
+
+    ! <prefix>.trl / four-record <prefix>.abz, the thick-restart layout (vecs: N x nvec)
+    integer(c_int) function nsx_hv_write_trl(h, prefix, nvec, evals, vecs, no_level, tt) &
+        bind(C, name='nsx_hv_write_trl')
+      import :: c_int, c_ptr, c_char, c_int32_t, c_double, c_float
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: prefix
+      integer(c_int32_t), value :: nvec, no_level
+      real(c_double), dimension(*), intent(in) :: evals, vecs
+      real(c_float), value :: tt
+    end function nsx_hv_write_trl
+
+    ! NuVec's step: <prefix>.abz with .lnz or .trl in, .xvc/.xva/.lev/.lp out
+    integer(c_int) function nsx_nuvec(h, prefix, nlev) bind(C, name='nsx_nuvec')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: prefix
+      integer(c_int32_t), intent(out) :: nlev
+    end function nsx_nuvec
+
+    ! batches: hs is an array of handles (type(c_ptr)), instances in that order
+    integer(c_int) function nsx_batch_create(hs, n, precision, b) bind(C, name='nsx_batch_create')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), dimension(*), intent(in) :: hs
+      integer(c_int32_t), value :: n
+      integer(c_int), value :: precision
+      type(c_ptr), intent(out) :: b
+    end function nsx_batch_create
+
+    subroutine nsx_batch_destroy(b) bind(C, name='nsx_batch_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end subroutine nsx_batch_destroy
+
+    ! evals, residuals: n*nev; converged_at: n; start, evecs may be c_null_ptr
+    integer(c_int) function nsx_batch_trl_solve(b, opt, start, evals, evecs, residuals, converged_at, stats) &
+        bind(C, name='nsx_batch_trl_solve')
+      import :: c_int, c_ptr, c_double, c_int32_t, nsx_trl_opts, nsx_trl_stats
+      type(c_ptr), value :: b
+      type(nsx_trl_opts), intent(in) :: opt
+      type(c_ptr), value :: start
+      real(c_double), dimension(*), intent(out) :: evals
+      type(c_ptr), value :: evecs
+      real(c_double), dimension(*), intent(out) :: residuals
+      integer(c_int32_t), dimension(*), intent(out) :: converged_at
+      type(nsx_trl_stats), intent(out) :: stats
+    end function nsx_batch_trl_solve
+
+    ! linear ensembles: n_oph .oph records and n_h H blocks (side, 2J, d) of a template
+    integer(c_int) function nsx_ens_layout(h, n_oph, n_h, h_desc, cap) bind(C, name='nsx_ens_layout')
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: n_oph
+      integer(c_int32_t), intent(out) :: n_h
+      integer(c_int32_t), dimension(*), intent(out) :: h_desc
+      integer(c_int32_t), value :: cap
+    end function nsx_ens_layout
+
+    integer(c_int) function nsx_ens_values(h, values) bind(C, name='nsx_ens_values')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), dimension(*), intent(out) :: values
+    end function nsx_ens_values
+
+    ! templates, bases: arrays of c_ptr (bases(t) = c_loc of a P x K_t row-major array);
+    ! theta: S x P row-major (theta(i, s) in Fortran order); cfg may be c_null_ptr
+    integer(c_int) function nsx_ens_create(templates, n_t, s, p, theta, bases, precision, cfg, b) &
+        bind(C, name='nsx_ens_create')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), dimension(*), intent(in) :: templates, bases
+      integer(c_int32_t), value :: n_t, s, p
+      real(c_double), dimension(*), intent(in) :: theta
+      integer(c_int), value :: precision
+      type(c_ptr), value :: cfg
+      type(c_ptr), intent(out) :: b
+    end function nsx_ens_create
+
+    integer(c_int) function nsx_ens_update(b, theta) bind(C, name='nsx_ens_update')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      real(c_double), dimension(*), intent(in) :: theta
+    end function nsx_ens_update
 
     type(c_ptr) function nsx_last_error() bind(C, name='nsx_last_error')
       import :: c_ptr
