@@ -48,14 +48,24 @@ constexpr int kTile = 64, kKc = 16, kGT = 256;
 // chunks of 16 through LDS (the load order follows whichever stride is unit).
 // Epilogue: + bias[n] (optional), store to C (ldc); with Zact, also store the
 // activation (SiLU when act) to Zact.  accumulate: C += result.
+// Split K (part != null): block z of the grid's third dimension sums K range
+// [z kchunk, (z+1) kchunk) and stores its raw M x N partial at part + z M N;
+// k_gemm_reduce then sums the partials in order and applies the epilogue.
 template <typename TA, typename TB>
 __global__ __launch_bounds__(kGT) void k_gemm(int M, int N, int K, const TA* __restrict__ A, int64_t sam,
                                               int64_t sak, const TB* __restrict__ B, int64_t sbk, int64_t sbn,
                                               float* __restrict__ C, int64_t ldc, int accumulate,
-                                              const float* __restrict__ bias, int act, float* __restrict__ Zact) {
+                                              const float* __restrict__ bias, int act, float* __restrict__ Zact,
+                                              int kchunk, float* __restrict__ part) {
   __shared__ float As[kKc][kTile + 1];
   __shared__ float Bs[kKc][kTile + 1];
   const int tm = blockIdx.y * kTile, tn = blockIdx.x * kTile;
+  if (part) {  // this block's K range
+    const int kb = (int)blockIdx.z * kchunk;
+    A += (int64_t)kb * sak;
+    B += (int64_t)kb * sbk;
+    K = min(K - kb, kchunk);
+  }
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   float acc[4][4];
 #pragma unroll
@@ -98,6 +108,10 @@ __global__ __launch_bounds__(kGT) void k_gemm(int M, int N, int K, const TA* __r
       const int n = tn + tx + 16 * j;
       if (n >= N) continue;
       float v = acc[i][j];
+      if (part) {
+        part[((int64_t)blockIdx.z * M + m) * N + n] = v;
+        continue;
+      }
       if (bias) v += bias[n];
       float* c = C + (int64_t)m * ldc + n;
       if (accumulate) v += *c;
@@ -105,6 +119,24 @@ __global__ __launch_bounds__(kGT) void k_gemm(int M, int N, int K, const TA* __r
       if (Zact) Zact[(int64_t)m * ldc + n] = act ? silu(v) : v;
     }
   }
+}
+
+// Sum of the split-K partials (in split order) with k_gemm's epilogue
+__global__ __launch_bounds__(256) void k_gemm_reduce(int M, int N, int splits, const float* __restrict__ part,
+                                                     float* __restrict__ C, int64_t ldc, int accumulate,
+                                                     const float* __restrict__ bias, int act,
+                                                     float* __restrict__ Zact) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t mn = (int64_t)M * N;
+  if (i >= mn) return;
+  const int m = (int)(i / N), n = (int)(i - (int64_t)m * N);
+  float v = 0.0f;
+  for (int z = 0; z < splits; ++z) v += part[z * mn + i];
+  if (bias) v += bias[n];
+  float* c = C + (int64_t)m * ldc + n;
+  if (accumulate) v += *c;
+  *c = v;
+  if (Zact) Zact[(int64_t)m * ldc + n] = act ? silu(v) : v;
 }
 
 // db[n] = sum_m dZ[m * ld + n]
@@ -199,6 +231,8 @@ struct nsx_mlp {
   float* X = nullptr;    // input rows (cap x n_in)
   float* Y = nullptr;    // target rows (cap x n_out)
   hipStream_t s = nullptr;
+  float* ws = nullptr;   // split-K partials of the long-K products
+  size_t ws_elems = 0;
   int layers() const { return (int)width.size() - 1; }
 };
 
@@ -250,29 +284,73 @@ int ensure_rows(nsx_mlp* h, int rows) {
   return NSX_OK;
 }
 
+// Split-K sizing: few output tiles over a long K (the input gradient of the
+// output layer: 32 rows x 512 columns over n_out) would leave the GPU idle
+// with one block per tile, so K is cut into chunks of at least kSplitK, up to
+// kMaxSplits, each block storing a partial.
+constexpr int kSplitK = 512, kMaxSplits = 512, kSplitTiles = 128;
+
+// The workspace the split-K partials of an M x N product need (elements);
+// 0 when the product runs without splitting.
+int64_t split_elems(int M, int N, int K, int* splits, int* kchunk) {
+  const int64_t tiles = (int64_t)((N + kTile - 1) / kTile) * ((M + kTile - 1) / kTile);
+  *splits = 1;
+  *kchunk = K;
+  if (tiles >= kSplitTiles || K < 2 * kSplitK) return 0;
+  int S = (int)std::min<int64_t>((K + kSplitK - 1) / kSplitK, kMaxSplits);
+  const int kc = (K + S - 1) / S;
+  S = (K + kc - 1) / kc;
+  *splits = S;
+  *kchunk = kc;
+  return (int64_t)S * M * N;
+}
+
 template <typename TA, typename TB>
-void gemm(hipStream_t s, int M, int N, int K, const TA* A, int64_t sam, int64_t sak, const TB* B, int64_t sbk,
-          int64_t sbn, float* C, int64_t ldc, int accumulate, const float* bias, int act, float* Zact) {
-  const dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
-  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(kGT), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
-                     accumulate, bias, act, Zact);
+int gemm(nsx_mlp* h, int M, int N, int K, const TA* A, int64_t sam, int64_t sak, const TB* B, int64_t sbk,
+         int64_t sbn, float* C, int64_t ldc, int accumulate, const float* bias, int act, float* Zact) {
+  int S, kc;
+  const int64_t need = split_elems(M, N, K, &S, &kc);
+  if (need > 0) {
+    if ((size_t)need > h->ws_elems) {
+      MLP_HIP(hipStreamSynchronize(h->s));
+      (void)hipFree(h->ws);
+      h->ws = nullptr;
+      h->ws_elems = 0;
+      MLP_HIP(hipMalloc(&h->ws, sizeof(float) * (size_t)need));
+      h->ws_elems = (size_t)need;
+    }
+    const dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, S);
+    hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(kGT), 0, h->s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
+                       accumulate, bias, act, Zact, kc, h->ws);
+    const int64_t mn = (int64_t)M * N;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, h->s, M, N, S, h->ws, C, ldc,
+                       accumulate, bias, act, Zact);
+  } else {
+    const dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
+    hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(kGT), 0, h->s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
+                       accumulate, bias, act, Zact, K, (float*)nullptr);
+  }
+  MLP_HIP(hipGetLastError());
+  return NSX_OK;
 }
 
 // forward over `rows` rows already in h->X; output in h->A[L-1]
-void forward(nsx_mlp* h, int rows) {
+int forward(nsx_mlp* h, int rows) {
   const int L = h->layers();
   const float* in = h->X;
   for (int l = 0; l < L; ++l) {
     const int ni = h->width[l], no = h->width[l + 1];
     // Z = in . W^T + b ; A = silu(Z) (hidden) or Z (output)
-    gemm<float, uint16_t>(h->s, rows, no, ni, in, ni, 1, h->pbf + h->w_off[l], 1, ni, h->Z[l], no, 0,
-                          h->p + h->b_off[l], l + 1 < L ? 1 : 0, h->A[l]);
+    const int rc = gemm<float, uint16_t>(h, rows, no, ni, in, ni, 1, h->pbf + h->w_off[l], 1, ni, h->Z[l], no, 0,
+                                         h->p + h->b_off[l], l + 1 < L ? 1 : 0, h->A[l]);
+    if (rc) return rc;
     in = h->A[l];
   }
+  return NSX_OK;
 }
 
 // backward from dY (in h->dA, rows x n_out) into h->g; needs forward() first
-void backward(nsx_mlp* h, int rows) {
+int backward(nsx_mlp* h, int rows) {
   const int L = h->layers();
   float* d = h->dA;  // dZ of layer l
   float* e = h->dB;
@@ -280,16 +358,19 @@ void backward(nsx_mlp* h, int rows) {
     const int ni = h->width[l], no = h->width[l + 1];
     const float* in = l > 0 ? h->A[l - 1] : h->X;
     // dW[o, i] = sum_r dZ[r, o] in[r, i]
-    gemm<float, float>(h->s, no, ni, rows, d, 1, no, in, ni, 1, h->g + h->w_off[l], ni, 0, nullptr, 0, nullptr);
+    int rc = gemm<float, float>(h, no, ni, rows, d, 1, no, in, ni, 1, h->g + h->w_off[l], ni, 0, nullptr, 0, nullptr);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_colsum, dim3((no + 255) / 256), dim3(256), 0, h->s, d, rows, no, (int64_t)no,
                        h->g + h->b_off[l]);
     if (l == 0) break;
     // dA_prev[r, i] = sum_o dZ[r, o] W[o, i]; then dZ_prev = dA_prev * silu'(Z_prev)
-    gemm<float, uint16_t>(h->s, rows, ni, no, d, no, 1, h->pbf + h->w_off[l], ni, 1, e, ni, 0, nullptr, 0, nullptr);
+    rc = gemm<float, uint16_t>(h, rows, ni, no, d, no, 1, h->pbf + h->w_off[l], ni, 1, e, ni, 0, nullptr, 0, nullptr);
+    if (rc) return rc;
     const int64_t n = (int64_t)rows * ni;
     hipLaunchKernelGGL(k_silu_back, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->s, e, h->Z[l - 1], n);
     std::swap(d, e);
   }
+  return NSX_OK;
 }
 
 // loss and dY for rows in h->X / h->Y; returns sum of squares through *sq
@@ -370,6 +451,7 @@ void nsx_mlp_destroy(nsx_mlp* h) {
   if (!h) return;
   if (h->s) (void)hipStreamSynchronize(h->s);
   free_rows(h);
+  (void)hipFree(h->ws);
   (void)hipFree(h->p);
   (void)hipFree(h->g);
   (void)hipFree(h->m1);
@@ -410,11 +492,11 @@ int nsx_mlp_grad(nsx_mlp* h, const float* X, const float* Y, int32_t rows, float
   if (rc) return rc;
   MLP_HIP(hipMemcpyAsync(h->X, X, sizeof(float) * (size_t)rows * h->width[0], hipMemcpyHostToDevice, h->s));
   MLP_HIP(hipMemcpyAsync(h->Y, Y, sizeof(float) * (size_t)rows * h->width.back(), hipMemcpyHostToDevice, h->s));
-  forward(h, rows);
+  if ((rc = forward(h, rows))) return rc;
   double sq = 0.0;
   rc = loss_grad(h, rows, &sq);
   if (rc) return rc;
-  backward(h, rows);
+  if ((rc = backward(h, rows))) return rc;
   MLP_HIP(hipGetLastError());
   if (grad) MLP_HIP(hipMemcpyAsync(grad, h->g, sizeof(float) * h->n_params, hipMemcpyDeviceToHost, h->s));
   MLP_HIP(hipStreamSynchronize(h->s));
@@ -462,12 +544,12 @@ int nsx_mlp_train(nsx_mlp* h, const float* X, const float* Y, int32_t rows, int3
                          h->s, dX, didx + r0, h->X, nb, nin);
       hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 64), nb), dim3(256), 0,
                          h->s, dY, didx + r0, h->Y, nb, nout);
-      forward(h, nb);
+      if ((rc = forward(h, nb))) { cleanup(); return rc; }
       double sq = 0.0;
       rc = loss_grad(h, nb, epoch_loss ? &sq : nullptr);
       if (rc) { cleanup(); return rc; }
       ep_sq += sq;
-      backward(h, nb);
+      if ((rc = backward(h, nb))) { cleanup(); return rc; }
       ++h->step;
       const float c1 = 1.0f - std::pow(b1, (float)h->step), c2 = 1.0f - std::pow(b2, (float)h->step);
       hipLaunchKernelGGL(k_adam, dim3((unsigned)((h->n_params + 255) / 256)), dim3(256), 0, h->s, h->p, h->g, h->m1,
@@ -487,7 +569,7 @@ int nsx_mlp_predict(nsx_mlp* h, const float* X, int32_t rows, float* Y) {
   int rc = ensure_rows(h, rows);
   if (rc) return rc;
   MLP_HIP(hipMemcpyAsync(h->X, X, sizeof(float) * (size_t)rows * h->width[0], hipMemcpyHostToDevice, h->s));
-  forward(h, rows);
+  if ((rc = forward(h, rows))) return rc;
   MLP_HIP(hipGetLastError());
   MLP_HIP(hipMemcpyAsync(Y, h->A[h->layers() - 1], sizeof(float) * (size_t)rows * h->width.back(),
                          hipMemcpyDeviceToHost, h->s));

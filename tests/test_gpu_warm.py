@@ -12,7 +12,10 @@ from oracle import mlp as omlp
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(6, 32, 2, 10, 7), (66, 512, 3, 300, 40), (17, 64, 1, 1000, 130)])
+# the last two have an output layer long enough that the input-gradient
+# product of the output layer is split over K (partials summed in order)
+@pytest.mark.parametrize("shape", [(6, 32, 2, 10, 7), (66, 512, 3, 300, 40), (17, 64, 1, 1000, 130),
+                                   (20, 64, 2, 5000, 24), (33, 128, 1, 70000, 16)])
 def test_mlp_gradients_and_outputs_match_oracle(shape):
     from nushellx_amd.warm import MLP
     n_in, hid, layers, n_out, rows = shape
