@@ -284,8 +284,8 @@ int nsx_batch_trl_solve(nsx_batch *b, const nsx_trl_opts *opt, const double *sta
  * sample s) and works with nsx_batch_apply_dev, nsx_batch_trl_solve,
  * nsx_batch_time and the other batch calls; autotune and set_config are
  * refused (pass a tuning result at creation).  It runs on the stacked path
- * (block sides up to 40).  No counterpart in the reference, which reads one
- * file set per interaction.
+ * and the large-block tiles (any block side up to 2048).  No counterpart in
+ * the reference, which reads one file set per interaction.
  *
  * Value layout of a template (K_t = n_oph + sum d^2): the .oph record values in
  * file order (the raw me, before renorm), then per H block (side a, then side b,

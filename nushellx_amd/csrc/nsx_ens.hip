@@ -13,11 +13,13 @@
 //   Bhat^T[s] = sum_p renorm val[s, p] Bsrc_p           (fold, per recipe)
 //   H[s]      = the H columns of values[s]
 // after which the stacked kernels run all samples at fixed strides, every
-// sample's workgroups on one XCD.
+// sample's workgroups on one XCD, and the large-block tiles (blocks with a side
+// above 40) run every sample as the second grid dimension.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -275,6 +277,8 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   hp.cfg.stacked = 1;
   hp.cfg.deterministic = 0;
   hp.cfg.big_rows1 = precision == NSX_BF16X3 ? 2 * kBigTile : kBigTile;
+  // test hook, as for batches: a smaller block side above which the large-block tiles run
+  if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
   hp.ens = true;
   for (int t = 0; t < n_t; ++t) {
     hp.ens_t = t;
@@ -282,9 +286,10 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
     plan_add(hp, *tms[t]);
   }
   build_stacked(hp, tms);
-  if (!hp.cfg.stacked || !hp.items.empty()) {
-    set_error("linear ensembles run on the stacked path: a template has blocks with a side above 40 or "
-              "a-terms beyond a stacked chunk");
+  build_big(hp, tms);
+  if (!hp.cfg.stacked || (!hp.items.empty() && hp.g_tile2.empty())) {
+    set_error("linear ensembles run on the stacked path and the large-block tiles: a template has a-terms "
+              "beyond a stacked chunk or a block above the tiles' limit");
     return NSX_ECAP;
   }
   EnsData& E = b->ens;
@@ -328,7 +333,7 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
       }
   }
   // instances: the templates in order, sample-major
-  const int64_t N_t = hp.N, x_t = hp.x_size;
+  const int64_t N_t = hp.N, x_t = hp.x_size, xg_t = hp.xg_size;
   const std::vector<int64_t> vb_t = hp.vbase;
   hp.vbase.clear();
   for (int s = 0; s < S; ++s)
@@ -343,6 +348,7 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   }
   hp.N = (int64_t)S * N_t;
   hp.x_size = (int64_t)S * x_t;
+  hp.xg_size = (int64_t)S * xg_t;
   hp.flops_ref *= S;
   hp.flops_like *= S;
   hp.bytes_alg *= S;
@@ -372,6 +378,7 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   d.es.x = x_t;
   d.es.b = E.b_t;
   d.es.h = E.h_t;
+  d.es.xg = xg_t;
   d.es.n = S;
   if (!(e = up(E.theta, theta, (size_t)S * P)).empty()) { set_error(e); return NSX_EGPU; }
   if (!(e = up(E.basis, cat.data(), cat.size())).empty()) { set_error(e); return NSX_EGPU; }

@@ -1060,11 +1060,18 @@ __device__ __forceinline__ void big_zero(typename BigMma<C>::vec (&acc)[2][CB]) 
 template <typename T>
 __global__ __launch_bounds__(256) void k_big1(const GTile1* __restrict__ tiles,
                                               const GTask1* __restrict__ tasks, const T* __restrict__ V,
-                                              const T* __restrict__ Bt, typename BigIn<T>::C* __restrict__ X) {
+                                              const T* __restrict__ Bt, typename BigIn<T>::C* __restrict__ X,
+                                              EnsStride es) {
   using C = typename BigIn<T>::C;
   __shared__ C sP[2][kBigK * kBigLd], sQ[2][kBigK * kBigLd];  // double-buffered: one barrier per K step
   const GTile1 t = tiles[blockIdx.x];
   const GTask1 g = tasks[t.task];
+  {  // linear ensembles: blockIdx.y is the sample (strides zero otherwise)
+    const int64_t smp = blockIdx.y;
+    V += smp * es.v;
+    Bt += smp * es.b;
+    X += smp * es.xg;
+  }
   const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
@@ -1126,11 +1133,16 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
                                               const GTerm2* __restrict__ terms, const SXRef* __restrict__ xrefs,
                                               const T* __restrict__ Ad, const typename BigIn<T>::C* __restrict__ X,
-                                              double* __restrict__ W) {
+                                              double* __restrict__ W, EnsStride es) {
   using C = typename BigIn<T>::C;
   __shared__ C sP[2][kBigK * kBigLd], sQ[2][kBigK * kBigLd];  // double-buffered: one barrier per K step
   const GTile2 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  {  // linear ensembles: blockIdx.y is the sample (strides zero otherwise)
+    const int64_t smp = blockIdx.y;
+    X += smp * es.xg;
+    W += smp * es.v;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
   typename BigMma<C>::vec acc[2][2];
@@ -1213,11 +1225,17 @@ __global__ __launch_bounds__(256) void k_big2(const GTile2* __restrict__ tiles,
 // V_L (da x db), then V_L times H_b^T, where H_b is symmetric: H_b^T(k, j) = H_b(k, j).
 template <typename T>
 __global__ __launch_bounds__(256) void k_big0(const GTile3* __restrict__ tiles, const T* __restrict__ V,
-                                              const T* __restrict__ H, double* __restrict__ W) {
+                                              const T* __restrict__ H, double* __restrict__ W, EnsStride es) {
   using C = typename BigIn<T>::C;
   __shared__ C sP[kBigK * kBigLd], sQ[kBigK * kBigLd];
   const GTile3 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  {  // linear ensembles: blockIdx.y is the sample (strides zero otherwise)
+    const int64_t smp = blockIdx.y;
+    V += smp * es.v;
+    H += smp * es.h;
+    W += smp * es.v;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
   typename BigMma<C>::vec acc[2][2];
@@ -1316,11 +1334,17 @@ __device__ __forceinline__ void big_mma_bf(const BfTile<TM>& b, int wj, int wi, 
 template <int TM>
 __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tiles, const GTask1* __restrict__ tasks,
                                                  const Bf16x2* __restrict__ V, const Bf16x2* __restrict__ Bt,
-                                                 float* __restrict__ X) {
+                                                 float* __restrict__ X, EnsStride es) {
   constexpr int CB = TM / 32, QV = TM * kBigK / 256, VS = 256 / TM;
   __shared__ BfTile<TM> sb[2];  // double-buffered: one barrier per K step
   const GTile1 t = tiles[blockIdx.x];
   const GTask1 g = tasks[t.task];
+  {  // linear ensembles: blockIdx.y is the sample (strides zero otherwise)
+    const int64_t smp = blockIdx.y;
+    V += smp * es.v;
+    Bt += smp * es.b;
+    X += smp * es.xg;
+  }
   const int m = g.m, n = g.n, K = g.k, i0 = t.i0, j0 = t.j0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * (TM / 2);
@@ -1377,10 +1401,15 @@ __global__ __launch_bounds__(256) void k_big1_bf(const GTile1* __restrict__ tile
 // and split into a hi/lo pair while staging
 __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tiles, const GTerm2* __restrict__ terms,
                                                  const SXRef* __restrict__ xrefs, const Bf16x2* __restrict__ Ad,
-                                                 const float* __restrict__ X, double* __restrict__ W) {
+                                                 const float* __restrict__ X, double* __restrict__ W, EnsStride es) {
   __shared__ BfTile<> sb[2];
   const GTile2 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  {  // linear ensembles: blockIdx.y is the sample (strides zero otherwise)
+    const int64_t smp = blockIdx.y;
+    X += smp * es.xg;
+    W += smp * es.v;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
   f32x4 acc[2][2];
@@ -1456,10 +1485,16 @@ __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tile
 
 // G0 in BF16 hi/lo: W_L = H_a V_L + V_L H_b^T for one 64 x 64 tile, stored
 __global__ __launch_bounds__(256) void k_big0_bf(const GTile3* __restrict__ tiles, const Bf16x2* __restrict__ V,
-                                                 const Bf16x2* __restrict__ H, double* __restrict__ W) {
+                                                 const Bf16x2* __restrict__ H, double* __restrict__ W, EnsStride es) {
   __shared__ BfTile<> sb[1];
   const GTile3 t = tiles[blockIdx.x];
   const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  {  // linear ensembles: blockIdx.y is the sample (strides zero otherwise)
+    const int64_t smp = blockIdx.y;
+    V += smp * es.v;
+    H += smp * es.h;
+    W += smp * es.v;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
   f32x4 acc[2][2];
@@ -1639,20 +1674,20 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
     if constexpr (std::is_same<T, Bf16x2>::value) {  // BF16 pairs on the BF16 matrix cores
       if (d.n_gtile1 > 0) {
         if (d.big_rows1 == 2 * kBigTile)
-          hipLaunchKernelGGL(k_big1_bf<2 * kBigTile>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1,
-                             d.g_task1, Vp, Bh, XG);
+          hipLaunchKernelGGL(k_big1_bf<2 * kBigTile>, dim3((unsigned)d.n_gtile1, (unsigned)d.ens_s), dim3(256), 0, s,
+                             d.g_tile1, d.g_task1, Vp, Bh, XG, d.es);
         else
-          hipLaunchKernelGGL(k_big1_bf<kBigTile>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1,
-                             d.g_task1, Vp, Bh, XG);
+          hipLaunchKernelGGL(k_big1_bf<kBigTile>, dim3((unsigned)d.n_gtile1, (unsigned)d.ens_s), dim3(256), 0, s,
+                             d.g_tile1, d.g_task1, Vp, Bh, XG, d.es);
       }
-      hipLaunchKernelGGL(k_big2_bf, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2, d.g_xref, Ad,
-                         (const C*)XG, W);
+      hipLaunchKernelGGL(k_big2_bf, dim3((unsigned)d.n_gtile2, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile2,
+                         d.g_term2, d.g_xref, Ad, (const C*)XG, W, d.es);
     } else {
       if (d.n_gtile1 > 0)
-        hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1), dim3(256), 0, s, d.g_tile1, d.g_task1,
-                           Vp, Bh, XG);
-      hipLaunchKernelGGL(k_big2<T>, dim3((unsigned)d.n_gtile2), dim3(256), 0, s, d.g_tile2, d.g_term2,
-                         d.g_xref, Ad, (const C*)XG, W);
+        hipLaunchKernelGGL(k_big1<T>, dim3((unsigned)d.n_gtile1, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile1,
+                           d.g_task1, Vp, Bh, XG, d.es);
+      hipLaunchKernelGGL(k_big2<T>, dim3((unsigned)d.n_gtile2, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile2,
+                         d.g_term2, d.g_xref, Ad, (const C*)XG, W, d.es);
     }
     return hipGetLastError();
   }
@@ -1680,9 +1715,11 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
                        V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
   if (big) {
     if constexpr (std::is_same<T, Bf16x2>::value)
-      hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+      hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile3, V, H,
+                         W, d.es);
     else
-      hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+      hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile3, V, H,
+                         W, d.es);
   }
   return hipGetLastError();
 }
@@ -1756,9 +1793,11 @@ static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T*
   const bool big = big_like_active(d);
   if (big) {
     if constexpr (std::is_same<T, Bf16x2>::value)
-      hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+      hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile3, V, H,
+                         W, d.es);
     else
-      hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3), dim3(256), 0, s, d.g_tile3, V, H, W);
+      hipLaunchKernelGGL(k_big0<T>, dim3((unsigned)d.n_gtile3, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile3, V, H,
+                         W, d.es);
   }
   DevicePlan small = d;   // k_fuse alone (the large-block items run on the tiles below)
   small.n_gtile2 = 0;

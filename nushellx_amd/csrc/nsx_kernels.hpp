@@ -9,9 +9,9 @@
 namespace nsx {
 
 // Per-sample element strides of a linear ensemble (vectors, X, Bhat^T store,
-// H store) and its sample count n; all zero for an ordinary plan.
+// H store, large-block X) and its sample count n; all zero for an ordinary plan.
 struct EnsStride {
-  int64_t v = 0, x = 0, b = 0, h = 0;
+  int64_t v = 0, x = 0, b = 0, h = 0, xg = 0;
   int32_t n = 0, pad = 0;
 };
 
@@ -71,7 +71,8 @@ struct DevicePlan {
   // truncated plans: the window mask of the pn input and the masked FP64 copy
   uint8_t* vmask = nullptr;
   double* Vm = nullptr;
-  // linear ensemble: the stacked kernels run ens_s samples (grid y) at strides es
+  // linear ensemble: the stacked and large-block kernels run ens_s samples
+  // (grid y) at strides es
   int ens_s = 1;
   EnsStride es;
 };

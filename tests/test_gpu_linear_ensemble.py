@@ -132,3 +132,49 @@ def test_linear_ensemble_with_tuned_config(ens):
         lin.autotune(iters=1)
     ref.close()
     lin.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
+def test_linear_ensemble_on_large_block_tiles(ens, monkeypatch, prec, tol):
+    """Blocks with a side above 12 on the matrix-core tiles (every sample as the
+    second grid dimension), against the per-sample file batch in FP64."""
+    from nushellx_amd.hv import Batch
+    ref = Batch(ens["hams"])
+    v = np.random.default_rng(21).standard_normal(ref.N)
+    w_ref = _hv(ref, v)
+    ref.close()
+    monkeypatch.setenv("NSX_BIG_MIN", "12")
+    lin, _, _ = _linear(ens, prec)
+    w = _hv(lin, v)
+    for k, o in enumerate(lin.offsets):
+        n = ens["hams"][k].N
+        assert _rel(w[o:o + n], w_ref[o:o + n]) < tol, (prec, k)
+    lin.close()
+
+
+def test_linear_ensemble_pf44():
+    """pf 4p4n J=0 (blocks up to 99 on the large-block tiles): three samples
+    against their own file sets."""
+    from nushellx_amd.ensemble import build_instances, linear_ensemble, write_ensemble
+    from nushellx_amd.hv import Batch
+    from nushellx_amd.synth.interaction import random_interaction
+    from nushellx_amd.synth.mspace import PF, PNSpace
+    from nushellx_amd.synth.writer import SideCache
+    space, cache = PNSpace(PF), SideCache()
+    base = random_interaction(PF, seed=2024, spe_range=(-10.0, -4.0))
+    seeds = [301, 302, 303]
+    print("case linear ensemble pf44: per-sample files", flush=True)
+    entries = write_ensemble(space, [(4, 4)], [0], base, seeds, tempfile.mkdtemp(prefix="nsx_pf_files_"),
+                             cache=cache, real_kind=8)
+    ref = Batch(build_instances(entries, [0], threads=4))
+    print("case linear ensemble pf44: linear", flush=True)
+    lin, _, _ = linear_ensemble(space, [(4, 4)], [0], base, seeds, tempfile.mkdtemp(prefix="nsx_pf_lin_"),
+                                cache=cache)
+    assert lin.N == ref.N
+    v = np.random.default_rng(31).standard_normal(ref.N)
+    assert _rel(_hv(lin, v), _hv(ref, v)) < 1e-12
+    r = lin.lanczos(nev=3, tol=1e-6)
+    r_ref = ref.lanczos(nev=3, tol=1e-6)
+    assert np.abs(r["evals"] - r_ref["evals"]).max() < 1e-6
+    ref.close()
+    lin.close()
