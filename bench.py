@@ -259,8 +259,8 @@ def precision_study(log):
     return out
 
 
-def pf_warm(log, train_samples: int, nprot: int = 4, nneut: int = 4, chunk: int = 64, held_n: int = 8,
-            threads: int = 16, train_tol: float = 1e-6):
+def pf_warm(log, train_samples: int, nprot: int = 4, nneut: int = 4, chunk: int = 256, held_n: int = 8,
+            threads: int = 16, train_tol: float = 1e-5):
     """Learned warm start on pf 4p4n J=0 (N = 41,355; 195 TBMEs ~ N(0, 1.5 MeV),
     SPEs ~ U[-10, -4] MeV), or pf 6p6n J=0 (N = 1,777,116) with nprot = nneut
     = 6: network trained on the lowest-8 Lanczos eigenvectors of
@@ -519,9 +519,9 @@ def main():
                     help="skip the pf 6p6n J=0 H*v (N = 1.78 M; about a minute, mostly writing its files)")
     ap.add_argument("--pf66-warm-samples", type=int, default=0,
                     help="training samples of the pf 6p6n J=0 warm-start study (0 skips it; 16 take ~6 min)")
-    ap.add_argument("--pf-warm-samples", type=int, default=64,
-                    help="training samples of the pf 4p4n warm-start study (0 skips it; 64 take ~1 min, "
-                         "512 ~8 min)")
+    ap.add_argument("--pf-warm-samples", type=int, default=2000,
+                    help="training samples of the pf 4p4n warm-start study (0 skips it; solved 256 at a time "
+                         "as linear ensembles)")
     ap.add_argument("--no-det", action="store_true", help="skip the deterministic-mode timings")
     ap.add_argument("--shape", default="",
                     help='fixed plan shape for the headline batch instead of autotuning, as JSON, e.g. '

@@ -11,7 +11,7 @@ sign-aligned to those of a reference interaction and summed (normalised per
 instance).  A start vector in the span of the wanted states makes the Krylov
 space capture all of them within a few steps.
 
-The random-start thick-restart Lanczos stays the fallback and the judge: a
+The random-start thick-restart Lanczos stays the fallback and the arbiter: a
 predicted start vector whose Rayleigh-quotient residual is not below that of a
 random vector, per instance, is replaced by a random one; the solver's own
 residual test decides convergence either way.  The reference has no such
@@ -189,77 +189,71 @@ def solve_batch(mf, J2s, nev: int = 5, tol: float = 1e-6, precision: str = "fp64
     return r, b, hams
 
 
+def _linear_training_set(space, nprot, nneut, J2s, base_inter, seeds, workdir, name, nev, tol, rel_sigma, chunk,
+                         cache, log):
+    """Training targets from linear ensembles: the base interaction's files are
+    written once (densities included) and read as one template per J; the
+    samples are solved ``chunk`` at a time as one lockstep batch whose operators
+    are made on the device from their parameters (no files per sample)."""
+    from .ensemble import template_basis
+    from .hv import Batch, Hamiltonian
+    from .synth.writer import PNModel, SideCache, write_model
+    cache = cache or SideCache()
+    model = PNModel(space, nprot, nneut, base_inter)
+    mf = write_model(model, workdir, name, list(J2s), cache=cache, real_kind=8)
+    tmpl = [Hamiltonian(*mf.args(J2), device=False, template=True) for J2 in J2s]
+    basis = template_basis(tmpl[0], space, model, cache)  # every J reads the same side and interaction files
+    bases = [basis] * len(tmpl)
+    refb = Batch(tmpl)
+    ref = refb.lanczos(nev=nev, tol=tol, vectors=True)
+    ref["offsets"] = list(refb.offsets)
+    ref["dims"] = dims = [h.N for h in tmpl]
+    refb.close()
+    nJ = len(J2s)
+    feats, targs = [], []
+    seeds = list(seeds)
+    for c0 in range(0, len(seeds), chunk):
+        part = seeds[c0:c0 + chunk]
+        theta = np.stack([base_inter.perturbed(int(s), rel_sigma).parameter_vector() for s in part])
+        b = Batch.linear_ensemble(tmpl, theta, bases)
+        r = b.lanczos(nev=nev, tol=tol, vectors=True)
+        for k in range(len(part)):
+            if (r["converged_at"][k * nJ:(k + 1) * nJ] < 0).any():
+                continue  # unconverged samples are not used as targets
+            o0 = b.offsets[k * nJ]
+            vec = r["evecs"][:, o0:o0 + sum(dims)]
+            offs = [b.offsets[k * nJ + j] - o0 for j in range(nJ)]
+            feats.append(theta[k])
+            targs.append(align_and_sum(vec, ref["evecs"], offs, dims))
+        b.close()
+        if log:
+            log(f"training set: {min(c0 + chunk, len(seeds))}/{len(seeds)} samples solved")
+    for h in tmpl:
+        h.close()
+    return np.array(feats), np.array(targs), ref["evecs"], ref
+
+
 def build_training_set(space, nprot: int, nneut: int, J2s: Sequence[int], base_inter, seeds: Sequence[int],
                        workdir: str, name: str = "ws", nev: int = 8, tol: float = 1e-6, rel_sigma: float = 0.1,
-                       cache=None, log=None):
+                       cache=None, log=None, chunk: int = 256):
     """Features and targets from Lanczos solves of perturbed interactions.
 
     Targets sum the lowest ``nev`` eigenvectors; taking a few more states than
     are wanted keeps a level crossing just above the wanted ones from flipping
     the target between neighbouring samples.  The operator files of the
-    reference interaction are written once (the densities are
-    interaction-independent); each sample rewrites only its
-    interaction-dependent files.  Returns (features, targets, reference
-    eigenvectors, reference solve)."""
-    from .synth.writer import PNModel, SideCache, write_model
-    cache = cache or SideCache()
-    mf = write_model(PNModel(space, nprot, nneut, base_inter), workdir, name, list(J2s), cache=cache)
-    ref, b, hams = solve_batch(mf, J2s, nev=nev, tol=tol)
-    b.close()
-    feats, targs = [], []
-    for k, s in enumerate(seeds):
-        inter = base_inter.perturbed(int(s), rel_sigma)
-        mf = write_model(PNModel(space, nprot, nneut, inter), workdir, name, list(J2s), cache=cache,
-                         densities=False, xtra="y")
-        r, b, hams = solve_batch(mf, J2s, nev=nev, tol=tol)
-        b.close()
-        if (r["converged_at"] < 0).any():
-            continue  # unconverged samples are not used as targets
-        feats.append(inter.parameter_vector())
-        targs.append(align_and_sum(r["evecs"], ref["evecs"], r["offsets"], r["dims"]))
-        if log and (k + 1) % 32 == 0:
-            log(f"training set: {k + 1}/{len(seeds)} samples solved")
-    return np.array(feats), np.array(targs), ref["evecs"], ref
+    reference interaction are written once (in ``workdir`` under ``name``,
+    densities included, so that further interactions can be written beside
+    them without densities); the samples are solved ``chunk`` at a time as a
+    linear ensemble.  Returns (features, targets, reference eigenvectors,
+    reference solve)."""
+    return _linear_training_set(space, nprot, nneut, J2s, base_inter, seeds, workdir, name, nev, tol, rel_sigma,
+                                chunk, cache, log)
 
 
 def build_training_set_batched(space, nprot: int, nneut: int, J2s: Sequence[int], base_inter,
                                seeds: Sequence[int], workdir: str, nev: int = 8, tol: float = 1e-6,
                                rel_sigma: float = 0.1, chunk: int = 64, threads: int = 16, log=None):
-    """As ``build_training_set``, with the samples solved ``chunk`` at a time
-    as one lockstep batch of host-only instances (for large model spaces)."""
-    from .ensemble import build_instances, solve_ensemble, write_ensemble
-    from .synth.writer import SideCache
-    cache = SideCache()
-    ref_entries = write_ensemble(space, [(nprot, nneut)], J2s, base_inter, [0], workdir + "/ref",
-                                 rel_sigma=0.0, cache=cache)
-    ref_hams = build_instances(ref_entries, J2s, threads=threads)
-    ref = solve_ensemble(ref_hams, nev=nev, tol=tol, vectors=True, tune_on=None)
-    ref["batch"].close()
-    feats, targs = [], []
-    seeds = list(seeds)
-    for c0 in range(0, len(seeds), chunk):
-        part = seeds[c0:c0 + chunk]
-        entries = write_ensemble(space, [(nprot, nneut)], J2s, base_inter, part, f"{workdir}/c{c0}",
-                                 rel_sigma=rel_sigma, cache=cache)
-        hams = build_instances(entries, J2s, threads=threads)
-        r = solve_ensemble(hams, nev=nev, tol=tol, vectors=True)
-        b = r["batch"]
-        nJ = len(J2s)
-        for k, s in enumerate(part):
-            sl = slice(k * nJ, (k + 1) * nJ)
-            if (r["converged_at"][sl] < 0).any():
-                continue
-            o0 = b.offsets[k * nJ]
-            n_tot = sum(h.N for h in hams[sl])
-            vec = r["evecs"][:, o0:o0 + n_tot]
-            offs = [b.offsets[k * nJ + j] - o0 for j in range(nJ)]
-            feats.append(base_inter.perturbed(int(s), rel_sigma).parameter_vector())
-            targs.append(align_and_sum(vec, ref["evecs"], offs, [h.N for h in hams[sl]]))
-        b.close()
-        for h in hams:
-            h.close()
-        if log:
-            log(f"training set: {min(c0 + chunk, len(seeds))}/{len(seeds)} samples solved")
-    for h in ref_hams:
-        h.close()
-    return np.array(feats), np.array(targs), ref["evecs"], ref
+    """As ``build_training_set`` with a smaller default chunk (large model
+    spaces); ``threads`` is kept for callers and unused."""
+    return _linear_training_set(space, nprot, nneut, J2s, base_inter, seeds, workdir, "ws", nev, tol, rel_sigma,
+                                chunk, None, log)
