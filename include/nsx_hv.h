@@ -113,9 +113,12 @@ int nsx_batch_work(const nsx_batch *b, double *flops, double *flops_like, int64_
 int nsx_batch_apply_dev(nsx_batch *b, const double *d_in, double *d_out, void *stream);
 
 /* ---------------------------------------------------------------- tuning
- * Autotuner of the staged pn kernel: rebuilds the work plan for several phase
- * capacities and chunk lengths and times every (threads, load mode) instance
- * with HIP events on the instance's own operators; the fastest is kept.
+ * Autotuner: rebuilds the work plan for several shapes of the small-block
+ * paths (staged: phase capacities, chunk lengths, threads, load mode; stacked:
+ * chunk LDS and work; fused: round LDS and rounds) and, when the plan has
+ * blocks with a side above 40, of the large-block tiles (stage-1 rows,
+ * stage-2 K chunk), and times each with HIP events on the instance's own
+ * operators; the fastest is kept.
  * No counterpart in the reference (its loop nest has no tunable shape). */
 typedef struct {
   int32_t fat_cap;     /* doubles one phase stages in LDS */
@@ -137,6 +140,10 @@ typedef struct {
   int32_t deterministic; /* 1: bitwise reproducible H*v: the fused path for the small
                             blocks and one stage-2 workgroup per large-block W tile
                             (also set for any plan by NSX_DETERMINISTIC=1) */
+  int32_t big_rows1;   /* large-block stage-1 tile rows: 64, or 128 (BF16 hi/lo only);
+                          0 = by precision (128 for BF16 hi/lo, else 64) */
+  int32_t big_kchunk;  /* large-block stage-2 tiles: active K steps one workgroup takes
+                          (split-K chunk; 0 = default 128) */
 } nsx_tune_result;
 int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);
 int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);

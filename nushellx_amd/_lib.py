@@ -43,7 +43,8 @@ class TuneResult(C.Structure):
     _fields_ = [("fat_cap", C.c_int32), ("max_phases", C.c_int32), ("threads", C.c_int32),
                 ("mode", C.c_int32), ("atomic", C.c_int32), ("ms", C.c_double),
                 ("stacked", C.c_int32), ("stk_cap", C.c_int32), ("stk_work", C.c_int32),
-                ("f_rounds", C.c_int32), ("deterministic", C.c_int32)]
+                ("f_rounds", C.c_int32), ("deterministic", C.c_int32), ("big_rows1", C.c_int32),
+                ("big_kchunk", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

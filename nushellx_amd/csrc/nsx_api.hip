@@ -561,6 +561,8 @@ PlanConfig plan_config(const nsx_tune_result& r) {
   if (r.stk_work > 0) pc.stk_work = r.stk_work;
   if (r.f_rounds > 0) pc.f_rounds = r.f_rounds;
   pc.deterministic = r.deterministic ? 1 : 0;
+  if (r.big_rows1 == kBigTile || r.big_rows1 == 2 * kBigTile) pc.big_rows1_req = r.big_rows1;
+  if (r.big_kchunk > 0) pc.big_kchunk = r.big_kchunk;
   return pc;
 }
 
@@ -570,7 +572,8 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int prec
   const int n = (int)models.size();
   // BF16 stage-1 tiles are bound by operand traffic rather than the matrix
   // cores: 128-row tiles halve the re-reads of each Bhat^T panel
-  hp.cfg.big_rows1 = precision == NSX_BF16X3 ? 2 * kBigTile : kBigTile;
+  // 128-row stage-1 tiles exist in BF16 hi/lo only
+  hp.cfg.big_rows1 = precision == NSX_BF16X3 ? (hp.cfg.big_rows1_req ? hp.cfg.big_rows1_req : 2 * kBigTile) : kBigTile;
   // test hook: a smaller block side above which the large-block GEMM path runs,
   // so that small models exercise it
   if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
@@ -659,7 +662,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
   std::vector<double> h(N);
   for (int64_t i = 0; i < N; ++i) h[i] = std::sin(0.37 * double(i + 1));
   (void)hipMemcpy(in, h.data(), sizeof(double) * N, hipMemcpyHostToDevice);
-  nsx_tune_result best{0, 0, 0, 0, 0, 1e300, 0, 0, 0, 0, 0};
+  nsx_tune_result best{0, 0, 0, 0, 0, 1e300, 0, 0, 0, 0, 0, 0, 0};
   const int caps[] = {768, 1536, 2560, kFatCap};
   const int chunks[] = {1, 2, 4, 8};
   const int nts[] = {64, 128, 256};
@@ -677,7 +680,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
             d.staged_mode = mode;
             d.atomic = at;
             const double ms = time_hv(d, lp, precision, in, out, s, iters);
-            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms, 0, 0, 0, 0, 0};
+            if (ms > 0.0 && ms < best.ms) best = {cap, mp, nt, mode, at, ms, 0, 0, 0, 0, 0, 0, 0};
           }
     }
   // stacked two-stage path: chunk capacity (LDS doubles) x work units x threads
@@ -693,7 +696,7 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
         d.stk_nt = nt;
         d.atomic = 1;
         const double ms = time_hv(d, lp, precision, in, out, s, iters);
-        if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, nt, 0, 1, ms, 1, scap, swork, 0, 0};
+        if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, nt, 0, 1, ms, 1, scap, swork, 0, 0, 0, 0};
       }
     }
   // fused column path: LDS of a round x work of a task
@@ -708,18 +711,44 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
       if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
       if (!d.fused) break;  // a column exceeds the fused kernel's limits
       const double ms = time_hv(d, lp, precision, in, out, s, iters);
-      if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, kFuseThreads, 0, 1, ms, 2, fcap, 250000, frounds, 0};
+      if (ms > 0.0 && ms < best.ms)
+        best = {kFatCap, kMaxPhases, kFuseThreads, 0, 1, ms, 2, fcap, 250000, frounds, 0, 0, 0};
     }
-  (void)hipFree(in);
-  (void)hipFree(out);
-  if (best.ms >= 1e300) { set_error("autotune: no configuration ran"); return NSX_EGPU; }
+  if (best.ms >= 1e300) {
+    set_error("autotune: no configuration ran");
+    (void)hipFree(in);
+    (void)hipFree(out);
+    return NSX_EGPU;
+  }
   const PlanConfig cfg = plan_config(best);
   d.staged_nt = best.threads;
   d.staged_mode = best.mode;
   d.atomic = best.atomic;
   d.stk_nt = best.threads;
   std::string e = replan(models, cfg, precision, hp, d, lp);
-  if (!e.empty()) { set_error(e); return NSX_EGPU; }
+  if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+  // large-block tiles, on the chosen small-block shape: stage-2 K chunk and
+  // (BF16 hi/lo) stage-1 rows
+  if (d.n_gtile2 > 0 && !hp.cfg.deterministic) {
+    std::vector<int> rows = {kBigTile};
+    if (precision == NSX_BF16X3) rows.push_back(2 * kBigTile);
+    nsx_tune_result bb = best;
+    for (int r1 : rows)
+      for (int kc : {32, 64, 128, 256}) {
+        nsx_tune_result t = best;
+        t.big_rows1 = r1;
+        t.big_kchunk = kc;
+        e = replan(models, plan_config(t), precision, hp, d, lp);
+        if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+        t.ms = time_hv(d, lp, precision, in, out, s, iters);
+        if (t.ms > 0.0 && (bb.big_kchunk == 0 || t.ms < bb.ms)) bb = t;
+      }
+    best = bb;
+    e = replan(models, plan_config(best), precision, hp, d, lp);
+    if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+  }
+  (void)hipFree(in);
+  (void)hipFree(out);
   if (best_out) *best_out = best;
   return NSX_OK;
 }

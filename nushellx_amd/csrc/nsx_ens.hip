@@ -276,7 +276,7 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   hp.cfg = cfg ? plan_config(*cfg) : PlanConfig();
   hp.cfg.stacked = 1;
   hp.cfg.deterministic = 0;
-  hp.cfg.big_rows1 = precision == NSX_BF16X3 ? 2 * kBigTile : kBigTile;
+  hp.cfg.big_rows1 = precision == NSX_BF16X3 ? (hp.cfg.big_rows1_req ? hp.cfg.big_rows1_req : 2 * kBigTile) : kBigTile;
   // test hook, as for batches: a smaller block side above which the large-block tiles run
   if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
   hp.ens = true;

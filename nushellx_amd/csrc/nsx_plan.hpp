@@ -236,6 +236,7 @@ struct PlanConfig {
   // stage-2 large-block tiles take about this many active K steps each (split-K
   // over several workgroups adding into one W tile)
   int big_kchunk = kBigKChunk;
+  int big_rows1_req = 0;     // requested stage-1 rows (0: by precision)
   // bitwise reproducible H*v: the fused path for the small blocks and one
   // stage-2 workgroup per large W tile (its single atomic add lands on the
   // stored like-particle value, so the order of additions cannot vary)

@@ -315,7 +315,8 @@ class Batch:
     def set_config(self, cfg: dict) -> None:
         r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"],
                        cfg.get("atomic", 0), 0.0, cfg.get("stacked", 0), cfg.get("stk_cap", 0),
-                       cfg.get("stk_work", 0), cfg.get("f_rounds", 0), cfg.get("deterministic", 0))
+                       cfg.get("stk_work", 0), cfg.get("f_rounds", 0), cfg.get("deterministic", 0),
+                       cfg.get("big_rows1", 0), cfg.get("big_kchunk", 0))
         check(lib().nsx_batch_set_config(self._b, C.byref(r)), "nsx_batch_set_config")
 
     def lanczos(self, nev: int = 5, max_basis: int = 50, keep: int = 10, tol: float = 1e-6,

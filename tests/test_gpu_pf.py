@@ -28,6 +28,26 @@ def test_pf44_hv_matches_oracle(pf44):
         h.close()
 
 
+def test_pf44_autotuned_large_block_tiles(pf44):
+    """The autotuner also picks the large-block tile shape (stage-2 K chunk;
+    stage-1 rows in BF16 hi/lo); the tuned plan still equals the oracle."""
+    from nushellx_amd.hv import Hamiltonian
+    args = pf44.args(0)
+    ref = RefCPU(*args, fp64=True)
+    v = np.random.default_rng(6).standard_normal(ref.N)
+    w_ref = ref.apply(v)
+    for prec, tol in (("fp64", 1e-12), ("bf16x3", 2e-4)):
+        print(f"case pf44 autotune {prec}", flush=True)
+        h = Hamiltonian(*args, precision=prec)
+        cfg = h.autotune(iters=2)
+        print(f"{prec}: {cfg}", flush=True)
+        assert cfg["big_kchunk"] in (32, 64, 128, 256)
+        assert cfg["big_rows1"] in ((64,) if prec == "fp64" else (64, 128))
+        err = np.abs(h.multiply(v) - w_ref).max() / np.abs(w_ref).max()
+        assert err < tol, (prec, err)
+        h.close()
+
+
 def test_pf44_lanczos_matches_arpack(pf44):
     from scipy.sparse.linalg import LinearOperator, eigsh
 
