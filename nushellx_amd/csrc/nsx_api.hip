@@ -83,6 +83,21 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
         return "LDS request too large (fused kernel)";
     d.fused = 1;
   }
+  if (hp.cfg.stacked == 3 && !hp.m_task2.empty()) {  // matrix-core stacked path
+    if (!(e = up(d.m_unit1, hp.m_unit1)).empty()) return e;
+    if (!(e = up(d.s_term2, hp.s_term2)).empty()) return e;
+    if (!(e = up(d.s_xref, hp.s_xref)).empty()) return e;
+    if (!(e = up(d.m_task1, hp.m_task1)).empty()) return e;
+    if (!(e = up(d.m_task2, hp.m_task2)).empty()) return e;
+    d.n_mtask1 = (int)hp.m_task1.size();
+    d.n_mtask2 = (int)hp.m_task2.size();
+    d.mx_ks = hp.mx_ks;
+    d.s_lds1 = hp.s_lds1;
+    if (hp.x_size > 0 && hipMalloc(&d.X, xbytes * (size_t)hp.x_size) != hipSuccess)
+      return "hipMalloc failed (stacked X buffer)";
+    d.mstk = 1;
+    d.atomic = 1;
+  }
   if (d.stacked) {
     if (!(e = up(d.s_chunk1, hp.s_chunk1)).empty()) return e;
     if (!(e = up(d.s_unit1, hp.s_unit1)).empty()) return e;
@@ -157,6 +172,7 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.part_list); (void)hipFree(d.Ad); (void)hipFree(d.Bh); (void)hipFree(d.H); (void)hipFree(d.part);
   (void)hipFree(d.s_chunk1); (void)hipFree(d.s_unit1); (void)hipFree(d.s_chunk2); (void)hipFree(d.s_term2);
   (void)hipFree(d.s_xref); (void)hipFree(d.X);
+  (void)hipFree(d.m_task1); (void)hipFree(d.m_unit1); (void)hipFree(d.m_task2);
   (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
   (void)hipFree(d.g_tile3);
   (void)hipFree(d.g_xref); (void)hipFree(d.XG);
@@ -460,7 +476,7 @@ int nsx_dev_sync(void) {
 
 int nsx_batch_path(const nsx_batch* b, int32_t* path) {
   if (!b || !path) { set_error("null argument"); return NSX_EARG; }
-  *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : 0;
+  *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : b->dplan.mstk ? 3 : 0;
   // deterministic: the fused path for the small blocks and unsplit large tiles
   if (b->dplan.fused && b->hplan.cfg.deterministic) *path |= 4;
   return NSX_OK;
@@ -556,9 +572,14 @@ PlanConfig plan_config(const nsx_tune_result& r) {
   PlanConfig pc;
   pc.fat_cap = std::min(std::max(r.fat_cap, 256), kFatCap);
   pc.max_phases = std::max(r.max_phases, 1);
-  pc.stacked = r.stacked == 2 ? 2 : r.stacked ? 1 : 0;
-  if (r.stk_cap > 0) pc.stk_cap = r.stk_cap;
-  if (r.stk_work > 0) pc.stk_work = r.stk_work;
+  pc.stacked = r.stacked == 2 || r.stacked == 3 ? r.stacked : r.stacked ? 1 : 0;
+  if (pc.stacked == 3) {  // matrix-core stacked path: a-terms per stage-2 workgroup, products per stage-1 wavefront
+    if (r.stk_cap > 0) pc.mx_terms2 = r.stk_cap;
+    if (r.stk_work > 0) pc.mx_units1 = r.stk_work;
+  } else {
+    if (r.stk_cap > 0) pc.stk_cap = r.stk_cap;
+    if (r.stk_work > 0) pc.stk_work = r.stk_work;
+  }
   if (r.f_rounds > 0) pc.f_rounds = r.f_rounds;
   pc.deterministic = r.deterministic ? 1 : 0;
   if (r.big_rows1 == kBigTile || r.big_rows1 == 2 * kBigTile) pc.big_rows1_req = r.big_rows1;
@@ -589,6 +610,7 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int prec
   }
   auto finish = [&]() {
     if (hp.cfg.stacked == 1) build_stacked(hp, models);
+    if (hp.cfg.stacked == 3) build_mstk(hp, models);
     build_big(hp, models);
     // fused column path, or the staged kernel where a column exceeds its limits
     if (hp.cfg.stacked == 2 && !build_fused(hp, models, kFuseThreads)) hp.cfg.stacked = 0;
@@ -698,6 +720,21 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
         const double ms = time_hv(d, lp, precision, in, out, s, iters);
         if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, nt, 0, 1, ms, 1, scap, swork, 0, 0, 0, 0};
       }
+    }
+  // the stacked stages on the matrix cores: products per stage-1 wavefront x
+  // a-terms per stage-2 workgroup
+  for (int u1 : {2, 4})
+    for (int t2 : {8, 16}) {
+      PlanConfig cfg;
+      cfg.stacked = 3;
+      cfg.mx_units1 = u1;
+      cfg.mx_terms2 = t2;
+      std::string e = replan(models, cfg, precision, hp, d, lp);
+      if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+      if (!d.mstk) break;  // no small blocks
+      d.atomic = 1;
+      const double ms = time_hv(d, lp, precision, in, out, s, iters);
+      if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, 256, 0, 1, ms, 3, t2, u1, 0, 0, 0, 0};
     }
   // fused column path: LDS of a round x work of a task
   for (int fcap : {3072, 4096, 6144})

@@ -1564,6 +1564,203 @@ const void* k_fuse_ptr(int prec, int u) {
 #undef NSX_FUSE
 }
 
+// ---------------------------------------------------------------- matrix-core stacked path
+// The stacked stages on 16 x 16 x 4 matrix-core steps, one tile per
+// wavefront, with every operand loaded straight into the registers the
+// instruction reads (lane l holds row l % 16, K l / 16 of the first operand and
+// K l / 16, column l % 16 of the second): the small blocks need no LDS staging,
+// no per-element index search and no barrier.  Task, unit and term records are
+// wavefront-uniform (scalar loads).  The number of K steps NK of a product or
+// term is a template parameter of its body (dispatched on the uniform inner
+// size), so a body's loads are straight-line code: a lane outside its block
+// reads the block's first element (always in range) and uses zero, and all of
+// the loads of a step are in flight together.
+
+// calls f(std::integral_constant<int, NK>) for NK = n, 1 <= n <= MAXN
+template <int N, int MAXN, typename F>
+__device__ __forceinline__ void mx_dispatch(int n, F&& f) {
+  if constexpr (N <= MAXN) {
+    if (n == N) {
+      f(std::integral_constant<int, N>{});
+      return;
+    }
+    mx_dispatch<N + 1, MAXN>(n, f);
+  }
+}
+
+// Stage 1 (one wavefront per MTask1; workgroups [0, g_red) take the
+// like-particle blocks as in k_stk1): X_u^T[j][i] = (V_R Bhat^T)[i][j] for the
+// task's 16 columns and each of its row tiles (MUnit1), the next tile's V
+// operand loaded while the current one is multiplied and stored.
+template <typename T, int KS>
+__global__ __launch_bounds__(256) void k_mx1(const MTask1* __restrict__ tasks, int n_tasks,
+                                             const MUnit1* __restrict__ units, const T* __restrict__ V,
+                                             const T* __restrict__ Vp, const T* __restrict__ Bt,
+                                             typename BigIn<T>::C* __restrict__ X, const DBlock* __restrict__ blocks,
+                                             int n_blocks, int g_red, const T* __restrict__ H,
+                                             double* __restrict__ W, int lds_cap, int skip_large, EnsStride es) {
+  using C = typename BigIn<T>::C;
+  using M = BigMma<C>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int bx = blockIdx.x;
+  if (es.n > 1) {
+    int64_t smp;
+    bx = ens_block(es, smp);
+    if (bx < 0) return;
+    V += smp * es.v;
+    Vp += smp * es.v;
+    W += smp * es.v;
+    Bt += smp * es.b;
+    H += smp * es.h;
+    X += smp * es.x;
+  }
+  if (bx < g_red) {
+    if (bx < n_blocks)
+      reduce_block<T, 256>(blocks[bx], nullptr, nullptr, V, H, W, 0, reinterpret_cast<T*>(smem_raw), lds_cap,
+                           skip_large);
+    return;
+  }
+  const int n_wg = (n_tasks + 3) / 4;
+  const int wg = es.n > 1 ? bx - g_red : xcd_chunk(bx - g_red, n_wg);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, ti = wg * 4 + wave;
+  if (ti >= n_tasks) return;  // whole wavefronts; no barrier follows
+  const MTask1 t = tasks[ti];
+  const int kq = lane >> 4, l16 = lane & 15;
+  const int dbr = t.dbr, db = t.db, j = t.j0 + l16;
+  const bool jok = j < db;
+  mx_dispatch<1, KS>((dbr + 3) >> 2, [&](auto nk) {
+    constexpr int NK = decltype(nk)::value;
+    C b[NK];
+    const T* B = Bt + t.b_off;
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) {
+      const int k = 4 * ks + kq;
+      const bool ok = k < dbr && jok;
+      const C x = BigIn<T>::get(B[ok ? k + dbr * j : 0]);
+      b[ks] = ok ? x : C(0);
+    }
+    auto load_a = [&](const MUnit1& s, C (&a)[NK]) {
+      const T* Vs = Vp + s.v_off;
+      const bool iok = l16 < s.rows;
+#pragma unroll
+      for (int ks = 0; ks < NK; ++ks) {
+        const int k = 4 * ks + kq;
+        const bool ok = k < dbr && iok;
+        const C x = BigIn<T>::get(Vs[ok ? l16 + s.ld * k : 0]);
+        a[ks] = ok ? x : C(0);
+      }
+    };
+    MUnit1 cur = units[t.u_begin];
+    C a[NK];
+    load_a(cur, a);
+    for (int u = t.u_begin; u < t.u_end; ++u) {
+      const bool more = u + 1 < t.u_end;
+      const MUnit1 nxt = units[more ? u + 1 : u];
+      C an[NK];
+      load_a(nxt, an);  // the last one re-reads the current tile
+      typename M::vec acc = {C(0), C(0), C(0), C(0)};
+#pragma unroll
+      for (int ks = 0; ks < NK; ++ks) acc = M::mma(a[ks], b[ks], acc);
+      C* Xs = X + cur.x_off;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = M::row(lane, v);
+        if (r < cur.rows && jok) Xs[j + db * r] = acc[v];
+      }
+#pragma unroll
+      for (int ks = 0; ks < NK; ++ks) a[ks] = an[ks];
+      cur = nxt;
+    }
+  });
+}
+
+// Stage 2 (one workgroup per MTask2): the transposed tile
+// W_L^T[j][i] += sum_k Y_k^T[j][:] A_k^T[:][i], Y_k^T[j][k'] = sum_u r_u X_u^T[j][k'],
+// the wavefronts taking terms t_begin + w, + w + 4, ...; their four partial
+// tiles are summed in LDS and the sum is added into W with FP64 atomics.
+// The X references of a term are read RU at a time, all their K steps' loads
+// in flight together with A's.
+template <typename T, int KS, int RU>
+__global__ __launch_bounds__(256) void k_mx2(const MTask2* __restrict__ tasks, int n_tasks,
+                                             const STerm2* __restrict__ terms, const SXRef* __restrict__ xrefs,
+                                             const T* __restrict__ Ad, const typename BigIn<T>::C* __restrict__ X,
+                                             double* __restrict__ W, EnsStride es) {
+  using C = typename BigIn<T>::C;
+  using M = BigMma<C>;
+  __shared__ C red[4][4][64];
+  int ti;
+  if (es.n > 1) {
+    int64_t smp;
+    ti = ens_block(es, smp);
+    if (ti < 0) return;
+    X += smp * es.x;
+    W += smp * es.v;
+  } else {
+    ti = xcd_chunk(blockIdx.x, n_tasks);
+  }
+  if (ti >= n_tasks) return;  // whole workgroups
+  const MTask2 t = tasks[ti];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int kq = lane >> 4, l16 = lane & 15;
+  const int da = t.da, db = t.db, j = t.j0 + l16, i = t.i0 + l16;
+  const bool jok = j < db, iok = i < da;
+  typename M::vec acc = {C(0), C(0), C(0), C(0)};
+  for (int tt = t.t_begin + wave; tt < t.t_end; tt += 4) {
+    const STerm2 s = terms[tt];
+    const int dar = s.dar;
+    mx_dispatch<1, KS>((dar + 3) >> 2, [&](auto nk) {
+      constexpr int NK = decltype(nk)::value;
+      C a[NK], y[NK];
+      const T* A = Ad + s.a_off;
+#pragma unroll
+      for (int ks = 0; ks < NK; ++ks) {
+        const int k = 4 * ks + kq;
+        const bool ok = k < dar && iok;
+        const C x = BigIn<T>::get(A[ok ? i + da * k : 0]);
+        a[ks] = ok ? x : C(0);
+        y[ks] = C(0);
+      }
+      for (int x0 = s.x_begin; x0 < s.x_end; x0 += RU) {
+        const C* Xq[RU];
+        C r[RU], xv[RU][NK];
+#pragma unroll
+        for (int q = 0; q < RU; ++q) {
+          const bool ok = x0 + q < s.x_end;
+          const SXRef xr = xrefs[ok ? x0 + q : x0];
+          Xq[q] = X + xr.x_off;
+          r[q] = ok ? C(xr.r) : C(0);
+        }
+#pragma unroll
+        for (int q = 0; q < RU; ++q)
+#pragma unroll
+          for (int ks = 0; ks < NK; ++ks) {
+            const int k = 4 * ks + kq;
+            const bool ok = k < dar && jok;
+            const C x = Xq[q][ok ? j + db * k : 0];
+            xv[q][ks] = ok ? x : C(0);
+          }
+#pragma unroll
+        for (int q = 0; q < RU; ++q)
+#pragma unroll
+          for (int ks = 0; ks < NK; ++ks) y[ks] += r[q] * xv[q][ks];
+      }
+#pragma unroll
+      for (int ks = 0; ks < NK; ++ks) acc = M::mma(y[ks], a[ks], acc);
+    });
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[wave][v][lane] = acc[v];
+  __syncthreads();
+  {  // thread (wave w, lane) sums element v = w of the four partial tiles
+    const int v = wave;
+    const C sum = (red[0][v][lane] + red[1][v][lane]) + (red[2][v][lane] + red[3][v][lane]);
+    const int jr = t.j0 + M::row(lane, v);
+    if (jr < db && iok) unsafeAtomicAdd(W + t.w_off + i + (int64_t)da * jr, (double)sum);
+  }
+}
+
 // ---------------------------------------------------------------- instance tables
 struct StagedInst {
   int prec, nt, mu, mode, atomic;
@@ -1657,6 +1854,34 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
       hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2, gy), dim3(d.stk_nt), a2, lds2, s);
       if (e != hipSuccess) return e;
     }
+  } else if (d.mstk) {  // matrix-core stacked path for the small blocks (accumulates into W)
+    if (!at) return hipErrorInvalidConfiguration;
+    using Cx = typename BigIn<T>::C;
+    Cx* X = static_cast<Cx*>(d.X);
+    const int g_red = ((d.n_blocks + kXcds - 1) / kXcds) * kXcds;
+    const int g1 = (((d.n_mtask1 + 3) / 4 + kXcds - 1) / kXcds) * kXcds;
+    const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
+    const int skip = big_like_active(d) ? 1 : 0, cap = d.s_lds1;
+    const size_t lds1 = sizeof(T) * (size_t)cap;
+    if (g_red + g1 > 0) {
+      if (d.mx_ks <= 6)
+        hipLaunchKernelGGL((k_mx1<T, 6>), dim3((unsigned)(g_red + g1), gy), dim3(256), lds1, s, d.m_task1,
+                           d.n_mtask1, d.m_unit1, V, Vp, Bh, X, d.blocks, d.n_blocks, g_red, H, W, cap, skip, d.es);
+      else
+        hipLaunchKernelGGL((k_mx1<T, 10>), dim3((unsigned)(g_red + g1), gy), dim3(256), lds1, s, d.m_task1,
+                           d.n_mtask1, d.m_unit1, V, Vp, Bh, X, d.blocks, d.n_blocks, g_red, H, W, cap, skip, d.es);
+    }
+    if (d.n_mtask2 > 0) {
+      const unsigned g2 = (unsigned)(((d.n_mtask2 + kXcds - 1) / kXcds) * kXcds);
+      if (d.mx_ks <= 6)
+        hipLaunchKernelGGL((k_mx2<T, 6, 2>), dim3(g2, gy), dim3(256), 0, s, d.m_task2, d.n_mtask2, d.s_term2,
+                           d.s_xref, Ad, (const Cx*)X, W, d.es);
+      else
+        hipLaunchKernelGGL((k_mx2<T, 10, 2>), dim3(g2, gy), dim3(256), 0, s, d.m_task2, d.n_mtask2, d.s_term2,
+                           d.s_xref, Ad, (const Cx*)X, W, d.es);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   } else if (d.n_items2 > 0) {
     const int k = pick_staged(prec, d.staged_nt, d.staged_mode, at, d.max_units);
     if (k < 0) return hipErrorInvalidConfiguration;
@@ -1710,7 +1935,7 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
                          d.f_part, V, H, W, 1, big ? 1 : 0);
   }
   // the stacked stage-1 launch carries the small like-particle blocks
-  if (!d.stacked && !d.fused && d.n_blocks > 0)
+  if (!d.stacked && !d.mstk && !d.fused && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
                        V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
   if (big) {

@@ -43,6 +43,13 @@ struct DevicePlan {
   SXRef* s_xref = nullptr;
   int n_chunk1 = 0, n_chunk2 = 0, s_lds1 = 1, s_lds2 = 1;
   void* X = nullptr;         // x_size elements of the accumulation type (FP64, or FP32 in reduced precision)
+  // matrix-core stacked path (small blocks; uses s_term2, s_xref, X of the
+  // stacked path, X transposed per product)
+  int mstk = 0, mx_ks = 1;
+  MTask1* m_task1 = nullptr;
+  MUnit1* m_unit1 = nullptr;
+  MTask2* m_task2 = nullptr;
+  int n_mtask1 = 0, n_mtask2 = 0;
   // large-block GEMM path (FP64, atomic mode)
   GTask1* g_task1 = nullptr;
   GTile1* g_tile1 = nullptr;

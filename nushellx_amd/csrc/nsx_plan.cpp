@@ -318,17 +318,23 @@ void plan_add(HostPlan& p, const Model& m) {
   p.bytes_alg += 16 * m.N + 8 * (m.nnz_a + m.nnz_b + nnz_like + m.n_bterms);
 }
 
-void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
-  p.s_chunk1.clear(); p.s_unit1.clear(); p.s_chunk2.clear(); p.s_term2.clear(); p.s_xref.clear();
-  p.x_size = 0;
-  p.s_lds1 = p.s_lds2 = 1;
-  struct U1 { int64_t v_off; int dar, dbr, db; int64_t x_off; };
-  std::map<std::pair<int64_t, int64_t>, int64_t> x_of;        // (b_off, v_off) -> x_off
-  std::map<std::array<int64_t, 3>, std::vector<U1>> by_b;     // (b_off, dbr, db) -> pairs
-  struct T2 { int64_t a_off; int dar; std::vector<std::pair<int64_t, double>> x; };
+namespace {
+// The two stacked stages' work, grouped: distinct (V block, Bhat^T block)
+// products by Bhat^T block, and a-terms (with their X references) by left block.
+struct StkU1 { int64_t v_off; int dar, dbr, db; int64_t x_off; };
+struct StkT2 { int64_t a_off; int dar; std::vector<std::pair<int64_t, double>> x; };
+struct StkGroups {
+  std::map<std::array<int64_t, 3>, std::vector<StkU1>> by_b;  // (b_off, dbr, db) -> products
   // (instance, 2Jb, 2Ja, w_off, da, db) -> a-terms: left blocks of one instance
   // with the same Jb read the same X products, so their chunks are kept adjacent
-  std::map<std::array<int64_t, 6>, std::vector<T2>> by_w;
+  std::map<std::array<int64_t, 6>, std::vector<StkT2>> by_w;
+};
+
+// Fills g with the small-block items of `models` and sets p.x_size (X buffer
+// elements; each product is dar x db).
+void stacked_groups(HostPlan& p, const std::vector<const Model*>& models, StkGroups& g) {
+  p.x_size = 0;
+  std::map<std::pair<int64_t, int64_t>, int64_t> x_of;        // (b_off, v_off) -> x_off
   for (size_t mi = 0; mi < models.size(); ++mi) {
     const Model& m = *models[mi];
     const int64_t vb = p.vbase[mi];
@@ -341,7 +347,7 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
         for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
       if (std::max(std::max(da, dar), std::max(db, maxdbr)) > p.cfg.big_min) continue;  // direct kernel
       for (const ATerm& at : it.at) {
-        T2 t;
+        StkT2 t;
         t.a_off = a_off[at.adens];
         t.dar = dar;
         for (const RTerm& rt : at.rt) {
@@ -352,16 +358,28 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
             x = p.x_size;
             p.x_size += int64_t(dar) * db;
             x_of[{b, v}] = x;
-            by_b[{b, m.db[rt.R], db}].push_back({v, dar, m.db[rt.R], db, x});
+            g.by_b[{b, m.db[rt.R], db}].push_back({v, dar, m.db[rt.R], db, x});
           } else {
             x = f->second;
           }
           t.x.push_back({x, rt.r});
         }
-        by_w[{(int64_t)mi, m.bas_b[it.L], m.bas_a[it.L], vb + m.posnv[it.L], da, db}].push_back(std::move(t));
+        g.by_w[{(int64_t)mi, m.bas_b[it.L], m.bas_a[it.L], vb + m.posnv[it.L], da, db}].push_back(std::move(t));
       }
     }
   }
+}
+}  // namespace
+
+void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
+  p.s_chunk1.clear(); p.s_unit1.clear(); p.s_chunk2.clear(); p.s_term2.clear(); p.s_xref.clear();
+  p.s_lds1 = p.s_lds2 = 1;
+  using U1 = StkU1;
+  using T2 = StkT2;
+  StkGroups grp;
+  stacked_groups(p, models, grp);
+  auto& by_b = grp.by_b;
+  auto& by_w = grp.by_w;
   // an a-term's references must fit one chunk's table; otherwise keep the staged path
   for (auto& g : by_w)
     for (const T2& t : g.second)
@@ -439,6 +457,69 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models) {
       c.lds += need;
     }
     close();
+  }
+}
+
+void build_mstk(HostPlan& p, const std::vector<const Model*>& models) {
+  p.s_chunk1.clear(); p.s_unit1.clear(); p.s_chunk2.clear(); p.s_term2.clear(); p.s_xref.clear();
+  p.m_task1.clear(); p.m_unit1.clear(); p.m_task2.clear();
+  p.s_lds1 = p.s_lds2 = 1;
+  StkGroups grp;
+  stacked_groups(p, models, grp);
+  const int u1 = std::max(p.cfg.mx_units1, 1), t2 = std::max(p.cfg.mx_terms2, 1);
+  p.mx_ks = 1;
+  // the stage-1 launch carries the like-particle blocks: LDS for the largest
+  // small one (V_L, H_a, H_b), at most 16 KiB; larger ones read global memory
+  int need = 1;
+  for (const DBlock& b : p.blocks)
+    if (b.j0 == 0 && std::max(b.da, b.db) <= kSmallMax) need = std::max(need, b.da * b.db + b.da * b.da + b.db * b.db);
+  p.s_lds1 = std::min(need, 2048);
+  for (auto& g : grp.by_b) {
+    const int dbr = (int)g.first[1], db = (int)g.first[2];
+    p.mx_ks = std::max(p.mx_ks, (dbr + 3) / 4);
+    const int n = (int)g.second.size();
+    for (int u0 = 0; u0 < n; u0 += u1) {
+      const int ub = (int)p.m_unit1.size();
+      for (int u = u0; u < std::min(n, u0 + u1); ++u) {
+        const StkU1& x = g.second[u];
+        for (int i0 = 0; i0 < x.dar; i0 += kMxTile) {
+          MUnit1 s{};
+          s.v_off = x.v_off + i0; s.x_off = x.x_off + int64_t(db) * i0;
+          s.ld = x.dar; s.rows = std::min(kMxTile, x.dar - i0);
+          p.m_unit1.push_back(s);
+        }
+      }
+      for (int j0 = 0; j0 < db; j0 += kMxTile) {
+        MTask1 t{};
+        t.b_off = g.first[0]; t.dbr = dbr; t.db = db; t.j0 = j0;
+        t.u_begin = ub; t.u_end = (int)p.m_unit1.size();
+        p.m_task1.push_back(t);
+      }
+    }
+  }
+  for (auto& g : grp.by_w) {
+    const int da = (int)g.first[4], db = (int)g.first[5];
+    const int n = (int)g.second.size();
+    for (int q0 = 0; q0 < n; q0 += t2) {
+      const int tb = (int)p.s_term2.size();
+      for (int q = q0; q < std::min(n, q0 + t2); ++q) {
+        const StkT2& t = g.second[q];
+        p.mx_ks = std::max(p.mx_ks, (t.dar + 3) / 4);
+        STerm2 s{};
+        s.a_off = t.a_off; s.dar = t.dar;
+        s.x_begin = (int)p.s_xref.size();
+        for (auto& x : t.x) p.s_xref.push_back({x.first, x.second});
+        s.x_end = (int)p.s_xref.size();
+        p.s_term2.push_back(s);
+      }
+      for (int j0 = 0; j0 < db; j0 += kMxTile)
+        for (int i0 = 0; i0 < da; i0 += kMxTile) {
+          MTask2 t{};
+          t.w_off = g.first[3]; t.da = da; t.db = db; t.i0 = i0; t.j0 = j0;
+          t.t_begin = tb; t.t_end = (int)p.s_term2.size();
+          p.m_task2.push_back(t);
+        }
+    }
   }
 }
 

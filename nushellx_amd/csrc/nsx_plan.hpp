@@ -119,6 +119,37 @@ struct SXRef {
   double r;
 };
 
+// Matrix-core stacked path (small blocks): the two stacked stages as 16 x 16
+// matrix-core tiles held in registers, operands loaded straight from memory
+// (no LDS), X stored transposed (X_u^T, db x dar column-major) so that both
+// stages read and write 16 consecutive elements per row of lanes.
+//   stage 1: one wavefront per (Bhat^T block, 16-column tile j0, run of
+//            products sharing that block): Bhat^T's K steps stay in registers,
+//            and every 16-row tile of each V_R is multiplied against them;
+//   stage 2: one workgroup per (left block L, 16 x 16 tile (i0, j0) of W_L,
+//            run of a-terms): its four wavefronts take every fourth term,
+//            Y_k = sum_u r_u X_u is formed in registers as the matrix-core
+//            operand, the partial tiles are summed in LDS and added into W.
+// MUnit1 (one 16-row tile of a product) and STerm2 (a_off, dar, x_begin,
+// x_end) / SXRef describe the products and a-terms.
+constexpr int kMxTile = 16;     // output tile side (16 x 16 x 4 matrix-core steps)
+struct MTask1 {
+  int64_t b_off;            // Bhat^T (dbr x db, column-major) in Bh
+  int32_t dbr, db, j0;      // inner size, columns, first column of the tile
+  int32_t u_begin, u_end;   // MUnit1 range
+  int32_t pad;
+};
+struct MUnit1 {             // one 16-row tile of one product X_u = V_R Bhat^T
+  int64_t v_off;            // V_R rows i0.. (leading dimension ld = dar) in the batched vector
+  int64_t x_off;            // X_u^T rows i0.. (db x dar column-major: X_u^T + db * i0)
+  int32_t ld, rows;         // dar; rows of this tile (<= 16)
+};
+struct MTask2 {
+  int64_t w_off;            // W_L (da x db, column-major) in the batched vector
+  int32_t da, db, i0, j0;   // block, first row and column of the tile
+  int32_t t_begin, t_end;   // STerm2 range
+};
+
 // Fused small-block path (owner-computes, no atomics, deterministic):
 // one workgroup owns the output column Jb of one instance (the blocks
 // L = (Ja, Jb), concatenated into a column partial) for a range of the column's
@@ -225,6 +256,7 @@ struct PlanConfig {
   int fat_cap = kFatCap;     // doubles one staged phase may load (<= kFatCap)
   int max_phases = kMaxPhases;  // phases per staged workgroup
   int stacked = 0;           // 1: small blocks take the two-stage stacked path
+                             // (3: on the matrix cores; 2: the fused path)
   int stk_cap = 2048;        // stacked path: doubles one chunk may stage in LDS
   int stk_work = 512;        // stacked path: work units per chunk
   int big_min = kSmallMax;   // items with a side above this take the direct / large-block paths
@@ -241,6 +273,10 @@ struct PlanConfig {
   // stage-2 workgroup per large W tile (its single atomic add lands on the
   // stored like-particle value, so the order of additions cannot vary)
   int deterministic = 0;
+  // stacked == 3, the matrix-core stacked path: products per stage-1
+  // wavefront and a-terms per stage-2 workgroup
+  int mx_units1 = 4;
+  int mx_terms2 = 16;
 };
 
 // Linear ensembles: every sample's operators are linear in its parameters, so
@@ -299,6 +335,11 @@ struct HostPlan {
   std::vector<SXRef> s_xref;
   int64_t x_size = 0;          // X buffer elements
   int s_lds1 = 1, s_lds2 = 1;  // largest chunk LDS (doubles)
+  // matrix-core stacked path (filled by build_mstk; shares the arrays above)
+  std::vector<MTask1> m_task1;
+  std::vector<MUnit1> m_unit1;
+  std::vector<MTask2> m_task2;
+  int mx_ks = 1;               // largest inner size in K steps of 4
   // large-block path (filled by build_big)
   std::vector<GTask1> g_task1;
   std::vector<GTile1> g_tile1;
@@ -348,6 +389,9 @@ void plan_merge(HostPlan& p, const HostPlan& q);
 // (the instances planned into p, in order).  Clears p.cfg.stacked (the staged
 // kernel then runs) if an a-term has more references than a chunk can hold.
 void build_stacked(HostPlan& p, const std::vector<const Model*>& models);
+// Fills the matrix-core stacked path's arrays (m_unit1, s_term2, s_xref,
+// m_task1, m_task2, x_size) for the small-block items of `models`.
+void build_mstk(HostPlan& p, const std::vector<const Model*>& models);
 // Fills the fused-path arrays of p for the small-block items of `models`.
 // Returns false (and leaves them empty) if a column exceeds the kernel's
 // limits (blocks per column, output units) or the plan has items that only

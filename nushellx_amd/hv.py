@@ -307,10 +307,11 @@ class Batch:
         return r.as_dict()
 
     def path(self) -> str:
-        """The small-block pn path of the plan: "staged", "stacked" or "fused"."""
+        """The small-block pn path of the plan: "staged", "stacked", "fused" or
+        "stacked-mfma" (the stacked stages on the matrix cores)."""
         v = C.c_int32()
         check(lib().nsx_batch_path(self._b, C.byref(v)), "nsx_batch_path")
-        return ("staged", "stacked", "fused")[v.value & 3] + (" deterministic" if v.value & 4 else "")
+        return ("staged", "stacked", "fused", "stacked-mfma")[v.value & 3] + (" deterministic" if v.value & 4 else "")
 
     def set_config(self, cfg: dict) -> None:
         r = TuneResult(cfg["fat_cap"], cfg["max_phases"], cfg["threads"], cfg["mode"],

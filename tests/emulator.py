@@ -35,6 +35,8 @@ def lib():
         L.emu_sym_eig.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.emu_hv_fused.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int,
                                                       C.c_void_p]
+        L.emu_hv_mstk.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                                     C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -132,3 +134,16 @@ def emu_hv_stacked(args, v, cap=2048, work=512):
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out
+
+
+def emu_hv_mstk(args, v, units1=4, terms2=16, big_min=0):
+    """Emulated H*v through the matrix-core stacked plan (lane-level model of
+    k_mx1 / k_mx2); returns (w, [stage-1 tasks, stage-2 tasks, X size, W writes])."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    st = np.zeros(4, dtype=np.int64)
+    rc = lib().emu_hv_mstk(*[a.encode() for a in args], v.ctypes.data, out.ctypes.data, len(v), units1, terms2,
+                           big_min, st.ctypes.data)
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out, st

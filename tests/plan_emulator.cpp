@@ -251,6 +251,150 @@ extern "C" int emu_sym_eig(const double* A, int n, int full, double* w, double* 
   return nsx::sym_eig(A, n, full != 0, w, Z) ? 0 : 1;
 }
 
+// Matrix-core stacked path, emulated lane by lane as the k_mx1 / k_mx2
+// kernels run it (16 x 16 x 4 FP64 steps with their operand and result lane
+// layouts, X transposed per product), with every memory index checked against
+// its array.  Returns -4 on an out-of-range access.
+namespace {
+struct Mfma16 {  // one 16 x 16 x 4 FP64 step from the 64 lanes' operands
+  static void step(const double (&a)[64], const double (&b)[64], double (&acc)[64][4]) {
+    double A[16][4], B[4][16];
+    for (int l = 0; l < 64; ++l) { A[l % 16][l / 16] = a[l]; B[l / 16][l % 16] = b[l]; }
+    for (int l = 0; l < 64; ++l)
+      for (int v = 0; v < 4; ++v) {
+        const int row = l / 16 + 4 * v, col = l % 16;
+        double s = 0.0;
+        for (int k = 0; k < 4; ++k) s += A[row][k] * B[k][col];
+        acc[l][v] += s;
+      }
+  }
+  static int row(int l, int v) { return l / 16 + 4 * v; }
+};
+}  // namespace
+
+extern "C" int emu_hv_mstk(const char* nus, const char* opa, const char* opb, const char* intf,
+                           const double* vin, double* vout, int64_t n, int units1, int terms2, int big_min,
+                           int64_t* stats4) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  p.cfg.stacked = 3;
+  p.cfg.mx_units1 = units1;
+  p.cfg.mx_terms2 = terms2;
+  if (big_min > 0) p.cfg.big_min = big_min;
+  plan_add(p, m);
+  std::vector<const Model*> ms{&m};
+  build_mstk(p, ms);
+  build_big(p, ms);
+  if (!p.g_tile2.empty() || (!p.items.empty() && p.g_task1.empty() && p.cfg.big_min < kSmallMax)) {
+    // large blocks are not emulated here
+  }
+  std::vector<double> vm;
+  const double* vp = pn_input(p, vin, vm);
+  std::vector<double> X(p.x_size, std::nan(""));
+  bool oob = false;
+  auto chk = [&](int64_t i, int64_t size) { if (i < 0 || i >= size) oob = true; return i >= 0 && i < size; };
+  const int KS = p.mx_ks <= 6 ? 6 : 10;
+  if (p.mx_ks > 10) { g_err = "inner size above 40"; return -3; }
+  for (const MTask1& t : p.m_task1) {
+    const int nks = (t.dbr + 3) >> 2;
+    double b[10][64];
+    for (int l = 0; l < 64; ++l) {
+      const int kq = l >> 4, j = t.j0 + (l & 15);
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + kq;
+        b[ks][l] = 0.0;
+        const bool ok = k < t.dbr && j < t.db;
+        const int64_t ix = t.b_off + (ok ? k + int64_t(t.dbr) * j : 0);
+        if (ks < nks && chk(ix, (int64_t)p.Bh.size()) && ok) b[ks][l] = p.Bh[ix];
+      }
+    }
+    for (int u = t.u_begin; u < t.u_end; ++u) {
+      const MUnit1& s = p.m_unit1[u];
+      if (s.rows < 1 || s.rows > kMxTile) { g_err = "row tile out of range"; return -3; }
+      double a[10][64], acc[64][4] = {};
+      for (int l = 0; l < 64; ++l)
+        for (int ks = 0; ks < KS; ++ks) {
+          const int k = 4 * ks + (l >> 4), i = l & 15;
+          const bool ok = k < t.dbr && i < s.rows;
+          // the kernel reads the tile's first element for lanes outside it
+          const int64_t ix = s.v_off + (ok ? i + int64_t(s.ld) * k : 0);
+          a[ks][l] = 0.0;
+          if (ks < nks && chk(ix, n) && ok) a[ks][l] = vp[ix];
+        }
+      for (int ks = 0; ks < nks; ++ks) Mfma16::step(a[ks], b[ks], acc);
+      for (int l = 0; l < 64; ++l)
+        for (int v = 0; v < 4; ++v) {
+          const int r = Mfma16::row(l, v), j = t.j0 + (l & 15);
+          if (r < s.rows && j < t.db) {
+            const int64_t ix = s.x_off + j + int64_t(t.db) * r;
+            if (chk(ix, p.x_size)) X[ix] = acc[l][v];
+          }
+        }
+    }
+  }
+  std::fill(vout, vout + n, 0.0);
+  for (const DBlock& b : p.blocks)
+    for (int jj = 0; jj < b.nc; ++jj) {
+      const int j = b.j0 + jj;
+      for (int i = 0; i < b.da; ++i) {
+        double w = 0.0;
+        if (b.ha_off >= 0)
+          for (int k = 0; k < b.da; ++k) w += p.H[b.ha_off + i + int64_t(b.da) * k] * vin[b.v_off + k + int64_t(b.da) * j];
+        if (b.hb_off >= 0)
+          for (int k = 0; k < b.db; ++k) w += vin[b.v_off + i + int64_t(b.da) * k] * p.H[b.hb_off + j + int64_t(b.db) * k];
+        vout[b.v_off + i + int64_t(b.da) * j] = w;
+      }
+    }
+  int64_t writes = 0;
+  for (const MTask2& t : p.m_task2) {
+    double accw[4][64][4] = {};
+    for (int wave = 0; wave < 4; ++wave)
+      for (int tt = t.t_begin + wave; tt < t.t_end; tt += 4) {
+        const STerm2& s = p.s_term2[tt];
+        const int nks = (s.dar + 3) >> 2;
+        double a[10][64], y[10][64];
+        for (int l = 0; l < 64; ++l) {
+          const int kq = l >> 4, j = t.j0 + (l & 15), i = t.i0 + (l & 15);
+          for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + kq;
+            a[ks][l] = 0.0;
+            y[ks][l] = 0.0;
+            const bool aok = k < s.dar && i < t.da;
+            const int64_t ia = s.a_off + (aok ? i + int64_t(t.da) * k : 0);
+            if (ks < nks && chk(ia, (int64_t)p.Ad.size()) && aok) a[ks][l] = p.Ad[ia];
+            const bool xok = k < s.dar && j < t.db;
+            for (int x = s.x_begin; x < s.x_end; ++x) {
+              const int64_t ix = p.s_xref[x].x_off + (xok ? j + int64_t(t.db) * k : 0);
+              if (ks < nks && chk(ix, p.x_size) && xok) y[ks][l] += p.s_xref[x].r * X[ix];
+            }
+          }
+        }
+        for (int ks = 0; ks < nks; ++ks) Mfma16::step(y[ks], a[ks], accw[wave]);
+      }
+    for (int l = 0; l < 64; ++l)
+      for (int v = 0; v < 4; ++v) {
+        const int jr = t.j0 + Mfma16::row(l, v), i = t.i0 + (l & 15);
+        if (jr < t.db && i < t.da) {
+          const int64_t ix = t.w_off + i + int64_t(t.da) * jr;
+          if (chk(ix, n)) vout[ix] += (accw[0][l][v] + accw[1][l][v]) + (accw[2][l][v] + accw[3][l][v]);
+          ++writes;
+        }
+      }
+  }
+  if (stats4) {
+    stats4[0] = (int64_t)p.m_task1.size();
+    stats4[1] = (int64_t)p.m_task2.size();
+    stats4[2] = p.x_size;
+    stats4[3] = writes;
+  }
+  if (oob) { g_err = "out-of-range access in the matrix-core stacked path"; return -4; }
+  if (!p.items.empty()) { g_err = "large-block items are not emulated by emu_hv_mstk"; return -5; }
+  return 0;
+}
+
 // Stacked two-stage path: X = V Bhat^T per distinct pair, then W += A sum r X,
 // plus the like-particle part and the direct (large-block) items, all from the
 // plan arrays the stacked kernels read.

@@ -178,7 +178,9 @@ def test_solver_vectors_through_nuvec_files(models, tmp_path):
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
 def test_stacked_path_matches_refcpu(models, prec, tol):
     """Two-stage stacked path (X = V Bhat^T shared, then W += A sum r X) on a
-    batch of every sd case, for several chunk shapes, against the oracle."""
+    batch of every sd case (sd 6p6n has inner sizes above 24: ten K steps),
+    for several chunk shapes, on the VALU and on the matrix cores, against the
+    oracle."""
     from nushellx_amd.hv import Batch, DeviceBuffer, Hamiltonian, device_sync
     cases = [(n, J2) for n in ("sd22", "sd44", "sd66", "sd23") for J2 in models[n].J2s[:3]]
     hams = [Hamiltonian(*models[n].args(J2), precision=prec) for n, J2 in cases]
@@ -186,7 +188,11 @@ def test_stacked_path_matches_refcpu(models, prec, tol):
     rng = np.random.default_rng(21)
     for cfg in ({"stacked": 1, "threads": 256, "stk_cap": 2048, "stk_work": 512},
                 {"stacked": 1, "threads": 128, "stk_cap": 4096, "stk_work": 256},
-                {"stacked": 1, "threads": 256, "stk_cap": 1, "stk_work": 1}):
+                {"stacked": 1, "threads": 256, "stk_cap": 1, "stk_work": 1},
+                # on the matrix cores: a-terms per stage-2 workgroup, products per stage-1 wavefront
+                {"stacked": 3, "threads": 256, "stk_cap": 8, "stk_work": 4},
+                {"stacked": 3, "threads": 256, "stk_cap": 1, "stk_work": 1},
+                {"stacked": 3, "threads": 256, "stk_cap": 64, "stk_work": 32}):
         print(f"case stacked {prec} {cfg}", flush=True)
         b = Batch(hams, precision=prec, config=dict(cfg, fat_cap=5120, max_phases=6, mode=0, atomic=1))
         v = rng.standard_normal(b.N)

@@ -135,6 +135,25 @@ def test_linear_ensemble_with_tuned_config(ens):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
+def test_linear_ensemble_on_matrix_core_stacked_path(ens, prec, tol):
+    """Every sample through the stacked stages on the matrix cores (grid y the
+    sample), against the per-sample file batch in FP64."""
+    from nushellx_amd.hv import Batch
+    ref = Batch(ens["hams"])
+    cfg = {"fat_cap": 5120, "max_phases": 6, "threads": 256, "mode": 0, "atomic": 1, "stacked": 3,
+           "stk_cap": 8, "stk_work": 4}
+    lin, _, _ = _linear(ens, prec, config=cfg)
+    assert lin.path() == "stacked-mfma"
+    v = np.random.default_rng(19).standard_normal(ref.N)
+    w_ref, w = _hv(ref, v), _hv(lin, v)
+    for k, o in enumerate(ref.offsets):
+        n = ens["hams"][k].N
+        assert _rel(w[o:o + n], w_ref[o:o + n]) < tol, (prec, k)
+    ref.close()
+    lin.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
 def test_linear_ensemble_on_large_block_tiles(ens, monkeypatch, prec, tol):
     """Blocks with a side above 12 on the matrix-core tiles (every sample as the
     second grid dimension), against the per-sample file batch in FP64."""

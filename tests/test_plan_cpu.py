@@ -60,6 +60,26 @@ def test_stacked_plan_matches_refcpu(models):
                 assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, cap)
 
 
+def test_mstk_plan_matches_refcpu(models):
+    """The matrix-core stacked plan, executed lane by lane as its kernels run
+    it (16 x 16 x 4 steps with their operand and result lane layouts, X stored
+    transposed, four wavefronts per stage-2 tile), gives the oracle's H*v and
+    stays inside every array, for several task sizes, a truncated basis and
+    blocks up to the small-path limit of 40 (inner sizes of 10 K steps)."""
+    from tests.emulator import emu_hv_mstk
+    rng = np.random.default_rng(17)
+    cases = [("sd22", 0), ("sd22", 4), ("sd44", 0), ("sd44", 8), ("sd66", 4), ("sd23", 3), ("psd33t", 2)]
+    for name, J2 in cases:
+        args = models[name].args(J2)
+        r = RefCPU(*args, fp64=True)
+        v = rng.standard_normal(r.N)
+        ref = r.apply(v)
+        for units1, terms2 in ((1, 1), (4, 16), (64, 5)):
+            w, st = emu_hv_mstk(args, v, units1, terms2)
+            assert st[0] > 0 and st[1] > 0
+            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, units1, terms2)
+
+
 def test_fused_plan_matches_refcpu(models):
     """The fused column plan (per task: rounds of V_R / Bhat^T / A staging,
     X = V Bhat^T, Y = sum r X, W += A Y into the column's outputs; then the
