@@ -133,13 +133,17 @@ typedef struct {
                           2: fused column path (X and Y in LDS, column partials
                           summed in a fixed order: deterministic);
                           3: the stacked stages on the matrix cores (16 x 16 tiles in
-                          registers, no LDS staging; atomic accumulation) */
+                          registers, no LDS staging; atomic accumulation);
+                          4: both stages fused in one matrix-core kernel (X and Y
+                          stay in registers; atomic accumulation) */
   int32_t stk_cap;     /* stacked path: doubles a chunk stages in LDS (0 = default);
                           fused path: LDS elements of one round;
-                          matrix-core stacked path: a-terms per stage-2 workgroup */
+                          matrix-core stacked path: a-terms per stage-2 workgroup;
+                          fused matrix-core path: steps per wavefront (0 = default 2) */
   int32_t stk_work;    /* stacked path: work units per chunk (0 = default);
                           fused path: flops one workgroup takes, about;
-                          matrix-core stacked path: products per stage-1 wavefront */
+                          matrix-core stacked path: products per stage-1 wavefront;
+                          fused matrix-core path: output slots per task (2 or 4; 0 = 4) */
   int32_t f_rounds;    /* fused path: rounds one workgroup takes at most (0 = default) */
   int32_t deterministic; /* 1: bitwise reproducible H*v: the fused path for the small
                             blocks and one stage-2 workgroup per large-block W tile
@@ -153,7 +157,8 @@ int nsx_hv_autotune(nsx_hv *h, int32_t iters, nsx_tune_result *best);
 int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);
 int nsx_batch_set_config(nsx_batch *b, const nsx_tune_result *cfg);
 /* The small-block pn path the batch's plan runs: 0 staged kernel, 1 stacked
- * two-stage path, 2 fused column path, 3 stacked path on the matrix cores (a requested path falls back to the
+ * two-stage path, 2 fused column path, 3 stacked path on the matrix cores,
+ * 8 both stages fused on the matrix cores (a requested path falls back to the
  * staged kernel where its limits are exceeded), plus 4 when the plan is
  * deterministic.  No counterpart in the reference. */
 int nsx_batch_path(const nsx_batch *b, int32_t *path);

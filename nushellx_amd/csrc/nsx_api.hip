@@ -83,6 +83,16 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
         return "LDS request too large (fused kernel)";
     d.fused = 1;
   }
+  if (hp.cfg.stacked == 4 && !hp.f_mtask.empty()) {  // fused matrix-core path
+    if (!(e = up(d.f_mtask, hp.f_mtask)).empty()) return e;
+    if (!(e = up(d.f_mstep, hp.f_mstep)).empty()) return e;
+    if (!(e = up(d.f_mu, hp.f_mu)).empty()) return e;
+    d.n_mftask = (int)hp.f_mtask.size();
+    d.mx_ks = hp.mx_ks;
+    d.mxf_slots = hp.mxf_slots;
+    d.mxf = 1;
+    d.atomic = 1;
+  }
   if (hp.cfg.stacked == 3 && !hp.m_task2.empty()) {  // matrix-core stacked path
     if (!(e = up(d.m_unit1, hp.m_unit1)).empty()) return e;
     if (!(e = up(d.s_term2, hp.s_term2)).empty()) return e;
@@ -173,6 +183,7 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.s_chunk1); (void)hipFree(d.s_unit1); (void)hipFree(d.s_chunk2); (void)hipFree(d.s_term2);
   (void)hipFree(d.s_xref); (void)hipFree(d.X);
   (void)hipFree(d.m_task1); (void)hipFree(d.m_unit1); (void)hipFree(d.m_task2);
+  (void)hipFree(d.f_mtask); (void)hipFree(d.f_mstep); (void)hipFree(d.f_mu);
   (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
   (void)hipFree(d.g_tile3);
   (void)hipFree(d.g_xref); (void)hipFree(d.XG);
@@ -476,7 +487,7 @@ int nsx_dev_sync(void) {
 
 int nsx_batch_path(const nsx_batch* b, int32_t* path) {
   if (!b || !path) { set_error("null argument"); return NSX_EARG; }
-  *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : b->dplan.mstk ? 3 : 0;
+  *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : b->dplan.mstk ? 3 : b->dplan.mxf ? 8 : 0;
   // deterministic: the fused path for the small blocks and unsplit large tiles
   if (b->dplan.fused && b->hplan.cfg.deterministic) *path |= 4;
   return NSX_OK;
@@ -572,10 +583,13 @@ PlanConfig plan_config(const nsx_tune_result& r) {
   PlanConfig pc;
   pc.fat_cap = std::min(std::max(r.fat_cap, 256), kFatCap);
   pc.max_phases = std::max(r.max_phases, 1);
-  pc.stacked = r.stacked == 2 || r.stacked == 3 ? r.stacked : r.stacked ? 1 : 0;
+  pc.stacked = r.stacked >= 2 && r.stacked <= 4 ? r.stacked : r.stacked ? 1 : 0;
   if (pc.stacked == 3) {  // matrix-core stacked path: a-terms per stage-2 workgroup, products per stage-1 wavefront
     if (r.stk_cap > 0) pc.mx_terms2 = r.stk_cap;
     if (r.stk_work > 0) pc.mx_units1 = r.stk_work;
+  } else if (pc.stacked == 4) {  // fused matrix-core path: steps per wavefront, output slots per task
+    if (r.stk_cap > 0) pc.mxf_steps = r.stk_cap;
+    if (r.stk_work == 2) pc.mxf_slots = 2;
   } else {
     if (r.stk_cap > 0) pc.stk_cap = r.stk_cap;
     if (r.stk_work > 0) pc.stk_work = r.stk_work;
@@ -611,6 +625,10 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int prec
   auto finish = [&]() {
     if (hp.cfg.stacked == 1) build_stacked(hp, models);
     if (hp.cfg.stacked == 3) build_mstk(hp, models);
+    if (hp.cfg.stacked == 4 && !build_mxf(hp, models)) {  // a slab of another shape: the stacked path
+      hp.cfg.stacked = 1;
+      build_stacked(hp, models);
+    }
     build_big(hp, models);
     // fused column path, or the staged kernel where a column exceeds its limits
     if (hp.cfg.stacked == 2 && !build_fused(hp, models, kFuseThreads)) hp.cfg.stacked = 0;
@@ -735,6 +753,21 @@ static int autotune(const std::vector<const Model*>& models, int precision, Host
       d.atomic = 1;
       const double ms = time_hv(d, lp, precision, in, out, s, iters);
       if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, 256, 0, 1, ms, 3, t2, u1, 0, 0, 0, 0};
+    }
+  // the stacked stages fused on the matrix cores (X and Y in registers):
+  // steps per wavefront
+  for (int sl : {2, kMfSlots})
+    for (int fs : {1, 2, 4}) {
+      PlanConfig cfg;
+      cfg.stacked = 4;
+      cfg.mxf_steps = fs;
+      cfg.mxf_slots = sl;
+      std::string e = replan(models, cfg, precision, hp, d, lp);
+      if (!e.empty()) { set_error(e); (void)hipFree(in); (void)hipFree(out); return NSX_EGPU; }
+      if (!d.mxf) break;
+      d.atomic = 1;
+      const double ms = time_hv(d, lp, precision, in, out, s, iters);
+      if (ms > 0.0 && ms < best.ms) best = {kFatCap, kMaxPhases, 256, 0, 1, ms, 4, fs, sl, 0, 0, 0, 0};
     }
   // fused column path: LDS of a round x work of a task
   for (int fcap : {3072, 4096, 6144})

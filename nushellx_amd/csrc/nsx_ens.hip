@@ -274,7 +274,8 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   }
   HostPlan& hp = b->hplan;
   hp.cfg = cfg ? plan_config(*cfg) : PlanConfig();
-  hp.cfg.stacked = cfg && cfg->stacked == 3 ? 3 : 1;  // the stacked stages, on the VALU or the matrix cores
+  // the stacked stages: on the VALU, on the matrix cores, or fused on the matrix cores
+  hp.cfg.stacked = cfg && (cfg->stacked == 3 || cfg->stacked == 4) ? cfg->stacked : 1;
   hp.cfg.deterministic = 0;
   hp.cfg.big_rows1 = precision == NSX_BF16X3 ? (hp.cfg.big_rows1_req ? hp.cfg.big_rows1_req : 2 * kBigTile) : kBigTile;
   // test hook, as for batches: a smaller block side above which the large-block tiles run
@@ -285,9 +286,10 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
     hp.ens_opcol = (int32_t)opcol[t];
     plan_add(hp, *tms[t]);
   }
+  if (hp.cfg.stacked == 4 && !build_mxf(hp, tms)) hp.cfg.stacked = 1;
   if (hp.cfg.stacked == 3)
     build_mstk(hp, tms);
-  else
+  else if (hp.cfg.stacked == 1)
     build_stacked(hp, tms);
   build_big(hp, tms);
   if (!hp.cfg.stacked || (!hp.items.empty() && hp.g_tile2.empty())) {

@@ -367,12 +367,18 @@ __device__ __forceinline__ void reduce_block(DBlock b, const int64_t* __restrict
 
 constexpr int kReduceLds = 3 * kSmallMax * kSmallMax;
 
+// Linear ensembles (atomic mode, no partials): blockIdx.y is the sample, whose
+// vectors and H store sit at the strides es (zero for one sample).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_reduce(
     const DBlock* __restrict__ blocks, const int64_t* __restrict__ part_list,
     const double* __restrict__ part, const T* __restrict__ V, const T* __restrict__ H,
-    double* __restrict__ W, int use_partials, int skip_large) {
+    double* __restrict__ W, int use_partials, int skip_large, EnsStride es) {
   __shared__ T sm[kReduceLds];
+  const int64_t smp = blockIdx.y;
+  V += smp * es.v;
+  W += smp * es.v;
+  H += smp * es.h;
   reduce_block<T, kThreads>(blocks[blockIdx.x], part_list, part, V, H, W, use_partials, sm, kReduceLds,
                             skip_large);
 }
@@ -1761,6 +1767,145 @@ __global__ __launch_bounds__(256) void k_mx2(const MTask2* __restrict__ tasks, i
   }
 }
 
+// ---------------------------------------------------------------- fused matrix-core path
+// One workgroup per MFTask (one 16-row tile of Ja', one 16-column tile of Jb,
+// up to kMfSlots output tiles of the (instance, Ja', Jb) group); wavefront w
+// takes steps [p[w], p[w + 1]).  Per step (slab): every product tile
+// X_u = V_u Bhat_u^T on the matrix cores, Y_s += r_su X_u in registers, then
+// W_s^T += Y_s^T A_s^T with Y_s's result registers as the first operand (the
+// result register q of a 16 x 16 x 4 step holds, in lane l, row M::row(l, q)
+// and column l % 16, i.e. K step q of the next product when A is read at
+// those rows).  X and Y never leave the registers; the slots' tiles are summed
+// over the four wavefronts in LDS and added into W with FP64 atomics (the
+// like-particle part is stored before, by k_reduce).
+// Operand loads through a buffer resource on a wavefront-uniform base: one
+// 32-bit byte offset per lane, and a lane outside its block passes an offset
+// beyond the resource's range (kOobOff), which returns zero without touching
+// memory.  Word 3 is the gfx9 raw-buffer format (32-bit data).
+constexpr uint32_t kOobOff = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mk_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+template <typename X>
+struct BufLd;
+template <>
+struct BufLd<double> {
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+};
+template <>
+struct BufLd<float> {
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+};
+template <>
+struct BufLd<Bf16x2> {
+  __device__ static Bf16x2 ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return Bf16x2{__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0)};
+  }
+};
+// byte offset of element idx, or kOobOff when the lane is outside its block
+template <typename X>
+__device__ __forceinline__ uint32_t boff(bool ok, int idx) {
+  return ok ? (uint32_t)idx * (uint32_t)sizeof(X) : kOobOff;
+}
+
+template <typename T, int KS, int SL>
+__global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, int n_tasks,
+                                             const MFStep* __restrict__ steps, const MFU* __restrict__ us,
+                                             const T* __restrict__ Vp, const T* __restrict__ Ad,
+                                             const T* __restrict__ Bt, double* __restrict__ W, EnsStride es) {
+  using C = typename BigIn<T>::C;
+  using M = BigMma<C>;
+  using vec = typename M::vec;
+  __shared__ C red[4][SL][4][64];
+  int ti;
+  if (es.n > 1) {
+    int64_t smp;
+    ti = ens_block(es, smp);
+    if (ti < 0) return;
+    Vp += smp * es.v;
+    W += smp * es.v;
+    Bt += smp * es.b;
+  } else {
+    ti = xcd_chunk(blockIdx.x, n_tasks);
+  }
+  if (ti >= n_tasks) return;  // whole workgroups
+  const MFTask t = tasks[ti];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int kq = lane >> 4, l16 = lane & 15;
+  const bool aok = l16 < t.rows_a, jok = l16 < t.cols_j;
+  vec w[SL];
+#pragma unroll
+  for (int q = 0; q < SL; ++q) w[q] = vec{C(0), C(0), C(0), C(0)};
+  // the wavefront's step range, kept in scalar registers
+  const int st_begin = __builtin_amdgcn_readfirstlane(wave == 0 ? t.p[0] : wave == 1 ? t.p[1] : wave == 2 ? t.p[2] : t.p[3]);
+  const int st_end = __builtin_amdgcn_readfirstlane(wave == 0 ? t.p[1] : wave == 1 ? t.p[2] : wave == 2 ? t.p[3] : t.p[4]);
+  for (int st = st_begin; st < st_end; ++st) {
+    const MFStep S = steps[st];
+    vec y[SL];
+#pragma unroll
+    for (int q = 0; q < SL; ++q) y[q] = vec{C(0), C(0), C(0), C(0)};
+    MFU U = us[S.u_begin];  // the next record is requested a product ahead
+    for (int u = S.u_begin; u < S.u_end; ++u) {
+      const MFU Un = us[u + 1 < S.u_end ? u + 1 : u];
+      const int dbr = U.dbr;
+      vec x = {C(0), C(0), C(0), C(0)};
+      mx_dispatch<1, KS>((dbr + 3) >> 2, [&](auto nk) {
+        constexpr int NK = decltype(nk)::value;
+        C a[NK], b[NK];
+        const __amdgpu_buffer_rsrc_t rv = mk_rsrc(Vp + U.v_off), rb = mk_rsrc(Bt + U.b_off);
+#pragma unroll
+        for (int ks = 0; ks < NK; ++ks) {
+          const int k = 4 * ks + kq;
+          a[ks] = BigIn<T>::get(BufLd<T>::ld(rv, boff<T>(k < dbr && aok, l16 + t.ld_v * k)));
+          b[ks] = BigIn<T>::get(BufLd<T>::ld(rb, boff<T>(k < dbr && jok, k + dbr * l16)));
+        }
+#pragma unroll
+        for (int ks = 0; ks < NK; ++ks) x = M::mma(a[ks], b[ks], x);
+      });
+#pragma unroll
+      for (int q = 0; q < SL; ++q) {
+        const C r = C(U.r[q]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) y[q][v] += r * x[v];
+      }
+      U = Un;
+    }
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+      if (S.a_off[q] < 0) continue;  // slot not in this slab (uniform)
+      const __amdgpu_buffer_rsrc_t ra = mk_rsrc(Ad + S.a_off[q]);
+      const bool iok = l16 < t.rows_i[q];
+      C av[4];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int kp = M::row(lane, ks);
+        av[ks] = BigIn<T>::get(BufLd<T>::ld(ra, boff<T>(kp < t.rows_a && iok, l16 + t.da[q] * kp)));
+      }
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) w[q] = M::mma(y[q][ks], av[ks], w[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SL; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[wave][q][v][lane] = w[q][v];
+  __syncthreads();
+  // thread (wave w, lane) sums register v = w of every slot over the wavefronts
+  const int v = wave;
+  const int jr = M::row(lane, v);
+#pragma unroll
+  for (int q = 0; q < SL; ++q) {
+    if (t.w_off[q] < 0) continue;
+    const C sum = (red[0][q][v][lane] + red[1][q][v][lane]) + (red[2][q][v][lane] + red[3][q][v][lane]);
+    if (l16 < t.rows_i[q] && jr < t.cols_j) unsafeAtomicAdd(W + t.w_off[q] + l16 + (int64_t)t.da[q] * jr, (double)sum);
+  }
+}
+
 // ---------------------------------------------------------------- instance tables
 struct StagedInst {
   int prec, nt, mu, mode, atomic;
@@ -1854,6 +1999,23 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
       hipError_t e = hipLaunchKernel(k_stk_ptr(2, prec, d.stk_nt), dim3(g2, gy), dim3(d.stk_nt), a2, lds2, s);
       if (e != hipSuccess) return e;
     }
+  } else if (d.mxf) {  // fused matrix-core path for the small blocks (adds into W)
+    if (!at) return hipErrorInvalidConfiguration;
+    if (d.n_mftask > 0) {
+      const unsigned g = (unsigned)(((d.n_mftask + kXcds - 1) / kXcds) * kXcds);
+      const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
+#define NSX_MXF(KS, SL)                                                                                   \
+  hipLaunchKernelGGL((k_mxf<T, KS, SL>), dim3(g, gy), dim3(256), 0, s, d.f_mtask, d.n_mftask, d.f_mstep, d.f_mu, \
+                     Vp, Ad, Bh, W, d.es)
+      if (d.mx_ks <= 6) {
+        if (d.mxf_slots == 2) NSX_MXF(6, 2); else NSX_MXF(6, kMfSlots);
+      } else {
+        if (d.mxf_slots == 2) NSX_MXF(10, 2); else NSX_MXF(10, kMfSlots);
+      }
+#undef NSX_MXF
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   } else if (d.mstk) {  // matrix-core stacked path for the small blocks (accumulates into W)
     if (!at) return hipErrorInvalidConfiguration;
     using Cx = typename BigIn<T>::C;
@@ -1932,12 +2094,12 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
   if (d.fused) {  // W = column partials + like-particle part, stored (large blocks: k_big0)
     if (d.n_fblocks > 0)
       hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_fblocks), dim3(kThreads), 0, s, d.f_blocks, d.f_part_list,
-                         d.f_part, V, H, W, 1, big ? 1 : 0);
+                         d.f_part, V, H, W, 1, big ? 1 : 0, EnsStride());
   }
   // the stacked stage-1 launch carries the small like-particle blocks
   if (!d.stacked && !d.mstk && !d.fused && d.n_blocks > 0)
-    hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks), dim3(kThreads), 0, s, d.blocks, d.part_list, d.part,
-                       V, H, W, d.atomic ? 0 : 1, big ? 1 : 0);
+    hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks, (unsigned)d.ens_s), dim3(kThreads), 0, s, d.blocks,
+                       d.part_list, d.part, V, H, W, d.atomic ? 0 : 1, big ? 1 : 0, d.es);
   if (big) {
     if constexpr (std::is_same<T, Bf16x2>::value)
       hipLaunchKernelGGL(k_big0_bf, dim3((unsigned)d.n_gtile3, (unsigned)d.ens_s), dim3(256), 0, s, d.g_tile3, V, H,
@@ -2031,7 +2193,7 @@ static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T*
   if (e != hipSuccess) return e;
   if (d.n_fblocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_fblocks), dim3(kThreads), 0, s, d.f_blocks, d.f_part_list, d.f_part,
-                       V, H, W, 1, big ? 1 : 0);
+                       V, H, W, 1, big ? 1 : 0, EnsStride());
   if (d.n_gtile2 > 0) {
     DevicePlan lg = d;    // the large-block tiles alone
     lg.n_ftask = 0;

@@ -50,6 +50,13 @@ struct DevicePlan {
   MUnit1* m_unit1 = nullptr;
   MTask2* m_task2 = nullptr;
   int n_mtask1 = 0, n_mtask2 = 0;
+  // fused matrix-core path (small blocks; k_reduce stores the like-particle
+  // part first, k_mxf adds the pn part)
+  int mxf = 0, mxf_slots = kMfSlots;
+  MFTask* f_mtask = nullptr;
+  MFStep* f_mstep = nullptr;
+  MFU* f_mu = nullptr;
+  int n_mftask = 0;
   // large-block GEMM path (FP64, atomic mode)
   GTask1* g_task1 = nullptr;
   GTile1* g_tile1 = nullptr;

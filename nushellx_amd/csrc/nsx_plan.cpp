@@ -1,5 +1,7 @@
 #include "nsx_plan.hpp"
 
+#include <stdexcept>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -521,6 +523,151 @@ void build_mstk(HostPlan& p, const std::vector<const Model*>& models) {
         }
     }
   }
+}
+
+static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models) {
+  p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear();
+  p.mx_ks = 1;
+  p.mxf_slots = p.cfg.mxf_slots == 2 ? 2 : kMfSlots;
+  struct Cons { int64_t a = -1; std::map<int, double> r; };
+  struct Slab {
+    std::map<int, std::pair<int64_t, int>> prod;  // R -> (Bhat^T offset, d(Jb'))
+    std::map<int, Cons> cons;                     // 2Ja -> a-term
+  };
+  const size_t nslot = p.cfg.mxf_slots == 2 ? 2 : kMfSlots;
+  for (size_t mi = 0; mi < models.size(); ++mi) {
+    const Model& m = *models[mi];
+    const int64_t vb = p.vbase[mi];
+    const auto& a_off = p.inst_a_off[mi];
+    const auto& b_off = p.inst_b_off[mi];
+    // (Ja', 2Jb) -> (lam, sa, sa') -> slab; per group the left blocks by 2Ja
+    std::map<std::array<int, 2>, std::map<std::array<int, 3>, Slab>> groups;
+    std::map<std::array<int, 2>, std::map<int, int>> lblk;  // group -> 2Ja -> L
+    for (const Item& it : m.items) {
+      const int L = it.L, da = m.da[L], db = m.db[L], dar = m.A.d(it.ajr);
+      int maxdbr = 0;
+      for (const ATerm& at : it.at)
+        for (const RTerm& rt : at.rt) maxdbr = std::max(maxdbr, m.db[rt.R]);
+      if (std::max(std::max(da, dar), std::max(db, maxdbr)) > p.cfg.big_min) continue;  // large-block paths
+      const std::array<int, 2> g{it.ajr, m.bas_b[L]};
+      lblk[g][m.bas_a[L]] = L;
+      for (const ATerm& at : it.at) {
+        if (at.rt.empty()) continue;
+        const auto& bk = m.bhat_key[at.rt[0].bhat];
+        Slab& sl = groups[g][{bk[2], bk[3], bk[4]}];
+        Cons& c = sl.cons[m.bas_a[L]];
+        if (c.a >= 0) throw std::runtime_error("build_mxf: two a-terms of one left block in one slab");
+        c.a = a_off[at.adens];
+        for (const RTerm& rt : at.rt) {
+          const int64_t bo = b_off[rt.bhat];
+          auto f = sl.prod.find(rt.R);
+          if (f == sl.prod.end())
+            sl.prod[rt.R] = {bo, m.db[rt.R]};
+          else if (f->second.first != bo)
+            throw std::runtime_error("build_mxf: one right block with two Bhat blocks in one slab");
+          c.r[rt.R] += rt.r;
+        }
+      }
+    }
+    for (auto& g : groups) {
+      const int dar = m.A.d(g.first[0]);
+      const auto& lb = lblk[g.first];
+      const int db = m.db[lb.begin()->second];
+      struct Slot { int ja, L, i0; };
+      std::vector<Slot> slots;
+      for (auto& kv : lb)
+        for (int i0 = 0; i0 < m.da[kv.second]; i0 += kMxTile) slots.push_back({kv.first, kv.second, i0});
+      for (int a0 = 0; a0 < dar; a0 += kMxTile)
+        for (int j0 = 0; j0 < db; j0 += kMxTile)
+          for (size_t s0 = 0; s0 < slots.size(); s0 += nslot) {
+            const int ns = (int)std::min<size_t>(nslot, slots.size() - s0);
+            MFTask t{};
+            for (int q = 0; q < kMfSlots; ++q) {
+              t.w_off[q] = -1;
+              t.da[q] = 1;
+            }
+            for (int q = 0; q < ns; ++q) {
+              const Slot& sl = slots[s0 + q];
+              const int da = m.da[sl.L];
+              t.w_off[q] = vb + m.posnv[sl.L] + sl.i0 + int64_t(da) * j0;
+              t.da[q] = da;
+              t.rows_i[q] = std::min(kMxTile, da - sl.i0);
+            }
+            t.ld_v = dar;
+            t.rows_a = std::min(kMxTile, dar - a0);
+            t.db = db;
+            t.cols_j = std::min(kMxTile, db - j0);
+            // steps: the group's slabs that feed any slot of the task
+            std::vector<int> step_ids;
+            std::vector<double> cost;
+            for (auto& kv : g.second) {
+              const Slab& sl = kv.second;
+              bool any = false;
+              for (int q = 0; q < ns; ++q) any |= sl.cons.count(slots[s0 + q].ja) > 0;
+              if (!any) continue;
+              MFStep st{};
+              double c = 0.0;
+              for (int q = 0; q < kMfSlots; ++q) st.a_off[q] = -1;
+              for (int q = 0; q < ns; ++q) {
+                const Slot& so = slots[s0 + q];
+                auto f = sl.cons.find(so.ja);
+                if (f == sl.cons.end()) continue;
+                st.a_off[q] = f->second.a + so.i0 + int64_t(m.da[so.L]) * a0;
+                c += 4.0;
+              }
+              st.u_begin = (int)p.f_mu.size();
+              for (auto& pr : sl.prod) {
+                MFU u{};
+                u.v_off = vb + m.posnv[pr.first] + a0;
+                u.dbr = pr.second.second;
+                u.b_off = pr.second.first + int64_t(u.dbr) * j0;
+                for (int q = 0; q < ns; ++q) {
+                  auto f = sl.cons.find(slots[s0 + q].ja);
+                  if (f == sl.cons.end()) continue;
+                  auto fr = f->second.r.find(pr.first);
+                  if (fr != f->second.r.end()) u.r[q] = fr->second;
+                }
+                p.mx_ks = std::max(p.mx_ks, (u.dbr + 3) / 4);
+                c += (u.dbr + 3) / 4 + 1;
+                p.f_mu.push_back(u);
+              }
+              st.u_end = (int)p.f_mu.size();
+              step_ids.push_back((int)p.f_mstep.size());
+              cost.push_back(c);
+              p.f_mstep.push_back(st);
+            }
+            if (step_ids.empty()) continue;
+            // workgroups of at most 4 * mxf_steps steps (each a copy of the
+            // task's slots), four contiguous step ranges of about equal cost
+            const int per = p.cfg.mxf_steps > 0 ? 4 * p.cfg.mxf_steps : (int)step_ids.size();
+            for (size_t c0 = 0; c0 < step_ids.size(); c0 += per) {
+              const int n = (int)std::min<size_t>(per, step_ids.size() - c0);
+              const int first = step_ids[c0];
+              double total = 0.0;
+              for (int q = 0; q < n; ++q) total += cost[c0 + q];
+              t.p[0] = first;
+              double acc = 0.0;
+              int k = 0;
+              for (int w = 1; w < 4; ++w) {
+                while (k < n && acc + 0.5 * cost[c0 + k] < total * w / 4.0) acc += cost[c0 + k++];
+                t.p[w] = first + k;
+              }
+              t.p[4] = first + n;
+              p.f_mtask.push_back(t);
+            }
+          }
+    }
+  }
+}
+
+bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {
+  try {
+    build_mxf_impl(p, models);
+  } catch (const std::exception&) {
+    p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear();
+    return false;
+  }
+  return true;
 }
 
 bool build_fused(HostPlan& p, const std::vector<const Model*>& models, int threads) {

@@ -150,6 +150,39 @@ struct MTask2 {
   int32_t t_begin, t_end;   // STerm2 range
 };
 
+// Fused matrix-core path (small blocks; stacked == 4): X never leaves the
+// registers.  A slab (instance, Ja', Jb, lam, sa, sa') has products
+// X_u = V_(Ja',Jb'_u) Bhat_(Jb,Jb'_u)^T and consumers (the left blocks (Ja, Jb)
+// whose a-term A(Ja, Ja') reads them, each with its r_u).  A task is one
+// 16-row tile of Ja' (a0), one 16-column tile of Jb (j0) and up to kMfSlots
+// output slots (consumer Ja, 16-row tile i0 of W_(Ja,Jb)) of one (instance,
+// Ja', Jb) group; its steps are the group's slabs.  Per step a wavefront forms
+// every X_u tile on the matrix cores, accumulates Y_s += r_su X_u per slot in
+// registers (the 16 x 16 x 4 FP64 result layout of X is the operand layout of
+// the next product's K steps), then W_s^T += Y_s^T A_s^T; the slots' tiles stay
+// in registers across the steps.  The four wavefronts of a workgroup take every
+// fourth step of one task; their tiles are summed in LDS and added into W.
+constexpr int kMfSlots = 4;
+struct MFTask {
+  int64_t w_off[kMfSlots];  // W_(Ja,Jb) + i0 + da * j0 (slot's tile origin); < 0: unused slot
+  int32_t da[kMfSlots];     // leading dimension of the slot's W block and A
+  int32_t rows_i[kMfSlots]; // valid rows of the slot's tile (Ja states)
+  int32_t ld_v, rows_a;     // d(Ja') (V's leading dimension); valid Ja' rows of the tile
+  int32_t db, cols_j;       // d(Jb) (Bhat^T columns); valid Jb columns of the tile
+  int32_t p[5];             // wavefront w takes steps [p[w], p[w + 1])
+  int32_t pad;
+};
+struct MFStep {
+  int64_t a_off[kMfSlots];  // A(Ja, Ja') + i0 + da * a0 per slot; < 0: slot not in this slab
+  int32_t u_begin, u_end;   // MFU range
+};
+struct MFU {                // one product of a slab
+  int64_t v_off;            // V_(Ja',Jb') + a0 (leading dimension ld_v)
+  int64_t b_off;            // Bhat^T(Jb, Jb') + dbr * j0 (dbr x db column-major)
+  int32_t dbr, pad;         // d(Jb')
+  double r[kMfSlots];       // r_su (0 for slots not in this slab)
+};
+
 // Fused small-block path (owner-computes, no atomics, deterministic):
 // one workgroup owns the output column Jb of one instance (the blocks
 // L = (Ja, Jb), concatenated into a column partial) for a range of the column's
@@ -277,6 +310,12 @@ struct PlanConfig {
   // wavefront and a-terms per stage-2 workgroup
   int mx_units1 = 4;
   int mx_terms2 = 16;
+  // stacked == 4, the fused matrix-core path: steps (slabs) per wavefront, at
+  // most (0: a workgroup takes all steps of its task)
+  int mxf_steps = 2;
+  // output slots per fused task: 4 (kMfSlots), or 2 (fewer registers, more
+  // products formed twice)
+  int mxf_slots = kMfSlots;
 };
 
 // Linear ensembles: every sample's operators are linear in its parameters, so
@@ -340,6 +379,11 @@ struct HostPlan {
   std::vector<MUnit1> m_unit1;
   std::vector<MTask2> m_task2;
   int mx_ks = 1;               // largest inner size in K steps of 4
+  // fused matrix-core path (filled by build_mxf)
+  std::vector<MFTask> f_mtask;
+  std::vector<MFStep> f_mstep;
+  std::vector<MFU> f_mu;
+  int mxf_slots = kMfSlots;      // slots the plan's tasks use (2 or 4)
   // large-block path (filled by build_big)
   std::vector<GTask1> g_task1;
   std::vector<GTile1> g_tile1;
@@ -392,6 +436,10 @@ void build_stacked(HostPlan& p, const std::vector<const Model*>& models);
 // Fills the matrix-core stacked path's arrays (m_unit1, s_term2, s_xref,
 // m_task1, m_task2, x_size) for the small-block items of `models`.
 void build_mstk(HostPlan& p, const std::vector<const Model*>& models);
+// Fills the fused matrix-core path's arrays (f_mtask, f_mstep, f_mu, mx_ks)
+// for the small-block items of `models`; false (arrays empty) if a slab does
+// not have the shape the path assumes.
+bool build_mxf(HostPlan& p, const std::vector<const Model*>& models);
 // Fills the fused-path arrays of p for the small-block items of `models`.
 // Returns false (and leaves them empty) if a column exceeds the kernel's
 // limits (blocks per column, output units) or the plan has items that only

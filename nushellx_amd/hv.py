@@ -307,10 +307,13 @@ class Batch:
         return r.as_dict()
 
     def path(self) -> str:
-        """The small-block pn path of the plan: "staged", "stacked", "fused" or
-        "stacked-mfma" (the stacked stages on the matrix cores)."""
+        """The small-block pn path of the plan: "staged", "stacked", "fused",
+        "stacked-mfma" (the stacked stages on the matrix cores) or "fused-mfma"
+        (both stages in one matrix-core kernel)."""
         v = C.c_int32()
         check(lib().nsx_batch_path(self._b, C.byref(v)), "nsx_batch_path")
+        if v.value & 8:
+            return "fused-mfma"
         return ("staged", "stacked", "fused", "stacked-mfma")[v.value & 3] + (" deterministic" if v.value & 4 else "")
 
     def set_config(self, cfg: dict) -> None:

@@ -37,6 +37,7 @@ def lib():
                                                       C.c_void_p]
         L.emu_hv_mstk.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int,
                                                      C.c_void_p]
+        L.emu_hv_mxf.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -144,6 +145,19 @@ def emu_hv_mstk(args, v, units1=4, terms2=16, big_min=0):
     st = np.zeros(4, dtype=np.int64)
     rc = lib().emu_hv_mstk(*[a.encode() for a in args], v.ctypes.data, out.ctypes.data, len(v), units1, terms2,
                            big_min, st.ctypes.data)
+    if rc:
+        raise RuntimeError(lib().emu_last_error().decode())
+    return out, st
+
+
+def emu_hv_mxf(args, v, steps=2, slots=4):
+    """Emulated H*v through the fused matrix-core plan (lane-level model of
+    k_mxf); returns (w, [tasks, steps, products, largest inner size in K steps])."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty_like(v)
+    st = np.zeros(4, dtype=np.int64)
+    rc = lib().emu_hv_mxf(*[a.encode() for a in args], v.ctypes.data, out.ctypes.data, len(v), steps, slots,
+                          st.ctypes.data)
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out, st

@@ -395,6 +395,111 @@ extern "C" int emu_hv_mstk(const char* nus, const char* opa, const char* opb, co
   return 0;
 }
 
+// Fused matrix-core path, emulated lane by lane as k_mxf runs it: per step
+// the X_u tiles (16 x 16 x 4 FP64 steps), Y_s += r_su X_u in the result
+// layout, W_s^T += Y_s^T A_s^T with Y's result registers as the first operand's
+// K steps, the four wavefronts' tiles summed; every memory index checked.
+extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, const char* intf,
+                          const double* vin, double* vout, int64_t n, int steps_per_wave, int slots,
+                          int64_t* stats4) {
+  Model m;
+  std::string e = build_model(nus, opa, opb, intf, m);
+  if (!e.empty()) { g_err = e; return -1; }
+  if (m.N != n) { g_err = "dimension mismatch"; return -2; }
+  HostPlan p;
+  p.cfg.stacked = 4;
+  p.cfg.mxf_steps = steps_per_wave;
+  p.cfg.mxf_slots = slots;
+  plan_add(p, m);
+  std::vector<const Model*> ms{&m};
+  if (!build_mxf(p, ms)) { g_err = "build_mxf refused the model"; return -6; }
+  if (p.mx_ks > 10) { g_err = "inner size above 40"; return -3; }
+  std::vector<double> vm;
+  const double* vp = pn_input(p, vin, vm);
+  bool oob = false;
+  auto chk = [&](int64_t i, int64_t size) { if (i < 0 || i >= size) oob = true; return i >= 0 && i < size; };
+  std::fill(vout, vout + n, 0.0);
+  for (const DBlock& b : p.blocks)
+    for (int jj = 0; jj < b.nc; ++jj) {
+      const int j = b.j0 + jj;
+      for (int i = 0; i < b.da; ++i) {
+        double w = 0.0;
+        if (b.ha_off >= 0)
+          for (int k = 0; k < b.da; ++k) w += p.H[b.ha_off + i + int64_t(b.da) * k] * vin[b.v_off + k + int64_t(b.da) * j];
+        if (b.hb_off >= 0)
+          for (int k = 0; k < b.db; ++k) w += vin[b.v_off + i + int64_t(b.da) * k] * p.H[b.hb_off + j + int64_t(b.db) * k];
+        vout[b.v_off + i + int64_t(b.da) * j] = w;
+      }
+    }
+  int64_t steps = 0, prods = 0;
+  for (const MFTask& t : p.f_mtask) {
+    double Wt[4][kMfSlots][64][4] = {};
+    for (int wave = 0; wave < 4; ++wave)
+      for (int st = t.p[wave]; st < t.p[wave + 1]; ++st) {
+        const MFStep& S = p.f_mstep[st];
+        ++steps;
+        double Y[kMfSlots][64][4] = {};
+        for (int u = S.u_begin; u < S.u_end; ++u) {
+          const MFU& U = p.f_mu[u];
+          ++prods;
+          const int nk = (U.dbr + 3) >> 2;
+          double X[64][4] = {};
+          for (int ks = 0; ks < nk; ++ks) {
+            double a[64], b[64];
+            for (int l = 0; l < 64; ++l) {
+              const int k = 4 * ks + (l >> 4), l16 = l & 15;
+              const bool okv = k < U.dbr && l16 < t.rows_a;
+              const int64_t iv = U.v_off + (okv ? l16 + int64_t(t.ld_v) * k : 0);
+              a[l] = (chk(iv, n) && okv) ? vp[iv] : 0.0;
+              const bool okb = k < U.dbr && l16 < t.cols_j;
+              const int64_t ib = U.b_off + (okb ? k + int64_t(U.dbr) * l16 : 0);
+              b[l] = (chk(ib, (int64_t)p.Bh.size()) && okb) ? p.Bh[ib] : 0.0;
+            }
+            Mfma16::step(a, b, X);
+          }
+          for (int q = 0; q < kMfSlots; ++q)
+            for (int l = 0; l < 64; ++l)
+              for (int v = 0; v < 4; ++v) Y[q][l][v] += U.r[q] * X[l][v];
+        }
+        for (int q = 0; q < kMfSlots; ++q) {
+          if (S.a_off[q] < 0) continue;
+          for (int ks = 0; ks < 4; ++ks) {
+            double a[64], b[64];
+            for (int l = 0; l < 64; ++l) {
+              const int kp = Mfma16::row(l, ks), l16 = l & 15;
+              a[l] = Y[q][l][ks];
+              const bool ok = kp < t.rows_a && l16 < t.rows_i[q];
+              const int64_t ia = S.a_off[q] + (ok ? l16 + int64_t(t.da[q]) * kp : 0);
+              b[l] = (chk(ia, (int64_t)p.Ad.size()) && ok) ? p.Ad[ia] : 0.0;
+            }
+            Mfma16::step(a, b, Wt[wave][q]);
+          }
+        }
+      }
+    for (int q = 0; q < kMfSlots; ++q) {
+      if (t.w_off[q] < 0) continue;
+      for (int l = 0; l < 64; ++l)
+        for (int v = 0; v < 4; ++v) {
+          const int jr = Mfma16::row(l, v), i = l & 15;
+          if (i < t.rows_i[q] && jr < t.cols_j) {
+            const int64_t iw = t.w_off[q] + i + int64_t(t.da[q]) * jr;
+            if (chk(iw, n))
+              vout[iw] += (Wt[0][q][l][v] + Wt[1][q][l][v]) + (Wt[2][q][l][v] + Wt[3][q][l][v]);
+          }
+        }
+    }
+  }
+  if (stats4) {
+    stats4[0] = (int64_t)p.f_mtask.size();
+    stats4[1] = steps;
+    stats4[2] = prods;
+    stats4[3] = p.mx_ks;
+  }
+  if (oob) { g_err = "out-of-range access in the fused matrix-core path"; return -4; }
+  if (!p.items.empty()) { g_err = "large-block items are not emulated by emu_hv_mxf"; return -5; }
+  return 0;
+}
+
 // Stacked two-stage path: X = V Bhat^T per distinct pair, then W += A sum r X,
 // plus the like-particle part and the direct (large-block) items, all from the
 // plan arrays the stacked kernels read.

@@ -135,15 +135,16 @@ def test_linear_ensemble_with_tuned_config(ens):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
-def test_linear_ensemble_on_matrix_core_stacked_path(ens, prec, tol):
-    """Every sample through the stacked stages on the matrix cores (grid y the
-    sample), against the per-sample file batch in FP64."""
+@pytest.mark.parametrize("stacked,path", [(3, "stacked-mfma"), (4, "fused-mfma")])
+def test_linear_ensemble_on_matrix_core_paths(ens, prec, tol, stacked, path):
+    """Every sample through the stacked stages on the matrix cores, two-stage
+    and fused (grid y the sample), against the per-sample file batch in FP64."""
     from nushellx_amd.hv import Batch
     ref = Batch(ens["hams"])
-    cfg = {"fat_cap": 5120, "max_phases": 6, "threads": 256, "mode": 0, "atomic": 1, "stacked": 3,
-           "stk_cap": 8, "stk_work": 4}
+    cfg = {"fat_cap": 5120, "max_phases": 6, "threads": 256, "mode": 0, "atomic": 1, "stacked": stacked,
+           "stk_cap": 8 if stacked == 3 else 1, "stk_work": 4}
     lin, _, _ = _linear(ens, prec, config=cfg)
-    assert lin.path() == "stacked-mfma"
+    assert lin.path() == path
     v = np.random.default_rng(19).standard_normal(ref.N)
     w_ref, w = _hv(ref, v), _hv(lin, v)
     for k, o in enumerate(ref.offsets):

@@ -80,6 +80,26 @@ def test_mstk_plan_matches_refcpu(models):
             assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, units1, terms2)
 
 
+def test_mxf_plan_matches_refcpu(models):
+    """The fused matrix-core plan, executed lane by lane as its kernel runs it
+    (X_u tiles on 16 x 16 x 4 steps, Y_s = sum r X_u in the result layout fed
+    back as the next product's K steps, four wavefronts per task), gives the
+    oracle's H*v and stays inside every array, on blocks up to 33 (sd 6p6n:
+    several tiles per side and ten K steps), a truncated basis and an odd J."""
+    from tests.emulator import emu_hv_mxf
+    rng = np.random.default_rng(23)
+    cases = [("sd22", 0), ("sd22", 4), ("sd44", 0), ("sd44", 8), ("sd66", 4), ("sd23", 3), ("psd33t", 2)]
+    for name, J2 in cases:
+        args = models[name].args(J2)
+        r = RefCPU(*args, fp64=True)
+        v = rng.standard_normal(r.N)
+        ref = r.apply(v)
+        for steps, slots in ((0, 4), (1, 4), (3, 2), (1, 2)):
+            w, st = emu_hv_mxf(args, v, steps, slots)
+            assert st[0] > 0 and st[1] > 0
+            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, steps, slots)
+
+
 def test_fused_plan_matches_refcpu(models):
     """The fused column plan (per task: rounds of V_R / Bhat^T / A staging,
     X = V Bhat^T, Y = sum r X, W += A Y into the column's outputs; then the
