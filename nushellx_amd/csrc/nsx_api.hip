@@ -87,6 +87,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
     if (!(e = up(d.f_mtask, hp.f_mtask)).empty()) return e;
     if (!(e = up(d.f_mstep, hp.f_mstep)).empty()) return e;
     if (!(e = up(d.f_mu, hp.f_mu)).empty()) return e;
+    if (!(e = up(d.Bn, hp.Bn)).empty()) return e;
     d.n_mftask = (int)hp.f_mtask.size();
     d.mx_ks = hp.mx_ks;
     d.mxf_slots = hp.mxf_slots;
@@ -183,7 +184,7 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.s_chunk1); (void)hipFree(d.s_unit1); (void)hipFree(d.s_chunk2); (void)hipFree(d.s_term2);
   (void)hipFree(d.s_xref); (void)hipFree(d.X);
   (void)hipFree(d.m_task1); (void)hipFree(d.m_unit1); (void)hipFree(d.m_task2);
-  (void)hipFree(d.f_mtask); (void)hipFree(d.f_mstep); (void)hipFree(d.f_mu);
+  (void)hipFree(d.f_mtask); (void)hipFree(d.f_mstep); (void)hipFree(d.f_mu); (void)hipFree(d.Bn);
   (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
   (void)hipFree(d.g_tile3);
   (void)hipFree(d.g_xref); (void)hipFree(d.XG);

@@ -274,8 +274,10 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   }
   HostPlan& hp = b->hplan;
   hp.cfg = cfg ? plan_config(*cfg) : PlanConfig();
-  // the stacked stages: on the VALU, on the matrix cores, or fused on the matrix cores
-  hp.cfg.stacked = cfg && (cfg->stacked == 3 || cfg->stacked == 4) ? cfg->stacked : 1;
+  // the stacked stages, on the VALU or on the matrix cores (the fused
+  // matrix-core path reads an untransposed copy of the Bhat store, which the
+  // per-sample fold does not make: it takes the two-stage matrix-core path)
+  hp.cfg.stacked = cfg && (cfg->stacked == 3 || cfg->stacked == 4) ? 3 : 1;
   hp.cfg.deterministic = 0;
   hp.cfg.big_rows1 = precision == NSX_BF16X3 ? (hp.cfg.big_rows1_req ? hp.cfg.big_rows1_req : 2 * kBigTile) : kBigTile;
   // test hook, as for batches: a smaller block side above which the large-block tiles run
@@ -286,7 +288,6 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
     hp.ens_opcol = (int32_t)opcol[t];
     plan_add(hp, *tms[t]);
   }
-  if (hp.cfg.stacked == 4 && !build_mxf(hp, tms)) hp.cfg.stacked = 1;
   if (hp.cfg.stacked == 3)
     build_mstk(hp, tms);
   else if (hp.cfg.stacked == 1)

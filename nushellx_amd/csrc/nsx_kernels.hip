@@ -1862,7 +1862,7 @@ __global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, i
         for (int ks = 0; ks < NK; ++ks) {
           const int k = 4 * ks + kq;
           a[ks] = BigIn<T>::get(BufLd<T>::ld(rv, boff<T>(k < dbr && aok, l16 + t.ld_v * k)));
-          b[ks] = BigIn<T>::get(BufLd<T>::ld(rb, boff<T>(k < dbr && jok, k + dbr * l16)));
+          b[ks] = BigIn<T>::get(BufLd<T>::ld(rb, boff<T>(k < dbr && jok, l16 + t.db * k)));
         }
 #pragma unroll
         for (int ks = 0; ks < NK; ++ks) x = M::mma(a[ks], b[ks], x);
@@ -1957,7 +1957,7 @@ static bool big_like_active(const DevicePlan& d) { return d.atomic && d.n_gtile3
 // those blocks); Vp: the pn input (V masked on truncated plans)
 template <typename T>
 static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, const T* Ad, const T* Bh, const T* H,
-                       double* W, hipStream_t s) {
+                       double* W, hipStream_t s, const T* Bnp = nullptr) {
   const bool at = d.atomic != 0;
   if (d.fused) {  // column partials of the small blocks (summed by the reduce)
     if (d.n_ftask > 0) {
@@ -2001,12 +2001,13 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
     }
   } else if (d.mxf) {  // fused matrix-core path for the small blocks (adds into W)
     if (!at) return hipErrorInvalidConfiguration;
+    const T* Bn = Bnp;
     if (d.n_mftask > 0) {
       const unsigned g = (unsigned)(((d.n_mftask + kXcds - 1) / kXcds) * kXcds);
       const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
 #define NSX_MXF(KS, SL)                                                                                   \
   hipLaunchKernelGGL((k_mxf<T, KS, SL>), dim3(g, gy), dim3(256), 0, s, d.f_mtask, d.n_mftask, d.f_mstep, d.f_mu, \
-                     Vp, Ad, Bh, W, d.es)
+                     Vp, Ad, Bn, W, d.es)
       if (d.mx_ks <= 6) {
         if (d.mxf_slots == 2) NSX_MXF(6, 2); else NSX_MXF(6, kMfSlots);
       } else {
@@ -2121,13 +2122,13 @@ hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const do
                      hipStream_t s) {
   const void* Vp = pn_input(d, lp, prec, V);
   switch (prec) {
-    case 0: return pn_T<double>(d, 0, V, (const double*)Vp, d.Ad, d.Bh, d.H, W, s);
+    case 0: return pn_T<double>(d, 0, V, (const double*)Vp, d.Ad, d.Bh, d.H, W, s, d.Bn);
     case 1:
       return pn_T<float>(d, 1, (const float*)lp.vin, (const float*)Vp, (const float*)lp.Ad, (const float*)lp.Bh,
-                         (const float*)lp.H, W, s);
+                         (const float*)lp.H, W, s, (const float*)lp.Bn);
     default:
       return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)Vp, (const Bf16x2*)lp.Ad,
-                          (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s);
+                          (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s, (const Bf16x2*)lp.Bn);
   }
 }
 

@@ -53,6 +53,7 @@ struct DevicePlan {
   // fused matrix-core path (small blocks; k_reduce stores the like-particle
   // part first, k_mxf adds the pn part)
   int mxf = 0, mxf_slots = kMfSlots;
+  double* Bn = nullptr;      // untransposed Bhat blocks (FP64)
   MFTask* f_mtask = nullptr;
   MFStep* f_mstep = nullptr;
   MFU* f_mu = nullptr;
@@ -130,6 +131,7 @@ struct LowpPlan {
   void* H = nullptr;
   void* vin = nullptr;
   void* vin_m = nullptr;  // the masked input of the pn part (truncated plans)
+  void* Bn = nullptr;     // the fused path's untransposed Bhat blocks
 };
 std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp);
 void free_lowp(LowpPlan& lp);

@@ -29,12 +29,14 @@ std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp) {
     if (!(e = upload_as<float>(hp.Ad, lp.Ad, cf)).empty()) return e;
     if (!(e = upload_as<float>(hp.Bh, lp.Bh, cf)).empty()) return e;
     if (!(e = upload_as<float>(hp.H, lp.H, cf)).empty()) return e;
+    if (!(e = upload_as<float>(hp.Bn, lp.Bn, cf)).empty()) return e;
     esz = sizeof(float);
   } else if (precision == 2) {
     auto cb = [](double x) { return bf16x2_split((float)x); };
     if (!(e = upload_as<Bf16x2>(hp.Ad, lp.Ad, cb)).empty()) return e;
     if (!(e = upload_as<Bf16x2>(hp.Bh, lp.Bh, cb)).empty()) return e;
     if (!(e = upload_as<Bf16x2>(hp.H, lp.H, cb)).empty()) return e;
+    if (!(e = upload_as<Bf16x2>(hp.Bn, lp.Bn, cb)).empty()) return e;
     esz = sizeof(Bf16x2);
   } else {
     return "";
@@ -51,6 +53,7 @@ void free_lowp(LowpPlan& lp) {
   (void)hipFree(lp.H);
   (void)hipFree(lp.vin);
   (void)hipFree(lp.vin_m);
+  (void)hipFree(lp.Bn);
   lp = LowpPlan();
 }
 

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <unordered_map>
 #include <set>
 #include <string>
 
@@ -526,8 +527,20 @@ void build_mstk(HostPlan& p, const std::vector<const Model*>& models) {
 }
 
 static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models) {
-  p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear();
+  p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear();
   p.mx_ks = 1;
+  // Bhat^T block of the Bh store -> its untransposed copy in Bn
+  std::unordered_map<int64_t, int64_t> bn_of;
+  auto bn = [&](int64_t bo, int dbr, int db) {
+    auto f = bn_of.find(bo);
+    if (f != bn_of.end()) return f->second;
+    const int64_t o = (int64_t)p.Bn.size();
+    p.Bn.resize(p.Bn.size() + size_t(dbr) * db);
+    for (int j = 0; j < db; ++j)
+      for (int k = 0; k < dbr; ++k) p.Bn[o + j + int64_t(db) * k] = p.Bh[bo + k + int64_t(dbr) * j];
+    bn_of[bo] = o;
+    return o;
+  };
   p.mxf_slots = p.cfg.mxf_slots == 2 ? 2 : kMfSlots;
   struct Cons { int64_t a = -1; std::map<int, double> r; };
   struct Slab {
@@ -620,7 +633,7 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
                 MFU u{};
                 u.v_off = vb + m.posnv[pr.first] + a0;
                 u.dbr = pr.second.second;
-                u.b_off = pr.second.first + int64_t(u.dbr) * j0;
+                u.b_off = bn(pr.second.first, u.dbr, db) + j0;
                 for (int q = 0; q < ns; ++q) {
                   auto f = sl.cons.find(slots[s0 + q].ja);
                   if (f == sl.cons.end()) continue;
@@ -664,7 +677,7 @@ bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {
   try {
     build_mxf_impl(p, models);
   } catch (const std::exception&) {
-    p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear();
+    p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear();
     return false;
   }
   return true;

@@ -178,7 +178,7 @@ struct MFStep {
 };
 struct MFU {                // one product of a slab
   int64_t v_off;            // V_(Ja',Jb') + a0 (leading dimension ld_v)
-  int64_t b_off;            // Bhat^T(Jb, Jb') + dbr * j0 (dbr x db column-major)
+  int64_t b_off;            // Bhat(Jb, Jb') + j0 in the Bn store (db x dbr column-major)
   int32_t dbr, pad;         // d(Jb')
   double r[kMfSlots];       // r_su (0 for slots not in this slab)
 };
@@ -384,6 +384,9 @@ struct HostPlan {
   std::vector<MFStep> f_mstep;
   std::vector<MFU> f_mu;
   int mxf_slots = kMfSlots;      // slots the plan's tasks use (2 or 4)
+  // the Bhat blocks the fused path reads, untransposed (db x dbr column-major),
+  // so that a row of lanes reads 16 consecutive elements
+  std::vector<double> Bn;
   // large-block path (filled by build_big)
   std::vector<GTask1> g_task1;
   std::vector<GTile1> g_tile1;

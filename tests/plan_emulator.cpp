@@ -452,8 +452,8 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
               const int64_t iv = U.v_off + (okv ? l16 + int64_t(t.ld_v) * k : 0);
               a[l] = (chk(iv, n) && okv) ? vp[iv] : 0.0;
               const bool okb = k < U.dbr && l16 < t.cols_j;
-              const int64_t ib = U.b_off + (okb ? k + int64_t(U.dbr) * l16 : 0);
-              b[l] = (chk(ib, (int64_t)p.Bh.size()) && okb) ? p.Bh[ib] : 0.0;
+              const int64_t ib = U.b_off + (okb ? l16 + int64_t(t.db) * k : 0);
+              b[l] = (chk(ib, (int64_t)p.Bn.size()) && okb) ? p.Bn[ib] : 0.0;
             }
             Mfma16::step(a, b, X);
           }
