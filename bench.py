@@ -71,11 +71,16 @@ def cpu_baseline(mf, seconds: float = 10.0):
             "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
-def traffic_from_profile(stacked: bool):
+# profile summary of the pn kernels of each small-block path
+PN_PROFILE = {"stacked": "k_stk1_k_stk2", "staged": "k_pn_staged", "fused": "k_fuse_k_reduce",
+              "stacked-mfma": "k_mx1_k_mx2", "fused-mfma": "k_mxf"}
+
+
+def traffic_from_profile(path_name: str):
     """HBM-side bytes per launch of the pn kernels of the tuned path from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
     copy-kernel factor recorded there), or None when that summary is absent."""
-    name = "k_stk1_k_stk2" if stacked else "k_pn_staged"
+    name = PN_PROFILE.get(path_name, "")
     path = os.path.join(ROOT, "profiles", f"{name}_sd44_J0-8_pmc.txt")
     if not os.path.exists(path):
         return None
@@ -195,7 +200,8 @@ def ensemble(log, samples: int = 256):
         tuned = small.autotune(iters=5)
         small.close()
         t_tune = time.perf_counter() - t0
-        cfg = tuned if tuned.get("stacked") == 1 else None
+        # a stacked shape (VALU or matrix cores) carries over to the ensemble
+        cfg = tuned if tuned.get("stacked") in (1, 3, 4) else None
         b, _, info = linear_ensemble(space, nuclei, J2s, base, seeds, "", precision=prec, config=cfg,
                                      templates=templates, log=log)
         setup = info["templates_s"] + info["theta_s"] + info["batch_s"]
@@ -572,19 +578,28 @@ def main():
     # whole H*v, and its pn kernels alone (no per-launch event overhead)
     ms_hv, ms_pn, ms_red = batch.time(din, dout, warmup=10, iters=min(args.steps, 1000))
     path = batch.path()
-    stacked = path.startswith("stacked")
-    # the stacked stage-1 launch also carries the like-particle blocks, so its
-    # pn launches are the whole H*v; the other paths do that part in the reduce
+    pname = path.split()[0]
+    stacked = pname.startswith("stacked")
+    # the stacked stage-1 launch (VALU or matrix cores) also carries the
+    # like-particle blocks, so its pn launches are the whole H*v; the other
+    # paths do that part in the reduce launch
     flops_pn = flops if stacked else flops - flops_like
     achieved = flops_pn / (ms_pn * 1e-3) / 1e12
-    traffic = traffic_from_profile(stacked)
-    kernel = {"stacked": "k_stk1 + k_stk2 (stacked two-stage pn path with the like-particle blocks, FP64)",
-              "staged": "k_pn_staged (pn two-body part, FP64)",
-              "fused": "k_fuse (fused column path, pn two-body part, FP64)"}[path.split()[0]]
+    traffic = traffic_from_profile(pname)
+    kernel = {"stacked": "k_stk1 + k_stk2 (stacked two-stage pn path with the like-particle blocks, FP64 VALU)",
+              "staged": "k_pn_staged (pn two-body part, FP64 VALU)",
+              "fused": "k_fuse (fused column path, pn two-body part, FP64 VALU)",
+              "stacked-mfma": "k_mx1 + k_mx2 (stacked stages on FP64 16x16x4 matrix-core tiles, with the "
+                              "like-particle blocks)",
+              "fused-mfma": "k_mxf (pn two-body part: both stacked stages in one kernel on FP64 16x16x4 "
+                            "matrix-core tiles, X and Y in registers)"}[pname]
+    mfma = pname.endswith("-mfma")
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                "engine": "FP64 vector FMA (VALU), whose peak equals the FP64 matrix-core peak on gfx950; "
-                          "the kernels are bound by load latency and instruction issue, not by either peak",
+                "engine": ("FP64 matrix cores (v_mfma_f64_16x16x4f64); the tiles are bound by operand-load "
+                           "latency, not by the matrix-core peak" if mfma else
+                           "FP64 vector FMA (VALU), whose peak equals the FP64 matrix-core peak on gfx950; "
+                           "the kernels are bound by load latency and instruction issue, not by either peak"),
                 "kernel": kernel, "kernel_ms": ms_pn, "flops_per_launch": flops_pn,
                 "hbm_GBs_alg": (bytes_alg / (ms_hv * 1e-3)) / 1e9,
                 "hbm_GBs_counters": (traffic / (ms_pn * 1e-3)) / 1e9 if traffic else None}
