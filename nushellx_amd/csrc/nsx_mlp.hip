@@ -43,9 +43,16 @@ __device__ inline float silu_grad(float z) {
 
 constexpr int kTile = 64, kKc = 16, kGT = 256;
 
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
 // C[m, n] (+)= sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn].
-// 64 x 64 output tile per block, 16 x 16 threads with 4 x 4 outputs each, K in
-// chunks of 16 through LDS (the load order follows whichever stride is unit).
+// 64 x 64 output tile per block, K in chunks of 16 through LDS (the load order
+// follows whichever stride is unit), multiplied on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32): each of the four wavefronts holds a 32 x 32
+// quarter as 2 x 2 blocks of 16 x 16; lane l supplies A(row l % 16, k l / 16)
+// and B(k l / 16, column l % 16) from LDS and holds C(4 (l / 16) + v, l % 16).
+// BF16 weights are widened to FP32 (exactly) while staging, so the products
+// are those of the FP32 formulation.
 // Epilogue: + bias[n] (optional), store to C (ldc); with Zact, also store the
 // activation (SiLU when act) to Zact.  accumulate: C += result.
 // Split K (part != null): block z of the grid's third dimension sums K range
@@ -66,12 +73,13 @@ __global__ __launch_bounds__(kGT) void k_gemm(int M, int N, int K, const TA* __r
     B += (int64_t)kb * sbk;
     K = min(K - kb, kchunk);
   }
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  float acc[4][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4v acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int c = 0; c < 2; ++c) acc[r][c] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
   const bool a_k_fast = sak == 1, b_n_fast = sbn == 1;
   for (int k0 = 0; k0 < K; k0 += kKc) {
     for (int e = threadIdx.x; e < kKc * kTile; e += kGT) {
@@ -86,39 +94,39 @@ __global__ __launch_bounds__(kGT) void k_gemm(int M, int N, int K, const TA* __r
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < kKc; ++kk) {
-      float a[4], b[4];
+    for (int ks = 0; ks < kKc; ks += 4) {
+      const int kk = ks + lane / 16;
+      float a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+      for (int r = 0; r < 2; ++r) a[r] = As[kk][wm + 16 * r + lane % 16];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+      for (int c = 0; c < 2; ++c) b[c] = Bs[kk][wn + 16 * c + lane % 16];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], b[c], acc[r][c], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int m = tm + ty + 16 * i;
-    if (m >= M) continue;
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = tn + tx + 16 * j;
-      if (n >= N) continue;
-      float v = acc[i][j];
-      if (part) {
-        part[((int64_t)blockIdx.z * M + m) * N + n] = v;
-        continue;
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = tm + wm + 16 * r + 4 * (lane / 16) + v, n = tn + wn + 16 * c + lane % 16;
+        if (m >= M || n >= N) continue;
+        float val = acc[r][c][v];
+        if (part) {
+          part[((int64_t)blockIdx.z * M + m) * N + n] = val;
+          continue;
+        }
+        if (bias) val += bias[n];
+        float* cp = C + (int64_t)m * ldc + n;
+        if (accumulate) val += *cp;
+        *cp = val;
+        if (Zact) Zact[(int64_t)m * ldc + n] = act ? silu(val) : val;
       }
-      if (bias) v += bias[n];
-      float* c = C + (int64_t)m * ldc + n;
-      if (accumulate) v += *c;
-      *c = v;
-      if (Zact) Zact[(int64_t)m * ldc + n] = act ? silu(v) : v;
-    }
-  }
 }
 
 // Sum of the split-K partials (in split order) with k_gemm's epilogue
