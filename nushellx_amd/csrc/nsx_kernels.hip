@@ -1844,6 +1844,11 @@ __global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, i
   // the wavefront's step range, kept in scalar registers
   const int st_begin = __builtin_amdgcn_readfirstlane(wave == 0 ? t.p[0] : wave == 1 ? t.p[1] : wave == 2 ? t.p[2] : t.p[3]);
   const int st_end = __builtin_amdgcn_readfirstlane(wave == 0 ? t.p[1] : wave == 1 ? t.p[2] : wave == 2 ? t.p[3] : t.p[4]);
+  // K steps of the W products that can hold rows of the Ja' tile: in the FP64
+  // result layout register q holds rows 4q..4q+3, in the FP32 one rows q, q+4, ..
+  const int nka = sizeof(C) == 8 ? (t.rows_a + 3) >> 2 : min(4, t.rows_a);
+  mx_dispatch<1, 4>(nka, [&](auto nka_c) {
+  constexpr int NKA = decltype(nka_c)::value;
   for (int st = st_begin; st < st_end; ++st) {
     const MFStep S = steps[st];
     vec y[SL];
@@ -1880,16 +1885,17 @@ __global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, i
       if (S.a_off[q] < 0) continue;  // slot not in this slab (uniform)
       const __amdgpu_buffer_rsrc_t ra = mk_rsrc(Ad + S.a_off[q]);
       const bool iok = l16 < t.rows_i[q];
-      C av[4];
+      C av[NKA];
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
+      for (int ks = 0; ks < NKA; ++ks) {
         const int kp = M::row(lane, ks);
         av[ks] = BigIn<T>::get(BufLd<T>::ld(ra, boff<T>(kp < t.rows_a && iok, l16 + t.da[q] * kp)));
       }
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) w[q] = M::mma(y[q][ks], av[ks], w[q]);
+      for (int ks = 0; ks < NKA; ++ks) w[q] = M::mma(y[q][ks], av[ks], w[q]);
     }
   }
+  });
 #pragma unroll
   for (int q = 0; q < SL; ++q)
 #pragma unroll
