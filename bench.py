@@ -580,10 +580,10 @@ def main():
     path = batch.path()
     pname = path.split()[0]
     stacked = pname.startswith("stacked")
-    # the stacked stage-1 launch (VALU or matrix cores) also carries the
-    # like-particle blocks, so its pn launches are the whole H*v; the other
-    # paths do that part in the reduce launch
-    flops_pn = flops if stacked else flops - flops_like
+    # the stacked stage-1 launch (VALU or matrix cores) and the fused
+    # matrix-core path with "+like" also carry the like-particle blocks, so their
+    # pn launches are the whole H*v; the other paths do that part in the reduce
+    flops_pn = flops if stacked or "+like" in path else flops - flops_like
     achieved = flops_pn / (ms_pn * 1e-3) / 1e12
     traffic = traffic_from_profile(pname)
     kernel = {"stacked": "k_stk1 + k_stk2 (stacked two-stage pn path with the like-particle blocks, FP64 VALU)",

@@ -158,7 +158,8 @@ int nsx_batch_autotune(nsx_batch *b, int32_t iters, nsx_tune_result *best);
 int nsx_batch_set_config(nsx_batch *b, const nsx_tune_result *cfg);
 /* The small-block pn path the batch's plan runs: 0 staged kernel, 1 stacked
  * two-stage path, 2 fused column path, 3 stacked path on the matrix cores,
- * 8 both stages fused on the matrix cores (a requested path falls back to the
+ * 8 both stages fused on the matrix cores (plus 16 when that path also carries
+ * the like-particle part) (a requested path falls back to the
  * staged kernel where its limits are exceeded), plus 4 when the plan is
  * deterministic.  No counterpart in the reference. */
 int nsx_batch_path(const nsx_batch *b, int32_t *path);

@@ -91,6 +91,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
     d.n_mftask = (int)hp.f_mtask.size();
     d.mx_ks = hp.mx_ks;
     d.mxf_slots = hp.mxf_slots;
+    d.mxf_like = hp.mxf_like ? 1 : 0;
     d.mxf = 1;
     d.atomic = 1;
   }
@@ -489,6 +490,7 @@ int nsx_dev_sync(void) {
 int nsx_batch_path(const nsx_batch* b, int32_t* path) {
   if (!b || !path) { set_error("null argument"); return NSX_EARG; }
   *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : b->dplan.mstk ? 3 : b->dplan.mxf ? 8 : 0;
+  if (b->dplan.mxf && b->dplan.mxf_like) *path |= 16;  // the like-particle part rides in the fused path
   // deterministic: the fused path for the small blocks and unsplit large tiles
   if (b->dplan.fused && b->hplan.cfg.deterministic) *path |= 4;
   return NSX_OK;

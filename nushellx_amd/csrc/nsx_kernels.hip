@@ -367,6 +367,12 @@ __device__ __forceinline__ void reduce_block(DBlock b, const int64_t* __restrict
 
 constexpr int kReduceLds = 3 * kSmallMax * kSmallMax;
 
+// W = 0 before the fused matrix-core path adds every term (pn and like-particle)
+__global__ __launch_bounds__(kThreads) void k_zero_w(double* __restrict__ W, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) W[i] = 0.0;
+}
+
 // Linear ensembles (atomic mode, no partials): blockIdx.y is the sample, whose
 // vectors and H store sit at the strides es (zero for one sample).
 template <typename T>
@@ -2104,7 +2110,10 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
                          d.f_part, V, H, W, 1, big ? 1 : 0, EnsStride());
   }
   // the stacked stage-1 launch carries the small like-particle blocks
-  if (!d.stacked && !d.mstk && !d.fused && d.n_blocks > 0)
+  const bool like_in_pn = d.mxf && d.mxf_like;  // the fused matrix-core path adds the like-particle part
+  if (like_in_pn && d.N > 0)
+    hipLaunchKernelGGL(k_zero_w, dim3((unsigned)((d.N + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, W, d.N);
+  if (!d.stacked && !d.mstk && !d.fused && !like_in_pn && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks, (unsigned)d.ens_s), dim3(kThreads), 0, s, d.blocks,
                        d.part_list, d.part, V, H, W, d.atomic ? 0 : 1, big ? 1 : 0, d.es);
   if (big) {

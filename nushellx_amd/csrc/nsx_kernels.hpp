@@ -52,7 +52,7 @@ struct DevicePlan {
   int n_mtask1 = 0, n_mtask2 = 0;
   // fused matrix-core path (small blocks; k_reduce stores the like-particle
   // part first, k_mxf adds the pn part)
-  int mxf = 0, mxf_slots = kMfSlots;
+  int mxf = 0, mxf_slots = kMfSlots, mxf_like = 0;
   double* Bn = nullptr;      // untransposed Bhat blocks (FP64)
   MFTask* f_mtask = nullptr;
   MFStep* f_mstep = nullptr;

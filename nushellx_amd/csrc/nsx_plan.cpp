@@ -542,6 +542,32 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
     return o;
   };
   p.mxf_slots = p.cfg.mxf_slots == 2 ? 2 : kMfSlots;
+  // the like-particle part of the small blocks folded into the path (then W
+  // is cleared first instead of stored by k_reduce): not on truncated plans,
+  // whose pn part reads the masked vector, nor on linear ensembles
+  const bool fold_like = p.cfg.mxf_like && p.vmask.empty() && !p.ens;
+  p.mxf_like = fold_like;
+  std::unordered_map<int64_t, const DBlock*> blk_of;  // small block by its vector offset
+  if (fold_like)
+    for (const DBlock& b : p.blocks)
+      if (b.j0 == 0 && b.nc == b.db && std::max(b.da, b.db) <= kSmallMax) blk_of[b.v_off] = &b;
+  std::unordered_map<int, int64_t> id_bn, id_ad;
+  std::unordered_map<int64_t, int64_t> hb_bn, ha_ad;
+  auto ident = [](std::vector<double>& st, int d) {
+    const int64_t o = (int64_t)st.size();
+    st.resize(st.size() + size_t(d) * d, 0.0);
+    for (int i = 0; i < d; ++i) st[o + i + int64_t(d) * i] = 1.0;
+    return o;
+  };
+  auto copy_h = [&](std::vector<double>& st, int64_t h, int d) {
+    const int64_t o = (int64_t)st.size();
+    st.insert(st.end(), p.H.begin() + h, p.H.begin() + h + int64_t(d) * d);
+    return o;
+  };
+  auto ident_bn = [&](int d) { auto f = id_bn.find(d); return f != id_bn.end() ? f->second : id_bn[d] = ident(p.Bn, d); };
+  auto ident_ad = [&](int d) { auto f = id_ad.find(d); return f != id_ad.end() ? f->second : id_ad[d] = ident(p.Ad, d); };
+  auto h_in_bn = [&](int64_t h, int d) { auto f = hb_bn.find(h); return f != hb_bn.end() ? f->second : hb_bn[h] = copy_h(p.Bn, h, d); };
+  auto h_in_ad = [&](int64_t h, int d) { auto f = ha_ad.find(h); return f != ha_ad.end() ? f->second : ha_ad[h] = copy_h(p.Ad, h, d); };
   struct Cons { int64_t a = -1; std::map<int, double> r; };
   struct Slab {
     std::map<int, std::pair<int64_t, int>> prod;  // R -> (Bhat^T offset, d(Jb'))
@@ -579,6 +605,33 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
           else if (f->second.first != bo)
             throw std::runtime_error("build_mxf: one right block with two Bhat blocks in one slab");
           c.r[rt.R] += rt.r;
+        }
+      }
+    }
+    if (fold_like) {
+      // the like-particle part as two more slabs of the group (Ja, Jb) of each
+      // small block L: H_a V_L (product V_L I, consumer A = H_a) and V_L H_b^T
+      // (product V_L H_b^T, consumer A = I), each with r = 1
+      for (int L = 0; L < m.basd; ++L) {
+        auto f = blk_of.find(vb + m.posnv[L]);
+        if (f == blk_of.end()) continue;  // a large block (its like-particle tiles are k_big0's)
+        const DBlock& b = *f->second;
+        if (b.ha_off < 0 && b.hb_off < 0) continue;
+        const std::array<int, 2> g{m.bas_a[L], m.bas_b[L]};
+        lblk[g][m.bas_a[L]] = L;
+        if (b.ha_off >= 0) {
+          Slab& sl = groups[g][{-1, 0, 0}];
+          sl.prod[L] = {-(ident_bn(b.db) + 1), b.db};
+          Cons& c = sl.cons[m.bas_a[L]];
+          c.a = h_in_ad(b.ha_off, b.da);
+          c.r[L] = 1.0;
+        }
+        if (b.hb_off >= 0) {
+          Slab& sl = groups[g][{-2, 0, 0}];
+          sl.prod[L] = {-(h_in_bn(b.hb_off, b.db) + 1), b.db};
+          Cons& c = sl.cons[m.bas_a[L]];
+          c.a = ident_ad(b.da);
+          c.r[L] = 1.0;
         }
       }
     }
@@ -633,7 +686,8 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
                 MFU u{};
                 u.v_off = vb + m.posnv[pr.first] + a0;
                 u.dbr = pr.second.second;
-                u.b_off = bn(pr.second.first, u.dbr, db) + j0;
+                const int64_t bo = pr.second.first;  // < 0: already in the Bn store, at -bo - 1
+                u.b_off = (bo >= 0 ? bn(bo, u.dbr, db) : -bo - 1) + j0;
                 for (int q = 0; q < ns; ++q) {
                   auto f = sl.cons.find(slots[s0 + q].ja);
                   if (f == sl.cons.end()) continue;
@@ -674,10 +728,13 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
 }
 
 bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {
+  const size_t ad0 = p.Ad.size();
   try {
     build_mxf_impl(p, models);
   } catch (const std::exception&) {
     p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear();
+    p.Ad.resize(ad0);
+    p.mxf_like = false;
     return false;
   }
   return true;

@@ -316,6 +316,9 @@ struct PlanConfig {
   // output slots per fused task: 4 (kMfSlots), or 2 (fewer registers, more
   // products formed twice)
   int mxf_slots = kMfSlots;
+  // and the like-particle part of the small blocks folded in (two more slabs
+  // per block; W cleared first), where the plan allows it
+  int mxf_like = 1;
 };
 
 // Linear ensembles: every sample's operators are linear in its parameters, so
@@ -385,8 +388,10 @@ struct HostPlan {
   std::vector<MFU> f_mu;
   int mxf_slots = kMfSlots;      // slots the plan's tasks use (2 or 4)
   // the Bhat blocks the fused path reads, untransposed (db x dbr column-major),
-  // so that a row of lanes reads 16 consecutive elements
+  // so that a row of lanes reads 16 consecutive elements (and, with the
+  // like-particle part folded in, identities and H_b blocks)
   std::vector<double> Bn;
+  bool mxf_like = false;        // the fused path carries the like-particle part
   // large-block path (filled by build_big)
   std::vector<GTask1> g_task1;
   std::vector<GTile1> g_tile1;

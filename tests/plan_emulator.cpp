@@ -401,7 +401,7 @@ extern "C" int emu_hv_mstk(const char* nus, const char* opa, const char* opb, co
 // K steps, the four wavefronts' tiles summed; every memory index checked.
 extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, const char* intf,
                           const double* vin, double* vout, int64_t n, int steps_per_wave, int slots,
-                          int64_t* stats4) {
+                          int like, int64_t* stats4) {
   Model m;
   std::string e = build_model(nus, opa, opb, intf, m);
   if (!e.empty()) { g_err = e; return -1; }
@@ -410,6 +410,7 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
   p.cfg.stacked = 4;
   p.cfg.mxf_steps = steps_per_wave;
   p.cfg.mxf_slots = slots;
+  p.cfg.mxf_like = like;
   plan_add(p, m);
   std::vector<const Model*> ms{&m};
   if (!build_mxf(p, ms)) { g_err = "build_mxf refused the model"; return -6; }
@@ -419,6 +420,8 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
   bool oob = false;
   auto chk = [&](int64_t i, int64_t size) { if (i < 0 || i >= size) oob = true; return i >= 0 && i < size; };
   std::fill(vout, vout + n, 0.0);
+  // the like-particle part: stored first by k_reduce, or folded into the tasks
+  if (!p.mxf_like)
   for (const DBlock& b : p.blocks)
     for (int jj = 0; jj < b.nc; ++jj) {
       const int j = b.j0 + jj;
@@ -493,7 +496,7 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
     stats4[0] = (int64_t)p.f_mtask.size();
     stats4[1] = steps;
     stats4[2] = prods;
-    stats4[3] = p.mx_ks;
+    stats4[3] = p.mx_ks + (p.mxf_like ? 100 : 0);
   }
   if (oob) { g_err = "out-of-range access in the fused matrix-core path"; return -4; }
   if (!p.items.empty()) { g_err = "large-block items are not emulated by emu_hv_mxf"; return -5; }

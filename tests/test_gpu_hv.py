@@ -193,12 +193,15 @@ def test_stacked_path_matches_refcpu(models, prec, tol):
                 {"stacked": 3, "threads": 256, "stk_cap": 8, "stk_work": 4},
                 {"stacked": 3, "threads": 256, "stk_cap": 1, "stk_work": 1},
                 {"stacked": 3, "threads": 256, "stk_cap": 64, "stk_work": 32},
-                # both stages fused on the matrix cores: steps per wavefront, output slots per task
+                # both stages fused on the matrix cores (with the like-particle part): steps per
+                # wavefront, output slots per task
                 {"stacked": 4, "threads": 256, "stk_cap": 1, "stk_work": 2},
                 {"stacked": 4, "threads": 256, "stk_cap": 2, "stk_work": 4},
                 {"stacked": 4, "threads": 256, "stk_cap": 1000, "stk_work": 4}):
         print(f"case stacked {prec} {cfg}", flush=True)
         b = Batch(hams, precision=prec, config=dict(cfg, fat_cap=5120, max_phases=6, mode=0, atomic=1))
+        if cfg["stacked"] == 4:
+            assert b.path() == "fused-mfma +like"
         v = rng.standard_normal(b.N)
         din, dout = DeviceBuffer.from_array(v), DeviceBuffer(v.nbytes)
         b.multiply_dev(din, dout)

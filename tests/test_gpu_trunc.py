@@ -67,7 +67,11 @@ def _batch_hv(b, v):
 @pytest.mark.parametrize("config", [None, {"stacked": 1, "threads": 256, "fat_cap": 5120, "max_phases": 6,
                                            "mode": 0, "atomic": 1},
                                     {"stacked": 2, "threads": 256, "fat_cap": 5120, "max_phases": 6, "mode": 0,
-                                     "atomic": 1, "stk_cap": 4096, "stk_work": 150000}])
+                                     "atomic": 1, "stk_cap": 4096, "stk_work": 150000},
+                                    {"stacked": 3, "threads": 256, "fat_cap": 5120, "max_phases": 6, "mode": 0,
+                                     "atomic": 1, "stk_cap": 8, "stk_work": 4},
+                                    {"stacked": 4, "threads": 256, "fat_cap": 5120, "max_phases": 6, "mode": 0,
+                                     "atomic": 1, "stk_cap": 1, "stk_work": 4}])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
 def test_truncated_pn_input_is_masked(models, config, prec, tol):
     """A vector nonzero outside the window: the pn part sees only its window
@@ -78,6 +82,8 @@ def test_truncated_pn_input_is_masked(models, config, prec, tol):
     print(f"case trunc mask {prec} {config and config['stacked']}", flush=True)
     hams = [Hamiltonian(*models[n].args(J2), precision=prec, device=False) for n, J2 in cases]
     b = Batch(hams, precision=prec, config=config)
+    if config and config["stacked"] == 4:  # the like-particle part stays out of the fused path here
+        assert b.path() == "fused-mfma"
     v = np.random.default_rng(43).standard_normal(b.N)
     w = _batch_hv(b, v)
     for k, (n, J2) in enumerate(cases):

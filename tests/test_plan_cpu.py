@@ -85,7 +85,9 @@ def test_mxf_plan_matches_refcpu(models):
     (X_u tiles on 16 x 16 x 4 steps, Y_s = sum r X_u in the result layout fed
     back as the next product's K steps, four wavefronts per task), gives the
     oracle's H*v and stays inside every array, on blocks up to 33 (sd 6p6n:
-    several tiles per side and ten K steps), a truncated basis and an odd J."""
+    several tiles per side and ten K steps), a truncated basis and an odd J,
+    with the like-particle part stored first or folded into the tasks (two
+    more slabs per block: V I against H_a, V H_b^T against I)."""
     from tests.emulator import emu_hv_mxf
     rng = np.random.default_rng(23)
     cases = [("sd22", 0), ("sd22", 4), ("sd44", 0), ("sd44", 8), ("sd66", 4), ("sd23", 3), ("psd33t", 2)]
@@ -94,10 +96,12 @@ def test_mxf_plan_matches_refcpu(models):
         r = RefCPU(*args, fp64=True)
         v = rng.standard_normal(r.N)
         ref = r.apply(v)
-        for steps, slots in ((0, 4), (1, 4), (3, 2), (1, 2)):
-            w, st = emu_hv_mxf(args, v, steps, slots)
+        for steps, slots, like in ((0, 4, 0), (1, 4, 1), (3, 2, 1), (1, 2, 0)):
+            w, st = emu_hv_mxf(args, v, steps, slots, like)
             assert st[0] > 0 and st[1] > 0
-            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, steps, slots)
+            # folded where the plan allows it: not on the truncated basis
+            assert (st[3] >= 100) == (like == 1 and name != "psd33t"), (name, st)
+            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, steps, slots, like)
 
 
 def test_fused_plan_matches_refcpu(models):
