@@ -37,9 +37,10 @@ struct nsx_hv {
 // every sample are produced from them on the device.
 struct EnsData {
   int S = 0, P = 0;
-  int64_t K = 0, n_o = 0, b_t = 0, h_t = 0;
-  int n_fold = 0;
+  int64_t K = 0, n_o = 0, b_t = 0, h_t = 0, bn_t = 0;
+  int n_fold = 0, n_bn = 0;
   double *theta = nullptr, *basis = nullptr, *values = nullptr, *bsrc = nullptr;
+  int64_t* bn_src = nullptr;  // fused matrix-core plans: {Bh offset, Bn offset, dbr, db} per copied block
   nsx::EnsFold* fold = nullptr;
   nsx::EnsPair* pair = nullptr;
 };

@@ -101,6 +101,21 @@ __global__ __launch_bounds__(256) void k_ens_fold(const EnsFold* __restrict__ fo
   }
 }
 
+// The fused matrix-core path's untransposed copies of Bhat^T blocks, per
+// sample: Bn[s][bn + j + db k] = Bh[s][bo + k + dbr j]; one workgroup per
+// (block, sample)
+__global__ __launch_bounds__(256) void k_ens_bn(const int64_t* __restrict__ src, const double* __restrict__ Bh,
+                                                int64_t b_t, double* __restrict__ Bn, int64_t bn_t) {
+  const int64_t* q = src + 4 * (int64_t)blockIdx.x;
+  const int64_t bo = q[0], bn = q[1];
+  const int dbr = (int)q[2], db = (int)q[3];
+  const int64_t s = blockIdx.y;
+  for (int e = threadIdx.x; e < dbr * db; e += 256) {
+    const int k = e % dbr, j = e / dbr;
+    Bn[s * bn_t + bn + j + (int64_t)db * k] = Bh[s * b_t + bo + e];
+  }
+}
+
 // H[s][k] = values[s][n_o + k]
 __global__ __launch_bounds__(256) void k_ens_hcopy(const double* __restrict__ values, int64_t K, int64_t n_o,
                                                    int64_t h_t, int64_t total, double* __restrict__ H) {
@@ -156,6 +171,11 @@ int produce(nsx_batch* b) {
     hipLaunchKernelGGL(k_ens_fold, g, dim3(256), 0, s, E.fold, E.pair, E.bsrc, E.values, E.K, E.S, d.Bh, E.b_t);
     ENS_HIP(hipGetLastError());
   }
+  if (E.n_bn > 0) {
+    hipLaunchKernelGGL(k_ens_bn, dim3((unsigned)E.n_bn, (unsigned)E.S), dim3(256), 0, s, E.bn_src, d.Bh, E.b_t, d.Bn,
+                       E.bn_t);
+    ENS_HIP(hipGetLastError());
+  }
   const int64_t nh = (int64_t)E.S * E.h_t;
   if (nh > 0) {
     hipLaunchKernelGGL(k_ens_hcopy, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, E.values, E.K, E.n_o,
@@ -165,6 +185,7 @@ int produce(nsx_batch* b) {
   if (b->precision != NSX_FP64) {
     ENS_HIP(convert_to(b->precision, d.Bh, b->lowp.Bh, (int64_t)E.S * E.b_t, s));
     ENS_HIP(convert_to(b->precision, d.H, b->lowp.H, nh, s));
+    if (E.n_bn > 0) ENS_HIP(convert_to(b->precision, d.Bn, b->lowp.Bn, (int64_t)E.S * E.bn_t, s));
   }
   ENS_HIP(hipStreamSynchronize(s));
   return NSX_OK;
@@ -189,6 +210,7 @@ void free_ens(EnsData& e) {
   (void)hipFree(e.bsrc);
   (void)hipFree(e.fold);
   (void)hipFree(e.pair);
+  (void)hipFree(e.bn_src);
   e = EnsData();
 }
 }  // namespace nsx
@@ -274,10 +296,10 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   }
   HostPlan& hp = b->hplan;
   hp.cfg = cfg ? plan_config(*cfg) : PlanConfig();
-  // the stacked stages, on the VALU or on the matrix cores (the fused
-  // matrix-core path reads an untransposed copy of the Bhat store, which the
-  // per-sample fold does not make: it takes the two-stage matrix-core path)
-  hp.cfg.stacked = cfg && (cfg->stacked == 3 || cfg->stacked == 4) ? 3 : 1;
+  // the stacked stages: on the VALU, on the matrix cores, or fused on the
+  // matrix cores (whose untransposed Bhat copy k_ens_bn makes per sample; the
+  // like-particle part stays in k_reduce, the H store being per sample)
+  hp.cfg.stacked = cfg && (cfg->stacked == 3 || cfg->stacked == 4) ? cfg->stacked : 1;
   hp.cfg.deterministic = 0;
   hp.cfg.big_rows1 = precision == NSX_BF16X3 ? (hp.cfg.big_rows1_req ? hp.cfg.big_rows1_req : 2 * kBigTile) : kBigTile;
   // test hook, as for batches: a smaller block side above which the large-block tiles run
@@ -288,6 +310,7 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
     hp.ens_opcol = (int32_t)opcol[t];
     plan_add(hp, *tms[t]);
   }
+  if (hp.cfg.stacked == 4 && !build_mxf(hp, tms)) hp.cfg.stacked = 3;
   if (hp.cfg.stacked == 3)
     build_mstk(hp, tms);
   else if (hp.cfg.stacked == 1)
@@ -379,12 +402,30 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
     ENS_HIP(hipMemset(d.Bh, 0, sizeof(double) * (size_t)S * E.b_t));
   }
   if (E.h_t > 0) ENS_HIP(hipMalloc(&d.H, sizeof(double) * (size_t)S * E.h_t));
+  if (d.mxf && !hp.bn_src.empty()) {  // every sample's untransposed Bhat copy
+    E.bn_t = (int64_t)hp.Bn.size();
+    E.n_bn = (int)hp.bn_src.size();
+    (void)hipFree(d.Bn);
+    d.Bn = nullptr;
+    ENS_HIP(hipMalloc(&d.Bn, sizeof(double) * (size_t)S * E.bn_t));
+    ENS_HIP(hipMemset(d.Bn, 0, sizeof(double) * (size_t)S * E.bn_t));
+    if (precision != NSX_FP64) {
+      const size_t esz = precision == NSX_FP32 ? sizeof(float) : sizeof(Bf16x2);
+      (void)hipFree(b->lowp.Bn);
+      b->lowp.Bn = nullptr;
+      ENS_HIP(hipMalloc(&b->lowp.Bn, esz * (size_t)S * E.bn_t));
+    }
+    std::vector<int64_t> src;
+    for (const auto& q : hp.bn_src) src.insert(src.end(), q.begin(), q.end());
+    if (!(e = up(E.bn_src, src.data(), src.size())).empty()) { set_error(e); return NSX_EGPU; }
+  }
   d.ens_s = S;
   d.es.v = N_t;
   d.es.x = x_t;
   d.es.b = E.b_t;
   d.es.h = E.h_t;
   d.es.xg = xg_t;
+  d.es.bn = E.bn_t;
   d.es.n = S;
   if (!(e = up(E.theta, theta, (size_t)S * P)).empty()) { set_error(e); return NSX_EGPU; }
   if (!(e = up(E.basis, cat.data(), cat.size())).empty()) { set_error(e); return NSX_EGPU; }

@@ -1834,7 +1834,7 @@ __global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, i
     if (ti < 0) return;
     Vp += smp * es.v;
     W += smp * es.v;
-    Bt += smp * es.b;
+    Bt += smp * es.bn;
   } else {
     ti = xcd_chunk(blockIdx.x, n_tasks);
   }

@@ -9,9 +9,10 @@
 namespace nsx {
 
 // Per-sample element strides of a linear ensemble (vectors, X, Bhat^T store,
-// H store, large-block X) and its sample count n; all zero for an ordinary plan.
+// H store, large-block X, untransposed Bhat store) and its sample count n; all
+// zero for an ordinary plan.
 struct EnsStride {
-  int64_t v = 0, x = 0, b = 0, h = 0, xg = 0;
+  int64_t v = 0, x = 0, b = 0, h = 0, xg = 0, bn = 0;  // bn: the untransposed Bhat copy
   int32_t n = 0, pad = 0;
 };
 

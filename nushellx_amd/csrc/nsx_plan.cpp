@@ -527,7 +527,7 @@ void build_mstk(HostPlan& p, const std::vector<const Model*>& models) {
 }
 
 static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models) {
-  p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear();
+  p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear(); p.bn_src.clear();
   p.mx_ks = 1;
   // Bhat^T block of the Bh store -> its untransposed copy in Bn
   std::unordered_map<int64_t, int64_t> bn_of;
@@ -539,6 +539,7 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
     for (int j = 0; j < db; ++j)
       for (int k = 0; k < dbr; ++k) p.Bn[o + j + int64_t(db) * k] = p.Bh[bo + k + int64_t(dbr) * j];
     bn_of[bo] = o;
+    p.bn_src.push_back({bo, o, dbr, db});
     return o;
   };
   p.mxf_slots = p.cfg.mxf_slots == 2 ? 2 : kMfSlots;
@@ -732,7 +733,7 @@ bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {
   try {
     build_mxf_impl(p, models);
   } catch (const std::exception&) {
-    p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear();
+    p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear(); p.bn_src.clear();
     p.Ad.resize(ad0);
     p.mxf_like = false;
     return false;

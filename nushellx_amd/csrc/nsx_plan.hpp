@@ -10,6 +10,7 @@
 // V_L H_b^T is a reduce pass per block (inside the stacked stage-1 launch on
 // that path), or matrix-core tiles for large blocks.
 #pragma once
+#include <array>
 #include <cstdint>
 #include <unordered_map>
 #include <string>
@@ -392,6 +393,9 @@ struct HostPlan {
   // like-particle part folded in, identities and H_b blocks)
   std::vector<double> Bn;
   bool mxf_like = false;        // the fused path carries the like-particle part
+  // Bn blocks that are transposed copies of Bh blocks (linear ensembles make
+  // them per sample on the device): {Bh offset, Bn offset, dbr, db}
+  std::vector<std::array<int64_t, 4>> bn_src;
   // large-block path (filled by build_big)
   std::vector<GTask1> g_task1;
   std::vector<GTile1> g_tile1;

@@ -135,12 +135,12 @@ def test_linear_ensemble_with_tuned_config(ens):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 2e-4)])
-@pytest.mark.parametrize("stacked,path", [(3, "stacked-mfma"), (4, "stacked-mfma")])
+@pytest.mark.parametrize("stacked,path", [(3, "stacked-mfma"), (4, "fused-mfma")])
 def test_linear_ensemble_on_matrix_core_paths(ens, prec, tol, stacked, path):
-    """Every sample through the stacked stages on the matrix cores (grid y the
-    sample; a request for the fused matrix-core path, whose untransposed Bhat
-    copy the per-sample fold does not make, takes the two-stage one), against
-    the per-sample file batch in FP64."""
+    """Every sample through the stacked stages on the matrix cores, two-stage
+    and fused (grid y the sample; the fused path's untransposed Bhat copies made
+    per sample on the device, its like-particle part in k_reduce), against the
+    per-sample file batch in FP64, also after new parameters."""
     from nushellx_amd.hv import Batch
     ref = Batch(ens["hams"])
     cfg = {"fat_cap": 5120, "max_phases": 6, "threads": 256, "mode": 0, "atomic": 1, "stacked": stacked,
@@ -152,6 +152,12 @@ def test_linear_ensemble_on_matrix_core_paths(ens, prec, tol, stacked, path):
     for k, o in enumerate(ref.offsets):
         n = ens["hams"][k].N
         assert _rel(w[o:o + n], w_ref[o:o + n]) < tol, (prec, k)
+    ref.close()
+    # the operators re-made for other parameters of the same samples
+    lin.ens_update(np.stack([ens["base"].perturbed(s, 0.1).parameter_vector() for s in SEEDS]))
+    ref = Batch(ens["hams"])
+    w_ref, w = _hv(ref, v), _hv(lin, v)
+    assert _rel(w, w_ref) < tol
     ref.close()
     lin.close()
 
