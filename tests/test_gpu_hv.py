@@ -246,6 +246,39 @@ def test_trl_checkpoint_resume(models, tmp_path):
     assert np.abs(rb["evals"] - fb["evals"]).max() < 1e-8
 
 
+def test_lanczos_basis_stays_orthonormal(models, tmp_path):
+    """The classical Gram-Schmidt applied twice keeps the Krylov basis
+    orthonormal: after the thick restarts of a batched solve cut short, the
+    checkpointed restart basis of each instance (its
+    kept Ritz vectors and the next Lanczos vector, read from the end of the
+    checkpoint file) satisfies |Q^T Q - I| < 1e-12, and each kept vector is an
+    approximate eigenvector of the oracle's H in the Rayleigh-quotient sense
+    (Q^T H Q is diagonal up to the residual column)."""
+    from nushellx_amd.hv import Batch, Hamiltonian
+    cases = [("sd44", 0), ("sd44", 4), ("sd66", 0)]
+    hams = [Hamiltonian(*models[n].args(J2)) for n, J2 in cases]
+    b = Batch(hams)
+    ck = str(tmp_path / "orth.ckpt")
+    keep = 10
+    r = b.lanczos(nev=5, max_basis=50, keep=keep, tol=1e-12, max_iter=130, checkpoint=ck)
+    assert r["restarts"] >= 2
+    raw = np.fromfile(ck, dtype=np.uint8)
+    n, kk = b.N, keep
+    Q = raw[raw.size - 8 * n * (kk + 1):].view(np.float64).reshape(kk + 1, n).T
+    for k, (name, J2) in enumerate(cases):
+        o, ni = b.offsets[k], hams[k].N
+        Qi = Q[o:o + ni]
+        G = Qi.T @ Qi
+        err = np.abs(G - np.eye(kk + 1)).max()
+        print(f"case orth {name} J2={J2}: |QtQ - I| = {err:.2e}", flush=True)
+        assert err < 1e-12, (name, J2, err)
+        ref = RefCPU(*models[name].args(J2), fp64=True)
+        HQ = np.stack([ref.apply(Qi[:, c]) for c in range(kk)], axis=1)
+        R = Qi[:, :kk].T @ HQ
+        off = np.abs(R - np.diag(np.diag(R))).max()
+        assert off < 1e-8 * np.abs(np.diag(R)).max(), (name, J2, off)
+
+
 def test_standard_lanczos_matches_dense(models):
     """Standard Lanczos (no restarts, eigenvalue-stability test every 3 steps as
     Lnzs) reaches the dense eigenvalues; small spaces end exactly."""
