@@ -383,7 +383,8 @@ int nsx_ens_create(nsx_hv* const* tm, int32_t n_t, int32_t S, int32_t P, const d
   hp.bytes_alg *= S;
   std::string e;
   if (precision != NSX_FP64) {
-    if (!(e = build_lowp(hp, precision, b->lowp)).empty()) { set_error(e); return NSX_EGPU; }
+    // the operators are made per sample on the device: no panels
+    if (!(e = build_lowp(hp, precision, b->lowp, false)).empty()) { set_error(e); return NSX_EGPU; }
     const size_t esz = precision == NSX_FP32 ? sizeof(float) : sizeof(Bf16x2);
     (void)hipFree(b->lowp.Bh);
     (void)hipFree(b->lowp.H);

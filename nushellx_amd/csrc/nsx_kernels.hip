@@ -416,6 +416,15 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
   const int per = (n + kXcds - 1) / kXcds;
   return (b % kXcds) * per + b / kXcds;
 }
+// Runs of g consecutive tiles on one XCD, the runs dealt round-robin over the
+// XCDs: workgroup b (the (b / 8)-th on XCD b % 8) takes tile
+// ((b / 8) / g * 8 + b % 8) g + (b / 8) % g.  Neighbouring tiles that share
+// operands then share that XCD's L2, and the XCDs stay within g tiles of an
+// even load.  The grid is the tile count rounded up to 8 g (tiles beyond it exit).
+__device__ __forceinline__ int xcd_group(int b, int g) {
+  const int loc = b / kXcds;
+  return ((loc / g) * kXcds + b % kXcds) * g + loc % g;
+}
 
 // Linear ensembles launch (workgroups per sample) x (samples rounded up to 8):
 // the dispatch index L = blockIdx.y * gridDim.x + blockIdx.x runs on XCD L % 8,
@@ -1495,6 +1504,277 @@ __global__ __launch_bounds__(256) void k_big2_bf(const GTile2* __restrict__ tile
       }
 }
 
+// Operand loads through a buffer resource on a wavefront-uniform base: one
+// 32-bit byte offset per lane, and a lane outside its block passes an offset
+// beyond the resource's range (kOobOff), which returns zero without touching
+// memory.  Word 3 is the gfx9 raw-buffer format (32-bit data).
+constexpr uint32_t kOobOff = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mk_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+template <typename X>
+struct BufLd;
+template <>
+struct BufLd<double> {
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+};
+template <>
+struct BufLd<float> {
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+};
+template <>
+struct BufLd<Bf16x2> {
+  __device__ static Bf16x2 ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return Bf16x2{__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0)};
+  }
+};
+// byte offset of element idx, or kOobOff when the lane is outside its block
+template <typename X>
+__device__ __forceinline__ uint32_t boff(bool ok, int idx) {
+  return ok ? (uint32_t)idx * (uint32_t)sizeof(X) : kOobOff;
+}
+
+// ---------------------------------------------------------------- large blocks, BF16 panels
+// The same tiles with every operand read as 16-byte panel units (PanelBlk):
+// a unit is four K values of one row, hi halves then lo halves, exactly what
+// one lane of the BF16 matrix-core instruction takes for one 16-deep K step
+// (hi, then lo, four packed BF16 each).  Staging a K step of a 64-row operand
+// is one 16-byte load and one 16-byte LDS store per thread, in place of four
+// 4-byte loads and eight 2-byte stores; the tiles are bound by that staging,
+// not by the matrix cores.  Operands are read through buffer resources (a
+// lane outside its block passes kOobOff and reads zero).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// the hi (first two dwords) and lo (last two) halves of a unit as MFMA operands
+// (built from the named components: a bit cast of a swizzled subvector such as
+// u.zw reads the wrong dwords with this compiler)
+__device__ __forceinline__ bf16x4 unit_hi(u32x4 u) { return __builtin_bit_cast(bf16x4, u32x2{u.x, u.y}); }
+__device__ __forceinline__ bf16x4 unit_lo(u32x4 u) { return __builtin_bit_cast(bf16x4, u32x2{u.z, u.w}); }
+
+// One panel unit per thread, from column-major Bf16x2 blocks.
+__global__ __launch_bounds__(256) void k_panel_bf(const PanelChunk* __restrict__ chunks,
+                                                  const PanelBlk* __restrict__ blks,
+                                                  const Bf16x2* __restrict__ src, uint4* __restrict__ dst) {
+  const PanelChunk c = chunks[blockIdx.x];
+  const PanelBlk b = blks[c.blk];
+  const int nx = b.kcols ? b.rows : b.cols, K = b.kcols ? b.cols : b.rows;
+  const int u = c.u0 + (int)threadIdx.x;  // = 4 (s nx + x) + qq: neighbouring threads write neighbouring units
+  if (u >= nx * b.ldq) return;
+  const int x = (u % (4 * nx)) / 4, q = 4 * (u / (4 * nx)) + u % 4;
+  uint32_t h[4], l[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int kk = 4 * q + t;
+    const int64_t e = b.kcols ? x + (int64_t)b.rows * kk : kk + (int64_t)b.rows * x;
+    const uint32_t v = kk < K ? src[b.src + e].bits : 0u;
+    h[t] = v >> 16;
+    l[t] = v & 0xffffu;
+  }
+  dst[b.dst + u] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+}
+
+// byte offset of unit (row x, quad qq) of K step s in a panel of nx rows, or
+// kOobOff when the row is outside the block (nx <= 2048 rows, K <= 2048: a
+// panel is at most 16 MB)
+__device__ __forceinline__ uint32_t panel_off(bool ok, int nx, int s, int x, int qq) {
+  return ok ? 16u * (uint32_t)(4 * (s * nx + x) + qq) : kOobOff;
+}
+__device__ __forceinline__ u32x4 ld_unit(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// LDS tiles of panel units, [row][quad] with the quad swizzled by the row:
+// unit (row, g) at 4 row + (g ^ (((row >> 1) ^ (row >> 2)) & 3)).  A lane of
+// an MFMA reads unit (16-row block + lane % 16, lane / 16) with one 16-byte
+// read; the swizzle spreads the 16 rows of one quad over distinct banks (for
+// both 8- and 16-lane read groups), and a wavefront's staging store (16 rows
+// x 4 quads) is 1 KB of contiguous LDS.
+template <int TM = kBigTile>  // TM: rows of the column operand (Q)
+struct alignas(16) BfUTile {
+  u32x4 p[kBigTile * 4], q[TM * 4];
+};
+__device__ __forceinline__ int unit_slot(int row, int g) { return 4 * row + (g ^ (((row >> 1) ^ (row >> 2)) & 3)); }
+
+// a wavefront's 32 x (16 CB) block of D from a unit tile (as big_mma_bf)
+template <int TM, int CB = 2>
+__device__ __forceinline__ void big_mma_bfu(const BfUTile<TM>& b, int wj, int wi, int lane, f32x4 (&acc)[2][CB]) {
+  const int g = lane / 16, r16 = lane % 16;
+  bf16x4 ah[2], al[2], bh[CB], bl[CB];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const u32x4 u = b.p[unit_slot(wj + r * 16 + r16, g)];
+    ah[r] = unit_hi(u);
+    al[r] = unit_lo(u);
+  }
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const u32x4 u = b.q[unit_slot(wi + c * 16 + r16, g)];
+    bh[c] = unit_hi(u);
+    bl[c] = unit_lo(u);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[r], bh[c], acc[r][c], 0, 0, 0);
+      acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[r], bl[c], acc[r][c], 0, 0, 0);
+      acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[r], bh[c], acc[r][c], 0, 0, 0);
+    }
+}
+
+// G1 on panels: X = V_R Bhat^T for one TM x 64 tile.  Staging map: unit u of
+// a K step is (row u / 4, quad u % 4); the V_R tile has TM rows (TM / 64
+// units per thread), the Bhat^T tile 64 (one unit per thread).
+template <int TM>
+__global__ __launch_bounds__(256) void k_big1_bfp(const GTile1* __restrict__ tiles, const GTask1* __restrict__ tasks,
+                                                  const uint4* __restrict__ Vt, const uint4* __restrict__ Bp,
+                                                  float* __restrict__ X, int n_tiles, int grp) {
+  constexpr int CB = TM / 32, QV = TM / 64;
+  __shared__ BfUTile<TM> sb[2];  // double-buffered: one barrier per K step
+  const int ti = grp > 0 ? xcd_group((int)blockIdx.x, grp) : (int)blockIdx.x;
+  if (ti >= n_tiles) return;
+  const GTile1 t = tiles[ti];
+  const GTask1 g = tasks[t.task];
+  const int m = g.m, n = g.n, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * (TM / 2);
+  f32x4 acc[2][CB];
+  big_zero<float, CB>(acc);
+  const __amdgpu_buffer_rsrc_t rvs = mk_rsrc(Vt + g.v_off), rbs = mk_rsrc(Bp + g.b_off);
+  u32x4 rv[QV], rb;
+  auto load = [&](int s) {
+#pragma unroll
+    for (int q = 0; q < QV; ++q) {
+      const int u = tid + 256 * q, i = i0 + u / 4;
+      rv[q] = ld_unit(rvs, panel_off(i < m, m, s, i, u % 4));
+    }
+    const int j = j0 + tid / 4;
+    rb = ld_unit(rbs, panel_off(j < n, n, s, j, tid % 4));
+  };
+  auto stage = [&](BfUTile<TM>& b) {
+#pragma unroll
+    for (int q = 0; q < QV; ++q) {
+      const int u = tid + 256 * q;
+      b.q[unit_slot(u / 4, u % 4)] = rv[q];
+    }
+    b.p[unit_slot(tid / 4, tid % 4)] = rb;
+  };
+  uint64_t m0 = t.kmask[0], m1 = t.kmask[1];
+  int step = pop_step(m0, m1), cur = 0;
+  if (step >= 0) {
+    load(step);
+    stage(sb[0]);
+    __syncthreads();
+  }
+  while (step >= 0) {
+    step = pop_step(m0, m1);
+    if (step >= 0) load(step);
+    big_mma_bfu<TM, CB>(sb[cur], wj, wi, lane, acc);
+    if (step >= 0) stage(sb[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + 4 * (lane / 16) + v;
+        if (i < m && j < n) X[g.x_off + i + (int64_t)big_ldx(m) * j] = acc[r][c][v];
+      }
+}
+
+// G2 on panels: W_L += sum_k A_k Y_k.  The A tile is one panel unit per
+// thread; Y = sum_u r_u X_u is formed as four consecutive K values of one
+// column (one 16-byte load per X reference: X columns are padded with zeros
+// to whole K steps and 64-byte aligned) and split into hi/lo units.
+__global__ __launch_bounds__(256) void k_big2_bfp(const GTile2* __restrict__ tiles, const GTerm2* __restrict__ terms,
+                                                  const SXRef* __restrict__ xrefs, const uint4* __restrict__ Ap,
+                                                  const float* __restrict__ X, double* __restrict__ W, int n_tiles,
+                                                  int grp) {
+  __shared__ BfUTile<> sb[2];
+  const int ti = grp > 0 ? xcd_group((int)blockIdx.x, grp) : (int)blockIdx.x;
+  if (ti >= n_tiles) return;
+  const GTile2 t = tiles[ti];
+  const int da = t.da, db = t.db, i0 = t.i0, j0 = t.j0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = (wave & 1) * 32, wi = (wave >> 1) * 32;
+  const int ro = tid / 4, qo = tid % 4;  // staging map: row (A: i, Y: j) and K quad
+  f32x4 acc[2][2];
+  big_zero<float>(acc);
+  u32x4 ra;
+  float4 ry;
+  auto load = [&](const GTerm2& tm, int s) {
+    const int i = i0 + ro, j = j0 + ro;
+    ra = ld_unit(mk_rsrc(Ap + tm.a_off), panel_off(i < da, da, s, i, qo));
+    ry = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j < db) {
+      const int64_t col = kBigK * s + 4 * qo + (int64_t)big_ldx(tm.dar) * j;
+      for (int x = tm.x_begin; x < tm.x_end; ++x) {
+        const SXRef xr = xrefs[x];
+        const float r = (float)xr.r;
+        const float4 xv = *reinterpret_cast<const float4*>(X + xr.x_off + col);
+        ry.x = fmaf(r, xv.x, ry.x);
+        ry.y = fmaf(r, xv.y, ry.y);
+        ry.z = fmaf(r, xv.z, ry.z);
+        ry.w = fmaf(r, xv.w, ry.w);
+      }
+    }
+  };
+  auto stage = [&](BfUTile<>& b) {
+    const int o = unit_slot(ro, qo);
+    b.q[o] = ra;
+    const uint32_t s0 = bf16x2_split(ry.x).bits, s1 = bf16x2_split(ry.y).bits;
+    const uint32_t s2 = bf16x2_split(ry.z).bits, s3 = bf16x2_split(ry.w).bits;
+    b.p[o] = u32x4{(s0 >> 16) | (s1 & 0xffff0000u), (s2 >> 16) | (s3 & 0xffff0000u), (s0 & 0xffffu) | (s1 << 16),
+                   (s2 & 0xffffu) | (s3 << 16)};
+  };
+  int tc = t.t_begin;
+  GTerm2 tm = terms[tc];
+  uint64_t m0 = tm.kmask[0], m1 = tm.kmask[1];
+  int step = pop_step(m0, m1);
+  while (step < 0 && ++tc < t.t_end) {
+    tm = terms[tc];
+    m0 = tm.kmask[0];
+    m1 = tm.kmask[1];
+    step = pop_step(m0, m1);
+  }
+  int cur = 0;
+  if (step >= 0) {
+    load(tm, step);
+    stage(sb[0]);
+    __syncthreads();
+  }
+  while (step >= 0) {
+    step = pop_step(m0, m1);
+    while (step < 0 && ++tc < t.t_end) {
+      tm = terms[tc];
+      m0 = tm.kmask[0];
+      m1 = tm.kmask[1];
+      step = pop_step(m0, m1);
+    }
+    if (step >= 0) load(tm, step);
+    big_mma_bfu<kBigTile>(sb[cur], wj, wi, lane, acc);
+    if (step >= 0) stage(sb[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wi + c * 16 + lane % 16, j = j0 + wj + r * 16 + 4 * (lane / 16) + v;
+        if (i < da && j < db) unsafeAtomicAdd(W + t.w_off + i + (int64_t)da * j, (double)acc[r][c][v]);
+      }
+}
+
 // G0 in BF16 hi/lo: W_L = H_a V_L + V_L H_b^T for one 64 x 64 tile, stored
 __global__ __launch_bounds__(256) void k_big0_bf(const GTile3* __restrict__ tiles, const Bf16x2* __restrict__ V,
                                                  const Bf16x2* __restrict__ H, double* __restrict__ W, EnsStride es) {
@@ -1784,40 +2064,6 @@ __global__ __launch_bounds__(256) void k_mx2(const MTask2* __restrict__ tasks, i
 // those rows).  X and Y never leave the registers; the slots' tiles are summed
 // over the four wavefronts in LDS and added into W with FP64 atomics (the
 // like-particle part is stored before, by k_reduce).
-// Operand loads through a buffer resource on a wavefront-uniform base: one
-// 32-bit byte offset per lane, and a lane outside its block passes an offset
-// beyond the resource's range (kOobOff), which returns zero without touching
-// memory.  Word 3 is the gfx9 raw-buffer format (32-bit data).
-constexpr uint32_t kOobOff = 0x80000000u;
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t mk_rsrc(const void* base) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
-}
-template <typename X>
-struct BufLd;
-template <>
-struct BufLd<double> {
-  __device__ static double ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-  }
-};
-template <>
-struct BufLd<float> {
-  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-  }
-};
-template <>
-struct BufLd<Bf16x2> {
-  __device__ static Bf16x2 ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return Bf16x2{__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0)};
-  }
-};
-// byte offset of element idx, or kOobOff when the lane is outside its block
-template <typename X>
-__device__ __forceinline__ uint32_t boff(bool ok, int idx) {
-  return ok ? (uint32_t)idx * (uint32_t)sizeof(X) : kOobOff;
-}
-
 template <typename T, int KS, int SL>
 __global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, int n_tasks,
                                              const MFStep* __restrict__ steps, const MFU* __restrict__ us,
@@ -1969,7 +2215,7 @@ static bool big_like_active(const DevicePlan& d) { return d.atomic && d.n_gtile3
 // those blocks); Vp: the pn input (V masked on truncated plans)
 template <typename T>
 static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, const T* Ad, const T* Bh, const T* H,
-                       double* W, hipStream_t s, const T* Bnp = nullptr) {
+                       double* W, hipStream_t s, const T* Bnp = nullptr, const LowpPlan* lp = nullptr) {
   const bool at = d.atomic != 0;
   if (d.fused) {  // column partials of the small blocks (summed by the reduce)
     if (d.n_ftask > 0) {
@@ -2072,6 +2318,22 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
     using C = typename BigIn<T>::C;
     C* XG = static_cast<C*>(d.XG);
     if constexpr (std::is_same<T, Bf16x2>::value) {  // BF16 pairs on the BF16 matrix cores
+      if (lp && lp->Bp && d.ens_s == 1) {  // operands as panels
+        const int g1 = lp->xcd_grp1, g2 = lp->xcd_grp2;
+        auto grid = [](int n, int g) { return (unsigned)(g > 0 ? (n + kXcds * g - 1) / (kXcds * g) * (kXcds * g) : n); };
+        if (d.n_gtile1 > 0) {
+          const dim3 gr(grid(d.n_gtile1, g1));
+          if (d.big_rows1 == 2 * kBigTile)
+            hipLaunchKernelGGL(k_big1_bfp<2 * kBigTile>, gr, dim3(256), 0, s, d.g_tile1, lp->g_task1p, lp->Vt,
+                               lp->Bp, XG, d.n_gtile1, g1);
+          else
+            hipLaunchKernelGGL(k_big1_bfp<kBigTile>, gr, dim3(256), 0, s, d.g_tile1, lp->g_task1p, lp->Vt, lp->Bp,
+                               XG, d.n_gtile1, g1);
+        }
+        hipLaunchKernelGGL(k_big2_bfp, dim3(grid(d.n_gtile2, g2)), dim3(256), 0, s, d.g_tile2, lp->g_term2p,
+                           d.g_xref, lp->Ap, (const C*)XG, W, d.n_gtile2, g2);
+        return hipGetLastError();
+      }
       if (d.n_gtile1 > 0) {
         if (d.big_rows1 == 2 * kBigTile)
           hipLaunchKernelGGL(k_big1_bf<2 * kBigTile>, dim3((unsigned)d.n_gtile1, (unsigned)d.ens_s), dim3(256), 0, s,
@@ -2143,7 +2405,7 @@ hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const do
                          (const float*)lp.H, W, s, (const float*)lp.Bn);
     default:
       return pn_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)Vp, (const Bf16x2*)lp.Ad,
-                          (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s, (const Bf16x2*)lp.Bn);
+                          (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s, (const Bf16x2*)lp.Bn, &lp);
   }
 }
 
@@ -2154,6 +2416,13 @@ hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, cons
     case 1: return reduce_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.H, W, s);
     default: return reduce_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s);
   }
+}
+
+hipError_t launch_panels(const PanelChunk* chunks, int n_chunks, const PanelBlk* blks, const void* src, uint4* dst,
+                         hipStream_t s) {
+  if (n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_panel_bf, dim3((unsigned)n_chunks), dim3(256), 0, s, chunks, blks, (const Bf16x2*)src, dst);
+  return hipGetLastError();
 }
 
 hipError_t convert_to(int prec, const double* x, void* y, int64_t n, hipStream_t s) {
@@ -2182,6 +2451,9 @@ hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, con
     hipLaunchKernelGGL(k_convert<float>, g, dim3(256), 0, s, V, (float*)lp.vin, d.N);
   else
     hipLaunchKernelGGL(k_convert<Bf16x2>, g, dim3(256), 0, s, V, (Bf16x2*)lp.vin, d.N);
+  if (prec == 2 && lp.n_vt_chunk > 0)  // the large-block tiles' V_R panels, from the pn input
+    hipLaunchKernelGGL(k_panel_bf, dim3((unsigned)lp.n_vt_chunk), dim3(256), 0, s, lp.vt_chunk, lp.vt_blk,
+                       (const Bf16x2*)(d.vmask ? lp.vin_m : lp.vin), lp.Vt);
   return hipGetLastError();
 }
 
@@ -2192,7 +2464,7 @@ hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, con
 // large-block terms
 template <typename T>
 static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T* Vp, const T* Ad, const T* Bh,
-                             const T* H, double* W, hipStream_t s) {
+                             const T* H, double* W, hipStream_t s, const LowpPlan* lp) {
   const bool big = big_like_active(d);
   if (big) {
     if constexpr (std::is_same<T, Bf16x2>::value)
@@ -2213,7 +2485,7 @@ static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T*
   if (d.n_gtile2 > 0) {
     DevicePlan lg = d;    // the large-block tiles alone
     lg.n_ftask = 0;
-    return pn_T<T>(lg, prec, V, Vp, Ad, Bh, H, W, s);
+    return pn_T<T>(lg, prec, V, Vp, Ad, Bh, H, W, s, nullptr, lp);
   }
   return hipGetLastError();
 }
@@ -2225,13 +2497,13 @@ hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const do
   if (d.fused) {
     const void* Vp = pn_input(d, lp, prec, V);
     switch (prec) {
-      case 0: return fused_hv_T<double>(d, 0, V, (const double*)Vp, d.Ad, d.Bh, d.H, W, s);
+      case 0: return fused_hv_T<double>(d, 0, V, (const double*)Vp, d.Ad, d.Bh, d.H, W, s, &lp);
       case 1:
         return fused_hv_T<float>(d, 1, (const float*)lp.vin, (const float*)Vp, (const float*)lp.Ad,
-                                 (const float*)lp.Bh, (const float*)lp.H, W, s);
+                                 (const float*)lp.Bh, (const float*)lp.H, W, s, &lp);
       default:
         return fused_hv_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)Vp, (const Bf16x2*)lp.Ad,
-                                  (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s);
+                                  (const Bf16x2*)lp.Bh, (const Bf16x2*)lp.H, W, s, &lp);
     }
   }
   if (d.atomic) {

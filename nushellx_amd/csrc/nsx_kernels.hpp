@@ -123,6 +123,23 @@ __host__ __device__ inline Bf16x2 bf16x2_split(float x) {
   return Bf16x2{(hi << 16) | bf16_rne(rest)};
 }
 
+// BF16 panels of the large-block tiles: a matrix of nx rows and inner (K)
+// dimension K in 16-byte units of four K values, the hi halves of the four
+// pairs first (two dwords), then the lo halves, stored step-major: unit
+// (row x, 16-deep step s, quad qq) at dst + 4 (s nx + x) + qq holds K values
+// 16s + 4qq .. 16s + 4qq + 3 of row x (K padded with zeros to whole steps;
+// ldq = 4 ceil(K / 16) units per row).  A K step of 64 consecutive rows is
+// 4 KB of contiguous memory.  A source block is column-major (rows x cols);
+// kcols = 1: K runs along its columns (x = row), kcols = 0: K runs along its
+// rows (x = column).
+struct PanelBlk {
+  int64_t src, dst;            // element offset in the Bf16x2 store, unit offset in the panel store
+  int32_t rows, cols, kcols, ldq;
+};
+struct PanelChunk {            // 256 consecutive units of one block
+  int32_t blk, u0;
+};
+
 // Reduced-precision copies of the operators (FP32 or BF16 hi/lo) and the
 // converted input vector; structure arrays are shared with the FP64 plan.
 struct LowpPlan {
@@ -133,14 +150,32 @@ struct LowpPlan {
   void* vin = nullptr;
   void* vin_m = nullptr;  // the masked input of the pn part (truncated plans)
   void* Bn = nullptr;     // the fused path's untransposed Bhat blocks
+  // BF16 plans with large blocks: the tiles' operands as panels.  Bp (Bhat^T
+  // per stage-1 task, K = its rows) and Ap (A per stage-2 term, K = its
+  // columns) are made once; Vt (V_R per stage-1 task, K = its columns) is
+  // made from the pn input by every H*v.  g_task1p / g_term2p are the plan's
+  // records with v_off / b_off / a_off as unit offsets into these stores.
+  uint4 *Vt = nullptr, *Bp = nullptr, *Ap = nullptr;
+  GTask1* g_task1p = nullptr;
+  GTerm2* g_term2p = nullptr;
+  PanelBlk* vt_blk = nullptr;
+  PanelChunk* vt_chunk = nullptr;
+  int n_vt_chunk = 0;
+  // the panel tiles' XCD grouping (xcd_group; 0: the hardware's round-robin order)
+  int xcd_grp1 = 16, xcd_grp2 = 16;
 };
-std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp);
+// panels = false (linear ensembles, whose operators are made per sample on
+// the device) keeps the large-block tiles on the Bf16x2 stores
+std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp, bool panels = true);
 void free_lowp(LowpPlan& lp);
 
 // prec: 0 FP64, 1 FP32, 2 BF16 hi/lo.  launch_hv = convert + (pn, reduce) in mode order.
 hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s);
 hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s);
+// panel units of Bf16x2 blocks (256 units per chunk)
+hipError_t launch_panels(const PanelChunk* chunks, int n_chunks, const PanelBlk* blks, const void* src, uint4* dst,
+                         hipStream_t s);
 // n FP64 values -> FP32 or BF16 hi/lo (prec 1, 2), as the plan's operands are stored
 hipError_t convert_to(int prec, const double* x, void* y, int64_t n, hipStream_t s);
 hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
