@@ -2064,8 +2064,15 @@ __global__ __launch_bounds__(256) void k_mx2(const MTask2* __restrict__ tasks, i
 // those rows).  X and Y never leave the registers; the slots' tiles are summed
 // over the four wavefronts in LDS and added into W with FP64 atomics (the
 // like-particle part is stored before, by k_reduce).
+// Occupancy: the tiles are bound by the latency of dependent operand loads,
+// so with up to 6 K steps per product the register budget is capped for 4
+// wavefronts per SIMD with 4 slots (6 dwords spilled; the compiler's own
+// choice gives 3: sd 4p4n FP64 H*v 30.6 -> 27.5 us) and 5 with 2 (none).
+// With 10 K steps the caps stay where the registers fit without spills.
+template <int KS, int SL>
+constexpr int mxf_waves() { return KS <= 6 ? (SL == 2 ? 5 : 4) : (SL == 2 ? 4 : 3); }
 template <typename T, int KS, int SL>
-__global__ __launch_bounds__(256) void k_mxf(const MFTask* __restrict__ tasks, int n_tasks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<KS, SL>(), 8))) void k_mxf(const MFTask* __restrict__ tasks, int n_tasks,
                                              const MFStep* __restrict__ steps, const MFU* __restrict__ us,
                                              const T* __restrict__ Vp, const T* __restrict__ Ad,
                                              const T* __restrict__ Bt, double* __restrict__ W, EnsStride es) {
