@@ -1514,22 +1514,24 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t mk_rsrc(const void* base) {
 }
 template <typename X>
 struct BufLd;
+// soff: a wavefront-uniform byte offset added in the scalar offset field (a
+// lane passing kOobOff stays out of range whatever soff is)
 template <>
 struct BufLd<double> {
-  __device__ static double ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
   }
 };
 template <>
 struct BufLd<float> {
-  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
   }
 };
 template <>
 struct BufLd<Bf16x2> {
-  __device__ static Bf16x2 ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return Bf16x2{__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0)};
+  __device__ static Bf16x2 ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
+    return Bf16x2{__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0)};
   }
 };
 // byte offset of element idx, or kOobOff when the lane is outside its block
@@ -2106,6 +2108,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<K
   // K steps of the W products that can hold rows of the Ja' tile: in the FP64
   // result layout register q holds rows 4q..4q+3, in the FP32 one rows q, q+4, ..
   const int nka = sizeof(C) == 8 ? (t.rows_a + 3) >> 2 : min(4, t.rows_a);
+  // FP64 A operands: K step ks reads row kp = kq + 4 ks, so a lane's offsets
+  // are one base per slot plus the uniform 4 da ks rows (scalar offset); only
+  // the last step has rows beyond rows_a (lanes kq >= rows_a - 4 (nka - 1)).
+  // Two registers per slot instead of one per slot and step.
+  uint32_t abase[SL], alast[SL];
+  if constexpr (sizeof(C) == 8) {
+    const bool lok = kq < t.rows_a - 4 * (nka - 1);
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+      abase[q] = boff<T>(l16 < t.rows_i[q], l16 + t.da[q] * kq);
+      alast[q] = lok ? abase[q] : kOobOff;
+    }
+  }
   mx_dispatch<1, 4>(nka, [&](auto nka_c) {
   constexpr int NKA = decltype(nka_c)::value;
   for (int st = st_begin; st < st_end; ++st) {
@@ -2143,12 +2158,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<K
     for (int q = 0; q < SL; ++q) {
       if (S.a_off[q] < 0) continue;  // slot not in this slab (uniform)
       const __amdgpu_buffer_rsrc_t ra = mk_rsrc(Ad + S.a_off[q]);
-      const bool iok = l16 < t.rows_i[q];
       C av[NKA];
+      if constexpr (sizeof(C) == 8) {
 #pragma unroll
-      for (int ks = 0; ks < NKA; ++ks) {
-        const int kp = M::row(lane, ks);
-        av[ks] = BigIn<T>::get(BufLd<T>::ld(ra, boff<T>(kp < t.rows_a && iok, l16 + t.da[q] * kp)));
+        for (int ks = 0; ks < NKA; ++ks)
+          av[ks] = BigIn<T>::get(BufLd<T>::ld(ra, ks + 1 < NKA ? abase[q] : alast[q],
+                                              (uint32_t)(4 * ks * t.da[q]) * (uint32_t)sizeof(T)));
+      } else {
+        const bool iok = l16 < t.rows_i[q];
+#pragma unroll
+        for (int ks = 0; ks < NKA; ++ks) {
+          const int kp = M::row(lane, ks);
+          av[ks] = BigIn<T>::get(BufLd<T>::ld(ra, boff<T>(kp < t.rows_a && iok, l16 + t.da[q] * kp)));
+        }
       }
 #pragma unroll
       for (int ks = 0; ks < NKA; ++ks) w[q] = M::mma(y[q][ks], av[ks], w[q]);
