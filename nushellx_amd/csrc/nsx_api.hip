@@ -89,6 +89,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
     if (!(e = up(d.f_mu, hp.f_mu)).empty()) return e;
     if (!(e = up(d.Bn, hp.Bn)).empty()) return e;
     d.n_mftask = (int)hp.f_mtask.size();
+    for (int x = 0; x < 9; ++x) d.mxf_xcd.s[x] = hp.mxf_xcd[size_t(x)];
     d.mx_ks = hp.mx_ks;
     d.mxf_slots = hp.mxf_slots;
     d.mxf_like = hp.mxf_like ? 1 : 0;

@@ -2077,7 +2077,8 @@ template <typename T, int KS, int SL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<KS, SL>(), 8))) void k_mxf(const MFTask* __restrict__ tasks, int n_tasks,
                                              const MFStep* __restrict__ steps, const MFU* __restrict__ us,
                                              const T* __restrict__ Vp, const T* __restrict__ Ad,
-                                             const T* __restrict__ Bt, double* __restrict__ W, EnsStride es) {
+                                             const T* __restrict__ Bt, double* __restrict__ W, EnsStride es,
+                                             XcdSplit xs) {
   using C = typename BigIn<T>::C;
   using M = BigMma<C>;
   using vec = typename M::vec;
@@ -2090,10 +2091,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<K
     Vp += smp * es.v;
     W += smp * es.v;
     Bt += smp * es.bn;
-  } else {
-    ti = xcd_chunk(blockIdx.x, n_tasks);
+    if (ti >= n_tasks) return;  // whole workgroups
+  } else {  // workgroup b runs on XCD b % 8: the (b / 8)-th task of that XCD's range
+    const int x = (int)(blockIdx.x % kXcds);
+    ti = xs.s[x] + (int)(blockIdx.x / kXcds);
+    if (ti >= xs.s[x + 1]) return;
   }
-  if (ti >= n_tasks) return;  // whole workgroups
   const MFTask t = tasks[ti];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
@@ -2290,11 +2293,14 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
     if (!at) return hipErrorInvalidConfiguration;
     const T* Bn = Bnp;
     if (d.n_mftask > 0) {
-      const unsigned g = (unsigned)(((d.n_mftask + kXcds - 1) / kXcds) * kXcds);
+      int most = 0;  // the largest XCD range sets the grid (ensembles: tasks per sample)
+      for (int x = 0; x < kXcds; ++x) most = std::max(most, d.mxf_xcd.s[x + 1] - d.mxf_xcd.s[x]);
+      const unsigned g = d.ens_s > 1 ? (unsigned)(((d.n_mftask + kXcds - 1) / kXcds) * kXcds)
+                                     : (unsigned)(kXcds * most);
       const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
 #define NSX_MXF(KS, SL)                                                                                   \
   hipLaunchKernelGGL((k_mxf<T, KS, SL>), dim3(g, gy), dim3(256), 0, s, d.f_mtask, d.n_mftask, d.f_mstep, d.f_mu, \
-                     Vp, Ad, Bn, W, d.es)
+                     Vp, Ad, Bn, W, d.es, d.mxf_xcd)
       if (d.mx_ks <= 6) {
         if (d.mxf_slots == 2) NSX_MXF(6, 2); else NSX_MXF(6, kMfSlots);
       } else {

@@ -16,6 +16,11 @@ struct EnsStride {
   int32_t n = 0, pad = 0;
 };
 
+// per-XCD ranges of a task list, passed to a kernel by value
+struct XcdSplit {
+  int32_t s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
 struct DevicePlan {
   DItem2* items2 = nullptr;
   DPhase* phases = nullptr;
@@ -59,6 +64,7 @@ struct DevicePlan {
   MFStep* f_mstep = nullptr;
   MFU* f_mu = nullptr;
   int n_mftask = 0;
+  XcdSplit mxf_xcd;          // the tasks' XCD ranges (HostPlan::mxf_xcd)
   // large-block GEMM path (FP64, atomic mode)
   GTask1* g_task1 = nullptr;
   GTile1* g_tile1 = nullptr;

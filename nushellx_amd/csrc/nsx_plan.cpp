@@ -726,6 +726,39 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
           }
     }
   }
+  // XCD ranges: a task's time is about its longest wavefront's chain of
+  // dependent loads (one round per step for A, one per product); contiguous
+  // ranges keep neighbouring tasks (shared operators) in one XCD's L2, cut at
+  // equal shares of the summed chains, and the longest tasks of a range go
+  // first, so that the range does not end on one long task
+  const int nt = (int)p.f_mtask.size();
+  std::vector<double> crit(size_t(nt), 0.0);
+  double total = 0.0;
+  for (int i = 0; i < nt; ++i) {
+    const MFTask& t = p.f_mtask[size_t(i)];
+    for (int w = 0; w < 4; ++w) {
+      double c = 0.0;
+      for (int st = t.p[w]; st < t.p[w + 1]; ++st) c += 1.0 + (p.f_mstep[size_t(st)].u_end - p.f_mstep[size_t(st)].u_begin);
+      crit[size_t(i)] = std::max(crit[size_t(i)], c);
+    }
+    total += crit[size_t(i)];
+  }
+  p.mxf_xcd[0] = 0;
+  double acc = 0.0;
+  for (int i = 0, x = 1; x < 8; ++x) {
+    while (i < nt && acc + 0.5 * crit[size_t(i)] < total * x / 8.0) acc += crit[size_t(i++)];
+    p.mxf_xcd[size_t(x)] = i;
+  }
+  p.mxf_xcd[8] = nt;
+  std::vector<MFTask> sorted;
+  sorted.reserve(size_t(nt));
+  for (int x = 0; x < 8; ++x) {
+    std::vector<int> ord;
+    for (int i = p.mxf_xcd[size_t(x)]; i < p.mxf_xcd[size_t(x) + 1]; ++i) ord.push_back(i);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return crit[size_t(a)] > crit[size_t(b)]; });
+    for (int i : ord) sorted.push_back(p.f_mtask[size_t(i)]);
+  }
+  p.f_mtask.swap(sorted);
 }
 
 bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {

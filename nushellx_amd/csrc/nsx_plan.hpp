@@ -393,6 +393,9 @@ struct HostPlan {
   // like-particle part folded in, identities and H_b blocks)
   std::vector<double> Bn;
   bool mxf_like = false;        // the fused path carries the like-particle part
+  // f_mtask in 8 contiguous ranges of about equal cost, one per XCD (range x
+  // = [mxf_xcd[x], mxf_xcd[x + 1])), each with its longest tasks first
+  std::array<int32_t, 9> mxf_xcd{};
   // Bn blocks that are transposed copies of Bh blocks (linear ensembles make
   // them per sample on the device): {Bh offset, Bn offset, dbr, db}
   std::vector<std::array<int64_t, 4>> bn_src;
