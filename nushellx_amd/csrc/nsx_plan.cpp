@@ -726,11 +726,12 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
           }
     }
   }
-  // XCD ranges: a task's time is about its longest wavefront's chain of
-  // dependent loads (one round per step for A, one per product); contiguous
-  // ranges keep neighbouring tasks (shared operators) in one XCD's L2, cut at
-  // equal shares of the summed chains, and the longest tasks of a range go
-  // first, so that the range does not end on one long task
+  // XCD ranges: a task's time is taken as its longest wavefront's cost (the
+  // cost of the step split above: K steps + 1 per product, 4 per slot's A;
+  // counting dependent load rounds instead balanced worse); contiguous ranges
+  // keep neighbouring tasks (shared operators) in one XCD's L2, cut at equal
+  // shares of the summed costs, and the longest tasks of a range go first, so
+  // that the range does not end on one long task
   const int nt = (int)p.f_mtask.size();
   std::vector<double> crit(size_t(nt), 0.0);
   double total = 0.0;
@@ -738,7 +739,11 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
     const MFTask& t = p.f_mtask[size_t(i)];
     for (int w = 0; w < 4; ++w) {
       double c = 0.0;
-      for (int st = t.p[w]; st < t.p[w + 1]; ++st) c += 1.0 + (p.f_mstep[size_t(st)].u_end - p.f_mstep[size_t(st)].u_begin);
+      for (int st = t.p[w]; st < t.p[w + 1]; ++st) {
+        const MFStep& S = p.f_mstep[size_t(st)];
+        for (int u = S.u_begin; u < S.u_end; ++u) c += (p.f_mu[size_t(u)].dbr + 3) / 4 + 1;
+        for (int q = 0; q < kMfSlots; ++q) c += S.a_off[q] >= 0 ? 4.0 : 0.0;
+      }
       crit[size_t(i)] = std::max(crit[size_t(i)], c);
     }
     total += crit[size_t(i)];
