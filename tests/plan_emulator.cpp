@@ -2,6 +2,7 @@
 // plain loops, so the plan construction (block lists, phase runs, Bhat^T
 // layout, partial bookkeeping) is checked against the oracle without a GPU.
 // It mirrors the arithmetic of the HIP kernels, not their scheduling.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -435,7 +436,21 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
       }
     }
   int64_t steps = 0, prods = 0;
-  for (const MFTask& t : p.f_mtask) {
+  // the kernel's workgroup -> task map: workgroup b on XCD b % 8 takes the
+  // (b / 8)-th task of that XCD's range; the ranges must partition the tasks
+  const int ntask = (int)p.f_mtask.size();
+  int most = 0;
+  for (int x = 0; x < 8; ++x) {
+    if (p.mxf_xcd[x + 1] < p.mxf_xcd[x]) { g_err = "XCD ranges out of order"; return -7; }
+    most = std::max(most, p.mxf_xcd[x + 1] - p.mxf_xcd[x]);
+  }
+  if (p.mxf_xcd[0] != 0 || p.mxf_xcd[8] != ntask) { g_err = "XCD ranges do not cover the tasks"; return -7; }
+  std::vector<int> visits(size_t(ntask), 0);
+  for (int wg = 0; wg < 8 * most; ++wg) {
+    const int x = wg % 8, ti = p.mxf_xcd[x] + wg / 8;
+    if (ti >= p.mxf_xcd[x + 1]) continue;
+    ++visits[size_t(ti)];
+    const MFTask& t = p.f_mtask[size_t(ti)];
     double Wt[4][kMfSlots][64][4] = {};
     for (int wave = 0; wave < 4; ++wave)
       for (int st = t.p[wave]; st < t.p[wave + 1]; ++st) {
@@ -498,6 +513,8 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
     stats4[2] = prods;
     stats4[3] = p.mx_ks + (p.mxf_like ? 100 : 0);
   }
+  for (int v : visits)
+    if (v != 1) { g_err = "a task not run exactly once by the workgroup map"; return -7; }
   if (oob) { g_err = "out-of-range access in the fused matrix-core path"; return -4; }
   if (!p.items.empty()) { g_err = "large-block items are not emulated by emu_hv_mxf"; return -5; }
   return 0;

@@ -87,7 +87,9 @@ def test_mxf_plan_matches_refcpu(models):
     oracle's H*v and stays inside every array, on blocks up to 33 (sd 6p6n:
     several tiles per side and ten K steps), a truncated basis and an odd J,
     with the like-particle part stored first or folded into the tasks (two
-    more slabs per block: V I against H_a, V H_b^T against I)."""
+    more slabs per block: V I against H_a, V H_b^T against I).  The tasks are
+    reached through the kernel's workgroup map (workgroup b: the (b / 8)-th
+    task of XCD b % 8's range), which must run every task exactly once."""
     from tests.emulator import emu_hv_mxf
     rng = np.random.default_rng(23)
     cases = [("sd22", 0), ("sd22", 4), ("sd44", 0), ("sd44", 8), ("sd66", 4), ("sd23", 3), ("psd33t", 2)]
