@@ -2399,7 +2399,8 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
 }
 
 template <typename T>
-static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H, double* W, hipStream_t s) {
+static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H, double* W, hipStream_t s,
+                           bool w_zero) {
   const bool big = big_like_active(d);
   if (d.fused) {  // W = column partials + like-particle part, stored (large blocks: k_big0)
     if (d.n_fblocks > 0)
@@ -2408,7 +2409,7 @@ static hipError_t reduce_T(const DevicePlan& d, int prec, const T* V, const T* H
   }
   // the stacked stage-1 launch carries the small like-particle blocks
   const bool like_in_pn = d.mxf && d.mxf_like;  // the fused matrix-core path adds the like-particle part
-  if (like_in_pn && d.N > 0)
+  if (like_in_pn && d.N > 0 && !w_zero)
     hipLaunchKernelGGL(k_zero_w, dim3((unsigned)((d.N + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, W, d.N);
   if (!d.stacked && !d.mstk && !d.fused && !like_in_pn && d.n_blocks > 0)
     hipLaunchKernelGGL(k_reduce<T>, dim3(d.n_blocks, (unsigned)d.ens_s), dim3(kThreads), 0, s, d.blocks,
@@ -2445,11 +2446,11 @@ hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const do
 }
 
 hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
-                         hipStream_t s) {
+                         hipStream_t s, bool w_zero) {
   switch (prec) {
-    case 0: return reduce_T<double>(d, 0, V, d.H, W, s);
-    case 1: return reduce_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.H, W, s);
-    default: return reduce_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s);
+    case 0: return reduce_T<double>(d, 0, V, d.H, W, s, w_zero);
+    case 1: return reduce_T<float>(d, 1, (const float*)lp.vin, (const float*)lp.H, W, s, w_zero);
+    default: return reduce_T<Bf16x2>(d, 2, (const Bf16x2*)lp.vin, (const Bf16x2*)lp.H, W, s, w_zero);
   }
 }
 
@@ -2526,7 +2527,7 @@ static hipError_t fused_hv_T(const DevicePlan& d, int prec, const T* V, const T*
 }
 
 hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
-                     hipStream_t s) {
+                     hipStream_t s, bool w_zero) {
   hipError_t e = launch_convert(d, lp, prec, V, s);
   if (e != hipSuccess) return e;
   if (d.fused) {
@@ -2542,7 +2543,7 @@ hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const do
     }
   }
   if (d.atomic) {
-    e = launch_reduce(d, lp, prec, V, W, s);
+    e = launch_reduce(d, lp, prec, V, W, s, w_zero);
     if (e != hipSuccess) return e;
     return launch_pn(d, lp, prec, V, W, s);
   }

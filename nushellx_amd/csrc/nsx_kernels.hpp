@@ -176,8 +176,10 @@ std::string build_lowp(const HostPlan& hp, int precision, LowpPlan& lp, bool pan
 void free_lowp(LowpPlan& lp);
 
 // prec: 0 FP64, 1 FP32, 2 BF16 hi/lo.  launch_hv = convert + (pn, reduce) in mode order.
+// w_zero: the caller guarantees W == 0 on entry (the paths that accumulate
+// every term into W then skip clearing it).
 hipError_t launch_hv(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
-                     hipStream_t s);
+                     hipStream_t s, bool w_zero = false);
 hipError_t launch_convert(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, hipStream_t s);
 // panel units of Bf16x2 blocks (256 units per chunk)
 hipError_t launch_panels(const PanelChunk* chunks, int n_chunks, const PanelBlk* blks, const void* src, uint4* dst,
@@ -187,7 +189,7 @@ hipError_t convert_to(int prec, const double* x, void* y, int64_t n, hipStream_t
 hipError_t launch_pn(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
                      hipStream_t s);
 hipError_t launch_reduce(const DevicePlan& d, const LowpPlan& lp, int prec, const double* V, double* W,
-                         hipStream_t s);
+                         hipStream_t s, bool w_zero = false);
 int num_staged();
 const void* k_pn_staged_ptr(int i);
 const void* k_pn_partial_ptr(int prec, bool atomic);

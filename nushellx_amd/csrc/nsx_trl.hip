@@ -173,10 +173,12 @@ __global__ __launch_bounds__(kVT) void k_seg_sub_norm(double* __restrict__ w, co
 // Finishes step j: beta = sqrt(sum of the instance's nrm slots), alpha = c1[j] + c2[j]; the row that
 // starts an instance records alpha/beta, marks a breakdown (dim[i] = j + 1: an
 // invariant subspace was reached) and clears the other norm slot for step
-// j + 1; every row writes q_{j+1} = w / beta (0 once the instance stopped).
+// j + 1; every row writes q_{j+1} = w / beta (0 once the instance stopped)
+// and clears w, the output of the next step's H*v (which then needs no
+// clearing kernel of its own).
 // A row reading dim after the starting row updated it computes the same zero
 // scale, since the update happens only on breakdown.
-__global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, const double* __restrict__ w,
+__global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, double* __restrict__ w,
                                                     const int64_t* __restrict__ seg, const int* __restrict__ inst,
                                                     int64_t n, const double* __restrict__ c1,
                                                     const double* __restrict__ c2, int ldc, int j,
@@ -202,6 +204,7 @@ __global__ __launch_bounds__(kVT) void k_seg_finish(double* __restrict__ q, cons
     for (int q = 0; q < kNrmSlots; ++q) nrm_next[(int64_t)i * kNrmSlots + q] = 0.0;
   }
   q[r] = ok ? w[r] / b : 0.0;
+  w[r] = 0.0;
 }
 
 // truncated bases: W = 0 outside the N-hbar-omega window after every H*v, as
@@ -211,8 +214,8 @@ __global__ __launch_bounds__(kVT) void k_mask(double* __restrict__ w, const uint
   if (r < n && !keep[r]) w[r] = 0.0;
 }
 
-__global__ __launch_bounds__(kVT) void k_fill0(double* __restrict__ x, int n) {
-  const int i = blockIdx.x * kVT + threadIdx.x;
+__global__ __launch_bounds__(kVT) void k_fill0(double* __restrict__ x, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)kVT + threadIdx.x;
   if (i < n) x[i] = 0.0;
 }
 
@@ -443,11 +446,13 @@ static int trl_run(const DevicePlan& d, const LowpPlan& lp, int prec, hipStream_
   };
   // Lanczos steps [j0, j1) on the stream: H*v, two Gram-Schmidt passes, normalise
   auto launch_steps = [&](int j0, int j1) -> hipError_t {
-    // norm accumulator of the block's first step (later steps are cleared by k_seg_finish)
+    // norm accumulator of the block's first step and its H*v output (later
+    // steps: both cleared by k_seg_finish)
     hipLaunchKernelGGL(k_fill0, dim3((B * kNrmSlots + kVT - 1) / kVT), dim3(kVT), 0, s,
-                       scale.as<double>() + (size_t)(j0 & 1) * B * kNrmSlots, B * kNrmSlots);
+                       scale.as<double>() + (size_t)(j0 & 1) * B * kNrmSlots, (int64_t)B * kNrmSlots);
+    hipLaunchKernelGGL(k_fill0, gv, dim3(kVT), 0, s, W.as<double>(), N);
     for (int j = j0; j < j1; ++j) {
-      hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s);
+      hipError_t e = launch_hv(d, lp, prec, Q.as<double>() + ldq * j, W.as<double>(), s, true);
       if (e != hipSuccess) return e;
       if (any_masked) hipLaunchKernelGGL(k_mask, gv, dim3(kVT), 0, s, W.as<double>(), kmask.as<uint8_t>(), N);
       dots(j + 1, c1.as<double>());
