@@ -28,6 +28,31 @@ def test_pf44_hv_matches_oracle(pf44):
         h.close()
 
 
+@pytest.mark.parametrize("group", ["16", "0", "1,4"])
+def test_pf44_bf16_panel_tiles(pf44, monkeypatch, group):
+    """BF16 hi/lo large blocks on the panel tiles (operators and V_R as
+    step-major hi/lo units, tiles in XCD runs of the given lengths) against the
+    tiles on the Bf16x2 stores: the same products in the same order per tile,
+    so the two agree to the FP32 accumulation's rounding (the FP64 atomic adds
+    may come in another order); both against the FP64 oracle."""
+    from nushellx_amd.hv import Hamiltonian
+    args = pf44.args(0)
+    ref = RefCPU(*args, fp64=True)
+    v = np.random.default_rng(8).standard_normal(ref.N)
+    w_ref = ref.apply(v)
+    out = {}
+    for panels in ("1", "0"):
+        print(f"case pf44 bf16 panels={panels} group={group}", flush=True)
+        monkeypatch.setenv("NSX_BF_PANELS", panels)
+        monkeypatch.setenv("NSX_BIG_XCD_GROUP", group)
+        h = Hamiltonian(*args, precision="bf16x3")
+        out[panels] = h.multiply(v)
+        h.close()
+        err = np.abs(out[panels] - w_ref).max() / np.abs(w_ref).max()
+        assert err < 2e-4, (panels, err)
+    assert np.abs(out["1"] - out["0"]).max() <= 1e-9 * np.abs(w_ref).max()
+
+
 def test_pf44_autotuned_large_block_tiles(pf44):
     """The autotuner also picks the large-block tile shape (stage-2 K chunk;
     stage-1 rows in BF16 hi/lo); the tuned plan still equals the oracle."""
