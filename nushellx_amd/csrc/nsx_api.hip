@@ -90,6 +90,15 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
     if (!(e = up(d.Bn, hp.Bn)).empty()) return e;
     d.n_mftask = (int)hp.f_mtask.size();
     for (int x = 0; x < 9; ++x) d.mxf_xcd.s[x] = hp.mxf_xcd[size_t(x)];
+    if (hp.mxf_det) {  // deterministic: partial tiles summed in task order
+      if (!(e = up(d.f_dtile, hp.f_dtile)).empty()) return e;
+      if (!(e = up(d.f_dpart, hp.f_dpart)).empty()) return e;
+      const size_t np = (size_t)hp.f_mtask.size() * (size_t)hp.mxf_slots * 256;
+      if (hipMalloc(&d.f_dpsum, sizeof(double) * std::max<size_t>(np, 1)) != hipSuccess)
+        return "hipMalloc failed (deterministic partials)";
+      d.n_dtile = (int)hp.f_dtile.size();
+      d.mxf_det = 1;
+    }
     d.mx_ks = hp.mx_ks;
     d.mxf_slots = hp.mxf_slots;
     d.mxf_like = hp.mxf_like ? 1 : 0;
@@ -187,6 +196,7 @@ void free_plan(DevicePlan& d) {
   (void)hipFree(d.s_xref); (void)hipFree(d.X);
   (void)hipFree(d.m_task1); (void)hipFree(d.m_unit1); (void)hipFree(d.m_task2);
   (void)hipFree(d.f_mtask); (void)hipFree(d.f_mstep); (void)hipFree(d.f_mu); (void)hipFree(d.Bn);
+  (void)hipFree(d.f_dtile); (void)hipFree(d.f_dpart); (void)hipFree(d.f_dpsum);
   (void)hipFree(d.g_task1); (void)hipFree(d.g_tile1); (void)hipFree(d.g_tile2); (void)hipFree(d.g_term2);
   (void)hipFree(d.g_tile3);
   (void)hipFree(d.g_xref); (void)hipFree(d.XG);
@@ -492,8 +502,9 @@ int nsx_batch_path(const nsx_batch* b, int32_t* path) {
   if (!b || !path) { set_error("null argument"); return NSX_EARG; }
   *path = b->dplan.fused ? 2 : b->dplan.stacked ? 1 : b->dplan.mstk ? 3 : b->dplan.mxf ? 8 : 0;
   if (b->dplan.mxf && b->dplan.mxf_like) *path |= 16;  // the like-particle part rides in the fused path
-  // deterministic: the fused path for the small blocks and unsplit large tiles
-  if (b->dplan.fused && b->hplan.cfg.deterministic) *path |= 4;
+  // deterministic: the fused column path or the fused matrix-core path with
+  // partials for the small blocks, and unsplit large tiles
+  if ((b->dplan.fused && b->hplan.cfg.deterministic) || b->dplan.mxf_det) *path |= 4;
   return NSX_OK;
 }
 
@@ -617,21 +628,32 @@ void plan_models(const std::vector<const Model*>& models, HostPlan& hp, int prec
   // so that small models exercise it
   if (const char* v = std::getenv("NSX_BIG_MIN")) hp.cfg.big_min = std::min(kSmallMax, std::max(1, std::atoi(v)));
   if (const char* v = std::getenv("NSX_DETERMINISTIC")) hp.cfg.deterministic = std::atoi(v) != 0;
-  if (hp.cfg.deterministic) {
-    if (hp.cfg.stacked != 2) {  // the fused shape measured fastest on sd 4p4n J=0..8
-      hp.cfg.stk_cap = 6144;
-      hp.cfg.stk_work = 250000;
-      hp.cfg.f_rounds = 1;
-    }
+  // deterministic: the fused matrix-core path with partials (the fused column
+  // path where a tuning result asks for it, or a slab does not fit the tiles)
+  auto det_column_shape = [&]() {  // the fused column shape measured fastest on sd 4p4n J=0..8
     hp.cfg.stacked = 2;
+    hp.cfg.stk_cap = 6144;
+    hp.cfg.stk_work = 250000;
+    hp.cfg.f_rounds = 1;
+  };
+  if (hp.cfg.deterministic) {
+    if (hp.cfg.stacked != 2 && hp.cfg.stacked != 4) {
+      hp.cfg.stacked = 4;
+      hp.cfg.mxf_steps = 2;
+      hp.cfg.mxf_slots = kMfSlots;
+    }
     hp.cfg.big_kchunk = 1 << 30;
   }
   auto finish = [&]() {
     if (hp.cfg.stacked == 1) build_stacked(hp, models);
     if (hp.cfg.stacked == 3) build_mstk(hp, models);
     if (hp.cfg.stacked == 4 && !build_mxf(hp, models)) {  // a slab of another shape: the stacked path
-      hp.cfg.stacked = 1;
-      build_stacked(hp, models);
+      if (hp.cfg.deterministic) {
+        det_column_shape();
+      } else {
+        hp.cfg.stacked = 1;
+        build_stacked(hp, models);
+      }
     }
     build_big(hp, models);
     // fused column path, or the staged kernel where a column exceeds its limits

@@ -2078,7 +2078,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<K
                                              const MFStep* __restrict__ steps, const MFU* __restrict__ us,
                                              const T* __restrict__ Vp, const T* __restrict__ Ad,
                                              const T* __restrict__ Bt, double* __restrict__ W, EnsStride es,
-                                             XcdSplit xs) {
+                                             XcdSplit xs, double* __restrict__ P) {
   using C = typename BigIn<T>::C;
   using M = BigMma<C>;
   using vec = typename M::vec;
@@ -2192,8 +2192,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<K
   for (int q = 0; q < SL; ++q) {
     if (t.w_off[q] < 0) continue;
     const C sum = (red[0][q][v][lane] + red[1][q][v][lane]) + (red[2][q][v][lane] + red[3][q][v][lane]);
-    if (l16 < t.rows_i[q] && jr < t.cols_j) unsafeAtomicAdd(W + t.w_off[q] + l16 + (int64_t)t.da[q] * jr, (double)sum);
+    if (P)  // deterministic: the task's partial tile, element row + 16 col
+      P[((int64_t)ti * SL + q) * 256 + l16 + 16 * jr] = (double)sum;
+    else if (l16 < t.rows_i[q] && jr < t.cols_j)
+      unsafeAtomicAdd(W + t.w_off[q] + l16 + (int64_t)t.da[q] * jr, (double)sum);
   }
+}
+
+// Deterministic fused path: one workgroup per W tile adds the tile's partial
+// tiles in task order onto W (one addend onto the stored value)
+__global__ __launch_bounds__(256) void k_mxf_sum(const MFDTile* __restrict__ tiles, const int32_t* __restrict__ parts,
+                                                 const double* __restrict__ P, double* __restrict__ W) {
+  const MFDTile t = tiles[blockIdx.x];
+  const int e = threadIdx.x, r = e & 15, c = e >> 4;
+  if (r >= t.rows || c >= t.cols) return;
+  double s = 0.0;
+  for (int k = t.c_begin; k < t.c_end; ++k) s += P[(int64_t)parts[k] * 256 + e];
+  W[t.w_off + r + (int64_t)t.da * c] += s;
 }
 
 // ---------------------------------------------------------------- instance tables
@@ -2300,13 +2315,16 @@ static hipError_t pn_T(const DevicePlan& d, int prec, const T* V, const T* Vp, c
       const unsigned gy = d.ens_s > 1 ? (unsigned)((d.ens_s + kXcds - 1) / kXcds * kXcds) : 1u;
 #define NSX_MXF(KS, SL)                                                                                   \
   hipLaunchKernelGGL((k_mxf<T, KS, SL>), dim3(g, gy), dim3(256), 0, s, d.f_mtask, d.n_mftask, d.f_mstep, d.f_mu, \
-                     Vp, Ad, Bn, W, d.es, d.mxf_xcd)
+                     Vp, Ad, Bn, W, d.es, d.mxf_xcd, d.mxf_det ? d.f_dpsum : nullptr)
       if (d.mx_ks <= 6) {
         if (d.mxf_slots == 2) NSX_MXF(6, 2); else NSX_MXF(6, kMfSlots);
       } else {
         if (d.mxf_slots == 2) NSX_MXF(10, 2); else NSX_MXF(10, kMfSlots);
       }
 #undef NSX_MXF
+      if (d.mxf_det && d.n_dtile > 0)
+        hipLaunchKernelGGL(k_mxf_sum, dim3((unsigned)d.n_dtile), dim3(256), 0, s, d.f_dtile, d.f_dpart,
+                           (const double*)d.f_dpsum, W);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
     }

@@ -65,6 +65,11 @@ struct DevicePlan {
   MFU* f_mu = nullptr;
   int n_mftask = 0;
   XcdSplit mxf_xcd;          // the tasks' XCD ranges (HostPlan::mxf_xcd)
+  // deterministic fused path: per-task partial tiles and their W tiles
+  int mxf_det = 0, n_dtile = 0;
+  MFDTile* f_dtile = nullptr;
+  int32_t* f_dpart = nullptr;
+  double* f_dpsum = nullptr;  // n_mftask * mxf_slots * 256 partials
   // large-block GEMM path (FP64, atomic mode)
   GTask1* g_task1 = nullptr;
   GTile1* g_tile1 = nullptr;

@@ -528,6 +528,7 @@ void build_mstk(HostPlan& p, const std::vector<const Model*>& models) {
 
 static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models) {
   p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear(); p.bn_src.clear();
+  p.f_dtile.clear(); p.f_dpart.clear(); p.mxf_det = false;
   p.mx_ks = 1;
   // Bhat^T block of the Bh store -> its untransposed copy in Bn
   std::unordered_map<int64_t, int64_t> bn_of;
@@ -764,6 +765,29 @@ static void build_mxf_impl(HostPlan& p, const std::vector<const Model*>& models)
     for (int i : ord) sorted.push_back(p.f_mtask[size_t(i)]);
   }
   p.f_mtask.swap(sorted);
+  // deterministic mode: each W tile's partials, in task order
+  p.mxf_det = p.cfg.deterministic != 0;
+  if (p.mxf_det) {
+    const int sl = p.mxf_slots;
+    std::map<int64_t, MFDTile> tiles;
+    std::map<int64_t, std::vector<int32_t>> parts;
+    for (int i = 0; i < nt; ++i) {
+      const MFTask& t = p.f_mtask[size_t(i)];
+      for (int q = 0; q < sl; ++q) {
+        if (t.w_off[q] < 0) continue;
+        tiles[t.w_off[q]] = MFDTile{t.w_off[q], t.da[q], t.rows_i[q], t.cols_j, 0, 0, 0};
+        parts[t.w_off[q]].push_back(i * sl + q);
+      }
+    }
+    for (auto& kv : tiles) {
+      MFDTile d = kv.second;
+      d.c_begin = (int32_t)p.f_dpart.size();
+      const auto& pl = parts[kv.first];
+      p.f_dpart.insert(p.f_dpart.end(), pl.begin(), pl.end());
+      d.c_end = (int32_t)p.f_dpart.size();
+      p.f_dtile.push_back(d);
+    }
+  }
 }
 
 bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {
@@ -772,6 +796,7 @@ bool build_mxf(HostPlan& p, const std::vector<const Model*>& models) {
     build_mxf_impl(p, models);
   } catch (const std::exception&) {
     p.f_mtask.clear(); p.f_mstep.clear(); p.f_mu.clear(); p.Bn.clear(); p.bn_src.clear();
+    p.f_dtile.clear(); p.f_dpart.clear(); p.mxf_det = false;
     p.Ad.resize(ad0);
     p.mxf_like = false;
     return false;

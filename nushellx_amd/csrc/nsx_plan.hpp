@@ -183,6 +183,17 @@ struct MFU {                // one product of a slab
   int32_t dbr, pad;         // d(Jb')
   double r[kMfSlots];       // r_su (0 for slots not in this slab)
 };
+// Deterministic variant: every task stores its summed slot tiles as partials
+// (16 x 16, element row + 16 col, partial index task * slots + slot) instead
+// of adding them into W, and one workgroup per W tile adds the tile's
+// partials in task order onto W (cleared or holding the stored like-particle
+// part: one addend onto a stored value).
+struct MFDTile {
+  int64_t w_off;            // W_(Ja,Jb) + i0 + da * j0
+  int32_t da, rows, cols;   // leading dimension; valid rows and columns of the tile
+  int32_t c_begin, c_end;   // range of partial indices in f_dpart
+  int32_t pad;
+};
 
 // Fused small-block path (owner-computes, no atomics, deterministic):
 // one workgroup owns the output column Jb of one instance (the blocks
@@ -387,6 +398,9 @@ struct HostPlan {
   std::vector<MFTask> f_mtask;
   std::vector<MFStep> f_mstep;
   std::vector<MFU> f_mu;
+  bool mxf_det = false;          // deterministic fused path: tasks store partials (cfg.deterministic)
+  std::vector<MFDTile> f_dtile;  // W tiles with partials, in order of their offsets
+  std::vector<int32_t> f_dpart;  // their partial indices, in task order
   int mxf_slots = kMfSlots;      // slots the plan's tasks use (2 or 4)
   // the Bhat blocks the fused path reads, untransposed (db x dbr column-major),
   // so that a row of lanes reads 16 consecutive elements (and, with the

@@ -310,11 +310,12 @@ class Batch:
         """The small-block pn path of the plan: "staged", "stacked", "fused",
         "stacked-mfma" (the stacked stages on the matrix cores) or "fused-mfma"
         (both stages in one matrix-core kernel; " +like" when it also carries
-        the like-particle part)."""
+        the like-particle part), with " deterministic" for the plans of the
+        deterministic mode."""
         v = C.c_int32()
         check(lib().nsx_batch_path(self._b, C.byref(v)), "nsx_batch_path")
         if v.value & 8:
-            return "fused-mfma" + (" +like" if v.value & 16 else "")
+            return "fused-mfma" + (" +like" if v.value & 16 else "") + (" deterministic" if v.value & 4 else "")
         return ("staged", "stacked", "fused", "stacked-mfma")[v.value & 3] + (" deterministic" if v.value & 4 else "")
 
     def set_config(self, cfg: dict) -> None:

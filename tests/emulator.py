@@ -38,7 +38,7 @@ def lib():
         L.emu_hv_mstk.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int,
                                                      C.c_void_p]
         L.emu_hv_mxf.argtypes = [C.c_char_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int,
-                                                    C.c_void_p]
+                                                    C.c_int, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -151,7 +151,7 @@ def emu_hv_mstk(args, v, units1=4, terms2=16, big_min=0):
     return out, st
 
 
-def emu_hv_mxf(args, v, steps=2, slots=4, like=1):
+def emu_hv_mxf(args, v, steps=2, slots=4, like=1, det=0):
     """Emulated H*v through the fused matrix-core plan (lane-level model of
     k_mxf); returns (w, [tasks, steps, products, largest inner size in K steps
     + 100 when the like-particle part is folded into the tasks])."""
@@ -159,7 +159,7 @@ def emu_hv_mxf(args, v, steps=2, slots=4, like=1):
     out = np.empty_like(v)
     st = np.zeros(4, dtype=np.int64)
     rc = lib().emu_hv_mxf(*[a.encode() for a in args], v.ctypes.data, out.ctypes.data, len(v), steps, slots, like,
-                          st.ctypes.data)
+                          det, st.ctypes.data)
     if rc:
         raise RuntimeError(lib().emu_last_error().decode())
     return out, st

@@ -402,7 +402,7 @@ extern "C" int emu_hv_mstk(const char* nus, const char* opa, const char* opb, co
 // K steps, the four wavefronts' tiles summed; every memory index checked.
 extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, const char* intf,
                           const double* vin, double* vout, int64_t n, int steps_per_wave, int slots,
-                          int like, int64_t* stats4) {
+                          int like, int det, int64_t* stats4) {
   Model m;
   std::string e = build_model(nus, opa, opb, intf, m);
   if (!e.empty()) { g_err = e; return -1; }
@@ -412,9 +412,14 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
   p.cfg.mxf_steps = steps_per_wave;
   p.cfg.mxf_slots = slots;
   p.cfg.mxf_like = like;
+  p.cfg.deterministic = det;
   plan_add(p, m);
   std::vector<const Model*> ms{&m};
   if (!build_mxf(p, ms)) { g_err = "build_mxf refused the model"; return -6; }
+  if (p.mxf_det != (det != 0)) { g_err = "deterministic tables not built"; return -8; }
+  const int sl = p.mxf_slots;
+  // deterministic mode: the tasks' partial tiles (element row + 16 col)
+  std::vector<double> P(p.mxf_det ? p.f_mtask.size() * size_t(sl) * 256 : 0, 0.0);
   if (p.mx_ks > 10) { g_err = "inner size above 40"; return -3; }
   std::vector<double> vm;
   const double* vp = pn_input(p, vin, vm);
@@ -499,13 +504,35 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
       for (int l = 0; l < 64; ++l)
         for (int v = 0; v < 4; ++v) {
           const int jr = Mfma16::row(l, v), i = l & 15;
-          if (i < t.rows_i[q] && jr < t.cols_j) {
+          if (p.mxf_det) {
+            if (q >= sl) { oob = true; continue; }
+            P[(size_t(ti) * sl + q) * 256 + i + 16 * jr] = (Wt[0][q][l][v] + Wt[1][q][l][v]) + (Wt[2][q][l][v] + Wt[3][q][l][v]);
+          } else if (i < t.rows_i[q] && jr < t.cols_j) {
             const int64_t iw = t.w_off[q] + i + int64_t(t.da[q]) * jr;
             if (chk(iw, n))
               vout[iw] += (Wt[0][q][l][v] + Wt[1][q][l][v]) + (Wt[2][q][l][v] + Wt[3][q][l][v]);
           }
         }
     }
+  }
+  if (p.mxf_det) {  // k_mxf_sum: each W tile's partials in task order, each partial of a used slot once
+    std::vector<int> used(P.size() / 256, 0);
+    for (const MFDTile& d : p.f_dtile) {
+      for (int e = 0; e < 256; ++e) {
+        const int r = e & 15, c = e >> 4;
+        if (r >= d.rows || c >= d.cols) continue;
+        double sum = 0.0;
+        for (int k = d.c_begin; k < d.c_end; ++k)
+          if (chk(p.f_dpart[size_t(k)], (int64_t)used.size())) sum += P[size_t(p.f_dpart[size_t(k)]) * 256 + e];
+        const int64_t iw = d.w_off + r + int64_t(d.da) * c;
+        if (chk(iw, n)) vout[iw] += sum;
+      }
+      for (int k = d.c_begin; k < d.c_end; ++k)
+        if (chk(p.f_dpart[size_t(k)], (int64_t)used.size())) ++used[size_t(p.f_dpart[size_t(k)])];
+    }
+    for (size_t i = 0; i < p.f_mtask.size(); ++i)
+      for (int q = 0; q < sl; ++q)
+        if ((p.f_mtask[i].w_off[q] >= 0) != (used[i * sl + q] == 1)) { g_err = "a partial tile not summed exactly once"; return -8; }
   }
   if (stats4) {
     stats4[0] = (int64_t)p.f_mtask.size();

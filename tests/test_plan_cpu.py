@@ -89,7 +89,9 @@ def test_mxf_plan_matches_refcpu(models):
     with the like-particle part stored first or folded into the tasks (two
     more slabs per block: V I against H_a, V H_b^T against I).  The tasks are
     reached through the kernel's workgroup map (workgroup b: the (b / 8)-th
-    task of XCD b % 8's range), which must run every task exactly once."""
+    task of XCD b % 8's range), which must run every task exactly once; in
+    the deterministic variant every used slot's partial tile is summed into
+    its W tile exactly once."""
     from tests.emulator import emu_hv_mxf
     rng = np.random.default_rng(23)
     cases = [("sd22", 0), ("sd22", 4), ("sd44", 0), ("sd44", 8), ("sd66", 4), ("sd23", 3), ("psd33t", 2)]
@@ -98,12 +100,13 @@ def test_mxf_plan_matches_refcpu(models):
         r = RefCPU(*args, fp64=True)
         v = rng.standard_normal(r.N)
         ref = r.apply(v)
-        for steps, slots, like in ((0, 4, 0), (1, 4, 1), (3, 2, 1), (1, 2, 0)):
-            w, st = emu_hv_mxf(args, v, steps, slots, like)
+        for steps, slots, like, det in ((0, 4, 0, 0), (1, 4, 1, 0), (3, 2, 1, 0), (1, 2, 0, 0), (2, 4, 1, 1),
+                                        (1, 2, 0, 1)):
+            w, st = emu_hv_mxf(args, v, steps, slots, like, det)
             assert st[0] > 0 and st[1] > 0
             # folded where the plan allows it: not on the truncated basis
             assert (st[3] >= 100) == (like == 1 and name != "psd33t"), (name, st)
-            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, steps, slots, like)
+            assert np.abs(w - ref).max() <= 1e-12 * np.abs(ref).max(), (name, J2, steps, slots, like, det)
 
 
 def test_fused_plan_matches_refcpu(models):
