@@ -135,7 +135,8 @@ typedef struct {
                           3: the stacked stages on the matrix cores (16 x 16 tiles in
                           registers, no LDS staging; atomic accumulation);
                           4: both stages fused in one matrix-core kernel (X and Y
-                          stay in registers; atomic accumulation) */
+                          stay in registers; atomic accumulation, or per-task
+                          partial tiles summed in a fixed order when deterministic) */
   int32_t stk_cap;     /* stacked path: doubles a chunk stages in LDS (0 = default);
                           fused path: LDS elements of one round;
                           matrix-core stacked path: a-terms per stage-2 workgroup;
@@ -145,8 +146,11 @@ typedef struct {
                           matrix-core stacked path: products per stage-1 wavefront;
                           fused matrix-core path: output slots per task (2 or 4; 0 = 4) */
   int32_t f_rounds;    /* fused path: rounds one workgroup takes at most (0 = default) */
-  int32_t deterministic; /* 1: bitwise reproducible H*v: the fused path for the small
-                            blocks and one stage-2 workgroup per large-block W tile
+  int32_t deterministic; /* 1: bitwise reproducible H*v: for the small blocks the
+                            fused matrix-core path with partial tiles (stacked 4, or
+                            any stacked value but 2) or the fused column path
+                            (stacked 2, and where a slab does not fit the matrix-core
+                            tiles), and one stage-2 workgroup per large-block W tile
                             (also set for any plan by NSX_DETERMINISTIC=1) */
   int32_t big_rows1;   /* large-block stage-1 tile rows: 64, or 128 (BF16 hi/lo only);
                           0 = by precision (128 for BF16 hi/lo, else 64) */
