@@ -2200,15 +2200,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mxf_waves<K
 }
 
 // Deterministic fused path: one workgroup per W tile adds the tile's partial
-// tiles in task order onto W (one addend onto the stored value)
+// tiles onto W (one addend onto the stored value).  The sum has a fixed
+// order: partial number n of the tile's list goes to accumulator n % 4, the
+// accumulators meet as (s0 + s1) + (s2 + s3); eight partials are loaded at a
+// time, so a thread's loads are in flight together.
 __global__ __launch_bounds__(256) void k_mxf_sum(const MFDTile* __restrict__ tiles, const int32_t* __restrict__ parts,
                                                  const double* __restrict__ P, double* __restrict__ W) {
   const MFDTile t = tiles[blockIdx.x];
   const int e = threadIdx.x, r = e & 15, c = e >> 4;
   if (r >= t.rows || c >= t.cols) return;
-  double s = 0.0;
-  for (int k = t.c_begin; k < t.c_end; ++k) s += P[(int64_t)parts[k] * 256 + e];
-  W[t.w_off + r + (int64_t)t.da * c] += s;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int k = t.c_begin;
+  for (; k + 8 <= t.c_end; k += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = P[(int64_t)parts[k + u] * 256 + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u & 3] += x[u];
+  }
+  for (int u = 0; k < t.c_end; ++k, ++u) s[u & 3] += P[(int64_t)parts[k] * 256 + e];
+  W[t.w_off + r + (int64_t)t.da * c] += (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 // ---------------------------------------------------------------- instance tables

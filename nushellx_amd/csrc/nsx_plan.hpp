@@ -185,9 +185,10 @@ struct MFU {                // one product of a slab
 };
 // Deterministic variant: every task stores its summed slot tiles as partials
 // (16 x 16, element row + 16 col, partial index task * slots + slot) instead
-// of adding them into W, and one workgroup per W tile adds the tile's
-// partials in task order onto W (cleared or holding the stored like-particle
-// part: one addend onto a stored value).
+// of adding them into W, and one workgroup per W tile sums the tile's
+// partials (listed in task order) in a fixed order and adds the sum onto W
+// (cleared or holding the stored like-particle part: one addend onto a stored
+// value).
 struct MFDTile {
   int64_t w_off;            // W_(Ja,Jb) + i0 + da * j0
   int32_t da, rows, cols;   // leading dimension; valid rows and columns of the tile

@@ -515,17 +515,18 @@ extern "C" int emu_hv_mxf(const char* nus, const char* opa, const char* opb, con
         }
     }
   }
-  if (p.mxf_det) {  // k_mxf_sum: each W tile's partials in task order, each partial of a used slot once
+  if (p.mxf_det) {  // k_mxf_sum: each W tile's partials in its fixed order, each partial of a used slot once
     std::vector<int> used(P.size() / 256, 0);
     for (const MFDTile& d : p.f_dtile) {
       for (int e = 0; e < 256; ++e) {
         const int r = e & 15, c = e >> 4;
         if (r >= d.rows || c >= d.cols) continue;
-        double sum = 0.0;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // partial n of the list into accumulator n % 4
         for (int k = d.c_begin; k < d.c_end; ++k)
-          if (chk(p.f_dpart[size_t(k)], (int64_t)used.size())) sum += P[size_t(p.f_dpart[size_t(k)]) * 256 + e];
+          if (chk(p.f_dpart[size_t(k)], (int64_t)used.size()))
+            acc[(k - d.c_begin) & 3] += P[size_t(p.f_dpart[size_t(k)]) * 256 + e];
         const int64_t iw = d.w_off + r + int64_t(d.da) * c;
-        if (chk(iw, n)) vout[iw] += sum;
+        if (chk(iw, n)) vout[iw] += (acc[0] + acc[1]) + (acc[2] + acc[3]);
       }
       for (int k = d.c_begin; k < d.c_end; ++k)
         if (chk(p.f_dpart[size_t(k)], (int64_t)used.size())) ++used[size_t(p.f_dpart[size_t(k)])];
