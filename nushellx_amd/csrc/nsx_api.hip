@@ -90,7 +90,7 @@ std::string upload_plan(const HostPlan& hp, DevicePlan& d, int precision) {
     if (!(e = up(d.Bn, hp.Bn)).empty()) return e;
     d.n_mftask = (int)hp.f_mtask.size();
     for (int x = 0; x < 9; ++x) d.mxf_xcd.s[x] = hp.mxf_xcd[size_t(x)];
-    if (hp.mxf_det) {  // deterministic: partial tiles summed in task order
+    if (hp.mxf_det) {  // deterministic: per-task partial tiles, summed per W tile in a fixed order
       if (!(e = up(d.f_dtile, hp.f_dtile)).empty()) return e;
       if (!(e = up(d.f_dpart, hp.f_dpart)).empty()) return e;
       const size_t np = (size_t)hp.f_mtask.size() * (size_t)hp.mxf_slots * 256;
